@@ -1,0 +1,305 @@
+/*
+ * vfd.h - C ABI of the MI355X-native DFSPH step (HIP, gfx950).
+ *
+ * This is the drop-in boundary for VFD's DFSPH simulation component. The reference exposes it as a C++ class,
+ * DFSPHSimulation (VFD/Source/Simulation/DFSPH/DFSPHSimulator.h:10-56), forwarding to DFSPHImplementation
+ * (VFD/Source/Simulation/DFSPH/DFSPHImplementation.h:23-137). Every entry point below names the reference member it
+ * replaces. The ABI uses plain pointers and sizes only; every call returns an int status (0 = VFD_OK) and stores a
+ * message for vfd_last_error().
+ *
+ * Data conventions
+ *   - vec3 arrays are tightly packed float[3*n] (x0,y0,z0,x1,...), in the caller's ORIGINAL particle order. The
+ *     library keeps particles internally in Morton (cell) order and maps back on every read.
+ *   - A vfd_sim holds one or more scenes ("batch"). The plain functions act on scene 0; vfd_batch_* take a scene index.
+ *   - One vfd_sim must not be used from two threads at once. Work is enqueued on the stream set with vfd_set_stream
+ *     (default: the null stream of the current device).
+ */
+#ifndef VFD_H
+#define VFD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFD_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+    VFD_OK = 0,
+    VFD_ERROR_INVALID_ARGUMENT = 1,
+    VFD_ERROR_HIP = 2,
+    VFD_ERROR_STATE = 3,
+    VFD_ERROR_OUT_OF_MEMORY = 4,
+    VFD_ERROR_NOT_SUPPORTED = 5,
+    VFD_ERROR_NONFINITE = 6
+};
+
+/* Mirrors DFSPHSimulationDescription (VFD/Source/Simulation/DFSPH/Structures/DFSPHSimulationDescription.h:9-50),
+ * all 26 fields, same names and meaning. Booleans are int32 (0/1). Tolerances are percentages, as in the reference. */
+typedef struct vfd_desc {
+    /* Time step */
+    float TimeStepSize;
+    float MinTimeStepSize;
+    float MaxTimeStepSize;
+    float FrameLength;
+    uint32_t FrameCount;
+    /* Pressure solver */
+    uint32_t MinPressureSolverIterations;
+    uint32_t MaxPressureSolverIterations;
+    float MaxPressureSolverError;
+    /* Divergence solver */
+    int32_t EnableDivergenceSolverError;
+    uint32_t MinDivergenceSolverIterations;
+    uint32_t MaxDivergenceSolverIterations;
+    float MaxDivergenceSolverError;
+    /* Viscosity solver */
+    int32_t EnableViscositySolver;
+    uint32_t MinViscositySolverIterations;
+    uint32_t MaxViscositySolverIterations;
+    float MaxViscositySolverError;
+    float Viscosity;
+    float BoundaryViscosity;
+    float TangentialDistanceFactor;
+    /* Surface tension solver */
+    int32_t EnableSurfaceTensionSolver;
+    uint32_t SurfaceTensionSmoothPassCount;
+    float SurfaceTension;
+    int32_t TemporalSmoothing;
+    int32_t CSDFix;
+    int32_t CSD;
+    /* Scene */
+    float ParticleRadius;
+    float Gravity[3];
+} vfd_desc;
+
+/* Mirrors DFSPHSimulationInfo (VFD/Source/Simulation/DFSPH/Structures/DFSPHSimulationInfo.h:5-44): the constants
+ * derived by SetDescription (DFSPHImplementation.cu:318-368) and ComputeTimeStepSize (:395-428). */
+typedef struct vfd_info {
+    uint32_t ParticleCount;
+    uint32_t RigidBodyCount;
+    float SupportRadius;
+    float SupportRadius2;
+    float ParticleRadius;
+    float ParticleDiameter;
+    float TimeStepSize;
+    float TimeStepSize2;
+    float TimeStepSizeInverse;
+    float TimeStepSize2Inverse;
+    float Volume;
+    float Density0;
+    float ParticleMass;
+    float ParticleMassInverse;
+    float Viscosity;
+    float BoundaryViscosity;
+    float DynamicViscosity;
+    float DynamicBoundaryViscosity;
+    float TangentialDistanceFactor;
+    float TangentialDistance;
+    float SurfaceTension;
+    uint32_t SurfaceTensionSampleCount;
+    float ClassifierSlope;
+    float ClassifierConstant;
+    int32_t TemporalSmoothing;
+    float SmoothingFactor;
+    float Factor;
+    float NeighborParticleRadius;
+    float MonteCarloFactor;
+    float Gravity[3];
+} vfd_info;
+
+/* Mirrors DFSPHDebugInfo (VFD/Source/Simulation/DFSPH/Structures/DFSPHDebugInfo.h:8-28). Phase timers are wall
+ * milliseconds of the last step measured with HIP events (the reference uses steady_clock around synchronised
+ * launches). The last four members are additions. */
+typedef struct vfd_debug_info {
+    uint32_t IterationCount;
+    uint32_t DivergenceSolverIterationCount;
+    uint32_t PressureSolverIterationCount;
+    uint32_t ViscositySolverIterationCount;
+    float DivergenceSolverError;
+    float PressureSolverError;
+    float ViscositySolverError;
+    float FrameTime;
+    uint32_t FrameIndex;
+    float NeighborhoodSearchTime;
+    float BaseSolverTime;
+    float DivergenceSolverTime;
+    float SurfaceTensionSolverTime;
+    float ViscositySolverTime;
+    float PressureSolverTime;
+    /* additions */
+    float TimeStepSize;
+    float MaxVelocityMagnitude;
+    uint32_t MaxNeighborCount;
+    uint64_t TotalNeighborCount;
+} vfd_debug_info;
+
+/* Boundary shapes. The reference builds its volume map from a triangle mesh (RigidBodyDescription,
+ * VFD/Source/Simulation/DFSPH/RigidBody/RigidBody.cuh:12-21; mesh distance in Utility/SDF/MeshDistance.cpp); this
+ * build takes the exact signed distance of an analytic shape instead. d(x) < 0 inside the shape. */
+enum {
+    VFD_SHAPE_BOX = 0,      /* Center, HalfExtents */
+    VFD_SHAPE_SPHERE = 1,   /* Center, Radius */
+    VFD_SHAPE_PLANE = 2,    /* Center = a point on the plane, Normal (unit, pointing out of the solid); needs a domain */
+    VFD_SHAPE_CYLINDER = 3  /* Center, Radius, HalfExtents[1] = half height; axis along y */
+};
+
+/* Mirrors RigidBodyDescription (RigidBody.cuh:12-21): Inverted, Padding, CollisionMapResolution, with the mesh and
+ * transform replaced by an analytic shape. The volume-map domain is the shape's bounds grown by 8*SupportRadius +
+ * (Padding - ParticleRadius), as in RigidBody.cu:35-37, unless HasDomain is set. */
+typedef struct vfd_boundary_desc {
+    int32_t Shape;
+    float Center[3];
+    float HalfExtents[3];
+    float Radius;
+    float Normal[3];
+    int32_t Inverted;
+    float Padding;
+    uint32_t CollisionMapResolution[3];
+    int32_t HasDomain;
+    float DomainMin[3];
+    float DomainMax[3];
+} vfd_boundary_desc;
+
+/* Per-particle fields readable with vfd_get_field (original order). Components per particle in brackets. The names
+ * are the DFSPHParticle members (VFD/Source/Simulation/DFSPH/Structures/DFSPHParticle.h:8-32) plus solver vectors. */
+enum {
+    VFD_FIELD_POSITION = 0,                  /* [3] */
+    VFD_FIELD_VELOCITY = 1,                  /* [3] */
+    VFD_FIELD_ACCELERATION = 2,              /* [3] */
+    VFD_FIELD_PRESSURE_ACCELERATION = 3,     /* [3] */
+    VFD_FIELD_PRESSURE_RESIDUUM = 4,         /* [1] */
+    VFD_FIELD_DENSITY = 5,                   /* [1] */
+    VFD_FIELD_DENSITY_ADVECTION = 6,         /* [1] */
+    VFD_FIELD_PRESSURE_RHO2 = 7,             /* [1] kappa of the density solve */
+    VFD_FIELD_PRESSURE_RHO2_V = 8,           /* [1] kappa of the divergence solve */
+    VFD_FIELD_FACTOR = 9,                    /* [1] */
+    VFD_FIELD_VELOCITY_DIFFERENCE = 10,      /* [3] */
+    VFD_FIELD_MC_SURFACE_NORMAL = 11,        /* [3] */
+    VFD_FIELD_MC_SURFACE_NORMAL_SMOOTH = 12, /* [3] */
+    VFD_FIELD_MC_SURFACE_CURVATURE = 13,     /* [1] */
+    VFD_FIELD_MC_SURFACE_CURVATURE_SMOOTH = 14, /* [1] */
+    VFD_FIELD_DELTA_FINAL_CURVATURE = 15,    /* [1] */
+    VFD_FIELD_BOUNDARY_XJ = 16,              /* [3] per body */
+    VFD_FIELD_BOUNDARY_VOLUME = 17,          /* [1] per body */
+    VFD_FIELD_VISCOSITY_SOLUTION = 18,       /* [3] x of the viscosity solve (ViscosityGradientG) */
+    VFD_FIELD_VISCOSITY_PRECONDITIONER = 19, /* [9] row-major inverse diagonal block */
+    VFD_FIELD_COUNT = 20
+};
+
+/* Step phases, in the order of DFSPHImplementation::OnUpdate (DFSPHImplementation.cu:63-170). vfd_step_until runs
+ * the step up to and including the named phase (a testing aid; the next vfd_step starts a fresh step). */
+enum {
+    VFD_PHASE_SEARCH = 0,
+    VFD_PHASE_BOUNDARY = 1,
+    VFD_PHASE_DENSITY = 2,
+    VFD_PHASE_FACTOR = 3,
+    VFD_PHASE_DIVERGENCE = 4,
+    VFD_PHASE_CLEAR_ACCELERATION = 5,
+    VFD_PHASE_SURFACE_TENSION = 6,
+    VFD_PHASE_VISCOSITY = 7,
+    VFD_PHASE_TIME_STEP = 8,
+    VFD_PHASE_VELOCITY = 9,
+    VFD_PHASE_PRESSURE = 10,
+    VFD_PHASE_POSITION = 11,
+    VFD_PHASE_COUNT = 12
+};
+
+typedef struct vfd_sim vfd_sim;
+
+/* Library */
+int vfd_abi_version(void);
+const char* vfd_last_error(void);
+/* Fills the reference defaults of DFSPHSimulationDescription.h:11-49. */
+int vfd_desc_default(vfd_desc* desc);
+
+/* DFSPHSimulation(desc) (DFSPHSimulator.cpp:10-18) / DFSPHImplementation ctor (DFSPHImplementation.cu:14-21). */
+int vfd_create(const vfd_desc* desc, vfd_sim** out);
+/* Batch of scene_count independent scenes advanced by one set of launches. Every scene must use the same
+ * ParticleRadius (one kernel table). No reference counterpart. */
+int vfd_batch_create(const vfd_desc* descs, uint32_t scene_count, vfd_sim** out);
+/* ~DFSPHImplementation (DFSPHImplementation.cu:23-31). */
+int vfd_destroy(vfd_sim* sim);
+/* hipStream_t passed as void*. No reference counterpart (the reference uses the default stream). */
+int vfd_set_stream(vfd_sim* sim, void* stream);
+int vfd_scene_count(const vfd_sim* sim, uint32_t* count);
+
+/* SetDescription / GetDescription / GetInfo (DFSPHImplementation.cu:313-373). */
+int vfd_set_description(vfd_sim* sim, const vfd_desc* desc);
+int vfd_get_description(const vfd_sim* sim, vfd_desc* desc);
+int vfd_get_info(const vfd_sim* sim, vfd_info* info);
+int vfd_batch_set_description(vfd_sim* sim, uint32_t scene, const vfd_desc* desc);
+int vfd_batch_get_info(const vfd_sim* sim, uint32_t scene, vfd_info* info);
+
+/* SetFluidObjects (DFSPHImplementation.cu:172-253): positions of every fluid object concatenated, one velocity per
+ * particle (the reference holds one per object, FluidObject.h:21-31; expand it per particle). */
+int vfd_set_fluid(vfd_sim* sim, const float* positions, const float* velocities, uint32_t count);
+int vfd_batch_set_fluid(vfd_sim* sim, uint32_t scene, const float* positions, const float* velocities,
+                        uint32_t count);
+
+/* RigidBody ctor + SetRigidBodies (RigidBody.cu:10-73, DFSPHImplementation.cu:255-271): builds the two-field volume
+ * map (signed distance, boundary volume) on the GPU. Call after vfd_set_fluid, as in the reference. */
+int vfd_add_boundary(vfd_sim* sim, const vfd_boundary_desc* body);
+int vfd_batch_add_boundary(vfd_sim* sim, uint32_t scene, const vfd_boundary_desc* body);
+int vfd_clear_boundaries(vfd_sim* sim, uint32_t scene);
+/* Number of map nodes and the node coefficients of one field (0 = distance, 1 = volume), node layout of
+ * SDF::AddFunction (Utility/SDF/SDF.cu:45-139). nodes may be NULL to query the count. */
+int vfd_get_boundary_map(vfd_sim* sim, uint32_t scene, uint32_t body, uint32_t field, float* nodes,
+                         uint32_t* node_count);
+
+/* DFSPHImplementation::OnUpdate (DFSPHImplementation.cu:63-170): one step of every scene. */
+int vfd_step(vfd_sim* sim);
+/* Runs `steps` steps. */
+int vfd_advance(vfd_sim* sim, uint32_t steps);
+/* DFSPHImplementation::Simulate (DFSPHImplementation.cu:33-61): restores the baked particles, resets dt and runs
+ * until FrameCount frames are captured. Synchronous (the reference runs it on a worker thread). */
+int vfd_simulate(vfd_sim* sim);
+/* Testing aid: runs the current step up to and including `phase` (VFD_PHASE_*). */
+int vfd_step_until(vfd_sim* sim, int32_t phase);
+/* Enables/disables frame capture in vfd_step (default on, as in OnUpdate :148-167). */
+int vfd_set_frame_capture(vfd_sim* sim, int32_t enabled);
+
+/* Reads and writes of particle state, original order. vfd_get_particles = ConvertParticlesToBuffer + copy
+ * (DFSPHKernels.cu:6-22); any output pointer may be NULL. */
+int vfd_get_particles(vfd_sim* sim, float* positions, float* velocities, float* accelerations);
+int vfd_batch_get_particles(vfd_sim* sim, uint32_t scene, float* positions, float* velocities,
+                            float* accelerations);
+int vfd_get_field(vfd_sim* sim, uint32_t scene, int32_t field, uint32_t body, float* out);
+/* Writes one persistent field (POSITION, VELOCITY, VELOCITY_DIFFERENCE, DELTA_FINAL_CURVATURE). */
+int vfd_set_field(vfd_sim* sim, uint32_t scene, int32_t field, const float* values);
+int vfd_get_particle_count(const vfd_sim* sim, uint32_t scene, uint32_t* count);
+
+/* NeighborSet (ParticleSearch/NeighborSet.h:8-30) in original ids: counts[n], offsets[n] (exclusive scan) and
+ * ids[total]. ids may be NULL to query total only. */
+int vfd_get_neighbors(vfd_sim* sim, uint32_t scene, uint32_t* counts, uint32_t* offsets, uint32_t* ids,
+                      uint64_t* total);
+
+/* GetDebugInfo (DFSPHImplementation.cu:390-393), GetMaxVelocityMagnitude, GetCurrentTimeStepSize. */
+int vfd_get_debug_info(vfd_sim* sim, uint32_t scene, vfd_debug_info* info);
+
+/* DFSPHParticleBuffer (ParticleBuffer/DFSPHParticleBuffer.h:13-45): number of captured frames and one frame's
+ * {x, v, a} (36 bytes per particle, DFSPHParticleSimple) with its MaxVelocityMagnitude and CurrentTimeStep. */
+int vfd_get_frame_count(vfd_sim* sim, uint32_t scene, uint32_t* count);
+int vfd_get_frame(vfd_sim* sim, uint32_t scene, uint32_t frame, float* xva, float* max_velocity, float* time_step);
+
+/* Per-kernel timing with HIP events on the simulation's stream (no reference counterpart; the reference times
+ * phases with steady_clock around synchronised launches, Debug/Timer.h:8-33). Enabling resets the statistics. */
+typedef struct vfd_kernel_stat {
+    char name[48];
+    uint32_t launches;
+    uint32_t pad;
+    double total_ms;
+} vfd_kernel_stat;
+int vfd_set_profiling(vfd_sim* sim, int32_t enabled);
+int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint32_t* count);
+
+/* PrecomputedDFSPHCubicKernel (Kernel/DFSPHKernels.h:7-143): W[10000], GradientW[10001] and
+ * {Radius, Radius2, InvStepSize, WZero, K, L}. */
+int vfd_get_kernel_table(vfd_sim* sim, float* w, float* gradient_w, float* scalars);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VFD_H */

@@ -1,0 +1,1719 @@
+// dfsph_oracle.cpp - CPU oracle for the DFSPH step. TEST INFRASTRUCTURE ONLY.
+//
+// This file restates, on the CPU, the algorithm of VFD's GPU DFSPH step (Goubermouche/VFD; the reference has no CPU
+// path of its own). It exists to check the HIP product: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and only as the checker. The product (vfd_amd/) never links or calls it.
+//
+// One template, two instantiations:
+//   T = float   "FP32 mirror": the reference's operation order in IEEE single precision, compiled with
+//               -ffp-contract=off. Every fused multiply-add is an explicit std::fma at the places named below.
+//   T = double  "FP64 twin": the same code in double precision, used to state each field's FP32 error.
+//
+// Arithmetic contract shared with the HIP kernels (the reference's nvcc --use_fast_math build leaves these
+// unpinned; this build fixes them):
+//   dot3(a,b)  = fma(a.z,b.z, fma(a.y,b.y, a.x*b.x))      (every dot, length2 and neighbour-search distance)
+//   length(a)  = sqrt(dot3(a,a)), IEEE sqrt and division (no fast math)
+//   reductions: Jacobi residual sums and CG dot products accumulate per-particle float terms in double,
+//               sequentially here (the reference uses thrust float reductions; `minus` makes its residual
+//               order-dependent, DFSPHImplementation.cu:483-489)
+//   neighbour order: 27 cells in z,y,x order (ParticleSearchKernels.cu:301-303), within a cell ascending particle
+//               index (the reference's order there follows atomicAdd, ParticleSearchKernels.cu:77)
+//
+// Parity status of each function is stated in the function comments ("pinned by ..." names the test that pins it).
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../include/vfd.h"
+
+namespace orc {
+
+// Core/Math/Math.h:6 - EPS is a double literal, compared against float values promoted to double.
+static const double kEps = 1.0e-5;
+static const unsigned kMaxNeighbors = 70u;      // ParticleSearchKernels.cuh:7
+static const unsigned kMetaGroup = 8u;          // ParticleSearchKernels.cuh:8
+static const unsigned kMetaBlock = 512u;        // ParticleSearchKernels.cuh:9
+static const unsigned kTableResolution = 10000u;  // Kernel/DFSPHKernels.h:143
+static const unsigned kHaltonCount = 16384u;    // HaltonVec323.cuh: 49152 floats = 16384 points
+static const unsigned kHaltonMod = 49152u;      // DFSPHKernels.cu:923
+
+// ---------------------------------------------------------------------------------------------------------------
+// Small vector / matrix types (glm semantics where the reference relies on them)
+// ---------------------------------------------------------------------------------------------------------------
+template <class T> struct V3 {
+    T x, y, z;
+};
+template <class T> inline V3<T> v3(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T> inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> inline V3<T> operator-(V3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class T> inline V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> inline V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class T> inline V3<T> operator/(V3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class T> inline V3<T> cmul(V3<T> a, V3<T> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class T> inline bool is_zero(V3<T> a) { return a.x == T(0) && a.y == T(0) && a.z == T(0); }
+template <class T> inline T dot3(V3<T> a, V3<T> b) { return std::fma(a.z, b.z, std::fma(a.y, b.y, a.x * b.x)); }
+template <class T> inline T length2(V3<T> a) { return dot3(a, a); }
+template <class T> inline T length(V3<T> a) { return std::sqrt(dot3(a, a)); }
+// glm::cross
+template <class T> inline V3<T> cross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+// glm::normalize(v) = v * inversesqrt(dot(v,v)), inversesqrt(x) = 1/sqrt(x)
+template <class T> inline V3<T> normalize(V3<T> a) { return a * (T(1) / std::sqrt(dot3(a, a))); }
+
+template <class T> struct M3 {  // row-major
+    T m[9];
+};
+template <class T> inline M3<T> mzero() { M3<T> r; for (auto& e : r.m) e = T(0); return r; }
+
+// 3x3 inverse by the adjugate: inv = adj(A) / det(A). Follows glm::inverse's use at DFSPHKernels.cu:683.
+template <class T> M3<T> inverse3(const M3<T>& A) {
+    const T a = A.m[0], b = A.m[1], c = A.m[2];
+    const T d = A.m[3], e = A.m[4], f = A.m[5];
+    const T g = A.m[6], h = A.m[7], i = A.m[8];
+    const T c00 = e * i - f * h, c01 = f * g - d * i, c02 = d * h - e * g;
+    const T det = a * c00 + b * c01 + c * c02;
+    const T s = T(1) / det;
+    M3<T> r;
+    r.m[0] = c00 * s;
+    r.m[1] = (c * h - b * i) * s;
+    r.m[2] = (b * f - c * e) * s;
+    r.m[3] = c01 * s;
+    r.m[4] = (a * i - c * g) * s;
+    r.m[5] = (c * d - a * f) * s;
+    r.m[6] = c02 * s;
+    r.m[7] = (b * g - a * h) * s;
+    r.m[8] = (a * e - b * d) * s;
+    return r;
+}
+template <class T> inline V3<T> mulmv(const M3<T>& A, V3<T> v) {  // glm mat3 * vec3 (row i = sum_j A_ij v_j)
+    return {A.m[0] * v.x + A.m[1] * v.y + A.m[2] * v.z, A.m[3] * v.x + A.m[4] * v.y + A.m[5] * v.z,
+            A.m[6] * v.x + A.m[7] * v.y + A.m[8] * v.z};
+}
+
+// Math.h:19-33 GetOrthogonalVectors
+template <class T> inline void orthogonal_vectors(V3<T> vec, V3<T>& x, V3<T>& y) {
+    V3<T> v = v3<T>(1, 0, 0);
+    if ((double)std::fabs(dot3(v, vec)) > 0.999) v = v3<T>(0, 1, 0);
+    x = cross(vec, v);
+    y = cross(vec, x);
+    x = normalize(x);
+    y = normalize(y);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Tabulated cubic spline kernel. Kernel/DFSPHKernels.h:7-143 (SetRadius :13-41, GetW :43-65, GetGradientW :67-80,
+// CalculateW :86-104, CalculateGradientW :110-131). Pinned by tests/test_oracle_kernel.py (closed-form spline,
+// normalisation integral, W(R)=0, gradient finite differences) and by GPU bit-equality of the tables.
+// ---------------------------------------------------------------------------------------------------------------
+template <class T> struct CubicKernel {
+    std::vector<T> W, G;  // W[10000], G[10001]
+    T R = 0, R2 = 0, invStep = 0, W0 = 0, K = 0, L = 0;
+
+    T calcW(T r) const {
+        T res = 0;
+        const T q = r / R;
+        if (q <= T(1)) {
+            if (q <= T(0.5)) {
+                const T q2 = q * q;
+                const T q3 = q2 * q;
+                res = K * (T(6) * q3 - T(6) * q2 + T(1));
+            } else {
+                const T q1 = T(1) - q;
+                const T q3 = q1 * q1 * q1;
+                res = K * (T(2) * q3);
+            }
+        }
+        return res;
+    }
+    V3<T> calcGradW(V3<T> r) const {
+        const T rl = std::sqrt(dot3(r, r));
+        const T q = rl / R;
+        if (rl > T(1.0e-5) && q <= T(1)) {
+            const V3<T> gq = r * (T(1) / (rl * R));
+            if (q <= T(0.5)) {
+                const T s = L * q * (T(3) * q - T(2));
+                return s * gq;
+            }
+            const T f = T(1) - q;
+            const T s = L * (-f * f);
+            return s * gq;
+        }
+        return v3<T>(0, 0, 0);
+    }
+    void setRadius(T radius) {
+        const T pi = T(3.14159265358979323846);
+        R = radius;
+        R2 = R * R;
+        const T r3 = R * R * R;
+        K = T(8) / (pi * r3);
+        L = T(48) / (pi * r3);
+        W0 = calcW(std::sqrt(dot3(v3<T>(0, 0, 0), v3<T>(0, 0, 0))));
+        const T step = R / T(kTableResolution - 1);
+        invStep = T(1) / step;
+        W.assign(kTableResolution, T(0));
+        G.assign(kTableResolution + 1, T(0));
+        for (unsigned i = 0; i < kTableResolution; ++i) {
+            const T posX = step * T(i);
+            W[i] = calcW(posX);
+            G[i] = posX > T(1.0e-9) ? calcGradW(v3<T>(posX, 0, 0)).x / posX : T(0);
+        }
+        G[kTableResolution] = T(0);
+    }
+    unsigned index(T rl) const {
+        const unsigned p = (unsigned)(rl * invStep);
+        return p < kTableResolution - 2u ? p : kTableResolution - 2u;
+    }
+    T getW(T r) const {  // scalar overload, cutoff r <= R
+        if (r <= R) {
+            const unsigned p = index(r);
+            return T(0.5) * (W[p] + W[p + 1]);
+        }
+        return T(0);
+    }
+    T getW(V3<T> r) const {  // vector overload, cutoff |r|^2 <= R^2
+        const T r2 = dot3(r, r);
+        if (r2 <= R2) {
+            const unsigned p = index(std::sqrt(r2));
+            return T(0.5) * (W[p] + W[p + 1]);
+        }
+        return T(0);
+    }
+    V3<T> gradW(V3<T> r) const {  // cutoff |r| <= R
+        const T rl = std::sqrt(dot3(r, r));
+        if (rl <= R) {
+            const unsigned p = index(rl);
+            const T s = T(0.5) * (G[p] + G[p + 1]);
+            return s * r;
+        }
+        return v3<T>(0, 0, 0);
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------------
+// Halton sphere points. The reference stores 16,384 precomputed points (HaltonVec323.cuh); they are the closed form
+// H[i] = (sqrt(1-z^2) cos 2*pi*phi2(i), sqrt(1-z^2) sin 2*pi*phi2(i), z = 1 - 2*phi3(i)), phi_b = radical inverse,
+// rounded to 15 decimals as printed in the reference table.
+// Pinned by tests/golden/halton_samples.json (values read from the reference table).
+// ---------------------------------------------------------------------------------------------------------------
+static double radical_inverse(unsigned base, unsigned i) {
+    double inv = 1.0 / base, f = inv, r = 0.0;
+    while (i > 0) {
+        r += f * (double)(i % base);
+        i /= base;
+        f *= inv;
+    }
+    return r;
+}
+template <class T> std::vector<T> halton_table() {
+    std::vector<T> h(kHaltonMod);
+    const double twopi = 6.283185307179586476925286766559;
+    for (unsigned i = 0; i < kHaltonCount; ++i) {
+        const double z = 1.0 - 2.0 * radical_inverse(3, i);
+        const double rxy = std::sqrt(std::max(0.0, 1.0 - z * z));
+        const double a = twopi * radical_inverse(2, i);
+        // the reference table holds 15 decimals; rounding there reproduces its values (e.g. 0 for cos(pi/2))
+        h[3 * i + 0] = (T)(std::nearbyint(rxy * std::cos(a) * 1e15) / 1e15);
+        h[3 * i + 1] = (T)(std::nearbyint(rxy * std::sin(a) * 1e15) / 1e15);
+        h[3 * i + 2] = (T)(std::nearbyint(z * 1e15) / 1e15);
+    }
+    return h;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Gauss-Legendre rule (GaussQuadrature.cpp tables, p = 30 -> 16 points). Computed by Newton iteration on P_n.
+// Pinned by tests/golden/gauss_legendre_16.json (values read from the reference table).
+// ---------------------------------------------------------------------------------------------------------------
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < n; ++i) {
+        double z = std::cos(pi * (i + 0.75) / (n + 0.5));
+        double dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+        }
+        x[n - 1 - i] = z;  // ascending order, like the reference table
+        w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+    for (int i = 0; i < n / 2; ++i) {  // exact symmetry
+        const double xs = 0.5 * (x[n - 1 - i] - x[i]);
+        x[i] = -xs;
+        x[n - 1 - i] = xs;
+        const double ws = 0.5 * (w[i] + w[n - 1 - i]);
+        w[i] = ws;
+        w[n - 1 - i] = ws;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// 32-node cubic serendipity shape functions on [-1,1]^3 (Discregrid layout): SDF.cu:403-...,
+// SDFDeviceData.cuh:36-370. Node order: 8 corners (bit0 = x, bit1 = y, bit2 = z), then x-edge, y-edge and z-edge
+// nodes at -1/3 and +1/3. Written from the polynomials (corner: (1/64)(1+-x)(1+-y)(1+-z)(9(x^2+y^2+z^2)-19);
+// edge: (9/64)(1-s^2)(1+-3s)(1+-a)(1+-b)). Pinned by tests/test_oracle_sdf.py (partition of unity, cubic
+// reproduction, nodal interpolation, gradient vs finite differences).
+// ---------------------------------------------------------------------------------------------------------------
+template <class T> void shape_function(V3<T> xi, T N[32], V3<T> dN[32]) {
+    const T x = xi.x, y = xi.y, z = xi.z;
+    const T x2 = x * x, y2 = y * y, z2 = z * z;
+    const T mx = T(1) - x, px = T(1) + x, my = T(1) - y, py = T(1) + y, mz = T(1) - z, pz = T(1) + z;
+    const T lx[2] = {mx, px}, ly[2] = {my, py}, lz[2] = {mz, pz};
+    const T sg[2] = {T(-1), T(1)};
+    // corners
+    const T q = T(9) * (x2 + y2 + z2) - T(19);
+    for (int c = 0; c < 8; ++c) {
+        const int ix = c & 1, iy = (c >> 1) & 1, iz = (c >> 2) & 1;
+        const T a = lx[ix], b = ly[iy], d = lz[iz];
+        N[c] = T(1) / T(64) * q * a * b * d;
+        // d/dx [a b d q] = sx*b*d*q + a*b*d*18x
+        dN[c].x = (sg[ix] * q + T(18) * x * a) * b * d / T(64);
+        dN[c].y = (sg[iy] * q + T(18) * y * b) * a * d / T(64);
+        dN[c].z = (sg[iz] * q + T(18) * z * d) * a * b / T(64);
+    }
+    // 1D cubic factors for edge nodes at s = -1/3 (m) and s = +1/3 (p): e(s) = (1-s^2)(1 -+ 3s)
+    auto edge = [](T s, int side, T& f, T& df) {
+        const T t = side == 0 ? T(1) - T(3) * s : T(1) + T(3) * s;
+        const T ds = side == 0 ? T(-3) : T(3);
+        f = (T(1) - s * s) * t;
+        df = T(-2) * s * t + (T(1) - s * s) * ds;
+    };
+    const T k = T(9) / T(64);
+    // x edges: index 8 + 2*(iz + 2*iy) + side   (iy, iz select -1/+1 in y and z)
+    for (int iy = 0; iy < 2; ++iy)
+        for (int iz = 0; iz < 2; ++iz)
+            for (int side = 0; side < 2; ++side) {
+                T f, df;
+                edge(x, side, f, df);
+                const int n = 8 + 2 * (iz + 2 * iy) + side;
+                N[n] = k * f * ly[iy] * lz[iz];
+                dN[n] = v3<T>(k * df * ly[iy] * lz[iz], k * f * sg[iy] * lz[iz], k * f * ly[iy] * sg[iz]);
+            }
+    // y edges: index 16 + 2*(ix + 2*iz) + side
+    for (int iz = 0; iz < 2; ++iz)
+        for (int ix = 0; ix < 2; ++ix)
+            for (int side = 0; side < 2; ++side) {
+                T f, df;
+                edge(y, side, f, df);
+                const int n = 16 + 2 * (ix + 2 * iz) + side;
+                N[n] = k * f * lx[ix] * lz[iz];
+                dN[n] = v3<T>(k * f * sg[ix] * lz[iz], k * df * lx[ix] * lz[iz], k * f * lx[ix] * sg[iz]);
+            }
+    // z edges: index 24 + 2*(iy + 2*ix) + side
+    for (int ix = 0; ix < 2; ++ix)
+        for (int iy = 0; iy < 2; ++iy)
+            for (int side = 0; side < 2; ++side) {
+                T f, df;
+                edge(z, side, f, df);
+                const int n = 24 + 2 * (iy + 2 * ix) + side;
+                N[n] = k * f * lx[ix] * ly[iy];
+                dN[n] = v3<T>(k * f * sg[ix] * ly[iy], k * f * lx[ix] * sg[iy], k * df * lx[ix] * ly[iy]);
+            }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Analytic signed distances (stand-in for MeshDistance, Utility/SDF/MeshDistance.cpp). d < 0 inside the shape.
+// ---------------------------------------------------------------------------------------------------------------
+template <class T> T shape_distance(const vfd_boundary_desc& b, V3<T> p) {
+    const V3<T> c = v3<T>((T)b.Center[0], (T)b.Center[1], (T)b.Center[2]);
+    switch (b.Shape) {
+        case VFD_SHAPE_BOX: {
+            const V3<T> d = p - c;
+            const V3<T> q = v3<T>(std::fabs(d.x) - (T)b.HalfExtents[0], std::fabs(d.y) - (T)b.HalfExtents[1],
+                                  std::fabs(d.z) - (T)b.HalfExtents[2]);
+            const V3<T> qp = v3<T>(std::max(q.x, T(0)), std::max(q.y, T(0)), std::max(q.z, T(0)));
+            return std::sqrt(dot3(qp, qp)) + std::min(std::max(q.x, std::max(q.y, q.z)), T(0));
+        }
+        case VFD_SHAPE_SPHERE: {
+            const V3<T> d = p - c;
+            return std::sqrt(dot3(d, d)) - (T)b.Radius;
+        }
+        case VFD_SHAPE_PLANE: {
+            const V3<T> n = v3<T>((T)b.Normal[0], (T)b.Normal[1], (T)b.Normal[2]);
+            return dot3(p - c, n);
+        }
+        case VFD_SHAPE_CYLINDER: {
+            const V3<T> d = p - c;
+            const T rxz = std::sqrt(std::fma(d.z, d.z, d.x * d.x)) - (T)b.Radius;
+            const T ry = std::fabs(d.y) - (T)b.HalfExtents[1];
+            const T ox = std::max(rxz, T(0)), oy = std::max(ry, T(0));
+            return std::sqrt(std::fma(oy, oy, ox * ox)) + std::min(std::max(rxz, ry), T(0));
+        }
+    }
+    return T(0);
+}
+
+template <class T> struct Box {
+    V3<T> min, max;
+    bool contains(V3<T> v) const {  // BoundingBox.h:42-44
+        return min.x <= v.x && min.y <= v.y && min.z <= v.z && max.x >= v.x && max.y >= v.y && max.z >= v.z;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------------
+// Volume map: SDF(domain,res) SDF.cu:8-14, AddFunction :45-139, host Interpolate :141-215, IndexToNodePosition
+// :313-373, device DetermineShapeFunction / Interpolate SDFDeviceData.cuh:372-474. Field 0 = signed distance
+// sign*(d - (Padding - r)) (RigidBody.cu:39-42), field 1 = boundary volume 0.8 * GL16^3 integral (RigidBody.cu:44-72,
+// GaussQuadrature.cpp:5621-5655). Pinned by tests/test_oracle_sdf.py (closed-form plane volume, deep-solid limit,
+// node values = analytic distance).
+// ---------------------------------------------------------------------------------------------------------------
+template <class T> struct VolumeMap {
+    Box<T> domain;
+    unsigned res[3] = {0, 0, 0};
+    V3<T> cellSize, cellSizeInv;
+    unsigned cellCount = 0, nodeCount = 0;
+    std::vector<T> nodes[2];
+
+    void init(Box<T> d, const unsigned r[3]) {
+        domain = d;
+        res[0] = r[0];
+        res[1] = r[1];
+        res[2] = r[2];
+        const V3<T> diag = domain.max - domain.min;
+        cellSize = v3<T>(diag.x / T(res[0]), diag.y / T(res[1]), diag.z / T(res[2]));
+        cellSizeInv = v3<T>(T(1) / cellSize.x, T(1) / cellSize.y, T(1) / cellSize.z);
+        cellCount = res[0] * res[1] * res[2];
+        const unsigned nv = (res[0] + 1) * (res[1] + 1) * (res[2] + 1);
+        const unsigned nex = res[0] * (res[1] + 1) * (res[2] + 1);
+        const unsigned ney = (res[0] + 1) * res[1] * (res[2] + 1);
+        const unsigned nez = (res[0] + 1) * (res[1] + 1) * res[2];
+        nodeCount = nv + 2 * (nex + ney + nez);
+    }
+    V3<T> node_position(unsigned i) const {
+        const unsigned nx = res[0], ny = res[1], nz = res[2];
+        const unsigned nv = (nx + 1) * (ny + 1) * (nz + 1);
+        const unsigned nex = nx * (ny + 1) * (nz + 1), ney = (nx + 1) * ny * (nz + 1);
+        V3<T> idx;
+        V3<T> r;
+        if (i < nv) {
+            idx.z = T(i / ((ny + 1) * (nx + 1)));
+            const unsigned t = i % ((ny + 1) * (nx + 1));
+            idx.y = T(t / (nx + 1));
+            idx.x = T(t % (nx + 1));
+            r = domain.min + cmul(cellSize, idx);
+        } else if (i < nv + 2 * nex) {
+            i -= nv;
+            const unsigned e = i / 2;
+            idx.z = T(e / ((ny + 1) * nx));
+            const unsigned t = e % ((ny + 1) * nx);
+            idx.y = T(t / nx);
+            idx.x = T(t % nx);
+            r = domain.min + cmul(cellSize, idx);
+            r.x += (T(1) + T(i % 2)) / T(3) * cellSize.x;
+        } else if (i < nv + 2 * (nex + ney)) {
+            i -= nv + 2 * nex;
+            const unsigned e = i / 2;
+            idx.x = T(e / ((nz + 1) * ny));
+            const unsigned t = e % ((nz + 1) * ny);
+            idx.z = T(t / ny);
+            idx.y = T(t % ny);
+            r = domain.min + cmul(cellSize, idx);
+            r.y += (T(1) + T(i % 2)) / T(3) * cellSize.y;
+        } else {
+            i -= nv + 2 * (nex + ney);
+            const unsigned e = i / 2;
+            idx.y = T(e / ((nx + 1) * nz));
+            const unsigned t = e % ((nx + 1) * nz);
+            idx.x = T(t / nz);
+            idx.z = T(t % nz);
+            r = domain.min + cmul(cellSize, idx);
+            r.z += (T(1) + T(i % 2)) / T(3) * cellSize.z;
+        }
+        return r;
+    }
+    // SDF.cu:81-131 - the 32 node indices of cell (i,j,k)
+    void cell_nodes(unsigned i, unsigned j, unsigned k, unsigned out[32]) const {
+        const unsigned nx = res[0], ny = res[1], nz = res[2];
+        const unsigned nv = (nx + 1) * (ny + 1) * (nz + 1);
+        const unsigned nex = nx * (ny + 1) * (nz + 1), ney = (nx + 1) * ny * (nz + 1);
+        for (unsigned c = 0; c < 8; ++c) {
+            const unsigned di = c & 1u, dj = (c >> 1) & 1u, dk = (c >> 2) & 1u;
+            out[c] = (nx + 1) * (ny + 1) * (k + dk) + (nx + 1) * (j + dj) + (i + di);
+        }
+        for (unsigned e = 0; e < 4; ++e) {  // x edges at (j+dj, k+dk), e = dk + 2 dj
+            const unsigned dk = e & 1u, dj = e >> 1;
+            const unsigned b = nv + 2 * (nx * (ny + 1) * (k + dk) + nx * (j + dj) + i);
+            out[8 + 2 * e] = b;
+            out[9 + 2 * e] = b + 1;
+        }
+        for (unsigned e = 0; e < 4; ++e) {  // y edges at (i+di, k+dk), e = di + 2 dk
+            const unsigned di = e & 1u, dk = e >> 1;
+            const unsigned b = nv + 2 * nex + 2 * (ny * (nz + 1) * (i + di) + ny * (k + dk) + j);
+            out[16 + 2 * e] = b;
+            out[17 + 2 * e] = b + 1;
+        }
+        for (unsigned e = 0; e < 4; ++e) {  // z edges at (i+di, j+dj), e = dj + 2 di
+            const unsigned dj = e & 1u, di = e >> 1;
+            const unsigned b = nv + 2 * nex + 2 * ney + 2 * (nz * (nx + 1) * (j + dj) + nz * (i + di) + k);
+            out[24 + 2 * e] = b;
+            out[25 + 2 * e] = b + 1;
+        }
+    }
+    // Locate the cell, build xi and the shape functions. Returns false outside the domain.
+    bool locate(V3<T> x, unsigned cell[32], V3<T>& c0, T N[32], V3<T> dN[32]) const {
+        if (!domain.contains(x)) return false;
+        const V3<T> f = cmul(cellSizeInv, x - domain.min);
+        unsigned mi[3] = {(unsigned)f.x, (unsigned)f.y, (unsigned)f.z};
+        for (int a = 0; a < 3; ++a)
+            if (mi[a] >= res[a]) mi[a] = res[a] - 1;
+        const V3<T> origin = domain.min + cmul(v3<T>(T(mi[0]), T(mi[1]), T(mi[2])), cellSize);
+        const V3<T> top = origin + cellSize;
+        const V3<T> den = top - origin;
+        c0 = v3<T>(T(2) / den.x, T(2) / den.y, T(2) / den.z);
+        const V3<T> sum = top + origin;
+        const V3<T> c1 = v3<T>(sum.x / den.x, sum.y / den.y, sum.z / den.z);
+        const V3<T> xi = cmul(c0, x) - c1;
+        cell_nodes(mi[0], mi[1], mi[2], cell);
+        shape_function(xi, N, dN);
+        return true;
+    }
+    T interp(unsigned field, const unsigned cell[32], const T N[32]) const {
+        T phi = 0;
+        for (int k = 0; k < 32; ++k) phi += nodes[field][cell[k]] * N[k];
+        return phi;
+    }
+    T interp_grad(unsigned field, const unsigned cell[32], V3<T> c0, const T N[32], const V3<T> dN[32],
+                  V3<T>& grad) const {
+        T phi = 0;
+        grad = v3<T>(0, 0, 0);
+        for (int k = 0; k < 32; ++k) {
+            const T c = nodes[field][cell[k]];
+            phi += c * N[k];
+            grad = grad + c * dN[k];
+        }
+        grad = cmul(grad, c0);
+        return phi;
+    }
+    // host Interpolate(field, x) without gradient; FLT_MAX outside the domain
+    T interp_at(unsigned field, V3<T> x) const {
+        unsigned cell[32];
+        V3<T> c0;
+        T N[32];
+        V3<T> dN[32];
+        if (!locate(x, cell, c0, N, dN)) return T(FLT_MAX);
+        return interp(field, cell, N);
+    }
+};
+
+// Builds both fields of a body's volume map (RigidBody.cu:10-73).
+template <class T>
+void build_volume_map(VolumeMap<T>& map, const vfd_boundary_desc& b, const CubicKernel<T>& kernel, T particleRadius) {
+    const T R = kernel.R;
+    const T tol = (T)b.Padding - particleRadius;
+    const T sign = b.Inverted ? T(-1) : T(1);
+    Box<T> dom;
+    if (b.HasDomain) {
+        dom.min = v3<T>((T)b.DomainMin[0], (T)b.DomainMin[1], (T)b.DomainMin[2]);
+        dom.max = v3<T>((T)b.DomainMax[0], (T)b.DomainMax[1], (T)b.DomainMax[2]);
+    } else {
+        V3<T> lo, hi;
+        const V3<T> c = v3<T>((T)b.Center[0], (T)b.Center[1], (T)b.Center[2]);
+        V3<T> e;
+        if (b.Shape == VFD_SHAPE_SPHERE)
+            e = v3<T>((T)b.Radius, (T)b.Radius, (T)b.Radius);
+        else if (b.Shape == VFD_SHAPE_CYLINDER)
+            e = v3<T>((T)b.Radius, (T)b.HalfExtents[1], (T)b.Radius);
+        else
+            e = v3<T>((T)b.HalfExtents[0], (T)b.HalfExtents[1], (T)b.HalfExtents[2]);
+        lo = c - e;
+        hi = c + e;
+        const T grow = T(8) * R + tol;
+        dom.min = lo - v3<T>(grow, grow, grow);
+        dom.max = hi + v3<T>(grow, grow, grow);
+    }
+    map.init(dom, b.CollisionMapResolution);
+    map.nodes[0].assign(map.nodeCount, T(0));
+    map.nodes[1].assign(map.nodeCount, T(0));
+#pragma omp parallel for schedule(static)
+    for (long l = 0; l < (long)map.nodeCount; ++l) {
+        const V3<T> x = map.node_position((unsigned)l);
+        map.nodes[0][l] = sign * (shape_distance<T>(b, x) - tol);
+    }
+    // Gauss-Legendre 16 points on [-R, R]^3; nodes/weights stored in T like the reference's float table.
+    std::vector<double> gx, gw;
+    gauss_legendre(16, gx, gw);
+    T xs[16], ws[16];
+    for (int i = 0; i < 16; ++i) {
+        xs[i] = (T)gx[i];
+        ws[i] = (T)gw[i];
+    }
+    const double c0 = (double)R;  // 0.5 * Diagonal() of [-R, R]^3; c1 = 0
+    const T W0 = kernel.W0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long l = 0; l < (long)map.nodeCount; ++l) {
+        const V3<T> x = map.node_position((unsigned)l);
+        const T dX = map.interp_at(0, x);
+        if (dX > T(2) * R) {
+            map.nodes[1][l] = T(0);
+            continue;
+        }
+        T res = 0;
+        for (int i = 0; i < 16; ++i) {
+            const T wi = ws[i];
+            for (int j = 0; j < 16; ++j) {
+                const T wij = wi * ws[j];
+                for (int k = 0; k < 16; ++k) {
+                    const T wijk = wij * ws[k];
+                    const V3<T> xi = v3<T>((T)(c0 * (double)xs[i]), (T)(c0 * (double)xs[j]), (T)(c0 * (double)xs[k]));
+                    T f = 0;
+                    if (!(dot3(xi, xi) > R * R)) {
+                        const T dist = map.interp_at(0, x + xi);
+                        if (dist <= T(0))
+                            f = T(1);
+                        else if (dist < R)
+                            f = kernel.getW(dist) / W0;
+                    }
+                    res += wijk * f;
+                }
+            }
+        }
+        res = (T)((double)res * (c0 * c0 * c0));
+        map.nodes[1][l] = T(0.8) * res;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Neighbour search: ParticleSearch.cu:29-195, ParticleSearchKernels.cu:4-193.
+// ---------------------------------------------------------------------------------------------------------------
+inline unsigned part1by2(unsigned x) {  // spread the low 10 bits of x to every third bit
+    unsigned r = 0;
+    for (unsigned b = 0; b < 10; ++b) r |= ((x >> b) & 1u) << (3 * b);
+    return r;
+}
+inline unsigned morton3(unsigned x, unsigned y, unsigned z) { return (part1by2(z) << 2) + (part1by2(y) << 1) + part1by2(x); }
+
+template <class T> struct SearchGrid {
+    V3<T> gridMin, gridDelta;
+    unsigned dims[3], meta[3];
+    unsigned cellCount;
+    T R2;
+    unsigned cell_index(int cx, int cy, int cz) const {  // ToCellIndexMortonMetaGrid :24-38
+        const unsigned mx = (unsigned)cx / kMetaGroup, my = (unsigned)cy / kMetaGroup, mz = (unsigned)cz / kMetaGroup;
+        const unsigned lin = mz * meta[1] * meta[0] + my * meta[0] + mx;
+        return lin * kMetaBlock + morton3((unsigned)cx % kMetaGroup, (unsigned)cy % kMetaGroup, (unsigned)cz % kMetaGroup);
+    }
+};
+
+template <class T> SearchGrid<T> make_grid(const std::vector<V3<T>>& x, T R) {
+    int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (const V3<T>& p : x) {  // ComputeMinMaxKernel :40-62
+        const int c[3] = {(int)std::floor(p.x / R), (int)std::floor(p.y / R), (int)std::floor(p.z / R)};
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], c[a]);
+            mx[a] = std::max(mx[a], c[a]);
+        }
+    }
+    for (int a = 0; a < 3; ++a) mx[a] += 1;
+    const V3<T> bmin = v3<T>(T(mn[0]) * R, T(mn[1]) * R, T(mn[2]) * R);
+    const V3<T> bmax = v3<T>(T(mx[0]) * R, T(mx[1]) * R, T(mx[2]) * R);
+    SearchGrid<T> g;  // ComputeCellInformation ParticleSearch.cu:57-93
+    g.R2 = R * R;
+    V3<T> size = bmax - bmin;
+    g.dims[0] = (unsigned)std::ceil(size.x / R) + 4;
+    g.dims[1] = (unsigned)std::ceil(size.y / R) + 4;
+    g.dims[2] = (unsigned)std::ceil(size.z / R) + 4;
+    g.gridMin = bmin - v3<T>(R, R, R) * T(2);
+    for (int a = 0; a < 3; ++a) g.meta[a] = (unsigned)std::ceil(T(g.dims[a]) / T(kMetaGroup));
+    size = v3<T>(T(g.dims[0]) * R, T(g.dims[1]) * R, T(g.dims[2]) * R);
+    g.gridDelta = v3<T>(T(g.dims[0]) / size.x, T(g.dims[1]) / size.y, T(g.dims[2]) / size.z);
+    g.cellCount = g.meta[0] * g.meta[1] * g.meta[2] * kMetaBlock;
+    return g;
+}
+
+struct NeighborLists {
+    std::vector<uint32_t> counts, offsets, ids;
+};
+
+template <class T> void find_neighbors(const std::vector<V3<T>>& x, T R, NeighborLists& out) {
+    const unsigned n = (unsigned)x.size();
+    out.counts.assign(n, 0);
+    out.offsets.assign(n, 0);
+    out.ids.clear();
+    if (n == 0) return;
+    const SearchGrid<T> g = make_grid(x, R);
+    std::vector<uint32_t> cellOf(n), cellCount(g.cellCount, 0), cellStart(g.cellCount, 0), sorted(n);
+    for (unsigned i = 0; i < n; ++i) {  // InsertParticlesMortonKernel :64-78 (truncation of a positive value)
+        const V3<T> f = cmul(x[i] - g.gridMin, g.gridDelta);
+        cellOf[i] = g.cell_index((int)f.x, (int)f.y, (int)f.z);
+        cellCount[cellOf[i]]++;
+    }
+    uint32_t run = 0;
+    for (unsigned c = 0; c < g.cellCount; ++c) {
+        cellStart[c] = run;
+        run += cellCount[c];
+    }
+    {
+        std::vector<uint32_t> fill(cellStart);
+        for (unsigned i = 0; i < n; ++i) sorted[fill[cellOf[i]]++] = i;  // ascending index within each cell
+    }
+    std::vector<uint32_t> tmp((size_t)n * kMaxNeighbors);
+#pragma omp parallel for schedule(static)
+    for (long il = 0; il < (long)n; ++il) {  // ComputeCountsKernel :93-142 / NeighborhoodQueryWithCountsKernel :144-193
+        const unsigned i = (unsigned)il;
+        const V3<T> p = x[i];
+        const V3<T> f = cmul(p - g.gridMin, g.gridDelta);
+        const int gc[3] = {(int)std::floor(f.x), (int)std::floor(f.y), (int)std::floor(f.z)};
+        unsigned cnt = 0;
+        uint32_t* dst = &tmp[(size_t)i * kMaxNeighbors];
+        for (int dz = -1; dz < 2 && cnt < kMaxNeighbors; ++dz)
+            for (int dy = -1; dy < 2 && cnt < kMaxNeighbors; ++dy)
+                for (int dx = -1; dx < 2 && cnt < kMaxNeighbors; ++dx) {
+                    const int cx = gc[0] + dx, cy = gc[1] + dy, cz = gc[2] + dz;
+                    if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)g.dims[0] || cy >= (int)g.dims[1] ||
+                        cz >= (int)g.dims[2])
+                        continue;
+                    const unsigned c = g.cell_index(cx, cy, cz);
+                    for (uint32_t s = cellStart[c]; s < cellStart[c] + cellCount[c]; ++s) {
+                        const unsigned j = sorted[s];
+                        const V3<T> d = x[j] - p;
+                        const T d2 = std::fma(d.z, d.z, std::fma(d.y, d.y, d.x * d.x));
+                        if (d2 < g.R2 && d2 > T(0)) dst[cnt++] = j;
+                        if (cnt == kMaxNeighbors) break;
+                    }
+                }
+        out.counts[i] = cnt;
+    }
+    uint64_t total = 0;
+    for (unsigned i = 0; i < n; ++i) {
+        out.offsets[i] = (uint32_t)total;
+        total += out.counts[i];
+    }
+    out.ids.resize(total);
+    for (unsigned i = 0; i < n; ++i)
+        std::memcpy(&out.ids[out.offsets[i]], &tmp[(size_t)i * kMaxNeighbors], out.counts[i] * sizeof(uint32_t));
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// The simulation. DFSPHImplementation.cu, DFSPHKernels.cu.
+// ---------------------------------------------------------------------------------------------------------------
+template <class T> struct Particle {  // Structures/DFSPHParticle.h:8-32
+    V3<T> x, v, a, ap;
+    T residuum, density, densityAdv, kappa, kappaV, factor;
+    V3<T> dv;
+    V3<T> nMC, nS;
+    T kMC, kS, dK;
+};
+
+template <class T> struct Info {  // Structures/DFSPHSimulationInfo.h:5-44
+    unsigned particleCount = 0, bodyCount = 0;
+    T R, R2, r, diam, dt, dt2, dtInv, dt2Inv, V, rho0, m, mInv, nu, nuB, mu, muB, tdf, td;
+    T sigma;
+    unsigned S = 0;
+    T slope, constant;
+    bool temporal;
+    T smoothing, factor, NR, mcFactor;
+    V3<T> g;
+};
+
+template <class T> struct Body {
+    vfd_boundary_desc desc;
+    VolumeMap<T> map;
+    std::vector<V3<T>> xb;
+    std::vector<T> vb;
+};
+
+template <class T> class Sim {
+public:
+    vfd_desc desc{};
+    Info<T> info{};
+    CubicKernel<T> kernel;
+    std::vector<T> halton;
+    std::vector<Particle<T>> p;
+    std::vector<Body<T>> bodies;
+    NeighborLists nb;
+    std::vector<M3<T>> minv;
+    std::vector<V3<T>> vb_b, vb_g, vb_r, vb_p, vb_z, vb_tmp, vb_ap;
+    vfd_debug_info dbg{};
+    T maxVel = 0;
+    bool kernelSet = false;
+
+    explicit Sim(const vfd_desc& d) {
+        halton = halton_table<T>();
+        set_description(d);
+    }
+
+    // SetDescription DFSPHImplementation.cu:318-368 (+ the sample count / Monte-Carlo factor initialised from dt0,
+    // which the reference leaves uninitialised until the first ComputeTimeStepSize)
+    void set_description(const vfd_desc& d) {
+        desc = d;
+        const T r = (T)d.ParticleRadius;
+        if (!kernelSet || r != info.r) {
+            kernel.setRadius(T(4) * r);
+            kernelSet = true;
+        }
+        info.r = r;
+        info.diam = T(2) * r;
+        info.R = T(4) * r;
+        info.R2 = info.R * info.R;
+        info.V = T(0.8) * info.diam * info.diam * info.diam;
+        info.rho0 = T(1000);
+        info.m = info.V * info.rho0;
+        info.mInv = T(1) / info.m;
+        info.nu = (T)d.Viscosity;
+        info.nuB = (T)d.BoundaryViscosity;
+        info.mu = info.nu * info.rho0;
+        info.muB = info.nuB * info.rho0;
+        info.tdf = (T)d.TangentialDistanceFactor;
+        info.td = info.tdf * info.R;
+        set_dt((T)d.TimeStepSize);
+        info.sigma = (T)d.SurfaceTension;
+        info.slope = T(74.688796680497925);
+        info.constant = T(12);
+        info.temporal = d.TemporalSmoothing != 0;
+        info.smoothing = T(0.5);
+        info.factor = T(0.8);
+        info.NR = info.r * info.factor;
+        info.g = v3<T>((T)d.Gravity[0], (T)d.Gravity[1], (T)d.Gravity[2]);
+        update_sample_count();
+        info.mcFactor = std::asin(info.NR / info.r);
+    }
+    void set_dt(T dt) {
+        info.dt = dt;
+        info.dt2 = dt * dt;
+        info.dtInv = T(1) / info.dt;
+        info.dt2Inv = T(1) / info.dt2;
+    }
+    void update_sample_count() {
+        if (desc.CSDFix > 0)
+            info.S = (unsigned)desc.CSDFix;
+        else
+            info.S = (unsigned)(int)((T)desc.CSD * info.dt);
+    }
+
+    // SetFluidObjects DFSPHImplementation.cu:172-253
+    void set_fluid(const float* x, const float* v, unsigned n) {
+        p.assign(n, Particle<T>{});
+        for (unsigned i = 0; i < n; ++i) {
+            Particle<T>& q = p[i];
+            std::memset(&q, 0, sizeof(q));
+            q.x = v3<T>((T)x[3 * i], (T)x[3 * i + 1], (T)x[3 * i + 2]);
+            q.v = v ? v3<T>((T)v[3 * i], (T)v[3 * i + 1], (T)v[3 * i + 2]) : v3<T>(0, 0, 0);
+        }
+        info.particleCount = n;
+        for (auto& b : bodies) {
+            b.xb.assign(n, v3<T>(0, 0, 0));
+            b.vb.assign(n, T(0));
+        }
+        const size_t m = n;
+        minv.assign(m, mzero<T>());
+        vb_b.assign(m, v3<T>(0, 0, 0));
+        vb_g = vb_r = vb_p = vb_z = vb_tmp = vb_ap = vb_b;
+    }
+    void add_boundary(const vfd_boundary_desc& d) {
+        bodies.emplace_back();
+        Body<T>& b = bodies.back();
+        b.desc = d;
+        build_volume_map(b.map, d, kernel, info.r);
+        b.xb.assign(info.particleCount, v3<T>(0, 0, 0));
+        b.vb.assign(info.particleCount, T(0));
+        info.bodyCount = (unsigned)bodies.size();
+    }
+
+    // ----- kernels --------------------------------------------------------------------------------------------
+    // ComputeVolumeAndBoundaryKernel DFSPHKernels.cu:69-150
+    void boundary() {
+        const long n = (long)p.size();
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {
+            for (auto& b : bodies) {
+                V3<T>& xb = b.xb[i];
+                T& vb = b.vb[i];
+                xb = v3<T>(0, 0, 0);
+                vb = T(0);
+                V3<T> normal = v3<T>(0, 0, 0);
+                T dist = T(FLT_MAX);
+                unsigned cell[32];
+                V3<T> c0;
+                T N[32];
+                V3<T> dN[32];
+                if (b.map.locate(p[i].x, cell, c0, N, dN)) dist = b.map.interp_grad(0, cell, c0, N, dN, normal);
+                if (dist > T(0) && dist < info.R) {
+                    const T vol = b.map.interp(1, cell, N);
+                    if (vol > T(0) && vol != T(FLT_MAX)) {
+                        vb = vol;
+                        const T nl = length(normal);
+                        if (nl > T(1.0e-9)) {
+                            normal = normal / nl;
+                            const T d = std::max(dist + T(0.5) * info.r, info.diam);
+                            xb = p[i].x - d * normal;
+                        } else {
+                            vb = T(0);
+                        }
+                    } else {
+                        vb = T(0);
+                    }
+                } else if (dist <= T(0)) {
+                    if (dist != T(FLT_MAX)) {
+                        const T nl = length(normal);
+                        if (nl > T(1.0e-5)) {
+                            normal = normal / nl;
+                            T delta = info.diam - dist;
+                            delta = std::min(delta, T(0.1) * info.r);
+                            p[i].x = p[i].x + delta * normal;
+                            p[i].v = v3<T>(0, 0, 0);
+                        }
+                    }
+                    vb = T(0);
+                } else {
+                    vb = T(0);
+                }
+            }
+        }
+    }
+    unsigned cnt(long i) const { return nb.counts[i]; }
+    unsigned nbr(long i, unsigned j) const { return nb.ids[nb.offsets[i] + j]; }
+
+    // ComputeDensityKernel DFSPHKernels.cu:152-187
+    void density() {
+        const long n = (long)p.size();
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {
+            const V3<T> xi = p[i].x;
+            T d = info.V * kernel.W0;
+            for (unsigned j = 0; j < cnt(i); ++j) d += info.V * kernel.getW(xi - p[nbr(i, j)].x);
+            for (auto& b : bodies) {
+                const T vb = b.vb[i];
+                if (vb > T(0)) d += vb * kernel.getW(xi - b.xb[i]);
+            }
+            p[i].density = d * info.rho0;
+        }
+    }
+    // ComputeDFSPHFactorKernel :189-231
+    void factor() {
+        const long n = (long)p.size();
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {
+            const V3<T> xi = p[i].x;
+            V3<T> gi = v3<T>(0, 0, 0);
+            T sum = 0;
+            for (unsigned j = 0; j < cnt(i); ++j) {
+                const V3<T> gj = (-info.V) * kernel.gradW(xi - p[nbr(i, j)].x);
+                sum += dot3(gj, gj);
+                gi = gi - gj;
+            }
+            for (auto& b : bodies) {
+                const T vb = b.vb[i];
+                if (vb > T(0)) {
+                    const V3<T> gj = (-vb) * kernel.gradW(xi - b.xb[i]);
+                    gi = gi - gj;
+                }
+            }
+            sum += dot3(gi, gi);
+            p[i].factor = (double)sum > kEps ? T(1) / sum : T(0);
+        }
+    }
+    // v_i . sum_j gradW (velocity divergence term shared by :233-278 and :483-529)
+    T velocity_divergence(long i) const {
+        const V3<T> xi = p[i].x, vi = p[i].v;
+        T d = 0;
+        for (unsigned j = 0; j < cnt(i); ++j) {
+            const unsigned k = nbr(i, j);
+            d += dot3(vi - p[k].v, kernel.gradW(xi - p[k].x));
+        }
+        d *= info.V;
+        for (auto& b : bodies) {
+            const T vb = b.vb[i];
+            if (vb > T(0)) d += vb * dot3(vi, kernel.gradW(xi - b.xb[i]));
+        }
+        return d;
+    }
+    // pressure acceleration from kappa (:342-386, :388-432, :434-481, :555-602); divergence selects kappaV
+    V3<T> pressure_accel(long i, bool divergence) const {
+        const V3<T> xi = p[i].x;
+        const T ki = divergence ? p[i].kappaV : p[i].kappa;
+        V3<T> a = v3<T>(0, 0, 0);
+        for (unsigned j = 0; j < cnt(i); ++j) {
+            const unsigned k = nbr(i, j);
+            const T s = ki + (divergence ? p[k].kappaV : p[k].kappa);
+            if ((double)std::fabs(s) > kEps) {
+                const V3<T> gj = (-info.V) * kernel.gradW(xi - p[k].x);
+                a = a + s * gj;
+            }
+        }
+        if ((double)std::fabs(ki) > kEps) {
+            for (auto& b : bodies) {
+                const T vb = b.vb[i];
+                if (vb > T(0)) {
+                    const V3<T> gj = (-vb) * kernel.gradW(xi - b.xb[i]);
+                    a = a + ki * gj;
+                }
+            }
+        }
+        return a;
+    }
+    // ComputeDensityPressureForce :280-315
+    T density_pressure_force(long i) const {
+        const V3<T> xi = p[i].x, ai = p[i].ap;
+        T f = 0;
+        for (unsigned j = 0; j < cnt(i); ++j) {
+            const unsigned k = nbr(i, j);
+            f += dot3(ai - p[k].ap, kernel.gradW(xi - p[k].x));
+        }
+        f *= info.V;
+        for (auto& b : bodies) {
+            const T vb = b.vb[i];
+            if (vb > T(0)) f += vb * dot3(ai, kernel.gradW(xi - b.xb[i]));
+        }
+        return f;
+    }
+    // sequential residual error -rho0 * sum(r) / N in the build's FP64 accumulation (DFSPHImplementation.cu:483-491)
+    T residual_error() const {
+        double s = 0;
+        for (const auto& q : p) s += (double)(info.rho0 * q.residuum);
+        return (T)(-s) / (T)p.size();
+    }
+
+    // ComputeDivergence DFSPHImplementation.cu:506-575
+    void divergence_solve() {
+        if (!desc.EnableDivergenceSolverError) {
+            dbg.DivergenceSolverError = 0;
+            dbg.DivergenceSolverIterationCount = 0;
+            return;
+        }
+        const long n = (long)p.size();
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {  // ComputeDensityChangeKernel :483-529
+            T d = velocity_divergence(i);
+            d = cnt(i) < 20u ? T(0) : std::max(d, T(0));
+            p[i].densityAdv = d;
+            p[i].factor *= info.dtInv;
+            p[i].kappaV = d * p[i].factor;
+        }
+        const T eta = info.dtInv * (T)desc.MaxDivergenceSolverError * T(0.0001) * info.rho0;
+        unsigned it = 0;
+        T err = 0;
+        while ((err > eta || it < desc.MinDivergenceSolverIterations) && it < desc.MaxDivergenceSolverIterations) {
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) p[i].ap = pressure_accel(i, true);
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) {  // DivergenceSolveIterationKernel :531-553
+                const T f = density_pressure_force(i) * info.dt;
+                p[i].residuum = cnt(i) < 20u ? T(0) : std::min(-p[i].densityAdv - f, T(0));
+                p[i].kappaV -= p[i].residuum * p[i].factor;
+            }
+            err = residual_error();
+            ++it;
+        }
+        dbg.DivergenceSolverIterationCount = it;
+        dbg.DivergenceSolverError = (float)err;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {  // ComputePressureAccelerationAndFactorKernel :555-602
+            p[i].ap = pressure_accel(i, true);
+            p[i].v = p[i].v + info.dt * p[i].ap;
+            p[i].factor *= info.dt;
+        }
+    }
+    // ComputePressure DFSPHImplementation.cu:443-504
+    void pressure_solve() {
+        const long n = (long)p.size();
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {  // ComputeDensityAdvectionKernel :233-278
+            const T d = velocity_divergence(i);
+            p[i].densityAdv = p[i].density / info.rho0 + info.dt * d;
+            p[i].factor *= info.dt2Inv;
+            const T si = T(1) - p[i].densityAdv;
+            const T r = std::min(si, T(0));
+            p[i].kappa = -r * p[i].factor;
+        }
+        const T eta = (T)desc.MaxPressureSolverError * T(0.0001) * info.rho0;
+        unsigned it = 0;
+        T err = 0;
+        while ((err > eta || it < desc.MinPressureSolverIterations) && it < desc.MaxPressureSolverIterations) {
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) p[i].ap = pressure_accel(i, false);
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) {  // PressureSolveIterationKernel :317-340
+                const T f = density_pressure_force(i) * info.dt2;
+                p[i].residuum = std::min(T(1) - p[i].densityAdv - f, T(0));
+                p[i].kappa -= p[i].residuum * p[i].factor;
+            }
+            err = residual_error();
+            ++it;
+        }
+        dbg.PressureSolverIterationCount = it;
+        dbg.PressureSolverError = (float)err;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {  // ComputePressureAccelerationAndVelocityKernel :434-481
+            p[i].ap = pressure_accel(i, false);
+            p[i].v = p[i].v + info.dt * p[i].ap;
+        }
+    }
+
+    // boundary tangential samples shared by :636-681, :706-752, :793-839
+    struct Tangential {
+        bool ok;
+        V3<T> pd[4];
+        V3<T> gw[4];
+        T vol;
+    };
+    Tangential tangential(long i, const Body<T>& b) const {
+        Tangential t;
+        t.ok = false;
+        const V3<T> xb = b.xb[i];
+        const V3<T> xi = p[i].x;
+        V3<T> normal = -(xi - xb);
+        const T nl = length(normal);
+        if (!(nl > T(0.0001))) return t;
+        normal = normal / nl;
+        V3<T> t1, t2;
+        orthogonal_vectors(normal, t1, t2);
+        const V3<T> pos[4] = {xb - t1 * info.td, xb + t1 * info.td, xb - t2 * info.td, xb + t2 * info.td};
+        for (int k = 0; k < 4; ++k) {
+            t.pd[k] = xi - pos[k];
+            t.gw[k] = kernel.gradW(t.pd[k]);
+        }
+        t.vol = T(0.25) * b.vb[i];
+        t.ok = true;
+        return t;
+    }
+
+    // ComputeViscosity DFSPHImplementation.cu:577-615
+    void viscosity_solve() {
+        if (!desc.EnableViscositySolver) {
+            dbg.ViscositySolverError = 0;
+            dbg.ViscositySolverIterationCount = 0;
+            return;
+        }
+        const long n = (long)p.size();
+        const T d001 = T(0.01) * info.R2;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {  // ComputeViscosityPreconditionerKernel :604-684
+            M3<T> M = mzero<T>();
+            const V3<T> xi = p[i].x;
+            for (unsigned j = 0; j < cnt(i); ++j) {
+                const unsigned k = nbr(i, j);
+                const V3<T> xij = xi - p[k].x;
+                const V3<T> gw = kernel.gradW(xij);
+                const T s = T(10) * info.mu * (info.m / p[k].density) / (length2(xij) + d001);
+                const T a[3] = {xij.x, xij.y, xij.z}, g[3] = {gw.x, gw.y, gw.z};
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) M.m[3 * r + c] += s * (a[r] * g[c]);
+            }
+            if (info.muB != T(0)) {
+                for (auto& b : bodies) {
+                    if (!(b.vb[i] > T(0))) continue;
+                    const Tangential t = tangential(i, b);
+                    if (!t.ok) continue;
+                    const T a[3] = {t.pd[0].x, t.pd[0].y, t.pd[0].z};  // the reference uses direction 1 for all four
+                    for (int k = 0; k < 4; ++k) {
+                        const T s = T(10) * info.muB * t.vol / (length2(t.pd[k]) + d001);
+                        const T g[3] = {t.gw[k].x, t.gw[k].y, t.gw[k].z};
+                        for (int r = 0; r < 3; ++r)
+                            for (int c = 0; c < 3; ++c) M.m[3 * r + c] += s * (a[r] * g[c]);
+                    }
+                }
+            }
+            const T f = info.dt / p[i].density;
+            M3<T> A;
+            for (int e = 0; e < 9; ++e) A.m[e] = (e % 4 == 0 ? T(1) : T(0)) - f * M.m[e];
+            minv[i] = inverse3(A);
+            // ComputeViscosityGradientKernel :686-756 (the boundary rhs term is identically zero: dot(0, .) = 0)
+            vb_b[i] = p[i].v;
+            vb_g[i] = p[i].v + p[i].dv;
+        }
+        cg_solve();
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {  // ApplyViscosityForceKernel :844-862
+            const V3<T> d = vb_g[i] - p[i].v;
+            p[i].a = p[i].a + (T(1) / info.dt) * d;
+            p[i].dv = d;
+        }
+    }
+    // ComputeMatrixVecProdFunctionKernel :758-842
+    void matvec(const std::vector<V3<T>>& in, std::vector<V3<T>>& out) {
+        const long n = (long)p.size();
+        const T d001 = T(0.01) * info.R2;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {
+            const V3<T> xi = p[i].x, vi = in[i];
+            V3<T> acc = v3<T>(0, 0, 0);
+            for (unsigned j = 0; j < cnt(i); ++j) {
+                const unsigned k = nbr(i, j);
+                const V3<T> xij = xi - p[k].x;
+                const V3<T> gw = kernel.gradW(xij);
+                const T s = T(10) * info.mu * (info.m / p[k].density) * dot3(vi - in[k], xij) / (length2(xij) + d001);
+                acc = acc + s * gw;
+            }
+            if (info.muB != T(0)) {
+                for (auto& b : bodies) {
+                    if (!(b.vb[i] > T(0))) continue;
+                    const Tangential t = tangential(i, b);
+                    if (!t.ok) continue;
+                    V3<T> s4 = v3<T>(0, 0, 0);
+                    for (int k = 0; k < 4; ++k) {
+                        const T s = T(10) * info.muB * t.vol * dot3(vi, t.pd[k]) / (length2(t.pd[k]) + d001);
+                        s4 = k == 0 ? s * t.gw[k] : s4 + s * t.gw[k];
+                    }
+                    acc = acc + s4;
+                }
+            }
+            out[i] = vi - (info.dt / p[i].density) * acc;
+        }
+    }
+    static double dotsum(const std::vector<V3<T>>& a, const std::vector<V3<T>>& b) {
+        double s = 0;
+        for (size_t i = 0; i < a.size(); ++i) s += (double)dot3(a[i], b[i]);
+        return s;
+    }
+    // SolveViscosity DFSPHImplementation.cu:617-808 (preconditioned CG)
+    void cg_solve() {
+        const long n = (long)p.size();
+        dbg.ViscositySolverIterationCount = 0;
+        matvec(vb_g, vb_tmp);
+        for (long i = 0; i < n; ++i) vb_r[i] = vb_b[i] - vb_tmp[i];
+        const T rhs2 = (T)dotsum(vb_b, vb_b);
+        if (rhs2 == T(0)) {
+            for (long i = 0; i < n; ++i) vb_g[i] = v3<T>(0, 0, 0);
+            dbg.ViscositySolverError = 0;
+            return;
+        }
+        const T tol = (T)desc.MaxViscositySolverError;
+        const T thr = std::max(tol * tol * T(0.0001) * rhs2, std::numeric_limits<T>::min());
+        T res2 = (T)dotsum(vb_r, vb_r);
+        if (res2 < thr) {
+            dbg.ViscositySolverError = (float)std::sqrt(res2 / rhs2);
+            return;
+        }
+        for (long i = 0; i < n; ++i) vb_p[i] = mulmv(minv[i], vb_r[i]);
+        T absNew = std::fabs((T)dotsum(vb_r, vb_p));
+        unsigned it = 0;
+        while (it >= desc.MinViscositySolverIterations && it < desc.MaxViscositySolverIterations) {
+            matvec(vb_p, vb_ap);
+            const T alpha = absNew / (T)dotsum(vb_p, vb_ap);
+            for (long i = 0; i < n; ++i) {
+                vb_g[i] = vb_g[i] + vb_p[i] * alpha;
+                vb_r[i] = vb_r[i] - vb_ap[i] * alpha;
+            }
+            res2 = (T)dotsum(vb_r, vb_r);
+            if (res2 < thr) break;
+            for (long i = 0; i < n; ++i) vb_z[i] = mulmv(minv[i], vb_r[i]);
+            const T absOld = absNew;
+            absNew = std::fabs((T)dotsum(vb_r, vb_z));
+            const T beta = absNew / absOld;
+            for (long i = 0; i < n; ++i) vb_p[i] = vb_p[i] * beta + vb_z[i];
+            ++it;
+        }
+        dbg.ViscositySolverIterationCount = it;
+        dbg.ViscositySolverError = (float)std::sqrt(res2 / rhs2);
+    }
+
+    // SolveSurfaceTension DFSPHImplementation.cu:810-839; kernels DFSPHKernels.cu:864-1046
+    void surface_tension() {
+        if (!desc.EnableSurfaceTensionSolver) return;
+        const long n = (long)p.size();
+        const T R = info.R;
+#pragma omp parallel for schedule(dynamic, 256)
+        for (long i = 0; i < n; ++i) {  // ComputeSurfaceTensionClassificationKernel :881-958
+            Particle<T>& q = p[i];
+            q.nMC = v3<T>(0, 0, 0);
+            const V3<T> xi = q.x;
+            const unsigned c = cnt(i);
+            if (c == 0) {
+                q.kMC = T(1) / R;
+                continue;
+            }
+            V3<T> com = v3<T>(0, 0, 0);
+            for (unsigned j = 0; j < c; ++j) com = com + (p[nbr(i, j)].x - xi);
+            com = com / R;
+            const T ci = length(com) / (T)c;
+            if (!((T)c <= info.slope * ci + info.constant + T(0))) continue;
+            unsigned count = 0;
+            const unsigned S = info.S;
+            const unsigned s = (unsigned)i * S / 3u * 3u;
+            const T rr = info.NR / info.r;
+            const T lim = rr * rr * info.R2;
+            V3<T> nsum = v3<T>(0, 0, 0);
+            for (unsigned k = 0; k < S; ++k) {
+                const unsigned i3 = s + 3u * k;
+                const V3<T> pt = R * v3<T>(halton[i3 % kHaltonMod], halton[(i3 + 1) % kHaltonMod],
+                                           halton[(i3 + 2) % kHaltonMod]);
+                bool covered = false;
+                for (unsigned j = 0; j < c; ++j) {
+                    const V3<T> pd = p[nbr(i, j)].x - xi;
+                    if (length2(pt - pd) <= lim) {
+                        covered = true;
+                        break;
+                    }
+                }
+                if (covered) continue;
+                nsum = nsum + pt;
+                ++count;
+            }
+            if (count > 0) {
+                if (length2(nsum) > T(0)) nsum = normalize(nsum);
+                q.nMC = nsum;
+                q.kMC = T(1) / R * T(-2) * std::sqrt(T(1) - info.NR * info.NR / (info.r * info.r)) *
+                        std::cos(std::acos(T(1) - T(2) * ((T)count / (T)S)) + info.mcFactor);
+            } else {
+                q.nMC = v3<T>(0, 0, 0);
+                q.kMC = T(0);
+            }
+        }
+        const T sf = info.smoothing;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) {  // ComputeSurfaceTensionNormalsAndCurvatureKernel :960-1006
+            Particle<T>& q = p[i];
+            if (is_zero(q.nMC)) continue;
+            const V3<T> xi = q.x;
+            V3<T> nc = v3<T>(0, 0, 0);
+            T cc = 0, cf = 0;
+            for (unsigned j = 0; j < cnt(i); ++j) {
+                const Particle<T>& o = p[nbr(i, j)];
+                if (is_zero(o.nMC)) continue;
+                const T dist = length(o.x - xi);
+                const T num = T(1) - dist / R;
+                nc = nc + o.nMC * num;
+                cc += o.kMC * num;
+                cf += num;
+            }
+            if (length2(nc) > T(0)) nc = normalize(nc);
+            V3<T> ns = (T(1) - sf) * q.nMC + sf * nc;
+            if (length2(ns) > T(0)) ns = normalize(ns);
+            q.nS = ns;
+            q.kS = ((T(1) - sf) * q.kMC + sf * cc) / (T(1) - sf + sf * cf);
+        }
+        for (unsigned pass = 0; pass < desc.SurfaceTensionSmoothPassCount; ++pass) {
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) {  // ComputeSurfaceTensionBlendingKernel :1008-1046
+                Particle<T>& q = p[i];
+                if (!is_zero(q.nMC)) {
+                    T fk = q.kS;
+                    if (info.temporal) fk = T(0.05) * fk + T(0.95) * q.dK;
+                    const V3<T> force = q.nS * info.sigma * fk;
+                    q.a = q.a - info.mInv * force;
+                    q.dK = fk;
+                } else {
+                    T fk = 0;
+                    if (info.temporal) fk = T(0.95) * q.dK;
+                    q.dK = fk;
+                }
+            }
+        }
+    }
+
+    // ComputeTimeStepSize DFSPHImplementation.cu:395-441
+    void time_step() {
+        T mx = T(0.1);
+        for (const auto& q : p) mx = std::max(mx, length2(q.v + q.a * info.dt));
+        maxVel = mx;
+        if (maxVel < T(1.0e-9)) maxVel = T(1.0e-9);
+        T dt = T(0.4) * (info.diam / std::sqrt(maxVel));
+        dt = std::min(dt, (T)desc.MaxTimeStepSize);
+        dt = std::max(dt, (T)desc.MinTimeStepSize);
+        set_dt(dt);
+        update_sample_count();
+        info.mcFactor = std::asin(info.NR / info.r);
+        dbg.FrameTime += (float)info.dt;
+    }
+
+    // OnUpdate DFSPHImplementation.cu:63-170, stopping after `last` (VFD_PHASE_*)
+    void step(int last = VFD_PHASE_POSITION) {
+        if (p.empty()) return;
+        const long n = (long)p.size();
+        std::vector<V3<T>> xs(n);
+        for (long i = 0; i < n; ++i) xs[i] = p[i].x;
+        find_neighbors(xs, kernel.R, nb);
+        if (last == VFD_PHASE_SEARCH) return;
+        boundary();
+        if (last == VFD_PHASE_BOUNDARY) return;
+        density();
+        if (last == VFD_PHASE_DENSITY) return;
+        factor();
+        if (last == VFD_PHASE_FACTOR) return;
+        divergence_solve();
+        if (last == VFD_PHASE_DIVERGENCE) return;
+        for (auto& q : p) q.a = info.g;  // ClearAccelerationKernel :24-37
+        if (last == VFD_PHASE_CLEAR_ACCELERATION) return;
+        surface_tension();
+        if (last == VFD_PHASE_SURFACE_TENSION) return;
+        viscosity_solve();
+        if (last == VFD_PHASE_VISCOSITY) return;
+        time_step();
+        if (last == VFD_PHASE_TIME_STEP) return;
+        for (auto& q : p) q.v = q.v + info.dt * q.a;  // ComputeVelocityKernel :39-52
+        if (last == VFD_PHASE_VELOCITY) return;
+        pressure_solve();
+        if (last == VFD_PHASE_PRESSURE) return;
+        for (auto& q : p) q.x = q.x + info.dt * q.v;  // ComputePositionKernel :54-67
+        if (dbg.FrameIndex < desc.FrameCount && dbg.FrameTime >= desc.FrameLength) {
+            dbg.FrameTime = 0;
+            dbg.FrameIndex++;
+        }
+        dbg.IterationCount++;
+    }
+
+    // Dense viscosity system of the current state: A (3n x 3n, row-major) built by applying the matvec to unit
+    // vectors, and b (the right-hand side, = v). Test aid for tiny n.
+    void assemble_viscosity(double* A, double* b) {
+        const long n = (long)p.size();
+        std::vector<V3<T>> in(n, v3<T>(0, 0, 0)), out(n);
+        for (long k = 0; k < n; ++k) {
+            for (int c = 0; c < 3; ++c) {
+                in[k] = v3<T>(c == 0 ? T(1) : T(0), c == 1 ? T(1) : T(0), c == 2 ? T(1) : T(0));
+                matvec(in, out);
+                for (long i = 0; i < n; ++i) {
+                    A[(3 * i + 0) * 3 * n + 3 * k + c] = (double)out[i].x;
+                    A[(3 * i + 1) * 3 * n + 3 * k + c] = (double)out[i].y;
+                    A[(3 * i + 2) * 3 * n + 3 * k + c] = (double)out[i].z;
+                }
+                in[k] = v3<T>(0, 0, 0);
+            }
+        }
+        for (long i = 0; i < n; ++i) {
+            b[3 * i] = (double)p[i].v.x;
+            b[3 * i + 1] = (double)p[i].v.y;
+            b[3 * i + 2] = (double)p[i].v.z;
+        }
+    }
+
+    // field access -------------------------------------------------------------------------------------------
+    int get_field(int field, unsigned body, double* out) const {
+        const size_t n = p.size();
+        auto put3 = [&](size_t i, V3<T> v) {
+            out[3 * i] = (double)v.x;
+            out[3 * i + 1] = (double)v.y;
+            out[3 * i + 2] = (double)v.z;
+        };
+        for (size_t i = 0; i < n; ++i) {
+            const Particle<T>& q = p[i];
+            switch (field) {
+                case VFD_FIELD_POSITION: put3(i, q.x); break;
+                case VFD_FIELD_VELOCITY: put3(i, q.v); break;
+                case VFD_FIELD_ACCELERATION: put3(i, q.a); break;
+                case VFD_FIELD_PRESSURE_ACCELERATION: put3(i, q.ap); break;
+                case VFD_FIELD_PRESSURE_RESIDUUM: out[i] = (double)q.residuum; break;
+                case VFD_FIELD_DENSITY: out[i] = (double)q.density; break;
+                case VFD_FIELD_DENSITY_ADVECTION: out[i] = (double)q.densityAdv; break;
+                case VFD_FIELD_PRESSURE_RHO2: out[i] = (double)q.kappa; break;
+                case VFD_FIELD_PRESSURE_RHO2_V: out[i] = (double)q.kappaV; break;
+                case VFD_FIELD_FACTOR: out[i] = (double)q.factor; break;
+                case VFD_FIELD_VELOCITY_DIFFERENCE: put3(i, q.dv); break;
+                case VFD_FIELD_MC_SURFACE_NORMAL: put3(i, q.nMC); break;
+                case VFD_FIELD_MC_SURFACE_NORMAL_SMOOTH: put3(i, q.nS); break;
+                case VFD_FIELD_MC_SURFACE_CURVATURE: out[i] = (double)q.kMC; break;
+                case VFD_FIELD_MC_SURFACE_CURVATURE_SMOOTH: out[i] = (double)q.kS; break;
+                case VFD_FIELD_DELTA_FINAL_CURVATURE: out[i] = (double)q.dK; break;
+                case VFD_FIELD_BOUNDARY_XJ:
+                    if (body >= bodies.size()) return VFD_ERROR_INVALID_ARGUMENT;
+                    put3(i, bodies[body].xb[i]);
+                    break;
+                case VFD_FIELD_BOUNDARY_VOLUME:
+                    if (body >= bodies.size()) return VFD_ERROR_INVALID_ARGUMENT;
+                    out[i] = (double)bodies[body].vb[i];
+                    break;
+                case VFD_FIELD_VISCOSITY_SOLUTION: put3(i, vb_g[i]); break;
+                case VFD_FIELD_VISCOSITY_PRECONDITIONER:
+                    for (int e = 0; e < 9; ++e) out[9 * i + e] = (double)minv[i].m[e];
+                    break;
+                default: return VFD_ERROR_INVALID_ARGUMENT;
+            }
+        }
+        return VFD_OK;
+    }
+    int set_field(int field, const double* in) {
+        const size_t n = p.size();
+        for (size_t i = 0; i < n; ++i) {
+            const V3<T> v = v3<T>((T)in[3 * i], (T)in[3 * i + 1], (T)in[3 * i + 2]);
+            switch (field) {
+                case VFD_FIELD_POSITION: p[i].x = v; break;
+                case VFD_FIELD_VELOCITY: p[i].v = v; break;
+                case VFD_FIELD_VELOCITY_DIFFERENCE: p[i].dv = v; break;
+                case VFD_FIELD_DELTA_FINAL_CURVATURE: p[i].dK = (T)in[i]; break;
+                default: return VFD_ERROR_INVALID_ARGUMENT;
+            }
+        }
+        return VFD_OK;
+    }
+};
+
+struct Handle {
+    int fp64;
+    Sim<float>* f;
+    Sim<double>* d;
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+#define DISPATCH(h, ...)                            \
+    do {                                            \
+        Handle* hh = (Handle*)(h);                  \
+        if (!hh) return VFD_ERROR_INVALID_ARGUMENT; \
+        if (hh->fp64) {                             \
+            auto* s = hh->d;                        \
+            __VA_ARGS__;                            \
+        } else {                                    \
+            auto* s = hh->f;                        \
+            __VA_ARGS__;                            \
+        }                                           \
+    } while (0)
+
+extern "C" {
+
+void* orc_create(const vfd_desc* desc, int fp64) {
+    Handle* h = new Handle{fp64, nullptr, nullptr};
+    if (fp64)
+        h->d = new Sim<double>(*desc);
+    else
+        h->f = new Sim<float>(*desc);
+    return h;
+}
+void orc_destroy(void* hv) {
+    Handle* h = (Handle*)hv;
+    if (!h) return;
+    delete h->f;
+    delete h->d;
+    delete h;
+}
+int orc_set_description(void* h, const vfd_desc* d) { DISPATCH(h, s->set_description(*d)); return VFD_OK; }
+int orc_set_fluid(void* h, const float* x, const float* v, uint32_t n) {
+    DISPATCH(h, s->set_fluid(x, v, n));
+    return VFD_OK;
+}
+int orc_add_boundary(void* h, const vfd_boundary_desc* b) { DISPATCH(h, s->add_boundary(*b)); return VFD_OK; }
+int orc_step(void* h) { DISPATCH(h, s->step()); return VFD_OK; }
+int orc_step_until(void* h, int phase) { DISPATCH(h, s->step(phase)); return VFD_OK; }
+int orc_set_time_step(void* h, double dt) {
+    DISPATCH(h, { s->set_dt(dt); s->update_sample_count(); });
+    return VFD_OK;
+}
+int orc_get_field(void* h, int field, uint32_t body, double* out) {
+    DISPATCH(h, return s->get_field(field, body, out));
+    return VFD_OK;
+}
+int orc_set_field(void* h, int field, const double* in) {
+    DISPATCH(h, return s->set_field(field, in));
+    return VFD_OK;
+}
+int orc_particle_count(void* h, uint32_t* n) { DISPATCH(h, *n = (uint32_t)s->p.size()); return VFD_OK; }
+int orc_get_neighbors(void* h, uint32_t* counts, uint32_t* offsets, uint32_t* ids, uint64_t* total) {
+    DISPATCH(h, {
+        const NeighborLists& L = s->nb;
+        if (total) *total = L.ids.size();
+        if (counts) std::memcpy(counts, L.counts.data(), L.counts.size() * 4);
+        if (offsets) std::memcpy(offsets, L.offsets.data(), L.offsets.size() * 4);
+        if (ids) std::memcpy(ids, L.ids.data(), L.ids.size() * 4);
+    });
+    return VFD_OK;
+}
+int orc_get_debug_info(void* h, vfd_debug_info* out) {
+    DISPATCH(h, {
+        *out = s->dbg;
+        out->TimeStepSize = (float)s->info.dt;
+        out->MaxVelocityMagnitude = (float)s->maxVel;
+    });
+    return VFD_OK;
+}
+// info as doubles in vfd_info member order (numeric members only, gravity last)
+int orc_get_info(void* h, double* out) {
+    DISPATCH(h, {
+        const auto& I = s->info;
+        const double v[] = {(double)I.particleCount, (double)I.bodyCount, (double)I.R,  (double)I.R2,
+                            (double)I.r,  (double)I.diam, (double)I.dt, (double)I.dt2,
+                            (double)I.dtInv, (double)I.dt2Inv, (double)I.V,  (double)I.rho0,
+                            (double)I.m,  (double)I.mInv, (double)I.nu, (double)I.nuB,
+                            (double)I.mu, (double)I.muB, (double)I.tdf, (double)I.td,
+                            (double)I.sigma, (double)I.S, (double)I.slope, (double)I.constant,
+                            (double)I.temporal, (double)I.smoothing, (double)I.factor, (double)I.NR,
+                            (double)I.mcFactor, (double)I.g.x, (double)I.g.y, (double)I.g.z};
+        std::memcpy(out, v, sizeof(v));
+    });
+    return VFD_OK;
+}
+int orc_get_kernel_table(void* h, double* w, double* g, double* scalars) {
+    DISPATCH(h, {
+        const auto& K = s->kernel;
+        for (unsigned i = 0; i < kTableResolution; ++i) w[i] = (double)K.W[i];
+        for (unsigned i = 0; i <= kTableResolution; ++i) g[i] = (double)K.G[i];
+        scalars[0] = K.R;
+        scalars[1] = K.R2;
+        scalars[2] = K.invStep;
+        scalars[3] = K.W0;
+        scalars[4] = K.K;
+        scalars[5] = K.L;
+    });
+    return VFD_OK;
+}
+// kernel evaluation for tests: which = 0 W(vec), 1 W(scalar |r|), 2 gradW(vec)
+int orc_kernel_eval(void* h, int which, const double* r, uint32_t n, double* out) {
+    DISPATCH(h, {
+        using TT = std::remove_reference<decltype(s->kernel.R)>::type;
+        for (uint32_t i = 0; i < n; ++i) {
+            const V3<TT> v = v3<TT>((TT)r[3 * i], (TT)r[3 * i + 1], (TT)r[3 * i + 2]);
+            if (which == 0)
+                out[3 * i] = (double)s->kernel.getW(v);
+            else if (which == 1)
+                out[3 * i] = (double)s->kernel.getW((TT)r[3 * i]);
+            else {
+                const V3<TT> g = s->kernel.gradW(v);
+                out[3 * i] = g.x;
+                out[3 * i + 1] = g.y;
+                out[3 * i + 2] = g.z;
+            }
+        }
+    });
+    return VFD_OK;
+}
+int orc_get_boundary_map(void* h, uint32_t body, uint32_t field, double* nodes, uint32_t* count, double* geom) {
+    DISPATCH(h, {
+        if (body >= s->bodies.size() || field > 1) return VFD_ERROR_INVALID_ARGUMENT;
+        const auto& M = s->bodies[body].map;
+        if (count) *count = M.nodeCount;
+        if (nodes)
+            for (unsigned i = 0; i < M.nodeCount; ++i) nodes[i] = (double)M.nodes[field][i];
+        if (geom) {
+            const double gv[] = {(double)M.domain.min.x, (double)M.domain.min.y, (double)M.domain.min.z,
+                                 (double)M.domain.max.x, (double)M.domain.max.y, (double)M.domain.max.z,
+                                 (double)M.res[0], (double)M.res[1], (double)M.res[2]};
+            std::memcpy(geom, gv, sizeof(gv));
+        }
+    });
+    return VFD_OK;
+}
+// interpolate a body's map field at points (field 0 also returns the gradient)
+int orc_map_interpolate(void* h, uint32_t body, uint32_t field, const double* pts, uint32_t n, double* out) {
+    DISPATCH(h, {
+        using TT = std::remove_reference<decltype(s->kernel.R)>::type;
+        if (body >= s->bodies.size()) return VFD_ERROR_INVALID_ARGUMENT;
+        const auto& M = s->bodies[body].map;
+        for (uint32_t i = 0; i < n; ++i) {
+            const V3<TT> x = v3<TT>((TT)pts[3 * i], (TT)pts[3 * i + 1], (TT)pts[3 * i + 2]);
+            unsigned cell[32];
+            V3<TT> c0, grad = v3<TT>(0, 0, 0);
+            TT N[32];
+            V3<TT> dN[32];
+            double phi = (double)FLT_MAX;
+            if (M.locate(x, cell, c0, N, dN))
+                phi = field == 0 ? (double)M.interp_grad(0, cell, c0, N, dN, grad) : (double)M.interp(1, cell, N);
+            out[4 * i] = phi;
+            out[4 * i + 1] = grad.x;
+            out[4 * i + 2] = grad.y;
+            out[4 * i + 3] = grad.z;
+        }
+    });
+    return VFD_OK;
+}
+
+int orc_viscosity_system(void* h, double* A, double* b) {
+    DISPATCH(h, s->assemble_viscosity(A, b));
+    return VFD_OK;
+}
+
+// ----- unit-level entry points ---------------------------------------------------------------------------------
+void orc_halton(int fp64, double* out) {
+    if (fp64) {
+        auto h = halton_table<double>();
+        for (unsigned i = 0; i < kHaltonMod; ++i) out[i] = h[i];
+    } else {
+        auto h = halton_table<float>();
+        for (unsigned i = 0; i < kHaltonMod; ++i) out[i] = h[i];
+    }
+}
+void orc_gauss_legendre(int n, double* x, double* w) {
+    std::vector<double> gx, gw;
+    gauss_legendre(n, gx, gw);
+    for (int i = 0; i < n; ++i) {
+        x[i] = gx[i];
+        w[i] = gw[i];
+    }
+}
+void orc_shape_function(const double* xi, double* N, double* dN) {
+    double n[32];
+    V3<double> d[32];
+    shape_function(v3<double>(xi[0], xi[1], xi[2]), n, d);
+    for (int k = 0; k < 32; ++k) {
+        N[k] = n[k];
+        dN[3 * k] = d[k].x;
+        dN[3 * k + 1] = d[k].y;
+        dN[3 * k + 2] = d[k].z;
+    }
+}
+uint32_t orc_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3(x, y, z); }
+// cell node indices of a map of resolution res (SDF.cu:81-131)
+void orc_cell_nodes(const uint32_t* res, uint32_t i, uint32_t j, uint32_t k, uint32_t* out) {
+    VolumeMap<double> m;
+    Box<double> b{v3<double>(0, 0, 0), v3<double>(1, 1, 1)};
+    m.init(b, res);
+    m.cell_nodes(i, j, k, out);
+}
+void orc_node_position(const double* dmin, const double* dmax, const uint32_t* res, uint32_t i, double* out) {
+    VolumeMap<double> m;
+    Box<double> b{v3<double>(dmin[0], dmin[1], dmin[2]), v3<double>(dmax[0], dmax[1], dmax[2])};
+    m.init(b, res);
+    const V3<double> p = m.node_position(i);
+    out[0] = p.x;
+    out[1] = p.y;
+    out[2] = p.z;
+}
+uint32_t orc_node_count(const uint32_t* res) {
+    VolumeMap<double> m;
+    Box<double> b{v3<double>(0, 0, 0), v3<double>(1, 1, 1)};
+    m.init(b, res);
+    return m.nodeCount;
+}
+// O(N^2) neighbour sets (no grid) with the same distance expression; lists ascending by index, capped at cap.
+int orc_brute_force_neighbors(const float* x, uint32_t n, float R, uint32_t cap, uint32_t* counts, uint32_t* ids) {
+    const float R2 = R * R;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long il = 0; il < (long)n; ++il) {
+        const uint32_t i = (uint32_t)il;
+        uint32_t c = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+            const float dx = x[3 * j] - x[3 * i], dy = x[3 * j + 1] - x[3 * i + 1], dz = x[3 * j + 2] - x[3 * i + 2];
+            const float d2 = std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+            if (d2 < R2 && d2 > 0.0f) {
+                if (c < cap) ids[(size_t)i * cap + c] = j;
+                ++c;
+            }
+        }
+        counts[i] = c;
+    }
+    return VFD_OK;
+}
+// grid geometry of the search for positions x (tests): floats gridMin[3], gridDelta[3]; uints dims[3], meta[3]
+int orc_search_grid(const float* x, uint32_t n, float R, float* fout, uint32_t* uout) {
+    std::vector<V3<float>> xs(n);
+    for (uint32_t i = 0; i < n; ++i) xs[i] = v3<float>(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    const SearchGrid<float> g = make_grid(xs, R);
+    fout[0] = g.gridMin.x;
+    fout[1] = g.gridMin.y;
+    fout[2] = g.gridMin.z;
+    fout[3] = g.gridDelta.x;
+    fout[4] = g.gridDelta.y;
+    fout[5] = g.gridDelta.z;
+    for (int a = 0; a < 3; ++a) {
+        uout[a] = g.dims[a];
+        uout[3 + a] = g.meta[a];
+    }
+    uout[6] = g.cellCount;
+    return VFD_OK;
+}
+
+}  // extern "C"

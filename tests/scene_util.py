@@ -1,0 +1,52 @@
+"""Helpers that build the same scene on the HIP product and on the CPU oracle (tests only)."""
+import numpy as np
+
+from vfd_amd import scenes
+from vfd_amd.simulation import DFSPHSimulationDescription, RigidBodyDescription
+
+
+def small_scene(kind="dam", seed=7):
+    """Small scenes (<= ~8k particles) that exercise every solver: kind = dam (C1 settings), visc (viscosity +
+    boundary viscosity), st (surface tension + viscosity), all (everything, temporal smoothing, 2 blend passes)."""
+    if kind == "dam":
+        return scenes.dam_break_8k(seed=seed)
+    rng = np.random.default_rng(seed)
+    r = 0.02
+    x = scenes.lattice_block((-0.3, -0.84, -0.2), (16, 14, 16), 2 * r, 0.01 * r, rng)
+    v = np.zeros_like(x)
+    v[:, 1] = -0.5
+    v += rng.uniform(-0.05, 0.05, size=v.shape).astype(np.float32)
+    kw = dict(ParticleRadius=r, MinDivergenceSolverIterations=2, MinPressureSolverIterations=2,
+              MaxDivergenceSolverError=0.1, MaxPressureSolverError=0.01, Viscosity=0.05, BoundaryViscosity=0.05,
+              MaxViscositySolverError=0.1, EnableViscositySolver=1, EnableSurfaceTensionSolver=0)
+    if kind in ("st", "all"):
+        kw.update(EnableSurfaceTensionSolver=1, SurfaceTension=0.05, CSD=10000)
+    if kind == "all":
+        kw.update(TemporalSmoothing=1, SurfaceTensionSmoothPassCount=2, Viscosity=0.5, BoundaryViscosity=0.2)
+    desc = DFSPHSimulationDescription(**kw)
+    box = RigidBodyDescription("box", center=(0, 0, 0), half_extents=(1, 1, 1), inverted=True, resolution=(16, 16, 16))
+    ball = RigidBodyDescription("sphere", center=(0.5, -0.8, 0.3), radius=0.15, padding=2 * r, resolution=(10, 10, 10))
+    return [desc], [(x, v.astype(np.float32))], [[box, ball]]
+
+
+def make_gpu(descs, fluids, bodies):
+    from vfd_amd import DFSPHBatch, DFSPHSimulation
+    if len(descs) == 1:
+        sim = DFSPHSimulation(descs[0])
+        sim.set_fluid(*fluids[0])
+        sim.SetRigidBodies(bodies[0])
+        return sim
+    b = DFSPHBatch(descs)
+    for s, (f, bl) in enumerate(zip(fluids, bodies)):
+        b.scenes[s].set_fluid(*f)
+        b.scenes[s].SetRigidBodies(bl)
+    return b
+
+
+def make_oracle(descs, fluids, bodies, fp64=False, scene=0):
+    from oracle.oracle import OracleSim
+    o = OracleSim(descs[scene], fp64=fp64)
+    o.set_fluid(*fluids[scene])
+    for b in bodies[scene]:
+        o.add_boundary(b.to_abi())
+    return o

@@ -1,0 +1,338 @@
+// boundary.hip - SDF volume-map boundaries.
+//
+// Build (setup, once per body): RigidBody ctor (VFD/Source/Simulation/DFSPH/RigidBody/RigidBody.cu:10-73) builds a
+// two-field cubic-Lagrange volume map (Utility/SDF/SDF.cu:45-139): field 0 = sign * (d(x) - (Padding - r)),
+// field 1 = 0.8 * integral over the support ball of {1 if phi <= 0; W(phi)/W(0) if phi < R} by 16^3-point
+// Gauss-Legendre quadrature (Core/Math/GaussQuadrature.cpp:5621-5655). The reference evaluates this on the host
+// with OpenMP; here one GPU thread builds one node, keeping the reference's summation order (sequential in
+// i, j, k) so that the nodes reproduce the CPU result.
+//
+// Step: ComputeVolumeAndBoundaryKernel (DFSPHKernels.cu:69-150) per particle and body: map lookup, boundary
+// neighbour position x_b and volume V_b, push-out of particles inside the solid.
+#include "device.h"
+#include "kernels.h"
+
+namespace vfd {
+
+// ---- 32-node serendipity shape functions on [-1,1]^3 (SDFDeviceData.cuh:36-370) ---------------------------------
+// Node order: 8 corners (bit0 x, bit1 y, bit2 z), x-edge nodes 8..15 (pair index iz + 2*iy), y-edge nodes 16..23
+// (pair index ix + 2*iz), z-edge nodes 24..31 (pair index iy + 2*ix); in each pair the node at -1/3 comes first.
+__device__ __forceinline__ void edge1d(float s, int side, float& f, float& df) {
+    const float t = side == 0 ? 1.0f - 3.0f * s : 1.0f + 3.0f * s;
+    const float ds = side == 0 ? -3.0f : 3.0f;
+    f = (1.0f - s * s) * t;
+    df = -2.0f * s * t + (1.0f - s * s) * ds;
+}
+
+template <bool kGrad>
+__device__ __forceinline__ void shape_fn(f3 xi, float N[32], f3 dN[32]) {
+    const float x = xi.x, y = xi.y, z = xi.z;
+    const float x2 = x * x, y2 = y * y, z2 = z * z;
+    const float lx[2] = {1.0f - x, 1.0f + x}, ly[2] = {1.0f - y, 1.0f + y}, lz[2] = {1.0f - z, 1.0f + z};
+    const float sg[2] = {-1.0f, 1.0f};
+    const float q = 9.0f * (x2 + y2 + z2) - 19.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int ix = c & 1, iy = (c >> 1) & 1, iz = (c >> 2) & 1;
+        const float a = lx[ix], b = ly[iy], d = lz[iz];
+        N[c] = 1.0f / 64.0f * q * a * b * d;
+        if (kGrad) {
+            dN[c].x = (sg[ix] * q + 18.0f * x * a) * b * d / 64.0f;
+            dN[c].y = (sg[iy] * q + 18.0f * y * b) * a * d / 64.0f;
+            dN[c].z = (sg[iz] * q + 18.0f * z * d) * a * b / 64.0f;
+        }
+    }
+    const float k = 9.0f / 64.0f;
+#pragma unroll
+    for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+        for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                float f, df;
+                edge1d(x, side, f, df);
+                const int n = 8 + 2 * (iz + 2 * iy) + side;
+                N[n] = k * f * ly[iy] * lz[iz];
+                if (kGrad) dN[n] = mk3(k * df * ly[iy] * lz[iz], k * f * sg[iy] * lz[iz], k * f * ly[iy] * sg[iz]);
+            }
+#pragma unroll
+    for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
+        for (int ix = 0; ix < 2; ++ix)
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                float f, df;
+                edge1d(y, side, f, df);
+                const int n = 16 + 2 * (ix + 2 * iz) + side;
+                N[n] = k * f * lx[ix] * lz[iz];
+                if (kGrad) dN[n] = mk3(k * f * sg[ix] * lz[iz], k * df * lx[ix] * lz[iz], k * f * lx[ix] * sg[iz]);
+            }
+#pragma unroll
+    for (int ix = 0; ix < 2; ++ix)
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                float f, df;
+                edge1d(z, side, f, df);
+                const int n = 24 + 2 * (iy + 2 * ix) + side;
+                N[n] = k * f * lx[ix] * ly[iy];
+                if (kGrad) dN[n] = mk3(k * f * sg[ix] * ly[iy], k * f * lx[ix] * sg[iy], k * df * lx[ix] * ly[iy]);
+            }
+}
+
+// 32 node indices of cell (i,j,k) in the node layout of SDF::AddFunction (SDF.cu:81-131).
+__device__ __forceinline__ void cell_nodes(const MapDev& m, uint32_t i, uint32_t j, uint32_t k, uint32_t out[32]) {
+    const uint32_t nx = m.res[0], ny = m.res[1], nz = m.res[2];
+    const uint32_t nv = (nx + 1) * (ny + 1) * (nz + 1);
+    const uint32_t nex = nx * (ny + 1) * (nz + 1), ney = (nx + 1) * ny * (nz + 1);
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c)
+        out[c] = (nx + 1) * (ny + 1) * (k + ((c >> 2) & 1u)) + (nx + 1) * (j + ((c >> 1) & 1u)) + (i + (c & 1u));
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+        const uint32_t b = nv + 2 * (nx * (ny + 1) * (k + (e & 1u)) + nx * (j + (e >> 1)) + i);
+        out[8 + 2 * e] = b;
+        out[9 + 2 * e] = b + 1;
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+        const uint32_t b = nv + 2 * nex + 2 * (ny * (nz + 1) * (i + (e & 1u)) + ny * (k + (e >> 1)) + j);
+        out[16 + 2 * e] = b;
+        out[17 + 2 * e] = b + 1;
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+        const uint32_t b = nv + 2 * nex + 2 * ney + 2 * (nz * (nx + 1) * (j + (e & 1u)) + nz * (i + (e >> 1)) + k);
+        out[24 + 2 * e] = b;
+        out[25 + 2 * e] = b + 1;
+    }
+}
+
+// DetermineShapeFunction (SDFDeviceData.cuh:372-419): false outside the map domain.
+template <bool kGrad>
+__device__ __forceinline__ bool map_locate(const MapDev& m, f3 x, uint32_t cell[32], f3& c0, float N[32], f3 dN[32]) {
+    if (!(m.domMin[0] <= x.x && m.domMin[1] <= x.y && m.domMin[2] <= x.z && m.domMax[0] >= x.x &&
+          m.domMax[1] >= x.y && m.domMax[2] >= x.z))
+        return false;
+    uint32_t mi[3] = {(uint32_t)((x.x - m.domMin[0]) * m.cellSizeInv[0]),
+                      (uint32_t)((x.y - m.domMin[1]) * m.cellSizeInv[1]),
+                      (uint32_t)((x.z - m.domMin[2]) * m.cellSizeInv[2])};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (mi[a] >= m.res[a]) mi[a] = m.res[a] - 1;
+    const f3 origin = mk3(m.domMin[0] + (float)mi[0] * m.cellSize[0], m.domMin[1] + (float)mi[1] * m.cellSize[1],
+                          m.domMin[2] + (float)mi[2] * m.cellSize[2]);
+    const f3 top = origin + mk3(m.cellSize[0], m.cellSize[1], m.cellSize[2]);
+    const f3 den = top - origin;
+    c0 = mk3(2.0f / den.x, 2.0f / den.y, 2.0f / den.z);
+    const f3 sum = top + origin;
+    const f3 c1 = mk3(sum.x / den.x, sum.y / den.y, sum.z / den.z);
+    const f3 xi = cmul(c0, x) - c1;
+    cell_nodes(m, mi[0], mi[1], mi[2], cell);
+    shape_fn<kGrad>(xi, N, dN);
+    return true;
+}
+
+__device__ __forceinline__ f3 node_position(const MapDev& m, uint32_t i) {
+    const uint32_t nx = m.res[0], ny = m.res[1], nz = m.res[2];
+    const uint32_t nv = (nx + 1) * (ny + 1) * (nz + 1);
+    const uint32_t nex = nx * (ny + 1) * (nz + 1), ney = (nx + 1) * ny * (nz + 1);
+    const f3 cs = mk3(m.cellSize[0], m.cellSize[1], m.cellSize[2]);
+    const f3 mn = mk3(m.domMin[0], m.domMin[1], m.domMin[2]);
+    f3 idx;
+    f3 r;
+    if (i < nv) {
+        idx.z = (float)(i / ((ny + 1) * (nx + 1)));
+        const uint32_t t = i % ((ny + 1) * (nx + 1));
+        idx.y = (float)(t / (nx + 1));
+        idx.x = (float)(t % (nx + 1));
+        r = mn + cmul(cs, idx);
+    } else if (i < nv + 2 * nex) {
+        i -= nv;
+        const uint32_t e = i / 2;
+        idx.z = (float)(e / ((ny + 1) * nx));
+        const uint32_t t = e % ((ny + 1) * nx);
+        idx.y = (float)(t / nx);
+        idx.x = (float)(t % nx);
+        r = mn + cmul(cs, idx);
+        r.x += (1.0f + (float)(i % 2)) / 3.0f * cs.x;
+    } else if (i < nv + 2 * (nex + ney)) {
+        i -= nv + 2 * nex;
+        const uint32_t e = i / 2;
+        idx.x = (float)(e / ((nz + 1) * ny));
+        const uint32_t t = e % ((nz + 1) * ny);
+        idx.z = (float)(t / ny);
+        idx.y = (float)(t % ny);
+        r = mn + cmul(cs, idx);
+        r.y += (1.0f + (float)(i % 2)) / 3.0f * cs.y;
+    } else {
+        i -= nv + 2 * (nex + ney);
+        const uint32_t e = i / 2;
+        idx.y = (float)(e / ((nx + 1) * nz));
+        const uint32_t t = e % ((nx + 1) * nz);
+        idx.x = (float)(t / nz);
+        idx.z = (float)(t % nz);
+        r = mn + cmul(cs, idx);
+        r.z += (1.0f + (float)(i % 2)) / 3.0f * cs.z;
+    }
+    return r;
+}
+
+// exact signed distance of the analytic shape; d < 0 inside
+__device__ __forceinline__ float shape_distance(const ShapeDev& s, f3 p) {
+    const f3 d = p - mk3(s.c[0], s.c[1], s.c[2]);
+    switch (s.shape) {
+        case 0: {  // box
+            const f3 q = mk3(fabsf(d.x) - s.h[0], fabsf(d.y) - s.h[1], fabsf(d.z) - s.h[2]);
+            const f3 qp = mk3(fmaxf(q.x, 0.0f), fmaxf(q.y, 0.0f), fmaxf(q.z, 0.0f));
+            return sqrtf(dot3(qp, qp)) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f);
+        }
+        case 1:  // sphere
+            return sqrtf(dot3(d, d)) - s.radius;
+        case 2:  // plane
+            return dot3(d, mk3(s.n[0], s.n[1], s.n[2]));
+        default: {  // cylinder, axis y
+            const float rxz = sqrtf(fmaf(d.z, d.z, d.x * d.x)) - s.radius;
+            const float ry = fabsf(d.y) - s.h[1];
+            const float ox = fmaxf(rxz, 0.0f), oy = fmaxf(ry, 0.0f);
+            return sqrtf(fmaf(oy, oy, ox * ox)) + fminf(fmaxf(rxz, ry), 0.0f);
+        }
+    }
+}
+
+__device__ __forceinline__ float map_interp_at0(const MapDev& m, const float* nodes0, f3 x) {
+    uint32_t cell[32];
+    f3 c0;
+    float N[32];
+    f3 dN[32];
+    if (!map_locate<false>(m, x, cell, c0, N, dN)) return FLT_MAX;
+    float phi = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) phi += nodes0[cell[k]] * N[k];
+    return phi;
+}
+
+__global__ void k_map_field0(ShapeDev shape, MapDev map, float* nodes0) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= map.nodeCount) return;
+    const f3 x = node_position(map, l);
+    nodes0[l] = shape.sign * (shape_distance(shape, x) - shape.tol);
+}
+
+__global__ void __launch_bounds__(64) k_map_field1(MapDev map, const float* __restrict__ nodes0, float* nodes1,
+                                                   KTabHost kt, const float* __restrict__ glx,
+                                                   const float* __restrict__ glw) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= map.nodeCount) return;
+    const KTab k{kt.tab, kt.R, kt.R2, kt.invStep, kt.W0};
+    const float R = kt.R;
+    const f3 x = node_position(map, l);
+    const float dX = map_interp_at0(map, nodes0, x);
+    if (dX > 2.0f * R) {
+        nodes1[l] = 0.0f;
+        return;
+    }
+    const double c0 = (double)R;
+    float res = 0.0f;
+    for (int i = 0; i < 16; ++i) {
+        const float wi = glw[i];
+        for (int j = 0; j < 16; ++j) {
+            const float wij = wi * glw[j];
+            for (int kk = 0; kk < 16; ++kk) {
+                const float wijk = wij * glw[kk];
+                const f3 xi = mk3((float)(c0 * (double)glx[i]), (float)(c0 * (double)glx[j]), (float)(c0 * (double)glx[kk]));
+                float f = 0.0f;
+                if (!(dot3(xi, xi) > R * R)) {
+                    const float dist = map_interp_at0(map, nodes0, x + xi);
+                    if (dist <= 0.0f)
+                        f = 1.0f;
+                    else if (dist < R)
+                        f = getW_scalar(k, dist) / k.W0;
+                }
+                res += wijk * f;
+            }
+        }
+    }
+    res = (float)((double)res * (c0 * c0 * c0));
+    nodes1[l] = 0.8f * res;
+}
+
+void launch_map_build(hipStream_t st, const ShapeDev& shape, const MapDev& map, float* nodes0, float* nodes1,
+                      const KTabHost& kt, const float* glx, const float* glw) {
+    hipLaunchKernelGGL(k_map_field0, dim3((map.nodeCount + 255) / 256), dim3(256), 0, st, shape, map, nodes0);
+    hipLaunchKernelGGL(k_map_field1, dim3((map.nodeCount + 63) / 64), dim3(64), 0, st, map, (const float*)nodes0,
+                       nodes1, kt, glx, glw);
+}
+
+// ComputeVolumeAndBoundaryKernel (DFSPHKernels.cu:69-150).
+__global__ void __launch_bounds__(kTile) k_boundary(const SceneParams* __restrict__ P,
+                                                    const uint32_t* __restrict__ tileScene,
+                                                    const MapDev* __restrict__ maps, float4* pos, float4* vel,
+                                                    float4* xbOut, float* vbOut, uint32_t capacity) {
+    const TileCtx c = tile_ctx(tileScene, P);
+    if (!c.valid) return;
+    const SceneParams& p = P[c.s];
+    f3 x = ld3(pos[c.i]);
+    bool moved = false;
+    for (uint32_t b = 0; b < p.bodyCount; ++b) {
+        const MapDev& m = maps[c.s * kMaxBodies + b];
+        f3 xb = mk3(0.0f, 0.0f, 0.0f);
+        float vb = 0.0f;
+        f3 normal = mk3(0.0f, 0.0f, 0.0f);
+        float dist = FLT_MAX;
+        uint32_t cell[32];
+        f3 c0;
+        float N[32];
+        f3 dN[32];
+        if (map_locate<true>(m, x, cell, c0, N, dN)) {
+            float phi = 0.0f;
+            f3 g = mk3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                const float cv = m.nodes0[cell[k]];
+                phi += cv * N[k];
+                g = g + cv * dN[k];
+            }
+            normal = cmul(g, c0);
+            dist = phi;
+        }
+        if (dist > 0.0f && dist < p.R) {
+            float vol = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 32; ++k) vol += m.nodes1[cell[k]] * N[k];
+            if (vol > 0.0f && vol != FLT_MAX) {
+                vb = vol;
+                const float nl = length(normal);
+                if (nl > 1.0e-9f) {
+                    normal = normal / nl;
+                    const float d = fmaxf(dist + 0.5f * p.r, p.diam);
+                    xb = x - d * normal;
+                } else {
+                    vb = 0.0f;
+                }
+            }
+        } else if (dist <= 0.0f) {
+            const float nl = length(normal);
+            if (nl > 1.0e-5f) {
+                normal = normal / nl;
+                const float delta = fminf(p.diam - dist, 0.1f * p.r);
+                x = x + delta * normal;
+                moved = true;
+            }
+        }
+        xbOut[(size_t)b * capacity + c.i] = st4(xb);
+        vbOut[(size_t)b * capacity + c.i] = vb;
+    }
+    if (moved) {
+        pos[c.i] = st4(x);
+        vel[c.i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+void launch_boundary(const Launch& L, const MapDev* maps, float4* pos, float4* vel, float4* xb, float* vb) {
+    hipLaunchKernelGGL(k_boundary, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, maps, pos, vel, xb, vb,
+                       L.capacity);
+}
+
+}  // namespace vfd

@@ -1,0 +1,120 @@
+// kernels.h - host-side launchers of the HIP kernels (declarations shared by the .hip files and sim.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace vfd {
+
+// Launch geometry shared by per-particle kernels: one block of kTile threads per tile.
+struct Launch {
+    hipStream_t stream;
+    const SceneParams* P;        // device [sceneCount]
+    const uint32_t* tileScene;   // device [tiles]
+    uint32_t tiles;
+    uint32_t sceneCount;
+    uint32_t capacity;           // tiles * kTile
+};
+
+// Analytic shape + volume-map geometry of one boundary body (device copies live in an array indexed by
+// scene * kMaxBodies + body).
+struct ShapeDev {
+    int32_t shape;
+    float c[3], h[3], radius, n[3];
+    float sign, tol;
+};
+struct MapDev {
+    float domMin[3], domMax[3];
+    float cellSize[3], cellSizeInv[3];
+    uint32_t res[3];
+    uint32_t nodeCount;
+    const float* nodes0;  // signed distance
+    const float* nodes1;  // boundary volume
+};
+
+// Kernel table view (device pointer + scalars).
+struct KTabHost {
+    const float2* tab;
+    float R, R2, invStep, W0;
+};
+
+// ---- search.hip -----------------------------------------------------------------------------------------------
+uint32_t scan_scratch_words(uint32_t n);
+void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch, hipStream_t st);
+void launch_minmax(const Launch& L, const float4* pos, int* mm);
+void launch_sort(const Launch& L, const SceneGrid* G, uint32_t totalCells, const float4* posIn, const float4* velIn,
+                 const float4* dvIn, const float* dkIn, const uint32_t* idIn, float4* posOut, float4* velOut,
+                 float4* dvOut, float* dkOut, uint32_t* idOut, uint32_t* cellOf, uint32_t* rank, uint32_t* tmpId,
+                 uint32_t* cellCount, uint32_t* cellStart, uint32_t* scanScratch);
+void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, const uint32_t* cellStart,
+                        const uint32_t* cellCount, uint32_t* nbrCount, uint32_t* nbrList,
+                        unsigned long long* nbStats);
+
+// ---- boundary.hip ---------------------------------------------------------------------------------------------
+void launch_map_build(hipStream_t st, const ShapeDev& shape, const MapDev& map, float* nodes0, float* nodes1,
+                      const KTabHost& kt, const float* glx, const float* glw);
+void launch_boundary(const Launch& L, const MapDev* maps, float4* pos, float4* vel, float4* xb, float* vb);
+
+// ---- dfsph.hip ------------------------------------------------------------------------------------------------
+struct Fields {
+    float4* pos;      // working positions (sorted)
+    float4* posNext;  // end-of-step positions
+    float4* vel;
+    float4* acc;
+    float4* ap;       // pressure acceleration
+    float4* dv;       // VelocityDifference
+    float* dk;        // DeltaFinalCurvature
+    const uint32_t* id;
+    float* density;
+    float* factor;
+    float* densAdv;
+    float* kappa;
+    float* kappaV;
+    float* residuum;
+    float4* nMC;
+    float4* nS;
+    float* kMC;
+    float* kS;
+    float4* xb;       // [kMaxBodies][capacity]
+    float* vb;        // [kMaxBodies][capacity]
+    const uint32_t* nbrCount;
+    const uint32_t* nbrList;
+    // viscosity solver
+    float* minv;      // [9][capacity]
+    float4* cgX;      // solution (ViscosityGradientG)
+    float4* cgR;
+    float4* cgP;
+    float4* cgZ;
+    float4* cgAp;
+    // reductions
+    double* partials;      // [tiles][kPartialSlots]
+    float* partialsF;      // [tiles]
+    uint32_t* counters;    // [sceneCount]
+    SceneState* state;     // [sceneCount]
+    const float* halton;   // [49152]
+};
+
+void launch_density_factor(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_scene_init_div(const Launch& L, const Fields& F);
+void launch_div_init(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_div_accel(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_div_iter(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_div_final(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_surface_tension(const Launch& L, const Fields& F);
+void launch_visc_precond(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_cg_begin(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_cg_matvec(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_cg_update(const Launch& L, const Fields& F);
+void launch_cg_direction(const Launch& L, const Fields& F);
+void launch_visc_apply(const Launch& L, const Fields& F);
+void launch_time_step(const Launch& L, const Fields& F);
+void launch_velocity(const Launch& L, const Fields& F);
+void launch_scene_init_pres(const Launch& L, const Fields& F);
+void launch_pres_init(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_pres_accel(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_pres_iter(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_pres_final(const Launch& L, const KTabHost& kt, const Fields& F, int writeNext);
+void launch_scene_state_init(const Launch& L, SceneState* state, const float* dt0);
+
+}  // namespace vfd

@@ -1,0 +1,310 @@
+// search.hip - compact-grid neighbour search with a Morton-ordered structure-of-arrays reorder.
+//
+// Reference: ParticleSearch::FindNeighbors (VFD/Source/Simulation/DFSPH/ParticleSearch/ParticleSearch.cu:29-195,
+// ParticleSearchKernels.cu:4-193). Same grid (cell = support radius, origin two cells below the particles' minimum
+// cell, 8^3 Morton meta blocks), same strict test 0 < d^2 < R^2, same cap of 70 and the same traversal order (27 cells
+// z,y,x). Different from the reference by design:
+//   - particles are physically reordered by (cell, original id) every step, so every later neighbour loop reads
+//     contiguous cells; the reference keeps particles in input order and only sorts index arrays;
+//   - the order inside a cell is ascending original id (deterministic) instead of atomicAdd order;
+//   - lists hold slot indices in a transposed [j][i] layout with a fixed capacity of 70 per particle.
+#include "device.h"
+#include "kernels.h"
+
+namespace vfd {
+
+__device__ __forceinline__ unsigned spread3(unsigned x) {  // low 10 bits of x to every third bit
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+// cell (cx,cy,cz) -> meta-block linear index * 512 + Morton code inside the 8^3 block (ParticleSearchKernels.cu:24-38)
+__device__ __forceinline__ uint32_t cell_key(const SceneGrid& g, uint32_t cx, uint32_t cy, uint32_t cz) {
+    const uint32_t mx = cx >> 3, my = cy >> 3, mz = cz >> 3;
+    const uint32_t lin = (mz * g.meta[1] + my) * g.meta[0] + mx;
+    const uint32_t m = (spread3(cz & 7u) << 2) | (spread3(cy & 7u) << 1) | spread3(cx & 7u);
+    return g.cellBase + lin * kMetaBlock + m;
+}
+
+__global__ void k_minmax_init(int* mm, uint32_t sceneCount) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= sceneCount) return;
+    for (int a = 0; a < 3; ++a) {
+        mm[6 * s + a] = INT_MAX;
+        mm[6 * s + 3 + a] = INT_MIN;
+    }
+}
+
+// ComputeMinMaxKernel (ParticleSearchKernels.cu:40-62): integer cell bounds floor(x / R), one atomic per wave.
+__global__ void __launch_bounds__(kTile) k_minmax(const float4* __restrict__ pos, const SceneParams* __restrict__ P,
+                                                  const uint32_t* __restrict__ tileScene, int* mm) {
+    const TileCtx c = tile_ctx(tileScene, P);
+    const float R = P[c.s].R;
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    if (c.valid) {
+        const float4 x = pos[c.i];
+        lo[0] = hi[0] = (int)floorf(x.x / R);
+        lo[1] = hi[1] = (int)floorf(x.y / R);
+        lo[2] = hi[2] = (int)floorf(x.z / R);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min_i(lo[a]);
+        hi[a] = wave_max_i(hi[a]);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (lo[a] != INT_MAX) atomicMin(&mm[6 * c.s + a], lo[a]);
+            if (hi[a] != INT_MIN) atomicMax(&mm[6 * c.s + 3 + a], hi[a]);
+        }
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ uint32_t clampu(int v, uint32_t hi) { return v < 0 ? 0u : ((uint32_t)v >= hi ? hi - 1u : (uint32_t)v); }
+
+// InsertParticlesMortonKernel (ParticleSearchKernels.cu:64-78): cell key and arbitrary rank inside the cell.
+// Coordinates are clamped into the grid so that a non-finite position cannot index outside the cell arrays.
+__global__ void __launch_bounds__(kTile) k_insert(const float4* __restrict__ pos, const SceneParams* __restrict__ P,
+                                                  const SceneGrid* __restrict__ G,
+                                                  const uint32_t* __restrict__ tileScene, uint32_t* cellOf,
+                                                  uint32_t* rank, uint32_t* cellCount) {
+    const TileCtx c = tile_ctx(tileScene, P);
+    if (!c.valid) return;
+    const SceneGrid& g = G[c.s];
+    const float4 x = pos[c.i];
+    const float fx = (x.x - g.gridMin[0]) * g.gridDelta[0];
+    const float fy = (x.y - g.gridMin[1]) * g.gridDelta[1];
+    const float fz = (x.z - g.gridMin[2]) * g.gridDelta[2];
+    const uint32_t key = cell_key(g, clampu((int)fx, g.dims[0]), clampu((int)fy, g.dims[1]), clampu((int)fz, g.dims[2]));
+    cellOf[c.i] = key;
+    rank[c.i] = atomicAdd(&cellCount[key], 1u);
+}
+
+// Places every particle's original id at its provisional slot (CountingSortIndicesKernel, :80-91).
+__global__ void __launch_bounds__(kTile) k_scatter_ids(const SceneParams* __restrict__ P,
+                                                       const SceneGrid* __restrict__ G,
+                                                       const uint32_t* __restrict__ tileScene,
+                                                       const uint32_t* __restrict__ cellOf,
+                                                       const uint32_t* __restrict__ rank,
+                                                       const uint32_t* __restrict__ cellStart,
+                                                       const uint32_t* __restrict__ idOld, uint32_t* tmpId) {
+    const TileCtx c = tile_ctx(tileScene, P);
+    if (!c.valid) return;
+    const uint32_t key = cellOf[c.i];
+    tmpId[(uint32_t)((int32_t)cellStart[key] + G[c.s].startCorr) + rank[c.i]] = idOld[c.i];
+}
+
+// Final slot = cell start + number of cell members with a smaller original id. Moves the persistent state there.
+__global__ void __launch_bounds__(kTile) k_reorder(const SceneParams* __restrict__ P, const SceneGrid* __restrict__ G,
+                                                   const uint32_t* __restrict__ tileScene,
+                                                   const uint32_t* __restrict__ cellOf,
+                                                   const uint32_t* __restrict__ cellStart,
+                                                   const uint32_t* __restrict__ cellCount,
+                                                   const uint32_t* __restrict__ tmpId, const float4* __restrict__ posIn,
+                                                   const float4* __restrict__ velIn, const float4* __restrict__ dvIn,
+                                                   const float* __restrict__ dkIn, const uint32_t* __restrict__ idIn,
+                                                   float4* posOut, float4* velOut, float4* dvOut, float* dkOut,
+                                                   uint32_t* idOut) {
+    const TileCtx c = tile_ctx(tileScene, P);
+    if (!c.valid) return;
+    const uint32_t key = cellOf[c.i];
+    const uint32_t start = (uint32_t)((int32_t)cellStart[key] + G[c.s].startCorr);
+    const uint32_t n = cellCount[key];
+    const uint32_t me = idIn[c.i];
+    uint32_t r = 0;
+    for (uint32_t t = 0; t < n; ++t) r += tmpId[start + t] < me ? 1u : 0u;
+    const uint32_t dst = start + r;
+    posOut[dst] = posIn[c.i];
+    velOut[dst] = velIn[c.i];
+    dvOut[dst] = dvIn[c.i];
+    dkOut[dst] = dkIn[c.i];
+    idOut[dst] = me;
+}
+
+__device__ __forceinline__ uint32_t neighbor_list(const TileCtx& c, const SceneParams* __restrict__ P,
+                                                  const SceneGrid* __restrict__ G, const float4* __restrict__ pos,
+                                                  const uint32_t* __restrict__ cellStart,
+                                                  const uint32_t* __restrict__ cellCount, uint32_t* nbrList,
+                                                  uint32_t capacity) {
+    const SceneGrid& g = G[c.s];
+    const float R2 = P[c.s].R * P[c.s].R;
+    const float4 xi = pos[c.i];
+    const int gx = (int)floorf((xi.x - g.gridMin[0]) * g.gridDelta[0]);
+    const int gy = (int)floorf((xi.y - g.gridMin[1]) * g.gridDelta[1]);
+    const int gz = (int)floorf((xi.z - g.gridMin[2]) * g.gridDelta[2]);
+    uint32_t cnt = 0;
+    for (int dz = -1; dz < 2; ++dz) {
+        const int cz = gz + dz;
+        if (cz < 0 || cz >= (int)g.dims[2]) continue;
+        for (int dy = -1; dy < 2; ++dy) {
+            const int cy = gy + dy;
+            if (cy < 0 || cy >= (int)g.dims[1]) continue;
+            for (int dx = -1; dx < 2; ++dx) {
+                const int cx = gx + dx;
+                if (cx < 0 || cx >= (int)g.dims[0]) continue;
+                const uint32_t key = cell_key(g, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+                const uint32_t n = cellCount[key];
+                const uint32_t start = (uint32_t)((int32_t)cellStart[key] + g.startCorr);
+                for (uint32_t t = start; t < start + n; ++t) {
+                    const float4 xj = pos[t];
+                    const float ddx = xj.x - xi.x, ddy = xj.y - xi.y, ddz = xj.z - xi.z;
+                    const float d2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+                    if (d2 < R2 && d2 > 0.0f) {
+                        nbrList[(size_t)cnt * capacity + c.i] = t;
+                        if (++cnt == kMaxNeighbors) goto done;
+                    }
+                }
+            }
+        }
+    }
+done:
+    return cnt;
+}
+
+// ComputeCountsKernel + NeighborhoodQueryWithCountsKernel (ParticleSearchKernels.cu:93-193) in one pass: lists are
+// written to a fixed capacity of 70 per particle, transposed.
+__global__ void __launch_bounds__(kTile) k_build_lists(const float4* __restrict__ pos, const SceneParams* __restrict__ P,
+                                                       const SceneGrid* __restrict__ G,
+                                                       const uint32_t* __restrict__ tileScene,
+                                                       const uint32_t* __restrict__ cellStart,
+                                                       const uint32_t* __restrict__ cellCount, uint32_t* nbrCount,
+                                                       uint32_t* nbrList, uint32_t capacity,
+                                                       unsigned long long* nbStats) {
+    const TileCtx c = tile_ctx(tileScene, P);
+    uint32_t cnt = 0;
+    if (c.valid) cnt = neighbor_list(c, P, G, pos, cellStart, cellCount, nbrList, capacity);
+    if (c.valid) nbrCount[c.i] = cnt;
+    // per-scene total and maximum neighbour count (debug statistics)
+    int tot = (int)cnt, mx = (int)cnt;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        tot += __shfl_xor(tot, o, kWave);
+        mx = max(mx, __shfl_xor(mx, o, kWave));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && tot > 0) {
+        atomicAdd(&nbStats[2 * c.s], (unsigned long long)tot);
+        atomicMax(&nbStats[2 * c.s + 1], (unsigned long long)mx);
+    }
+}
+
+
+// ---- exclusive scan of uint32 (cell counts) --------------------------------------------------------------------
+constexpr unsigned kScanThreads = 256, kScanItems = 8, kScanChunk = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t wsum[kScanThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += t;
+    }
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int j = 0; j < (int)(kScanThreads / kWave); ++j) {
+        if (j < w) before += wsum[j];
+        tot += wsum[j];
+    }
+    __syncthreads();
+    total = tot;
+    return before + incl - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_chunks(const uint32_t* __restrict__ in, uint32_t* out,
+                                                              uint32_t* chunkSums, uint32_t n) {
+    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t local = 0;
+#pragma unroll
+    for (unsigned k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        local += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(local, total);
+#pragma unroll
+    for (unsigned k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) chunkSums[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of the chunk sums in place (any count, sequential over blocks of kScanThreads)
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(uint32_t* sums, uint32_t n) {
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < n; b += kScanThreads) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < n ? sums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, total);
+        if (i < n) sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t* out, const uint32_t* __restrict__ chunkSums,
+                                                           uint32_t n) {
+    const uint32_t add = chunkSums[blockIdx.x];
+    const uint32_t base = blockIdx.x * kScanChunk;
+    for (unsigned k = threadIdx.x; k < kScanChunk; k += kScanThreads)
+        if (base + k < n) out[base + k] += add;
+}
+
+uint32_t scan_scratch_words(uint32_t n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
+
+void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch, hipStream_t st) {
+    if (n == 0) return;
+    const uint32_t chunks = (n + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_scan_chunks, dim3(chunks), dim3(kScanThreads), 0, st, in, out, scratch, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, scratch, chunks);
+    hipLaunchKernelGGL(k_scan_add, dim3(chunks), dim3(kScanThreads), 0, st, out, (const uint32_t*)scratch, n);
+}
+
+// ---- host launchers ---------------------------------------------------------------------------------------------
+void launch_minmax(const Launch& L, const float4* pos, int* mm) {
+    hipLaunchKernelGGL(k_minmax_init, dim3((L.sceneCount + 63) / 64), dim3(64), 0, L.stream, mm, L.sceneCount);
+    hipLaunchKernelGGL(k_minmax, dim3(L.tiles), dim3(kTile), 0, L.stream, pos, L.P, L.tileScene, mm);
+}
+
+void launch_sort(const Launch& L, const SceneGrid* G, uint32_t totalCells, const float4* posIn, const float4* velIn,
+                 const float4* dvIn, const float* dkIn, const uint32_t* idIn, float4* posOut, float4* velOut,
+                 float4* dvOut, float* dkOut, uint32_t* idOut, uint32_t* cellOf, uint32_t* rank, uint32_t* tmpId,
+                 uint32_t* cellCount, uint32_t* cellStart, uint32_t* scanScratch) {
+    hipLaunchKernelGGL(k_fill_u32, dim3((totalCells + 255) / 256), dim3(256), 0, L.stream, cellCount, 0u, totalCells);
+    hipLaunchKernelGGL(k_insert, dim3(L.tiles), dim3(kTile), 0, L.stream, posIn, L.P, G, L.tileScene, cellOf, rank,
+                       cellCount);
+    exclusive_scan(cellCount, cellStart, totalCells, scanScratch, L.stream);
+    hipLaunchKernelGGL(k_scatter_ids, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, G, L.tileScene,
+                       (const uint32_t*)cellOf, (const uint32_t*)rank, (const uint32_t*)cellStart, idIn, tmpId);
+    hipLaunchKernelGGL(k_reorder, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, G, L.tileScene,
+                       (const uint32_t*)cellOf, (const uint32_t*)cellStart, (const uint32_t*)cellCount,
+                       (const uint32_t*)tmpId, posIn, velIn, dvIn, dkIn, idIn, posOut, velOut, dvOut, dkOut, idOut);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, const uint32_t* cellStart,
+                        const uint32_t* cellCount, uint32_t* nbrCount, uint32_t* nbrList,
+                        unsigned long long* nbStats) {
+    hipLaunchKernelGGL(k_fill_u64, dim3((2 * L.sceneCount + 63) / 64), dim3(64), 0, L.stream, nbStats,
+                       2 * L.sceneCount);
+    hipLaunchKernelGGL(k_build_lists, dim3(L.tiles), dim3(kTile), 0, L.stream, pos, L.P, G, L.tileScene, cellStart,
+                       cellCount, nbrCount, nbrList, L.capacity, nbStats);
+}
+
+}  // namespace vfd

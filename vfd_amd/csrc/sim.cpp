@@ -1,0 +1,1408 @@
+// sim.cpp - host driver of the MI355X DFSPH step and the C ABI declared in include/vfd.h.
+//
+// Replaces DFSPHImplementation (VFD/Source/Simulation/DFSPH/DFSPHImplementation.h:23-137, .cu:14-839) and the
+// DFSPHSimulation wrapper (DFSPHSimulator.cpp). The reference launches ~25 + 3*it_J + 13*it_CG kernels per step, each
+// followed by cudaDeviceSynchronize and scalar device-to-host copies. Here a step is a fixed sequence of launches on one
+// stream; the only host synchronisations are the grid-bounds read of the neighbour search, one poll of the solver state
+// every few Jacobi/CG iterations, and the end-of-step state read.
+//
+// Compiled with -ffp-contract=off: the host float arithmetic that derives the kernel table, grid and constants is the
+// reference's expression order in IEEE single precision.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vfd.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace vfd {
+
+static thread_local std::string g_lastError;
+
+static int fail(int code, const std::string& msg) {
+    g_lastError = msg;
+    return code;
+}
+
+#define VFD_HIP(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(VFD_ERROR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define VFD_TRY(expr)            \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_ != VFD_OK) return rc_; \
+    } while (0)
+
+template <class T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        if (count <= n && p) return VFD_OK;
+        release();
+        if (count == 0) return VFD_OK;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(VFD_ERROR_OUT_OF_MEMORY, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                                     " bytes failed: " + hipGetErrorString(e));
+        }
+        n = count;
+        return VFD_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+// ---- tables (host) -------------------------------------------------------------------------------------------------
+// PrecomputedDFSPHCubicKernel::SetRadius (Kernel/DFSPHKernels.h:13-41) in IEEE float.
+struct KernelTable {
+    float R = 0, R2 = 0, invStep = 0, W0 = 0, K = 0, L = 0;
+    std::vector<float> W, G;
+    float calcW(float r) const {
+        float res = 0.0f;
+        const float q = r / R;
+        if (q <= 1.0f) {
+            if (q <= 0.5f) {
+                const float q2 = q * q;
+                const float q3 = q2 * q;
+                res = K * (6.0f * q3 - 6.0f * q2 + 1.0f);
+            } else {
+                const float q1 = 1.0f - q;
+                const float q3 = q1 * q1 * q1;
+                res = K * (2.0f * q3);
+            }
+        }
+        return res;
+    }
+    // CalculateGradientW({x, 0, 0}).x: |r| = sqrt(fma(0,0,fma(0,0,x*x)))
+    float calcGradWx(float x) const {
+        const float rl = std::sqrt(std::fma(0.0f, 0.0f, std::fma(0.0f, 0.0f, x * x)));
+        const float q = rl / R;
+        if (rl > 1.0e-5f && q <= 1.0f) {
+            const float gq = x * (1.0f / (rl * R));
+            if (q <= 0.5f) {
+                const float s = L * q * (3.0f * q - 2.0f);
+                return s * gq;
+            }
+            const float f = 1.0f - q;
+            const float s = L * (-f * f);
+            return s * gq;
+        }
+        return 0.0f;
+    }
+    void set_radius(float radius) {
+        const float pi = 3.14159265358979323846f;
+        R = radius;
+        R2 = R * R;
+        const float r3 = R * R * R;
+        K = 8.0f / (pi * r3);
+        L = 48.0f / (pi * r3);
+        W0 = calcW(std::sqrt(std::fma(0.0f, 0.0f, std::fma(0.0f, 0.0f, 0.0f * 0.0f))));
+        const float step = R / (float)(kTableResolution - 1);
+        invStep = 1.0f / step;
+        W.assign(kTableResolution, 0.0f);
+        G.assign(kTableResolution + 1, 0.0f);
+        for (unsigned i = 0; i < kTableResolution; ++i) {
+            const float x = step * (float)i;
+            W[i] = calcW(x);
+            G[i] = x > 1.0e-9f ? calcGradWx(x) / x : 0.0f;
+        }
+        G[kTableResolution] = 0.0f;
+    }
+    // averaged entries: the value GetW/GetGradientW return for table index p
+    std::vector<float2> averaged() const {
+        std::vector<float2> t(kTableEntries);
+        for (unsigned p = 0; p < kTableEntries; ++p) {
+            t[p].x = 0.5f * (W[p] + W[p + 1]);
+            t[p].y = 0.5f * (G[p] + G[p + 1]);
+        }
+        return t;
+    }
+};
+
+// 16,384 unit-sphere points of HaltonVec323.cuh from their closed form (radical inverses in bases 2 and 3),
+// rounded to the 15 decimals the reference table holds.
+static std::vector<float> make_halton() {
+    std::vector<float> h(kHaltonFloats);
+    auto radinv = [](unsigned base, unsigned i) {
+        const double inv = 1.0 / base;
+        double f = inv, r = 0.0;
+        while (i) {
+            r += f * (double)(i % base);
+            i /= base;
+            f *= inv;
+        }
+        return r;
+    };
+    const double twopi = 6.283185307179586476925286766559;
+    for (unsigned i = 0; i < kHaltonFloats / 3; ++i) {
+        const double z = 1.0 - 2.0 * radinv(3, i);
+        const double rxy = std::sqrt(std::max(0.0, 1.0 - z * z));
+        const double a = twopi * radinv(2, i);
+        h[3 * i + 0] = (float)(std::nearbyint(rxy * std::cos(a) * 1e15) / 1e15);
+        h[3 * i + 1] = (float)(std::nearbyint(rxy * std::sin(a) * 1e15) / 1e15);
+        h[3 * i + 2] = (float)(std::nearbyint(z * 1e15) / 1e15);
+    }
+    return h;
+}
+
+// 16-point Gauss-Legendre rule on [-1,1] (GaussQuadrature.cpp, p = 30), ascending, by Newton iteration on P_16.
+static void make_gauss_legendre(std::vector<float>& xs, std::vector<float>& ws) {
+    const int n = 16;
+    std::vector<double> x(n), w(n);
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < n; ++i) {
+        double z = std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = z;
+        for (int k = 2; k <= n; ++k) {
+            const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (z * p1 - p0) / (z * z - 1.0);
+        x[n - 1 - i] = z;
+        w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+    for (int i = 0; i < n / 2; ++i) {
+        const double xm = 0.5 * (x[n - 1 - i] - x[i]), wm = 0.5 * (w[i] + w[n - 1 - i]);
+        x[i] = -xm;
+        x[n - 1 - i] = xm;
+        w[i] = w[n - 1 - i] = wm;
+    }
+    xs.resize(n);
+    ws.resize(n);
+    for (int i = 0; i < n; ++i) {
+        xs[i] = (float)x[i];
+        ws[i] = (float)w[i];
+    }
+}
+
+// ---- scenes -------------------------------------------------------------------------------------------------------
+struct Frame {
+    std::vector<float> xva;  // original order, {x, v, a} per particle (DFSPHParticleSimple)
+    float maxVel = 0, dt = 0;
+};
+
+struct Body {
+    vfd_boundary_desc desc{};
+    MapDev map{};
+    DevBuf<float> n0, n1;
+};
+
+struct Scene {
+    vfd_desc desc{};
+    std::vector<float> x0, v0;  // baked particles (SetFluidObjects keeps the host copy, :189-224)
+    uint32_t count = 0, base = 0, tileBegin = 0, tileEnd = 0;
+    std::vector<Body*> bodies;
+    vfd_debug_info dbg{};
+    std::vector<Frame> frames;
+    SceneState st{};  // host copy
+    SceneParams prm{};
+    SceneGrid grid{};
+    ~Scene() {
+        for (Body* b : bodies) delete b;
+    }
+};
+
+// SetDescription (DFSPHImplementation.cu:318-368)
+static void derive_params(const vfd_desc& d, SceneParams& p) {
+    const float r = d.ParticleRadius;
+    p.r = r;
+    p.diam = 2.0f * r;
+    p.R = 4.0f * r;
+    p.R2 = p.R * p.R;
+    p.V = 0.8f * p.diam * p.diam * p.diam;
+    p.rho0 = 1000.0f;
+    p.m = p.V * p.rho0;
+    p.mInv = 1.0f / p.m;
+    p.mu = d.Viscosity * p.rho0;
+    p.muB = d.BoundaryViscosity * p.rho0;
+    p.td = d.TangentialDistanceFactor * p.R;
+    p.sigma = d.SurfaceTension;
+    p.slope = 74.688796680497925f;
+    p.constant = 12.0f;
+    p.smoothing = 0.5f;
+    p.NR = r * 0.8f;
+    p.mcFactor = std::asin(p.NR / r);
+    p.gx = d.Gravity[0];
+    p.gy = d.Gravity[1];
+    p.gz = d.Gravity[2];
+    p.divTol = d.MaxDivergenceSolverError;
+    p.presTol = d.MaxPressureSolverError;
+    p.viscTol = d.MaxViscositySolverError;
+    p.dtMin = d.MinTimeStepSize;
+    p.dtMax = d.MaxTimeStepSize;
+    p.frameLength = d.FrameLength;
+    p.minDiv = d.MinDivergenceSolverIterations;
+    p.maxDiv = d.MaxDivergenceSolverIterations;
+    p.minPres = d.MinPressureSolverIterations;
+    p.maxPres = d.MaxPressureSolverIterations;
+    p.minVisc = d.MinViscositySolverIterations;
+    p.maxVisc = d.MaxViscositySolverIterations;
+    p.stPasses = d.SurfaceTensionSmoothPassCount;
+    p.temporal = d.TemporalSmoothing ? 1u : 0u;
+    p.enableDiv = d.EnableDivergenceSolverError ? 1 : 0;
+    p.enableVisc = d.EnableViscositySolver ? 1 : 0;
+    p.enableST = d.EnableSurfaceTensionSolver ? 1 : 0;
+    p.csdFix = d.CSDFix;
+    p.csd = d.CSD;
+}
+
+static void set_time_step(SceneState& st, const vfd_desc& d, float dt) {
+    st.dt = dt;
+    st.dt2 = dt * dt;
+    st.dtInv = 1.0f / dt;
+    st.dt2Inv = 1.0f / st.dt2;
+    // SurfaceTensionSampleCount from dt (the reference sets it only in ComputeTimeStepSize, :413-420)
+    st.S = d.CSDFix > 0 ? (uint32_t)d.CSDFix : (uint32_t)(int32_t)((float)d.CSD * dt);
+}
+
+static bool valid_desc(const vfd_desc& d, std::string& why) {
+    if (!(d.ParticleRadius > 0.0f) || !std::isfinite(d.ParticleRadius)) {
+        why = "ParticleRadius must be positive and finite";
+        return false;
+    }
+    if (!(d.TimeStepSize > 0.0f) || !(d.MinTimeStepSize > 0.0f) || !(d.MaxTimeStepSize > 0.0f)) {
+        why = "time step sizes must be positive";
+        return false;
+    }
+    return true;
+}
+
+// ---- simulation ---------------------------------------------------------------------------------------------------
+class Simulation {
+public:
+    std::vector<Scene*> sc;
+    hipStream_t stream = nullptr;
+    bool capture = true;
+    bool committed = false;
+
+    KernelTable kt;
+    DevBuf<float2> dTab;
+    DevBuf<float> dHalton, dGlx, dGlw;
+
+    uint32_t tiles = 0, cap = 0;
+    int cur = 0;
+    DevBuf<SceneParams> dP;
+    DevBuf<uint32_t> dTileScene;
+    DevBuf<SceneState> dState;
+    DevBuf<SceneGrid> dGrid;
+    DevBuf<MapDev> dMaps;
+    DevBuf<int> dMM;
+    DevBuf<float4> pos, posNext, vel[2], dv[2], acc, ap, nMC, nS, xb, cgX, cgR, cgP, cgZ, cgAp;
+    DevBuf<float> dk[2], density, factor, densAdv, kappa, kappaV, residuum, kMC, kS, vb, minv;
+    DevBuf<uint32_t> id[2], nbrCount, nbrList, cellOf, rank, tmpId, cellCount, cellStart, scanScratch, counters;
+    DevBuf<double> partials;
+    DevBuf<float> partialsF;
+    DevBuf<unsigned long long> nbStats;
+    uint32_t cellCap = 0;
+    hipEvent_t ev[8] = {};
+    bool eventsMade = false;
+
+    // per-kernel timing: an event pair around each launch, resolved at the end of a step
+    struct KStat {
+        std::string name;
+        uint32_t launches = 0;
+        double ms = 0.0;
+    };
+    bool profiling = false;
+    std::vector<KStat> kstats;
+    std::vector<hipEvent_t> evPool;
+    std::vector<std::pair<size_t, size_t>> pending;  // (stat index, first event of the pair)
+    size_t evUsed = 0;
+
+    ~Simulation() {
+        if (eventsMade)
+            for (auto& e : ev) (void)hipEventDestroy(e);
+        for (auto& e : evPool) (void)hipEventDestroy(e);
+        for (Scene* s : sc) delete s;
+    }
+
+    size_t stat_index(const char* name) {
+        for (size_t i = 0; i < kstats.size(); ++i)
+            if (kstats[i].name == name) return i;
+        kstats.push_back(KStat{name, 0, 0.0});
+        return kstats.size() - 1;
+    }
+    // Runs `fn` (which enqueues launches on the stream) between two recorded events when profiling.
+    template <class Fn> int timed(const char* name, Fn fn) {
+        if (!profiling) {
+            fn();
+            return VFD_OK;
+        }
+        while (evPool.size() < evUsed + 2) {
+            hipEvent_t e;
+            VFD_HIP(hipEventCreate(&e));
+            evPool.push_back(e);
+        }
+        const size_t first = evUsed;
+        evUsed += 2;
+        VFD_HIP(hipEventRecord(evPool[first], stream));
+        fn();
+        VFD_HIP(hipEventRecord(evPool[first + 1], stream));
+        pending.emplace_back(stat_index(name), first);
+        return VFD_OK;
+    }
+    int resolve_profile() {
+        if (pending.empty()) return VFD_OK;
+        VFD_HIP(hipStreamSynchronize(stream));
+        for (const auto& pr : pending) {
+            float ms = 0.0f;
+            VFD_HIP(hipEventElapsedTime(&ms, evPool[pr.second], evPool[pr.second + 1]));
+            kstats[pr.first].launches++;
+            kstats[pr.first].ms += ms;
+        }
+        pending.clear();
+        evUsed = 0;
+        return VFD_OK;
+    }
+
+    int init(const vfd_desc* descs, uint32_t n) {
+        if (n == 0) return fail(VFD_ERROR_INVALID_ARGUMENT, "scene_count must be positive");
+        for (uint32_t i = 0; i < n; ++i) {
+            std::string why;
+            if (!valid_desc(descs[i], why)) return fail(VFD_ERROR_INVALID_ARGUMENT, why);
+            if (descs[i].ParticleRadius != descs[0].ParticleRadius)
+                return fail(VFD_ERROR_NOT_SUPPORTED, "all scenes of a batch must use the same ParticleRadius");
+        }
+        int dev = 0;
+        VFD_HIP(hipGetDevice(&dev));
+        for (uint32_t i = 0; i < n; ++i) {
+            Scene* s = new Scene();
+            s->desc = descs[i];
+            derive_params(descs[i], s->prm);
+            set_time_step(s->st, descs[i], descs[i].TimeStepSize);
+            sc.push_back(s);
+        }
+        VFD_TRY(build_tables(descs[0].ParticleRadius));
+        for (auto& e : ev) VFD_HIP(hipEventCreate(&e));
+        eventsMade = true;
+        return VFD_OK;
+    }
+
+    int build_tables(float r) {
+        kt.set_radius(4.0f * r);
+        const std::vector<float2> avg = kt.averaged();
+        VFD_TRY(dTab.alloc(avg.size()));
+        VFD_HIP(hipMemcpy(dTab.p, avg.data(), avg.size() * sizeof(float2), hipMemcpyHostToDevice));
+        if (!dHalton.p) {
+            const std::vector<float> h = make_halton();
+            VFD_TRY(dHalton.alloc(h.size()));
+            VFD_HIP(hipMemcpy(dHalton.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+            std::vector<float> gx, gw;
+            make_gauss_legendre(gx, gw);
+            VFD_TRY(dGlx.alloc(16));
+            VFD_TRY(dGlw.alloc(16));
+            VFD_HIP(hipMemcpy(dGlx.p, gx.data(), 16 * sizeof(float), hipMemcpyHostToDevice));
+            VFD_HIP(hipMemcpy(dGlw.p, gw.data(), 16 * sizeof(float), hipMemcpyHostToDevice));
+        }
+        return VFD_OK;
+    }
+
+    KTabHost ktab() const { return KTabHost{dTab.p, kt.R, kt.R2, kt.invStep, kt.W0}; }
+
+    int set_description(uint32_t s, const vfd_desc& d) {
+        std::string why;
+        if (!valid_desc(d, why)) return fail(VFD_ERROR_INVALID_ARGUMENT, why);
+        if (d.ParticleRadius != sc[s]->desc.ParticleRadius) {
+            if (sc.size() > 1) return fail(VFD_ERROR_NOT_SUPPORTED, "a batch cannot change ParticleRadius");
+            if (!sc[s]->bodies.empty())
+                return fail(VFD_ERROR_STATE, "ParticleRadius cannot change after boundaries are built");
+            VFD_TRY(build_tables(d.ParticleRadius));
+        }
+        sc[s]->desc = d;
+        const SceneParams old = sc[s]->prm;
+        derive_params(d, sc[s]->prm);
+        sc[s]->prm.base = old.base;
+        sc[s]->prm.count = old.count;
+        sc[s]->prm.tileBegin = old.tileBegin;
+        sc[s]->prm.tileEnd = old.tileEnd;
+        sc[s]->prm.bodyCount = old.bodyCount;
+        set_time_step(sc[s]->st, d, d.TimeStepSize);  // SetDescription resets the time step (:352-356)
+        if (committed) {
+            VFD_HIP(hipMemcpyAsync(dP.p + s, &sc[s]->prm, sizeof(SceneParams), hipMemcpyHostToDevice, stream));
+            VFD_TRY(push_state());
+        }
+        return VFD_OK;
+    }
+
+    int set_fluid(uint32_t s, const float* x, const float* v, uint32_t n) {
+        if (n && !x) return fail(VFD_ERROR_INVALID_ARGUMENT, "positions must not be NULL");
+        for (size_t i = 0; i < (size_t)3 * n; ++i)
+            if (!std::isfinite(x[i]) || (v && !std::isfinite(v[i])))
+                return fail(VFD_ERROR_INVALID_ARGUMENT, "non-finite particle position or velocity");
+        Scene& S = *sc[s];
+        S.x0.assign(x, x + (size_t)3 * n);
+        if (v)
+            S.v0.assign(v, v + (size_t)3 * n);
+        else
+            S.v0.assign((size_t)3 * n, 0.0f);
+        S.count = n;
+        S.frames.clear();
+        S.dbg = vfd_debug_info{};
+        committed = false;
+        return VFD_OK;
+    }
+
+    // RigidBody ctor (RigidBody.cu:10-73): volume map domain, resolution and both fields, built on the GPU.
+    int add_boundary(uint32_t s, const vfd_boundary_desc& b) {
+        Scene& S = *sc[s];
+        if (S.bodies.size() >= kMaxBodies)
+            return fail(VFD_ERROR_NOT_SUPPORTED, "at most " + std::to_string(kMaxBodies) + " boundary bodies per scene");
+        if (b.Shape < VFD_SHAPE_BOX || b.Shape > VFD_SHAPE_CYLINDER)
+            return fail(VFD_ERROR_INVALID_ARGUMENT, "unknown boundary shape");
+        if (b.Shape == VFD_SHAPE_PLANE && !b.HasDomain)
+            return fail(VFD_ERROR_INVALID_ARGUMENT, "a plane boundary needs an explicit map domain");
+        for (int a = 0; a < 3; ++a)
+            if (b.CollisionMapResolution[a] == 0)
+                return fail(VFD_ERROR_INVALID_ARGUMENT, "CollisionMapResolution must be positive");
+        const uint64_t cells = (uint64_t)b.CollisionMapResolution[0] * b.CollisionMapResolution[1] *
+                               b.CollisionMapResolution[2];
+        if (cells > (1u << 26)) return fail(VFD_ERROR_INVALID_ARGUMENT, "CollisionMapResolution too large");
+        const float r = S.desc.ParticleRadius, R = kt.R;
+        const float tol = b.Padding - r;
+        ShapeDev sh{};
+        sh.shape = b.Shape;
+        for (int a = 0; a < 3; ++a) {
+            sh.c[a] = b.Center[a];
+            sh.h[a] = b.HalfExtents[a];
+            sh.n[a] = b.Normal[a];
+        }
+        sh.radius = b.Radius;
+        sh.sign = b.Inverted ? -1.0f : 1.0f;
+        sh.tol = tol;
+        Body* body = new Body();
+        body->desc = b;
+        MapDev& m = body->map;
+        if (b.HasDomain) {
+            for (int a = 0; a < 3; ++a) {
+                m.domMin[a] = b.DomainMin[a];
+                m.domMax[a] = b.DomainMax[a];
+            }
+        } else {
+            float e[3];
+            if (b.Shape == VFD_SHAPE_SPHERE)
+                e[0] = e[1] = e[2] = b.Radius;
+            else if (b.Shape == VFD_SHAPE_CYLINDER) {
+                e[0] = e[2] = b.Radius;
+                e[1] = b.HalfExtents[1];
+            } else
+                for (int a = 0; a < 3; ++a) e[a] = b.HalfExtents[a];
+            const float grow = 8.0f * R + tol;  // RigidBody.cu:35-37
+            for (int a = 0; a < 3; ++a) {
+                m.domMin[a] = (b.Center[a] - e[a]) - grow;
+                m.domMax[a] = (b.Center[a] + e[a]) + grow;
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            m.res[a] = b.CollisionMapResolution[a];
+            m.cellSize[a] = (m.domMax[a] - m.domMin[a]) / (float)m.res[a];
+            m.cellSizeInv[a] = 1.0f / m.cellSize[a];
+        }
+        const uint32_t nx = m.res[0], ny = m.res[1], nz = m.res[2];
+        m.nodeCount = (nx + 1) * (ny + 1) * (nz + 1) +
+                      2 * (nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz);
+        int rc = body->n0.alloc(m.nodeCount);
+        if (rc == VFD_OK) rc = body->n1.alloc(m.nodeCount);
+        if (rc != VFD_OK) {
+            delete body;
+            return rc;
+        }
+        m.nodes0 = body->n0.p;
+        m.nodes1 = body->n1.p;
+        launch_map_build(stream, sh, m, body->n0.p, body->n1.p, ktab(), dGlx.p, dGlw.p);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            delete body;
+            return fail(VFD_ERROR_HIP, std::string("volume map build: ") + hipGetErrorString(e));
+        }
+        S.bodies.push_back(body);
+        S.prm.bodyCount = (uint32_t)S.bodies.size();
+        committed = false;
+        return VFD_OK;
+    }
+
+    int clear_boundaries(uint32_t s) {
+        for (Body* b : sc[s]->bodies) delete b;
+        sc[s]->bodies.clear();
+        sc[s]->prm.bodyCount = 0;
+        committed = false;
+        return VFD_OK;
+    }
+
+    // Allocates the packed particle arrays for the current scenes and uploads the baked particles.
+    int commit() {
+        tiles = 0;
+        for (Scene* s : sc) {
+            s->base = tiles * kTile;
+            s->tileBegin = tiles;
+            tiles += (s->count + kTile - 1) / kTile;
+            s->tileEnd = tiles;
+            s->prm.base = s->base;
+            s->prm.count = s->count;
+            s->prm.tileBegin = s->tileBegin;
+            s->prm.tileEnd = s->tileEnd;
+            s->prm.bodyCount = (uint32_t)s->bodies.size();
+        }
+        cap = tiles * kTile;
+        const uint32_t S = (uint32_t)sc.size();
+        std::vector<uint32_t> ts(tiles);
+        for (uint32_t k = 0; k < S; ++k)
+            for (uint32_t t = sc[k]->tileBegin; t < sc[k]->tileEnd; ++t) ts[t] = k;
+        const size_t c = std::max<size_t>(cap, 1);
+        VFD_TRY(dP.alloc(S));
+        VFD_TRY(dState.alloc(S));
+        VFD_TRY(dGrid.alloc(S));
+        VFD_TRY(dMaps.alloc((size_t)S * kMaxBodies));
+        VFD_TRY(dMM.alloc((size_t)S * 6));
+        VFD_TRY(counters.alloc(S));
+        VFD_TRY(dTileScene.alloc(std::max<uint32_t>(tiles, 1)));
+        DevBuf<float4>* f4[] = {&pos, &posNext, &vel[0], &vel[1], &dv[0], &dv[1], &acc, &ap, &nMC, &nS,
+                                &cgX, &cgR, &cgP, &cgZ, &cgAp};
+        for (auto* b : f4) VFD_TRY(b->alloc(c));
+        DevBuf<float>* f1[] = {&dk[0], &dk[1], &density, &factor, &densAdv, &kappa, &kappaV, &residuum, &kMC, &kS};
+        for (auto* b : f1) VFD_TRY(b->alloc(c));
+        DevBuf<uint32_t>* u1[] = {&id[0], &id[1], &nbrCount, &cellOf, &rank, &tmpId};
+        for (auto* b : u1) VFD_TRY(b->alloc(c));
+        VFD_TRY(xb.alloc(c * kMaxBodies));
+        VFD_TRY(vb.alloc(c * kMaxBodies));
+        VFD_TRY(minv.alloc(c * 9));
+        VFD_TRY(nbrList.alloc(c * kMaxNeighbors));
+        VFD_TRY(partials.alloc((size_t)std::max<uint32_t>(tiles, 1) * kPartialSlots));
+        VFD_TRY(partialsF.alloc(std::max<uint32_t>(tiles, 1)));
+        VFD_TRY(nbStats.alloc((size_t)2 * S));
+        if (tiles) VFD_HIP(hipMemcpy(dTileScene.p, ts.data(), tiles * sizeof(uint32_t), hipMemcpyHostToDevice));
+        std::vector<MapDev> maps((size_t)S * kMaxBodies);
+        for (uint32_t k = 0; k < S; ++k)
+            for (size_t b = 0; b < sc[k]->bodies.size(); ++b) maps[(size_t)k * kMaxBodies + b] = sc[k]->bodies[b]->map;
+        VFD_HIP(hipMemcpy(dMaps.p, maps.data(), maps.size() * sizeof(MapDev), hipMemcpyHostToDevice));
+        // every particle field starts at zero, as SetFluidObjects initialises DFSPHParticle (:198-222)
+        VFD_HIP(hipMemset(counters.p, 0, S * sizeof(uint32_t)));
+        VFD_HIP(hipMemset(xb.p, 0, c * kMaxBodies * sizeof(float4)));
+        VFD_HIP(hipMemset(vb.p, 0, c * kMaxBodies * sizeof(float)));
+        VFD_HIP(hipMemset(minv.p, 0, c * 9 * sizeof(float)));
+        VFD_HIP(hipMemset(nbrCount.p, 0, c * sizeof(uint32_t)));
+        for (auto* b : f4) VFD_HIP(hipMemset(b->p, 0, c * sizeof(float4)));
+        for (auto* b : f1) VFD_HIP(hipMemset(b->p, 0, c * sizeof(float)));
+        std::vector<SceneParams> prm(S);
+        for (uint32_t k = 0; k < S; ++k) prm[k] = sc[k]->prm;
+        VFD_HIP(hipMemcpy(dP.p, prm.data(), S * sizeof(SceneParams), hipMemcpyHostToDevice));
+        committed = true;
+        VFD_TRY(upload_baked());
+        return VFD_OK;
+    }
+
+    // Simulate() start (DFSPHImplementation.cu:36-45): baked particles, dv = dk = 0, dt = TimeStepSize.
+    int upload_baked() {
+        const size_t c = std::max<size_t>(cap, 1);
+        std::vector<float4> x(c, make_float4(0, 0, 0, 0)), v(c, make_float4(0, 0, 0, 0));
+        std::vector<uint32_t> ids(c, 0);
+        for (Scene* s : sc) {
+            for (uint32_t i = 0; i < s->count; ++i) {
+                x[s->base + i] = make_float4(s->x0[3 * i], s->x0[3 * i + 1], s->x0[3 * i + 2], 0.0f);
+                v[s->base + i] = make_float4(s->v0[3 * i], s->v0[3 * i + 1], s->v0[3 * i + 2], 0.0f);
+                ids[s->base + i] = i;
+            }
+            s->dbg = vfd_debug_info{};
+            s->frames.clear();
+        }
+        cur = 0;
+        VFD_HIP(hipMemcpy(posNext.p, x.data(), c * sizeof(float4), hipMemcpyHostToDevice));
+        VFD_HIP(hipMemcpy(pos.p, x.data(), c * sizeof(float4), hipMemcpyHostToDevice));
+        VFD_HIP(hipMemcpy(vel[0].p, v.data(), c * sizeof(float4), hipMemcpyHostToDevice));
+        VFD_HIP(hipMemcpy(id[0].p, ids.data(), c * sizeof(uint32_t), hipMemcpyHostToDevice));
+        VFD_HIP(hipMemset(dv[0].p, 0, c * sizeof(float4)));
+        VFD_HIP(hipMemset(dk[0].p, 0, c * sizeof(float)));
+        for (Scene* s : sc) set_time_step(s->st, s->desc, s->desc.TimeStepSize);
+        return push_state();
+    }
+
+    int push_state() {
+        std::vector<SceneState> st(sc.size());
+        for (size_t k = 0; k < sc.size(); ++k) st[k] = sc[k]->st;
+        VFD_HIP(hipMemcpyAsync(dState.p, st.data(), st.size() * sizeof(SceneState), hipMemcpyHostToDevice, stream));
+        VFD_HIP(hipStreamSynchronize(stream));
+        return VFD_OK;
+    }
+    int pull_state() {
+        std::vector<SceneState> st(sc.size());
+        VFD_HIP(hipMemcpyAsync(st.data(), dState.p, st.size() * sizeof(SceneState), hipMemcpyDeviceToHost, stream));
+        VFD_HIP(hipStreamSynchronize(stream));
+        for (size_t k = 0; k < sc.size(); ++k) sc[k]->st = st[k];
+        return VFD_OK;
+    }
+
+    Launch launch() const {
+        return Launch{stream, dP.p, dTileScene.p, tiles, (uint32_t)sc.size(), cap};
+    }
+    Fields fields() {
+        Fields F{};
+        F.pos = pos.p;
+        F.posNext = posNext.p;
+        F.vel = vel[cur].p;
+        F.acc = acc.p;
+        F.ap = ap.p;
+        F.dv = dv[cur].p;
+        F.dk = dk[cur].p;
+        F.id = id[cur].p;
+        F.density = density.p;
+        F.factor = factor.p;
+        F.densAdv = densAdv.p;
+        F.kappa = kappa.p;
+        F.kappaV = kappaV.p;
+        F.residuum = residuum.p;
+        F.nMC = nMC.p;
+        F.nS = nS.p;
+        F.kMC = kMC.p;
+        F.kS = kS.p;
+        F.xb = xb.p;
+        F.vb = vb.p;
+        F.nbrCount = nbrCount.p;
+        F.nbrList = nbrList.p;
+        F.minv = minv.p;
+        F.cgX = cgX.p;
+        F.cgR = cgR.p;
+        F.cgP = cgP.p;
+        F.cgZ = cgZ.p;
+        F.cgAp = cgAp.p;
+        F.partials = partials.p;
+        F.partialsF = partialsF.p;
+        F.counters = counters.p;
+        F.state = dState.p;
+        F.halton = dHalton.p;
+        return F;
+    }
+
+    // ParticleSearch::ComputeMinMax + ComputeCellInformation (ParticleSearch.cu:29-93) on the host, per scene.
+    int search() {
+        const Launch L = launch();
+        const uint32_t S = (uint32_t)sc.size();
+        VFD_TRY(timed("minmax", [&] { launch_minmax(L, posNext.p, dMM.p); }));
+        std::vector<int> mm((size_t)S * 6);
+        VFD_HIP(hipMemcpyAsync(mm.data(), dMM.p, mm.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+        VFD_HIP(hipStreamSynchronize(stream));
+        uint32_t cellTotal = 0, prefix = 0;
+        std::vector<SceneGrid> grids(S);
+        for (uint32_t k = 0; k < S; ++k) {
+            Scene& s = *sc[k];
+            SceneGrid& g = grids[k];
+            std::memset(&g, 0, sizeof(g));
+            g.cellBase = cellTotal;
+            g.startCorr = (int32_t)s.base - (int32_t)prefix;
+            prefix += s.count;
+            if (s.count == 0) continue;
+            const float R = s.prm.R;
+            const int* lo = &mm[6 * k];
+            const int* hi = &mm[6 * k + 3];
+            float bmin[3], bmax[3];
+            for (int a = 0; a < 3; ++a) {
+                bmin[a] = (float)lo[a] * R;
+                bmax[a] = (float)(hi[a] + 1) * R;
+            }
+            uint64_t cells = kMetaBlock;
+            for (int a = 0; a < 3; ++a) {
+                const float size = bmax[a] - bmin[a];
+                g.dims[a] = (uint32_t)std::ceil(size / R) + 4;
+                g.gridMin[a] = bmin[a] - R * 2.0f;
+                g.meta[a] = (uint32_t)std::ceil((float)g.dims[a] / (float)kMetaGroup);
+                const float gs = (float)g.dims[a] * R;
+                g.gridDelta[a] = (float)g.dims[a] / gs;
+                cells *= g.meta[a];
+            }
+            if (cells + cellTotal > 0xF0000000ull)
+                return fail(VFD_ERROR_NOT_SUPPORTED, "particle cloud too sparse for the neighbour grid");
+            g.cellCount = (uint32_t)cells;
+            cellTotal += g.cellCount;
+            s.grid = g;
+        }
+        if (cellTotal > cellCap) {
+            const uint32_t want = cellTotal + cellTotal / 4;
+            VFD_TRY(cellCount.alloc(want));
+            VFD_TRY(cellStart.alloc(want));
+            VFD_TRY(scanScratch.alloc(scan_scratch_words(want)));
+            cellCap = want;
+        }
+        VFD_HIP(hipMemcpyAsync(dGrid.p, grids.data(), S * sizeof(SceneGrid), hipMemcpyHostToDevice, stream));
+        const int alt = cur ^ 1;
+        VFD_TRY(timed("sort", [&] {
+            launch_sort(L, dGrid.p, cellTotal, posNext.p, vel[cur].p, dv[cur].p, dk[cur].p, id[cur].p, pos.p,
+                        vel[alt].p, dv[alt].p, dk[alt].p, id[alt].p, cellOf.p, rank.p, tmpId.p, cellCount.p,
+                        cellStart.p, scanScratch.p);
+        }));
+        cur = alt;
+        VFD_TRY(timed("build_lists", [&] {
+            launch_build_lists(L, dGrid.p, pos.p, cellStart.p, cellCount.p, nbrCount.p, nbrList.p, nbStats.p);
+        }));
+        VFD_HIP(hipGetLastError());
+        return VFD_OK;
+    }
+
+    template <class Fn>
+    int poll_loop(Fn enqueue, uint32_t first, int which) {
+        uint32_t chunk = std::max<uint32_t>(first, 1);
+        for (;;) {
+            for (uint32_t c = 0; c < chunk; ++c) enqueue();
+            VFD_HIP(hipGetLastError());
+            VFD_TRY(pull_state());
+            bool any = false;
+            for (Scene* s : sc) {
+                const uint32_t a = which == 0 ? s->st.div.active : which == 1 ? s->st.pres.active : s->st.cg.active;
+                any = any || (a && s->count);
+            }
+            if (!any) return VFD_OK;
+            chunk = 4;
+        }
+    }
+
+    // OnUpdate (DFSPHImplementation.cu:63-170), stopping after phase `last`.
+    int step(int last) {
+        if (!committed) VFD_TRY(commit());
+        if (tiles == 0) return VFD_OK;  // OnUpdate returns for N = 0 (:65-67)
+        const Launch L = launch();
+        const KTabHost K = ktab();
+        bool anyST = false, anyVisc = false;
+        uint32_t minDiv = 0, minPres = 0, minVisc = 1;
+        for (Scene* s : sc) {
+            anyST = anyST || (s->desc.EnableSurfaceTensionSolver && s->count);
+            anyVisc = anyVisc || (s->desc.EnableViscositySolver && s->count);
+            minDiv = std::max(minDiv, s->desc.MinDivergenceSolverIterations);
+            minPres = std::max(minPres, s->desc.MinPressureSolverIterations);
+        }
+        VFD_HIP(hipEventRecord(ev[0], stream));
+        VFD_TRY(search());
+        VFD_HIP(hipEventRecord(ev[1], stream));
+        if (last == VFD_PHASE_SEARCH) return finish_partial();
+        Fields F = fields();
+        VFD_TRY(timed("boundary", [&] { launch_boundary(L, dMaps.p, pos.p, vel[cur].p, xb.p, vb.p); }));
+        if (last == VFD_PHASE_BOUNDARY) return finish_partial();
+        VFD_TRY(timed("density_factor", [&] { launch_density_factor(L, K, F); }));
+        VFD_HIP(hipEventRecord(ev[2], stream));
+        if (last == VFD_PHASE_DENSITY || last == VFD_PHASE_FACTOR) return finish_partial();
+        // divergence solve (ComputeDivergence :506-575)
+        launch_scene_init_div(L, F);
+        VFD_TRY(timed("div_init", [&] { launch_div_init(L, K, F); }));
+        VFD_TRY(poll_loop(
+            [&] {
+                (void)timed("div_accel", [&] { launch_div_accel(L, K, F); });
+                (void)timed("div_iter", [&] { launch_div_iter(L, K, F); });
+            },
+            minDiv, 0));
+        VFD_TRY(timed("div_final", [&] { launch_div_final(L, K, F); }));  // also ClearAccelerationKernel
+        VFD_HIP(hipEventRecord(ev[3], stream));
+        if (last == VFD_PHASE_DIVERGENCE || last == VFD_PHASE_CLEAR_ACCELERATION) return finish_partial();
+        if (anyST) VFD_TRY(timed("surface_tension", [&] { launch_surface_tension(L, F); }));
+        VFD_HIP(hipEventRecord(ev[4], stream));
+        if (last == VFD_PHASE_SURFACE_TENSION) return finish_partial();
+        if (anyVisc) {
+            VFD_TRY(timed("visc_precond", [&] { launch_visc_precond(L, K, F); }));
+            VFD_TRY(timed("cg_begin", [&] { launch_cg_begin(L, K, F); }));
+            VFD_TRY(poll_loop(
+                [&] {
+                    (void)timed("cg_matvec", [&] { launch_cg_matvec(L, K, F); });
+                    (void)timed("cg_update", [&] { launch_cg_update(L, F); });
+                    (void)timed("cg_direction", [&] { launch_cg_direction(L, F); });
+                },
+                minVisc, 2));
+            VFD_TRY(timed("visc_apply", [&] { launch_visc_apply(L, F); }));
+        }
+        VFD_HIP(hipEventRecord(ev[5], stream));
+        if (last == VFD_PHASE_VISCOSITY) return finish_partial();
+        VFD_TRY(timed("time_step", [&] { launch_time_step(L, F); }));
+        if (last == VFD_PHASE_TIME_STEP) return finish_partial();
+        VFD_TRY(timed("velocity", [&] { launch_velocity(L, F); }));
+        if (last == VFD_PHASE_VELOCITY) return finish_partial();
+        launch_scene_init_pres(L, F);
+        VFD_TRY(timed("pres_init", [&] { launch_pres_init(L, K, F); }));
+        VFD_TRY(poll_loop(
+            [&] {
+                (void)timed("pres_accel", [&] { launch_pres_accel(L, K, F); });
+                (void)timed("pres_iter", [&] { launch_pres_iter(L, K, F); });
+            },
+            minPres, 1));
+        VFD_TRY(timed("pres_final", [&] { launch_pres_final(L, K, F, last >= VFD_PHASE_POSITION ? 1 : 0); }));
+        VFD_HIP(hipEventRecord(ev[6], stream));
+        if (last == VFD_PHASE_PRESSURE) return finish_partial();
+        VFD_HIP(hipGetLastError());
+        VFD_TRY(pull_state());
+        solver_debug();
+        VFD_TRY(neighbor_stats());
+        VFD_TRY(resolve_profile());
+        float t[6] = {0, 0, 0, 0, 0, 0};
+        VFD_HIP(hipEventElapsedTime(&t[0], ev[0], ev[1]));
+        VFD_HIP(hipEventElapsedTime(&t[1], ev[1], ev[2]));
+        VFD_HIP(hipEventElapsedTime(&t[2], ev[2], ev[3]));
+        VFD_HIP(hipEventElapsedTime(&t[3], ev[3], ev[4]));
+        VFD_HIP(hipEventElapsedTime(&t[4], ev[4], ev[5]));
+        VFD_HIP(hipEventElapsedTime(&t[5], ev[5], ev[6]));
+        for (Scene* s : sc) {
+            if (s->count == 0) continue;
+            vfd_debug_info& d = s->dbg;
+            d.NeighborhoodSearchTime = t[0];
+            d.BaseSolverTime = t[1];
+            d.DivergenceSolverTime = t[2];
+            d.SurfaceTensionSolverTime = t[3];
+            d.ViscositySolverTime = t[4];
+            d.PressureSolverTime = t[5];
+            d.FrameTime += s->st.dt;  // ComputeTimeStepSize :427
+        }
+        // frame capture (:148-167)
+        for (size_t k = 0; k < sc.size(); ++k) {
+            Scene* s = sc[k];
+            if (s->count == 0) continue;
+            vfd_debug_info& d = s->dbg;
+            if (d.FrameIndex < s->desc.FrameCount && d.FrameTime >= s->desc.FrameLength) {
+                if (capture) {
+                    Frame fr;
+                    fr.maxVel = s->st.maxVel;
+                    fr.dt = s->st.dt;
+                    VFD_TRY(frame_data((uint32_t)k, fr.xva));
+                    s->frames.push_back(std::move(fr));
+                }
+                d.FrameTime = 0.0f;
+                d.FrameIndex++;
+            }
+            d.IterationCount++;
+        }
+        return VFD_OK;
+    }
+
+    // after a partial step the working positions become the state the next step starts from
+    int finish_partial() {
+        VFD_HIP(hipMemcpyAsync(posNext.p, pos.p, (size_t)cap * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        VFD_HIP(hipGetLastError());
+        VFD_TRY(pull_state());
+        solver_debug();
+        VFD_TRY(neighbor_stats());
+        return resolve_profile();
+    }
+
+    int neighbor_stats() {
+        std::vector<unsigned long long> nb(2 * sc.size());
+        VFD_HIP(hipMemcpyAsync(nb.data(), nbStats.p, nb.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               stream));
+        VFD_HIP(hipStreamSynchronize(stream));
+        for (size_t k = 0; k < sc.size(); ++k) {
+            sc[k]->dbg.TotalNeighborCount = nb[2 * k];
+            sc[k]->dbg.MaxNeighborCount = (uint32_t)nb[2 * k + 1];
+        }
+        return VFD_OK;
+    }
+
+    // solver iteration counts and errors of the last (possibly partial) step, DFSPHDebugInfo semantics
+    void solver_debug() {
+        for (Scene* s : sc) {
+            vfd_debug_info& d = s->dbg;
+            d.DivergenceSolverIterationCount = s->desc.EnableDivergenceSolverError ? s->st.div.it : 0;
+            d.DivergenceSolverError = s->desc.EnableDivergenceSolverError ? s->st.div.err : 0.0f;
+            d.PressureSolverIterationCount = s->st.pres.it;
+            d.PressureSolverError = s->st.pres.err;
+            d.ViscositySolverIterationCount = s->desc.EnableViscositySolver ? s->st.cg.it : 0;
+            d.ViscositySolverError = s->desc.EnableViscositySolver ? s->st.cg.err : 0.0f;
+            d.TimeStepSize = s->st.dt;
+            d.MaxVelocityMagnitude = s->st.maxVel;
+        }
+    }
+
+    // ---- reads in original order --------------------------------------------------------------------------------
+    template <class T> int fetch(const T* dptr, uint32_t s, std::vector<T>& out) {
+        const Scene& S = *sc[s];
+        out.resize(S.count);
+        if (S.count)
+            VFD_HIP(hipMemcpyAsync(out.data(), dptr + S.base, S.count * sizeof(T), hipMemcpyDeviceToHost, stream));
+        VFD_HIP(hipStreamSynchronize(stream));
+        return VFD_OK;
+    }
+    int ids(uint32_t s, std::vector<uint32_t>& out) { return fetch(id[cur].p, s, out); }
+
+    int get_field(uint32_t s, int field, uint32_t body, float* out) {
+        if (!committed) VFD_TRY(commit());
+        const Scene& S = *sc[s];
+        std::vector<uint32_t> idv;
+        VFD_TRY(ids(s, idv));
+        auto put4 = [&](const float4* d, int comps) -> int {
+            std::vector<float4> v;
+            VFD_TRY(fetch(d, s, v));
+            for (uint32_t k = 0; k < S.count; ++k) {
+                const uint32_t i = idv[k];
+                out[comps * i] = v[k].x;
+                if (comps > 1) {
+                    out[comps * i + 1] = v[k].y;
+                    out[comps * i + 2] = v[k].z;
+                }
+            }
+            return VFD_OK;
+        };
+        auto put1 = [&](const float* d) -> int {
+            std::vector<float> v;
+            VFD_TRY(fetch(d, s, v));
+            for (uint32_t k = 0; k < S.count; ++k) out[idv[k]] = v[k];
+            return VFD_OK;
+        };
+        switch (field) {
+            case VFD_FIELD_POSITION: return put4(posNext.p, 3);
+            case VFD_FIELD_VELOCITY: return put4(vel[cur].p, 3);
+            case VFD_FIELD_ACCELERATION: return put4(acc.p, 3);
+            case VFD_FIELD_PRESSURE_ACCELERATION: return put4(ap.p, 3);
+            case VFD_FIELD_PRESSURE_RESIDUUM: return put1(residuum.p);
+            case VFD_FIELD_DENSITY: return put1(density.p);
+            case VFD_FIELD_DENSITY_ADVECTION: return put1(densAdv.p);
+            case VFD_FIELD_PRESSURE_RHO2: return put1(kappa.p);
+            case VFD_FIELD_PRESSURE_RHO2_V: return put1(kappaV.p);
+            case VFD_FIELD_FACTOR: return put1(factor.p);
+            case VFD_FIELD_VELOCITY_DIFFERENCE: return put4(dv[cur].p, 3);
+            case VFD_FIELD_MC_SURFACE_NORMAL: return put4(nMC.p, 3);
+            case VFD_FIELD_MC_SURFACE_NORMAL_SMOOTH: return put4(nS.p, 3);
+            case VFD_FIELD_MC_SURFACE_CURVATURE: return put1(kMC.p);
+            case VFD_FIELD_MC_SURFACE_CURVATURE_SMOOTH: return put1(kS.p);
+            case VFD_FIELD_DELTA_FINAL_CURVATURE: return put1(dk[cur].p);
+            case VFD_FIELD_BOUNDARY_XJ:
+                if (body >= kMaxBodies) return fail(VFD_ERROR_INVALID_ARGUMENT, "body out of range");
+                return put4(xb.p + (size_t)body * cap, 3);
+            case VFD_FIELD_BOUNDARY_VOLUME:
+                if (body >= kMaxBodies) return fail(VFD_ERROR_INVALID_ARGUMENT, "body out of range");
+                return put1(vb.p + (size_t)body * cap);
+            case VFD_FIELD_VISCOSITY_SOLUTION: return put4(cgX.p, 3);
+            case VFD_FIELD_VISCOSITY_PRECONDITIONER: {
+                for (int e = 0; e < 9; ++e) {
+                    std::vector<float> v;
+                    VFD_TRY(fetch(minv.p + (size_t)e * cap, s, v));
+                    for (uint32_t k = 0; k < S.count; ++k) out[9 * idv[k] + e] = v[k];
+                }
+                return VFD_OK;
+            }
+        }
+        return fail(VFD_ERROR_INVALID_ARGUMENT, "unknown field");
+    }
+
+    int set_field(uint32_t s, int field, const float* in) {
+        if (!committed) VFD_TRY(commit());
+        const Scene& S = *sc[s];
+        std::vector<uint32_t> idv;
+        VFD_TRY(ids(s, idv));
+        auto write4 = [&](float4* d) -> int {
+            std::vector<float4> v(S.count);
+            for (uint32_t k = 0; k < S.count; ++k) {
+                const uint32_t i = idv[k];
+                v[k] = make_float4(in[3 * i], in[3 * i + 1], in[3 * i + 2], 0.0f);
+            }
+            if (S.count) VFD_HIP(hipMemcpy(d + S.base, v.data(), S.count * sizeof(float4), hipMemcpyHostToDevice));
+            return VFD_OK;
+        };
+        switch (field) {
+            case VFD_FIELD_POSITION:
+                VFD_TRY(write4(posNext.p));
+                return write4(pos.p);
+            case VFD_FIELD_VELOCITY: return write4(vel[cur].p);
+            case VFD_FIELD_VELOCITY_DIFFERENCE: return write4(dv[cur].p);
+            case VFD_FIELD_DELTA_FINAL_CURVATURE: {
+                std::vector<float> v(S.count);
+                for (uint32_t k = 0; k < S.count; ++k) v[k] = in[idv[k]];
+                if (S.count)
+                    VFD_HIP(hipMemcpy(dk[cur].p + S.base, v.data(), S.count * sizeof(float), hipMemcpyHostToDevice));
+                return VFD_OK;
+            }
+        }
+        return fail(VFD_ERROR_INVALID_ARGUMENT, "field is not writable");
+    }
+
+    int frame_data(uint32_t s, std::vector<float>& xva) {
+        const Scene& S = *sc[s];
+        std::vector<uint32_t> idv;
+        VFD_TRY(ids(s, idv));
+        std::vector<float4> x, v, a;
+        VFD_TRY(fetch(posNext.p, s, x));
+        VFD_TRY(fetch(vel[cur].p, s, v));
+        VFD_TRY(fetch(acc.p, s, a));
+        xva.assign((size_t)9 * S.count, 0.0f);
+        for (uint32_t k = 0; k < S.count; ++k) {
+            float* o = &xva[(size_t)9 * idv[k]];
+            o[0] = x[k].x, o[1] = x[k].y, o[2] = x[k].z;
+            o[3] = v[k].x, o[4] = v[k].y, o[5] = v[k].z;
+            o[6] = a[k].x, o[7] = a[k].y, o[8] = a[k].z;
+        }
+        return VFD_OK;
+    }
+
+    int get_neighbors(uint32_t s, uint32_t* counts, uint32_t* offsets, uint32_t* out, uint64_t* total) {
+        if (!committed) VFD_TRY(commit());
+        const Scene& S = *sc[s];
+        std::vector<uint32_t> idv, cnt;
+        VFD_TRY(ids(s, idv));
+        VFD_TRY(fetch(nbrCount.p, s, cnt));
+        std::vector<uint32_t> cOrig(S.count), off(S.count);
+        for (uint32_t k = 0; k < S.count; ++k) cOrig[idv[k]] = cnt[k];
+        uint64_t run = 0;
+        for (uint32_t i = 0; i < S.count; ++i) {
+            off[i] = (uint32_t)run;
+            run += cOrig[i];
+        }
+        if (total) *total = run;
+        if (counts) std::memcpy(counts, cOrig.data(), S.count * sizeof(uint32_t));
+        if (offsets) std::memcpy(offsets, off.data(), S.count * sizeof(uint32_t));
+        if (!out) return VFD_OK;
+        std::vector<uint32_t> rows((size_t)kMaxNeighbors * S.count);
+        for (uint32_t j = 0; j < kMaxNeighbors; ++j)
+            if (S.count)
+                VFD_HIP(hipMemcpyAsync(&rows[(size_t)j * S.count], nbrList.p + (size_t)j * cap + S.base,
+                                       S.count * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        VFD_HIP(hipStreamSynchronize(stream));
+        for (uint32_t k = 0; k < S.count; ++k) {
+            const uint32_t i = idv[k];
+            for (uint32_t j = 0; j < cnt[k]; ++j) {
+                const uint32_t t = rows[(size_t)j * S.count + k];
+                if (t < S.base || t >= S.base + S.count) return fail(VFD_ERROR_STATE, "neighbour outside its scene");
+                out[off[i] + j] = idv[t - S.base];
+            }
+        }
+        return VFD_OK;
+    }
+};
+
+}  // namespace vfd
+
+using vfd::Simulation;
+using vfd::fail;
+
+struct vfd_sim {
+    Simulation impl;
+};
+
+#define CHECK_SIM(sim)                                                                 \
+    do {                                                                               \
+        if (!(sim)) return fail(VFD_ERROR_INVALID_ARGUMENT, "vfd_sim handle is NULL"); \
+    } while (0)
+#define CHECK_SCENE(sim, s)                                                                   \
+    do {                                                                                      \
+        CHECK_SIM(sim);                                                                       \
+        if ((s) >= (sim)->impl.sc.size()) return fail(VFD_ERROR_INVALID_ARGUMENT, "scene index out of range"); \
+    } while (0)
+
+static int guard(int rc) { return rc; }
+
+extern "C" {
+
+int vfd_abi_version(void) { return VFD_ABI_VERSION; }
+const char* vfd_last_error(void) { return vfd::g_lastError.c_str(); }
+
+int vfd_desc_default(vfd_desc* d) {
+    if (!d) return fail(VFD_ERROR_INVALID_ARGUMENT, "desc is NULL");
+    *d = vfd_desc{};
+    d->TimeStepSize = 0.001f;
+    d->MinTimeStepSize = 0.0001f;
+    d->MaxTimeStepSize = 0.005f;
+    d->FrameLength = 0.0016f;
+    d->FrameCount = 200u;
+    d->MinPressureSolverIterations = 0u;
+    d->MaxPressureSolverIterations = 100u;
+    d->MaxPressureSolverError = 10.0f;
+    d->EnableDivergenceSolverError = 1;
+    d->MinDivergenceSolverIterations = 0u;
+    d->MaxDivergenceSolverIterations = 100u;
+    d->MaxDivergenceSolverError = 10.0f;
+    d->EnableViscositySolver = 1;
+    d->MinViscositySolverIterations = 0u;
+    d->MaxViscositySolverIterations = 100u;
+    d->MaxViscositySolverError = 0.1f;
+    d->Viscosity = 10.0f;
+    d->BoundaryViscosity = 10.0f;
+    d->TangentialDistanceFactor = 0.5f;
+    d->EnableSurfaceTensionSolver = 1;
+    d->SurfaceTensionSmoothPassCount = 1u;
+    d->SurfaceTension = 1.0f;
+    d->TemporalSmoothing = 0;
+    d->CSDFix = -1;
+    d->CSD = 10000;
+    d->ParticleRadius = 0.025f;
+    d->Gravity[0] = 0.0f;
+    d->Gravity[1] = -9.81f;
+    d->Gravity[2] = 0.0f;
+    return VFD_OK;
+}
+
+int vfd_batch_create(const vfd_desc* descs, uint32_t n, vfd_sim** out) {
+    if (!descs || !out) return fail(VFD_ERROR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    vfd_sim* s = new vfd_sim();
+    const int rc = s->impl.init(descs, n);
+    if (rc != VFD_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return VFD_OK;
+}
+int vfd_create(const vfd_desc* desc, vfd_sim** out) { return vfd_batch_create(desc, 1, out); }
+int vfd_destroy(vfd_sim* sim) {
+    if (sim) {
+        if (sim->impl.stream) (void)hipStreamSynchronize(sim->impl.stream);
+        delete sim;
+    }
+    return VFD_OK;
+}
+int vfd_set_stream(vfd_sim* sim, void* stream) {
+    CHECK_SIM(sim);
+    sim->impl.stream = (hipStream_t)stream;
+    return VFD_OK;
+}
+int vfd_scene_count(const vfd_sim* sim, uint32_t* count) {
+    CHECK_SIM(sim);
+    if (!count) return fail(VFD_ERROR_INVALID_ARGUMENT, "count is NULL");
+    *count = (uint32_t)sim->impl.sc.size();
+    return VFD_OK;
+}
+int vfd_batch_set_description(vfd_sim* sim, uint32_t scene, const vfd_desc* desc) {
+    CHECK_SCENE(sim, scene);
+    if (!desc) return fail(VFD_ERROR_INVALID_ARGUMENT, "desc is NULL");
+    return guard(sim->impl.set_description(scene, *desc));
+}
+int vfd_set_description(vfd_sim* sim, const vfd_desc* desc) { return vfd_batch_set_description(sim, 0, desc); }
+int vfd_get_description(const vfd_sim* sim, vfd_desc* desc) {
+    CHECK_SIM(sim);
+    if (!desc) return fail(VFD_ERROR_INVALID_ARGUMENT, "desc is NULL");
+    *desc = sim->impl.sc[0]->desc;
+    return VFD_OK;
+}
+int vfd_batch_get_info(const vfd_sim* sim, uint32_t scene, vfd_info* info) {
+    CHECK_SCENE(sim, scene);
+    if (!info) return fail(VFD_ERROR_INVALID_ARGUMENT, "info is NULL");
+    const vfd::Scene& S = *sim->impl.sc[scene];
+    const vfd::SceneParams& p = S.prm;
+    vfd_info& o = *info;
+    o.ParticleCount = S.count;
+    o.RigidBodyCount = (uint32_t)S.bodies.size();
+    o.SupportRadius = p.R;
+    o.SupportRadius2 = p.R2;
+    o.ParticleRadius = p.r;
+    o.ParticleDiameter = p.diam;
+    o.TimeStepSize = S.st.dt;
+    o.TimeStepSize2 = S.st.dt2;
+    o.TimeStepSizeInverse = S.st.dtInv;
+    o.TimeStepSize2Inverse = S.st.dt2Inv;
+    o.Volume = p.V;
+    o.Density0 = p.rho0;
+    o.ParticleMass = p.m;
+    o.ParticleMassInverse = p.mInv;
+    o.Viscosity = S.desc.Viscosity;
+    o.BoundaryViscosity = S.desc.BoundaryViscosity;
+    o.DynamicViscosity = p.mu;
+    o.DynamicBoundaryViscosity = p.muB;
+    o.TangentialDistanceFactor = S.desc.TangentialDistanceFactor;
+    o.TangentialDistance = p.td;
+    o.SurfaceTension = p.sigma;
+    o.SurfaceTensionSampleCount = S.st.S;
+    o.ClassifierSlope = p.slope;
+    o.ClassifierConstant = p.constant;
+    o.TemporalSmoothing = (int32_t)p.temporal;
+    o.SmoothingFactor = p.smoothing;
+    o.Factor = 0.8f;
+    o.NeighborParticleRadius = p.NR;
+    o.MonteCarloFactor = p.mcFactor;
+    o.Gravity[0] = p.gx;
+    o.Gravity[1] = p.gy;
+    o.Gravity[2] = p.gz;
+    return VFD_OK;
+}
+int vfd_get_info(const vfd_sim* sim, vfd_info* info) { return vfd_batch_get_info(sim, 0, info); }
+
+int vfd_batch_set_fluid(vfd_sim* sim, uint32_t scene, const float* x, const float* v, uint32_t n) {
+    CHECK_SCENE(sim, scene);
+    return guard(sim->impl.set_fluid(scene, x, v, n));
+}
+int vfd_set_fluid(vfd_sim* sim, const float* x, const float* v, uint32_t n) {
+    return vfd_batch_set_fluid(sim, 0, x, v, n);
+}
+int vfd_batch_add_boundary(vfd_sim* sim, uint32_t scene, const vfd_boundary_desc* body) {
+    CHECK_SCENE(sim, scene);
+    if (!body) return fail(VFD_ERROR_INVALID_ARGUMENT, "body is NULL");
+    return guard(sim->impl.add_boundary(scene, *body));
+}
+int vfd_add_boundary(vfd_sim* sim, const vfd_boundary_desc* body) { return vfd_batch_add_boundary(sim, 0, body); }
+int vfd_clear_boundaries(vfd_sim* sim, uint32_t scene) {
+    CHECK_SCENE(sim, scene);
+    return sim->impl.clear_boundaries(scene);
+}
+int vfd_get_boundary_map(vfd_sim* sim, uint32_t scene, uint32_t body, uint32_t field, float* nodes,
+                         uint32_t* node_count) {
+    CHECK_SCENE(sim, scene);
+    const vfd::Scene& S = *sim->impl.sc[scene];
+    if (body >= S.bodies.size() || field > 1) return fail(VFD_ERROR_INVALID_ARGUMENT, "body or field out of range");
+    const vfd::Body& b = *S.bodies[body];
+    if (node_count) *node_count = b.map.nodeCount;
+    if (nodes) {
+        hipError_t e = hipMemcpy(nodes, field == 0 ? b.n0.p : b.n1.p, b.map.nodeCount * sizeof(float),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail(VFD_ERROR_HIP, hipGetErrorString(e));
+    }
+    return VFD_OK;
+}
+
+int vfd_step(vfd_sim* sim) {
+    CHECK_SIM(sim);
+    return guard(sim->impl.step(VFD_PHASE_POSITION));
+}
+int vfd_advance(vfd_sim* sim, uint32_t steps) {
+    CHECK_SIM(sim);
+    for (uint32_t i = 0; i < steps; ++i) {
+        const int rc = sim->impl.step(VFD_PHASE_POSITION);
+        if (rc != VFD_OK) return rc;
+    }
+    return VFD_OK;
+}
+int vfd_step_until(vfd_sim* sim, int32_t phase) {
+    CHECK_SIM(sim);
+    if (phase < 0 || phase >= VFD_PHASE_COUNT) return fail(VFD_ERROR_INVALID_ARGUMENT, "phase out of range");
+    return guard(sim->impl.step(phase));
+}
+int vfd_simulate(vfd_sim* sim) {
+    CHECK_SIM(sim);
+    Simulation& S = sim->impl;
+    if (!S.committed) {
+        const int rc = S.commit();
+        if (rc != VFD_OK) return rc;
+    } else {
+        const int rc = S.upload_baked();
+        if (rc != VFD_OK) return rc;
+    }
+    for (;;) {
+        bool done = true;
+        for (vfd::Scene* s : S.sc)
+            if (s->count && s->dbg.FrameIndex < s->desc.FrameCount) done = false;
+        if (done) return VFD_OK;
+        const int rc = S.step(VFD_PHASE_POSITION);
+        if (rc != VFD_OK) return rc;
+    }
+}
+int vfd_set_frame_capture(vfd_sim* sim, int32_t enabled) {
+    CHECK_SIM(sim);
+    sim->impl.capture = enabled != 0;
+    return VFD_OK;
+}
+
+int vfd_batch_get_particles(vfd_sim* sim, uint32_t scene, float* x, float* v, float* a) {
+    CHECK_SCENE(sim, scene);
+    if (x) {
+        const int rc = sim->impl.get_field(scene, VFD_FIELD_POSITION, 0, x);
+        if (rc != VFD_OK) return rc;
+    }
+    if (v) {
+        const int rc = sim->impl.get_field(scene, VFD_FIELD_VELOCITY, 0, v);
+        if (rc != VFD_OK) return rc;
+    }
+    if (a) {
+        const int rc = sim->impl.get_field(scene, VFD_FIELD_ACCELERATION, 0, a);
+        if (rc != VFD_OK) return rc;
+    }
+    return VFD_OK;
+}
+int vfd_get_particles(vfd_sim* sim, float* x, float* v, float* a) { return vfd_batch_get_particles(sim, 0, x, v, a); }
+int vfd_get_field(vfd_sim* sim, uint32_t scene, int32_t field, uint32_t body, float* out) {
+    CHECK_SCENE(sim, scene);
+    if (!out) return fail(VFD_ERROR_INVALID_ARGUMENT, "out is NULL");
+    return guard(sim->impl.get_field(scene, field, body, out));
+}
+int vfd_set_field(vfd_sim* sim, uint32_t scene, int32_t field, const float* values) {
+    CHECK_SCENE(sim, scene);
+    if (!values) return fail(VFD_ERROR_INVALID_ARGUMENT, "values is NULL");
+    return guard(sim->impl.set_field(scene, field, values));
+}
+int vfd_get_particle_count(const vfd_sim* sim, uint32_t scene, uint32_t* count) {
+    CHECK_SCENE(sim, scene);
+    if (!count) return fail(VFD_ERROR_INVALID_ARGUMENT, "count is NULL");
+    *count = sim->impl.sc[scene]->count;
+    return VFD_OK;
+}
+int vfd_get_neighbors(vfd_sim* sim, uint32_t scene, uint32_t* counts, uint32_t* offsets, uint32_t* ids,
+                      uint64_t* total) {
+    CHECK_SCENE(sim, scene);
+    return guard(sim->impl.get_neighbors(scene, counts, offsets, ids, total));
+}
+int vfd_get_debug_info(vfd_sim* sim, uint32_t scene, vfd_debug_info* info) {
+    CHECK_SCENE(sim, scene);
+    if (!info) return fail(VFD_ERROR_INVALID_ARGUMENT, "info is NULL");
+    *info = sim->impl.sc[scene]->dbg;
+    info->TimeStepSize = sim->impl.sc[scene]->st.dt;
+    info->MaxVelocityMagnitude = sim->impl.sc[scene]->st.maxVel;
+    return VFD_OK;
+}
+int vfd_get_frame_count(vfd_sim* sim, uint32_t scene, uint32_t* count) {
+    CHECK_SCENE(sim, scene);
+    if (!count) return fail(VFD_ERROR_INVALID_ARGUMENT, "count is NULL");
+    *count = (uint32_t)sim->impl.sc[scene]->frames.size();
+    return VFD_OK;
+}
+int vfd_get_frame(vfd_sim* sim, uint32_t scene, uint32_t frame, float* xva, float* max_velocity, float* time_step) {
+    CHECK_SCENE(sim, scene);
+    const auto& fr = sim->impl.sc[scene]->frames;
+    if (frame >= fr.size()) return fail(VFD_ERROR_INVALID_ARGUMENT, "frame index out of range");
+    if (xva) std::memcpy(xva, fr[frame].xva.data(), fr[frame].xva.size() * sizeof(float));
+    if (max_velocity) *max_velocity = fr[frame].maxVel;
+    if (time_step) *time_step = fr[frame].dt;
+    return VFD_OK;
+}
+int vfd_set_profiling(vfd_sim* sim, int32_t enabled) {
+    CHECK_SIM(sim);
+    sim->impl.profiling = enabled != 0;
+    sim->impl.kstats.clear();
+    sim->impl.pending.clear();
+    sim->impl.evUsed = 0;
+    return VFD_OK;
+}
+int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint32_t* count) {
+    CHECK_SIM(sim);
+    const auto& ks = sim->impl.kstats;
+    if (count) *count = (uint32_t)ks.size();
+    if (out)
+        for (uint32_t i = 0; i < capacity && i < ks.size(); ++i) {
+            std::memset(&out[i], 0, sizeof(vfd_kernel_stat));
+            std::strncpy(out[i].name, ks[i].name.c_str(), sizeof(out[i].name) - 1);
+            out[i].launches = ks[i].launches;
+            out[i].total_ms = ks[i].ms;
+        }
+    return VFD_OK;
+}
+int vfd_get_kernel_table(vfd_sim* sim, float* w, float* gradient_w, float* scalars) {
+    CHECK_SIM(sim);
+    const vfd::KernelTable& k = sim->impl.kt;
+    if (w) std::memcpy(w, k.W.data(), k.W.size() * sizeof(float));
+    if (gradient_w) std::memcpy(gradient_w, k.G.data(), k.G.size() * sizeof(float));
+    if (scalars) {
+        scalars[0] = k.R;
+        scalars[1] = k.R2;
+        scalars[2] = k.invStep;
+        scalars[3] = k.W0;
+        scalars[4] = k.K;
+        scalars[5] = k.L;
+    }
+    return VFD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+"""Synthetic scenes for the five benchmark configurations (seeded, reproducible).
+
+The reference samples fluid volumes from meshes (ParticleSampler::SampleMeshVolume, Utility/Sampler/
+ParticleSampler.cpp:7-91) and builds boundaries from meshes (RigidBody.cu:10-73). These scenes use cubic lattices
+and analytic boundary shapes instead; every choice the configuration text leaves open is fixed here:
+
+  - Jacobi solves run at least MinPressure/MinDivergenceSolverIterations = 2 (with the reference default 0 both
+    loops run zero iterations, DFSPHImplementation.cu:458-461, 529-532);
+  - "dynamic viscosity mu" maps to Viscosity = mu / rho0 (the code multiplies by rho0, :347) and
+    BoundaryViscosity = Viscosity;
+  - a CG tolerance of 1e-3 is MaxViscositySolverError = 0.1 (threshold tol^2 * 1e-4 * |b|^2, :661);
+  - fluid blocks start one support radius (4r) away from walls: the reference's single-point boundary term
+    V_b W(x - x_b) (DFSPHKernels.cu:176-184) gives a lattice layer at distance 2r from a wall about 2.5 rho0, which
+    the first pressure solve turns into velocities of metres per second.
+
+Each builder returns (descs, fluids, bodies): one vfd_desc per scene, one (positions, velocities) pair per scene and
+one list of RigidBodyDescription per scene.
+"""
+import math
+
+import numpy as np
+
+from .simulation import DFSPHSimulationDescription, RigidBodyDescription
+
+RHO0 = 1000.0
+
+
+def lattice_block(origin, counts, spacing, jitter, rng):
+    """Cubic lattice of counts[0] x counts[1] x counts[2] points; x fastest. Uniform jitter in [-jitter, jitter]."""
+    nx, ny, nz = counts
+    i = np.arange(nx, dtype=np.float64)
+    j = np.arange(ny, dtype=np.float64)
+    k = np.arange(nz, dtype=np.float64)
+    Z, Y, X = np.meshgrid(k, j, i, indexing="ij")
+    p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1) * spacing + np.asarray(origin, np.float64)
+    if jitter:
+        p = p + rng.uniform(-jitter, jitter, size=p.shape)
+    return p.astype(np.float32)
+
+
+def dam_break_8k(seed=1):
+    """Config 1: 20^3 = 8,000 particles, r = 0.025, spacing 2r, jitter +-0.01r, in the corner of a 2 m box
+    ([-1, 1]^3, the span of the reference's Cube.obj) given as an inverted-box volume map, the block one support radius (4r) off the walls; no viscosity, no surface
+    tension; divergence tolerance 0.1 %, density tolerance 0.01 %; CFL 0.4, dt <= 5 ms."""
+    rng = np.random.default_rng(seed)
+    r = 0.025
+    desc = DFSPHSimulationDescription(
+        ParticleRadius=r, TimeStepSize=0.001, MaxTimeStepSize=0.005, MinTimeStepSize=0.0001,
+        EnableViscositySolver=0, EnableSurfaceTensionSolver=0, Viscosity=0.0, BoundaryViscosity=0.0,
+        MaxDivergenceSolverError=0.1, MaxPressureSolverError=0.01,
+        MinDivergenceSolverIterations=2, MinPressureSolverIterations=2, FrameCount=200)
+    origin = (-1.0 + 4 * r,) * 3  # one support radius from the walls
+    x = lattice_block(origin, (20, 20, 20), 2 * r, 0.01 * r, rng)
+    v = np.zeros_like(x)
+    box = RigidBodyDescription("box", center=(0, 0, 0), half_extents=(1, 1, 1), inverted=True, resolution=(20, 20, 20))
+    return [desc], [(x, v)], [[box]]
+
+
+def viscous_coil_250k(seed=2, n=250_000):
+    """Config 2: a 250k-particle column (radius 0.1 m, r = 0.005) whose bottom starts 0.5 m above a ground plane,
+    falling at 1 m/s plus seeded jitter; mu = 50 (nu = 0.05), CG tolerance 1e-3, surface tension 0.05; ground plane
+    and walls (an inverted box) as volume maps."""
+    rng = np.random.default_rng(seed)
+    r = 0.005
+    d = 2 * r
+    rc = 0.1
+    m = int(math.ceil(rc / d)) + 1
+    g = np.arange(-m, m + 1) * d
+    gx, gz = np.meshgrid(g, g, indexing="ij")
+    disc = np.stack([gx.ravel(), gz.ravel()], axis=1)
+    disc = disc[(disc ** 2).sum(1) <= (rc - r) ** 2]
+    layers = int(math.ceil(n / len(disc)))
+    ys = 0.5 + r + np.arange(layers) * d
+    pts = np.concatenate([np.stack([disc[:, 0], np.full(len(disc), y), disc[:, 1]], 1) for y in ys])[:n]
+    pts = pts + rng.uniform(-0.01 * r, 0.01 * r, size=pts.shape)
+    x = pts.astype(np.float32)
+    v = np.zeros_like(x)
+    v[:, 1] = -1.0
+    v += rng.uniform(-0.01, 0.01, size=v.shape).astype(np.float32)
+    top = float(x[:, 1].max()) + 0.5
+    desc = DFSPHSimulationDescription(
+        ParticleRadius=r, Viscosity=50.0 / RHO0, BoundaryViscosity=50.0 / RHO0, MaxViscositySolverError=0.1,
+        EnableSurfaceTensionSolver=1, SurfaceTension=0.05, CSD=10000,
+        MinDivergenceSolverIterations=2, MinPressureSolverIterations=2,
+        MaxDivergenceSolverError=0.1, MaxPressureSolverError=0.01, FrameCount=500)
+    plane = RigidBodyDescription("plane", center=(0, 0, 0), normal=(0, 1, 0), resolution=(24, 6, 24),
+                                 domain=((-0.6, -0.12, -0.6), (0.6, 0.12, 0.6)))
+    walls = RigidBodyDescription("box", center=(0, top / 2 - 0.05, 0), half_extents=(0.5, top / 2 + 0.05, 0.5),
+                                 inverted=True, resolution=(16, max(16, int(top / 0.06)), 16))
+    return [desc], [(x, v.astype(np.float32))], [[plane, walls]]
+
+
+def dam_break_2m(seed=3):
+    """Config 3: 2,000,000 particles (200 x 100 x 100 cubic lattice, r = 0.004, spacing 2r, jitter +-0.01r) in a
+    2 x 1 x 1 m tank (inverted box) with a sphere obstacle of radius 0.15; viscosity 0.01. The block is 1.6 x 0.8 x
+    0.8 m (51 % of the tank; 40 % cannot hold 2M particles on a 2r cubic lattice)."""
+    rng = np.random.default_rng(seed)
+    r = 0.004
+    desc = DFSPHSimulationDescription(
+        ParticleRadius=r, Viscosity=0.01, BoundaryViscosity=0.01, MaxViscositySolverError=0.1,
+        EnableSurfaceTensionSolver=0, MaxDivergenceSolverError=0.1, MaxPressureSolverError=0.01,
+        MinDivergenceSolverIterations=2, MinPressureSolverIterations=2, FrameCount=100)
+    x = lattice_block((4 * r, 4 * r, 4 * r), (200, 100, 100), 2 * r, 0.01 * r, rng)
+    v = np.zeros_like(x)
+    tank = RigidBodyDescription("box", center=(1.0, 0.5, 0.5), half_extents=(1.0, 0.5, 0.5), inverted=True,
+                                resolution=(40, 24, 24))
+    # Padding 2r keeps particle centres r outside the sphere (with Padding 0 a solid admits them r deep,
+    # RigidBody.cu:29,39-42)
+    sphere = RigidBodyDescription("sphere", center=(1.8, 0.15, 0.5), radius=0.15, padding=2 * r,
+                                  resolution=(16, 16, 16))
+    return [desc], [(x, v)], [[tank, sphere]]
+
+
+def batched_uneven(seed=4, scenes=64, r=0.01):
+    """Config 4: 64 dam breaks with N_s = round(exp U(ln 5e3, ln 6e4)) and mu_s = exp U(ln 0.1, ln 100), one seed;
+    every scene has its own box."""
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(math.log(5e3), math.log(6e4), size=scenes))).astype(int)
+    mus = np.exp(rng.uniform(math.log(0.1), math.log(100.0), size=scenes))
+    descs, fluids, bodies = [], [], []
+    d = 2 * r
+    for s in range(scenes):
+        n = int(ns[s])
+        side = int(math.ceil(n ** (1.0 / 3.0)))
+        counts = (side, side + 1, side)
+        x = lattice_block((2 * d, 2 * d, 2 * d), counts, d, 0.01 * r, rng)[:n]
+        descs.append(DFSPHSimulationDescription(
+            ParticleRadius=r, Viscosity=float(mus[s]) / RHO0, BoundaryViscosity=float(mus[s]) / RHO0,
+            EnableSurfaceTensionSolver=0, MaxViscositySolverError=0.1, MaxDivergenceSolverError=0.1,
+            MaxPressureSolverError=0.01, MinDivergenceSolverIterations=2, MinPressureSolverIterations=2,
+            FrameCount=200))
+        fluids.append((x, np.zeros_like(x)))
+        w = side * d + 4 * d
+        bodies.append([RigidBodyDescription("box", center=(w, 0.75 * w, w / 2), half_extents=(w, 0.75 * w, w / 2),
+                                            inverted=True, resolution=(16, 12, 8))])
+    return descs, fluids, bodies
+
+
+def falling_viscous_block(seed=5, n_side=(50, 40, 50), mu=20.0):
+    """Config 5: 100k particles in a falling viscous block with a smooth seeded initial velocity (8 random Fourier
+    modes, amplitude 0.5 m/s); target mu = 20."""
+    rng = np.random.default_rng(seed)
+    r = 0.01
+    x = lattice_block((0.3, 0.3, 0.3), n_side, 2 * r, 0.01 * r, rng)
+    k = rng.normal(size=(8, 3)) * 2 * math.pi / 1.0
+    ph = rng.uniform(0, 2 * math.pi, size=8)
+    amp = rng.normal(size=(8, 3))
+    xd = x.astype(np.float64)
+    v = np.zeros_like(xd)
+    for m in range(8):
+        v += np.sin(xd @ k[m] + ph[m])[:, None] * amp[m]
+    v *= 0.5 / max(np.abs(v).max(), 1e-12)
+    desc = DFSPHSimulationDescription(
+        ParticleRadius=r, Viscosity=mu / RHO0, BoundaryViscosity=mu / RHO0, EnableSurfaceTensionSolver=0,
+        MaxViscositySolverError=0.1, MaxDivergenceSolverError=0.1, MaxPressureSolverError=0.01,
+        MinDivergenceSolverIterations=2, MinPressureSolverIterations=2, FrameCount=60)
+    box = RigidBodyDescription("box", center=(1, 1, 1), half_extents=(1, 1, 1), inverted=True, resolution=(20, 20, 20))
+    return [desc], [(x, v.astype(np.float32))], [[box]]
+
+
+CONFIGS = {
+    "dam_break_8k": dam_break_8k,
+    "viscous_coil_250k": viscous_coil_250k,
+    "dam_break_2m": dam_break_2m,
+    "batched_uneven_64": batched_uneven,
+    "falling_viscous_block_100k": falling_viscous_block,
+}

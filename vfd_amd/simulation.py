@@ -1,0 +1,287 @@
+"""Host-side mirror of VFD's DFSPH simulation component, on top of the C ABI (include/vfd.h).
+
+The reference's interface is the C++ class DFSPHSimulation (VFD/Source/Simulation/DFSPH/DFSPHSimulator.h:10-56),
+driven by the editor (Editor/Panels/ComponentPanel.cpp:608-667): construct it from a DFSPHSimulationDescription,
+SetFluidObjects, SetRigidBodies, then Simulate (or OnUpdate per step) and read the particle frame buffer and debug
+info. This module keeps those names and their meaning; errors raise VfdError instead of exiting the process.
+"""
+import numpy as np
+
+from . import _abi
+from ._abi import check, ptr
+
+
+class FluidObject:
+    """FluidObject (FluidObject/FluidObject.h:21-31): sampled positions and one initial velocity.
+
+    `velocity` may also be an (n, 3) array of per-particle velocities (an extension)."""
+
+    def __init__(self, positions, velocity=(0.0, 0.0, 0.0)):
+        self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        v = np.asarray(velocity, dtype=np.float32)
+        if v.ndim == 1:
+            v = np.broadcast_to(v, self.positions.shape)
+        self.velocities = np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
+        if self.velocities.shape != self.positions.shape:
+            raise ValueError("velocity must be a vec3 or one vec3 per position")
+
+    def GetPositions(self):
+        return self.positions
+
+    def GetPositionCount(self):
+        return len(self.positions)
+
+    def GetVelocity(self):
+        return self.velocities
+
+
+class RigidBodyDescription:
+    """RigidBodyDescription (RigidBody/RigidBody.cuh:12-21): Inverted, Padding, CollisionMapResolution; the mesh and
+    transform are replaced by an analytic shape ("box", "sphere", "plane", "cylinder")."""
+
+    def __init__(self, shape, center=(0, 0, 0), half_extents=(0, 0, 0), radius=0.0, normal=(0, 1, 0),
+                 inverted=False, padding=0.0, resolution=(20, 20, 20), domain=None):
+        self.shape = shape
+        self.center = center
+        self.half_extents = half_extents
+        self.radius = radius
+        self.normal = normal
+        self.inverted = inverted
+        self.padding = padding
+        self.resolution = resolution
+        self.domain = domain
+
+    def to_abi(self):
+        return _abi.make_boundary(self.shape, self.center, self.half_extents, self.radius, self.normal,
+                                  self.inverted, self.padding, self.resolution, self.domain)
+
+
+def DFSPHSimulationDescription(**fields):
+    """DFSPHSimulationDescription with the reference defaults (DFSPHSimulationDescription.h:11-49)."""
+    return _abi.make_desc(**fields)
+
+
+class DFSPHParticleFrame:
+    """DFSPHParticleFrame (ParticleBuffer/DFSPHParticleBuffer.h:13-19)."""
+
+    def __init__(self, xva, max_velocity, time_step):
+        self.ParticleData = xva  # (n, 9): position, velocity, acceleration (DFSPHParticleSimple)
+        self.MaxVelocityMagnitude = max_velocity
+        self.CurrentTimeStep = time_step
+
+
+class DFSPHSimulation:
+    """DFSPHSimulation (DFSPHSimulator.h:10-56) backed by the HIP library. `scene` selects one scene of a batch."""
+
+    def __init__(self, desc=None, _handle=None, _scene=0, _owner=None):
+        self._lib = _abi.load_library()
+        self._scene = _scene
+        self._owner = _owner
+        if _handle is not None:
+            self._h = _handle
+            return
+        if desc is None:
+            desc = DFSPHSimulationDescription()
+        h = _abi.ctypes.c_void_p()
+        check(self._lib.vfd_create(_abi.ctypes.byref(desc), _abi.ctypes.byref(h)), "vfd_create")
+        self._h = h
+
+    def close(self):
+        if self._owner is None and getattr(self, "_h", None):
+            self._lib.vfd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- description / info ------------------------------------------------------------------------------------
+    def SetDescription(self, desc):
+        check(self._lib.vfd_batch_set_description(self._h, self._scene, _abi.ctypes.byref(desc)),
+              "vfd_set_description")
+
+    def GetDescription(self):
+        d = _abi.Desc()
+        check(self._lib.vfd_get_description(self._h, _abi.ctypes.byref(d)), "vfd_get_description")
+        return d
+
+    def GetInfo(self):
+        i = _abi.Info()
+        check(self._lib.vfd_batch_get_info(self._h, self._scene, _abi.ctypes.byref(i)), "vfd_get_info")
+        return i
+
+    # ---- set-up -------------------------------------------------------------------------------------------------
+    def SetFluidObjects(self, fluid_objects):
+        xs = [f.positions for f in fluid_objects]
+        vs = [f.velocities for f in fluid_objects]
+        x = np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros((0, 3), np.float32), dtype=np.float32)
+        v = np.ascontiguousarray(np.concatenate(vs) if vs else np.zeros((0, 3), np.float32), dtype=np.float32)
+        self.set_fluid(x, v)
+
+    def set_fluid(self, positions, velocities=None):
+        x = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        v = None if velocities is None else np.ascontiguousarray(velocities, dtype=np.float32).reshape(-1, 3)
+        check(self._lib.vfd_batch_set_fluid(self._h, self._scene, ptr(x), ptr(v), len(x)), "vfd_set_fluid")
+
+    def SetRigidBodies(self, rigid_bodies):
+        check(self._lib.vfd_clear_boundaries(self._h, self._scene), "vfd_clear_boundaries")
+        for rb in rigid_bodies:
+            b = rb.to_abi() if isinstance(rb, RigidBodyDescription) else rb
+            check(self._lib.vfd_batch_add_boundary(self._h, self._scene, _abi.ctypes.byref(b)),
+                  "vfd_add_boundary")
+
+    # ---- stepping -----------------------------------------------------------------------------------------------
+    def OnUpdate(self):
+        check(self._lib.vfd_step(self._h), "vfd_step")
+
+    def Simulate(self):
+        check(self._lib.vfd_simulate(self._h), "vfd_simulate")
+
+    def advance(self, steps):
+        check(self._lib.vfd_advance(self._h, int(steps)), "vfd_advance")
+
+    def step_until(self, phase):
+        p = _abi.PHASES.index(phase) if isinstance(phase, str) else int(phase)
+        check(self._lib.vfd_step_until(self._h, p), "vfd_step_until")
+
+    def set_frame_capture(self, enabled):
+        check(self._lib.vfd_set_frame_capture(self._h, 1 if enabled else 0), "vfd_set_frame_capture")
+
+    def synchronize(self):
+        pass  # every ABI call returns after its work is complete or enqueued behind a synchronising read
+
+    # ---- getters ------------------------------------------------------------------------------------------------
+    def GetParticleCount(self):
+        n = _abi.ctypes.c_uint32()
+        check(self._lib.vfd_get_particle_count(self._h, self._scene, _abi.ctypes.byref(n)), "vfd_get_particle_count")
+        return n.value
+
+    def GetParticleRadius(self):
+        return self.GetInfo().ParticleRadius
+
+    def GetDebugInfo(self):
+        d = _abi.DebugInfo()
+        check(self._lib.vfd_get_debug_info(self._h, self._scene, _abi.ctypes.byref(d)), "vfd_get_debug_info")
+        return d
+
+    def GetMaxVelocityMagnitude(self):
+        return self.GetDebugInfo().MaxVelocityMagnitude
+
+    def GetCurrentTimeStepSize(self):
+        return self.GetDebugInfo().TimeStepSize
+
+    def GetRigidBodyCount(self):
+        return self.GetInfo().RigidBodyCount
+
+    def GetKernel(self):
+        w = np.zeros(10000, np.float32)
+        g = np.zeros(10001, np.float32)
+        s = np.zeros(6, np.float32)
+        check(self._lib.vfd_get_kernel_table(self._h, ptr(w), ptr(g), ptr(s)), "vfd_get_kernel_table")
+        return w, g, dict(zip(["Radius", "Radius2", "InvStepSize", "WZero", "K", "L"], s.tolist()))
+
+    def field(self, name, body=0):
+        fid, comps = _abi.FIELDS[name]
+        n = self.GetParticleCount()
+        out = np.zeros((n, comps), np.float32)
+        check(self._lib.vfd_get_field(self._h, self._scene, fid, body, ptr(out)), "vfd_get_field(%s)" % name)
+        return out if comps > 1 else out[:, 0]
+
+    def set_field(self, name, values):
+        fid, comps = _abi.FIELDS[name]
+        a = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        check(self._lib.vfd_set_field(self._h, self._scene, fid, ptr(a)), "vfd_set_field(%s)" % name)
+
+    def particles(self):
+        n = self.GetParticleCount()
+        x = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        a = np.zeros((n, 3), np.float32)
+        check(self._lib.vfd_batch_get_particles(self._h, self._scene, ptr(x), ptr(v), ptr(a)), "vfd_get_particles")
+        return x, v, a
+
+    def neighbors(self):
+        n = self.GetParticleCount()
+        counts = np.zeros(n, np.uint32)
+        offsets = np.zeros(n, np.uint32)
+        total = _abi.ctypes.c_uint64()
+        check(self._lib.vfd_get_neighbors(self._h, self._scene, ptr(counts), ptr(offsets), None,
+                                          _abi.ctypes.byref(total)), "vfd_get_neighbors")
+        ids = np.zeros(max(total.value, 1), np.uint32)
+        check(self._lib.vfd_get_neighbors(self._h, self._scene, ptr(counts), ptr(offsets), ptr(ids),
+                                          _abi.ctypes.byref(total)), "vfd_get_neighbors")
+        return counts, offsets, ids[:total.value]
+
+    def boundary_map(self, body, field):
+        cnt = _abi.ctypes.c_uint32()
+        check(self._lib.vfd_get_boundary_map(self._h, self._scene, body, field, None, _abi.ctypes.byref(cnt)),
+              "vfd_get_boundary_map")
+        out = np.zeros(cnt.value, np.float32)
+        check(self._lib.vfd_get_boundary_map(self._h, self._scene, body, field, ptr(out), _abi.ctypes.byref(cnt)),
+              "vfd_get_boundary_map")
+        return out
+
+    def set_profiling(self, enabled):
+        check(self._lib.vfd_set_profiling(self._h, 1 if enabled else 0), "vfd_set_profiling")
+
+    def profile(self):
+        """{kernel: (launches, total_ms)} accumulated since profiling was enabled."""
+        n = _abi.ctypes.c_uint32()
+        check(self._lib.vfd_get_profile(self._h, None, 0, _abi.ctypes.byref(n)), "vfd_get_profile")
+        arr = (_abi.KernelStat * max(n.value, 1))()
+        check(self._lib.vfd_get_profile(self._h, arr, n.value, _abi.ctypes.byref(n)), "vfd_get_profile")
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
+
+    def GetParticleFrameBuffer(self):
+        cnt = _abi.ctypes.c_uint32()
+        check(self._lib.vfd_get_frame_count(self._h, self._scene, _abi.ctypes.byref(cnt)), "vfd_get_frame_count")
+        n = self.GetParticleCount()
+        frames = []
+        for k in range(cnt.value):
+            xva = np.zeros((n, 9), np.float32)
+            mv = _abi.ctypes.c_float()
+            dt = _abi.ctypes.c_float()
+            check(self._lib.vfd_get_frame(self._h, self._scene, k, ptr(xva), _abi.ctypes.byref(mv),
+                                          _abi.ctypes.byref(dt)), "vfd_get_frame")
+            frames.append(DFSPHParticleFrame(xva, mv.value, dt.value))
+        return frames
+
+
+class DFSPHBatch:
+    """Many independent scenes advanced by one set of launches (no reference counterpart)."""
+
+    def __init__(self, descs):
+        self._lib = _abi.load_library()
+        arr = (_abi.Desc * len(descs))(*descs)
+        h = _abi.ctypes.c_void_p()
+        check(self._lib.vfd_batch_create(arr, len(descs), _abi.ctypes.byref(h)), "vfd_batch_create")
+        self._h = h
+        self.scenes = [DFSPHSimulation(_handle=h, _scene=i, _owner=self) for i in range(len(descs))]
+
+    def step(self):
+        check(self._lib.vfd_step(self._h), "vfd_step")
+
+    def advance(self, steps):
+        check(self._lib.vfd_advance(self._h, int(steps)), "vfd_advance")
+
+    def set_frame_capture(self, enabled):
+        check(self._lib.vfd_set_frame_capture(self._h, 1 if enabled else 0), "vfd_set_frame_capture")
+
+    def set_profiling(self, enabled):
+        self.scenes[0].set_profiling(enabled)
+
+    def profile(self):
+        return self.scenes[0].profile()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.vfd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
