@@ -53,29 +53,37 @@ def bytes_per_particle(kernel, nbar, bodies):
     return table.get(kernel)
 
 
-def cpu_baseline(cfg, steps):
+def cpu_baseline(cfg, steps, snap):
+    """The CPU oracle (FP32 mirror) advanced `steps` steps from the GPU's state `snap`, so both sides do the same
+    solver iterations; returns its throughput and its positions after the first step."""
     from oracle.oracle import OracleSim, build_oracle
     from vfd_amd import scenes
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         build_oracle()
     descs, fluids, bodies = scenes.CONFIGS[cfg]()
     o = OracleSim(descs[0])
-    o.set_fluid(*fluids[0])
+    o.set_fluid(snap["x"], snap["v"])
+    o.set_field("velocity_difference", snap["dv"])
+    o.set_field("delta_final_curvature", snap["dk"])
+    o.set_time_step(snap["dt"])
     for b in bodies[0]:
         o.add_boundary(b.to_abi())
     t0 = time.perf_counter()
     its = []
-    for _ in range(steps):
+    x1 = None
+    for k in range(steps):
         o.step()
+        if k == 0:
+            x1 = o.field("position")
         d = o.debug_info()
         its.append((d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount))
     dt = time.perf_counter() - t0
-    n = len(fluids[0][0])
+    n = len(snap["x"])
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": n * steps / dt, "unit": "particle-updates/s", "cores": cores, "kind": "port",
-            "sample": "%s, %d particles, %d steps from the initial state (setup excluded), iterations "
-                      "div/pres/visc per step %s" % (cfg, n, steps, its),
-            "seconds": dt}
+            "sample": "%s, %d particles, %d step(s) from the GPU state after the warm-up steps (setup excluded); "
+                      "solver iterations div/pres/visc per step %s" % (cfg, n, steps, its),
+            "seconds": dt, "x1": x1, "iterations": its}
 
 
 def main():
@@ -85,7 +93,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="dam_break_2m")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the DFSPH step runs on a single MI355X (multi-GPU sharding is a design, "
@@ -103,6 +111,13 @@ def main():
     n = sim.GetParticleCount()
     print("[bench] %s: %d particles, warmup %d steps" % (args.config, n, args.warmup), file=sys.stderr, flush=True)
     sim.advance(args.warmup)
+    # state after warm-up (for the CPU baseline) and one untimed check step
+    snap = {"x": sim.field("position"), "v": sim.field("velocity"), "dv": sim.field("velocity_difference"),
+            "dk": sim.field("delta_final_curvature"), "dt": sim.GetDebugInfo().TimeStepSize}
+    sim.OnUpdate()
+    d = sim.GetDebugInfo()
+    check_its = (d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount)
+    x1_gpu = sim.field("position")
 
     sim.set_profiling(True)
     its = []
@@ -163,8 +178,13 @@ def main():
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline: oracle, %d steps" % args.cpu_steps, file=sys.stderr, flush=True)
-        cb = cpu_baseline(args.config, args.cpu_steps)
+        cb = cpu_baseline(args.config, args.cpu_steps, snap)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        r = descs[0].ParticleRadius
+        out["parity_full_size"] = {
+            "check": "one step from the same state on GPU and CPU oracle",
+            "max_position_difference_in_particle_radii": float(np.abs(x1_gpu - cb["x1"]).max() / r),
+            "iterations_gpu": check_its, "iterations_cpu": cb["iterations"][0]}
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -1438,13 +1438,16 @@ public:
     }
     int set_field(int field, const double* in) {
         const size_t n = p.size();
+        if (field == VFD_FIELD_DELTA_FINAL_CURVATURE) {
+            for (size_t i = 0; i < n; ++i) p[i].dK = (T)in[i];
+            return VFD_OK;
+        }
         for (size_t i = 0; i < n; ++i) {
             const V3<T> v = v3<T>((T)in[3 * i], (T)in[3 * i + 1], (T)in[3 * i + 2]);
             switch (field) {
                 case VFD_FIELD_POSITION: p[i].x = v; break;
                 case VFD_FIELD_VELOCITY: p[i].v = v; break;
                 case VFD_FIELD_VELOCITY_DIFFERENCE: p[i].dv = v; break;
-                case VFD_FIELD_DELTA_FINAL_CURVATURE: p[i].dK = (T)in[i]; break;
                 default: return VFD_ERROR_INVALID_ARGUMENT;
             }
         }

@@ -120,75 +120,41 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     }
 }
 
-// Per-scene "last tile finalises" reduction. Every tile of scene s writes its NV partials to
-// partials[tile*kPartialSlots + i], then bumps counters[s]; the tile that completes the count sums the scene's
-// partials in tile order (deterministic) and returns true in thread 0 with the totals in out[]. The counter is reset
-// for the next use. Must be called by all threads of the block.
+// Per-tile partial sums: the block reduces NV doubles in a fixed order and thread 0 stores them at
+// partials[tile * kPartialSlots + i]. A separate finalize kernel (one block per scene) adds a scene's tiles in tile
+// order, so every reduction is deterministic and needs no device-scope fence (on the multi-XCD MI355X an agent-scope
+// release writes back the L2; one per block serialises thousands of them).
 template <int NV>
-__device__ __forceinline__ bool tile_reduce_last(double (&v)[NV], double* partials, uint32_t* counters, uint32_t s,
-                                                 uint32_t tileBegin, uint32_t tileEnd, double (&out)[NV]) {
-    __shared__ int isLast;
+__device__ __forceinline__ void tile_partials(double (&v)[NV], double* partials, uint32_t tile) {
     block_sum<NV>(v);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
-        for (int i = 0; i < NV; ++i) partials[(size_t)blockIdx.x * kPartialSlots + i] = v[i];
-        __threadfence();
-        const uint32_t done = atomicAdd(&counters[s], 1u);
-        isLast = (done == tileEnd - tileBegin - 1u) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!isLast) return false;
-    __threadfence();
-    // sum partials of the scene's tiles: thread t takes tiles t, t+kTile, ...; then fixed-order block sum
-    double acc[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
-    for (uint32_t t = tileBegin + threadIdx.x; t < tileEnd; t += kTile)
-#pragma unroll
-        for (int i = 0; i < NV; ++i) acc[i] += partials[(size_t)t * kPartialSlots + i];
-    block_sum<NV>(acc);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) out[i] = acc[i];
-        counters[s] = 0u;
-    }
-    return threadIdx.x == 0;
+        for (int i = 0; i < NV; ++i) partials[(size_t)tile * kPartialSlots + i] = v[i];
+    __syncthreads();  // the block-sum scratch is reused by the next tile of a persistent block
 }
 
-// Same pattern for a float max.
-__device__ __forceinline__ bool tile_max_last(float v, float* partials, uint32_t* counters, uint32_t s,
-                                              uint32_t tileBegin, uint32_t tileEnd, float& out) {
+// Inside a finalize block (kTile threads) for scene s: sum of the scene's tile partials, valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void scene_sum(const double* __restrict__ partials, uint32_t tileBegin, uint32_t tileEnd,
+                                          double (&out)[NV]) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) out[i] = 0.0;
+    for (uint32_t t = tileBegin + threadIdx.x; t < tileEnd; t += kTile)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) out[i] += partials[(size_t)t * kPartialSlots + i];
+    block_sum<NV>(out);
+}
+
+__device__ __forceinline__ float block_max(float v) {
     __shared__ float sh[kTile / kWave];
-    __shared__ int isLast;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     v = wave_max(v);
     if (lane == 0) sh[w] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float m = sh[0];
-        for (int j = 1; j < (int)(kTile / kWave); ++j) m = fmaxf(m, sh[j]);
-        partials[blockIdx.x] = m;
-        __threadfence();
-        const uint32_t done = atomicAdd(&counters[s], 1u);
-        isLast = (done == tileEnd - tileBegin - 1u) ? 1 : 0;
-    }
+    float m = sh[0];
+    for (int j = 1; j < (int)(kTile / kWave); ++j) m = fmaxf(m, sh[j]);
     __syncthreads();
-    if (!isLast) return false;
-    __threadfence();
-    float m = -FLT_MAX;
-    for (uint32_t t = tileBegin + threadIdx.x; t < tileEnd; t += kTile)
-        m = fmaxf(m, partials[t]);
-    m = wave_max(m);
-    __syncthreads();
-    if (lane == 0) sh[w] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float r = sh[0];
-        for (int j = 1; j < (int)(kTile / kWave); ++j) r = fmaxf(r, sh[j]);
-        out = r;
-        counters[s] = 0u;
-    }
-    return threadIdx.x == 0;
+    return m;
 }
 
 // Tile bookkeeping shared by every per-particle kernel.
@@ -203,6 +169,72 @@ __device__ __forceinline__ TileCtx tile_ctx(const uint32_t* tileScene, const Sce
     c.i = blockIdx.x * kTile + threadIdx.x;
     c.valid = c.i < P[c.s].base + P[c.s].count;
     return c;
+}
+__device__ __forceinline__ TileCtx tile_ctx(uint32_t tile, const uint32_t* tileScene, const SceneParams* P) {
+    TileCtx c;
+    c.s = tileScene[tile];
+    c.i = tile * kTile + threadIdx.x;
+    c.valid = c.i < P[c.s].base + P[c.s].count;
+    return c;
+}
+
+// XCD-aware tile loop. The tiles (Morton-ordered particle ranges) are cut into 8 contiguous chunks of `per` tiles,
+// one per XCD; work-group q runs on XCD q % 8 (round-robin dispatch), so chunk q % 8 stays in one XCD's L2. With a
+// grid smaller than 8 * per the blocks loop (persistent); the grid must be a multiple of 8.
+struct TileGrid {
+    uint32_t tiles, per;
+    __host__ __device__ TileGrid() = default;
+    template <class T> __host__ __device__ TileGrid(const T& t) : tiles(t.tiles), per(t.per) {}
+};
+template <class Body>
+__device__ __forceinline__ void tile_loop(const TileGrid& tg, Body&& body) {
+    for (uint32_t q = blockIdx.x; q < 8u * tg.per; q += gridDim.x) {
+        const uint32_t k = q >> 3, tile = (q & 7u) * tg.per + k;
+        if (tile < tg.tiles) body(tile);
+    }
+}
+
+// Neighbour j of particle i in the transposed lists; f(t) is called in list order, four list entries fetched per
+// trip so that their gathers can be in flight together.
+template <class Fn>
+__device__ __forceinline__ void each_neighbor(const uint32_t* __restrict__ list, uint32_t cap, uint32_t i, uint32_t n,
+                                              Fn&& f) {
+    uint32_t j = 0;
+    for (; j + 4 <= n; j += 4) {
+        const uint32_t t0 = list[(size_t)j * cap + i], t1 = list[(size_t)(j + 1) * cap + i];
+        const uint32_t t2 = list[(size_t)(j + 2) * cap + i], t3 = list[(size_t)(j + 3) * cap + i];
+        f(t0);
+        f(t1);
+        f(t2);
+        f(t3);
+    }
+    for (; j < n; ++j) f(list[(size_t)j * cap + i]);
+}
+
+// Kernel table staged in LDS: tab[p] (float2 {W, G}) or G only, loaded once per block.
+constexpr unsigned kLdsG = kTableEntries * sizeof(float);
+constexpr unsigned kLdsWG = kTableEntries * sizeof(float2);
+__device__ __forceinline__ void load_table_g(float* sG, const float2* __restrict__ tab) {
+    for (uint32_t p = threadIdx.x; p < kTableEntries; p += blockDim.x) sG[p] = tab[p].y;
+    __syncthreads();
+}
+__device__ __forceinline__ void load_table_wg(float2* sWG, const float2* __restrict__ tab) {
+    for (uint32_t p = threadIdx.x; p < kTableEntries; p += blockDim.x) sWG[p] = tab[p];
+    __syncthreads();
+}
+// gradW(r) with the G table in LDS (cutoff |r| <= R), same values as gradW(KTab, r)
+struct KLds {
+    const float* sG;
+    float R, invStep;
+};
+__device__ __forceinline__ f3 gradW(const KLds& k, f3 r) {
+    const float rl = sqrtf(dot3(r, r));
+    if (rl <= k.R) {
+        const unsigned p0 = (unsigned)(rl * k.invStep);
+        const unsigned p = p0 < kTableEntries - 1u ? p0 : kTableEntries - 1u;
+        return k.sG[p] * r;
+    }
+    return mk3(0.0f, 0.0f, 0.0f);
 }
 
 }  // namespace vfd

@@ -5,9 +5,9 @@
 // below names the reference kernel(s) it replaces. Differences by design:
 //   - structure-of-arrays in Morton slot order, transposed neighbour lists (see common.h);
 //   - density and factor share one neighbour loop (one table fetch per pair gives W and gradW);
-//   - solver control lives on the device: every reduction is a deterministic FP64 per-tile sum finished by the last
-//     tile of each scene, which also evaluates the reference's loop condition, so the host only polls every few
-//     iterations instead of copying a scalar back per iteration;
+//   - solver control lives on the device: every reduction is a deterministic FP64 per-tile sum that a one-block-per-
+//     scene finalize kernel adds up in tile order; that kernel also evaluates the reference's loop condition, so the
+//     host only polls every few iterations instead of copying a scalar back per iteration;
 //   - ClearAcceleration is fused into the divergence finaliser; the position update into the pressure finaliser.
 // Floating-point expressions keep the reference's association order (no implicit contraction; -ffp-contract=off).
 #include "device.h"
@@ -21,57 +21,59 @@ __device__ __forceinline__ uint32_t nbr(const Fields& F, uint32_t cap, uint32_t 
 }
 
 // ---- ComputeDensityKernel (DFSPHKernels.cu:152-187) + ComputeDFSPHFactorKernel (:189-231) ----------------------
+// One table fetch per pair gives W (cutoff r^2 <= R^2) and gradW (cutoff |r| <= R); the {W, G} table sits in LDS.
 __global__ void __launch_bounds__(kTile) k_density_factor(const SceneParams* __restrict__ P,
                                                           const uint32_t* __restrict__ tileScene, KTabHost kh,
-                                                          Fields F, uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    if (!c.valid) return;
-    const SceneParams& p = P[c.s];
-    const KTab k = ktab(kh);
-    const f3 xi = ld3(F.pos[c.i]);
-    const float V = p.V;
-    float dens = V * k.W0;
-    f3 gi = mk3(0.0f, 0.0f, 0.0f);
-    float sum = 0.0f;
-    const uint32_t n = F.nbrCount[c.i];
-    for (uint32_t j = 0; j < n; ++j) {
-        const f3 r = xi - ld3(F.pos[nbr(F, cap, c.i, j)]);
-        const Pair q = pair_lookup(k, r);
-        dens += V * q.W;
-        const f3 gj = (-V) * (q.G * r);
-        sum += dot3(gj, gj);
-        gi = gi - gj;
-    }
-    for (uint32_t b = 0; b < p.bodyCount; ++b) {
-        const float vb = F.vb[(size_t)b * cap + c.i];
-        if (vb > 0.0f) {
-            const f3 r = xi - ld3(F.xb[(size_t)b * cap + c.i]);
-            dens += vb * getW(k, r);
-            const f3 gj = (-vb) * gradW(k, r);
+                                                          Fields F, uint32_t cap, TileGrid tg) {
+    extern __shared__ float2 sWG[];
+    load_table_wg(sWG, kh.tab);
+    const KTab k{sWG, kh.R, kh.R2, kh.invStep, kh.W0};
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        if (!c.valid) return;
+        const SceneParams& p = P[c.s];
+        const f3 xi = ld3(F.pos[c.i]);
+        const float V = p.V;
+        float dens = V * k.W0;
+        f3 gi = mk3(0.0f, 0.0f, 0.0f);
+        float sum = 0.0f;
+        each_neighbor(F.nbrList, cap, c.i, F.nbrCount[c.i], [&](uint32_t t) {
+            const f3 r = xi - ld3(F.pos[t]);
+            const Pair q = pair_lookup(k, r);
+            dens += V * q.W;
+            const f3 gj = (-V) * (q.G * r);
+            sum += dot3(gj, gj);
             gi = gi - gj;
+        });
+        for (uint32_t b = 0; b < p.bodyCount; ++b) {
+            const float vb = F.vb[(size_t)b * cap + c.i];
+            if (vb > 0.0f) {
+                const f3 r = xi - ld3(F.xb[(size_t)b * cap + c.i]);
+                dens += vb * getW(k, r);
+                const f3 gj = (-vb) * gradW(k, r);
+                gi = gi - gj;
+            }
         }
-    }
-    sum += dot3(gi, gi);
-    F.density[c.i] = dens * p.rho0;
-    F.factor[c.i] = (double)sum > kEps ? 1.0f / sum : 0.0f;
+        sum += dot3(gi, gi);
+        F.density[c.i] = dens * p.rho0;
+        F.factor[c.i] = (double)sum > kEps ? 1.0f / sum : 0.0f;
+    });
 }
 
 void launch_density_factor(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_density_factor, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F,
-                       L.capacity);
+    hipLaunchKernelGGL(k_density_factor, dim3(L.grid2), dim3(kTile), kLdsWG, L.stream, L.P, L.tileScene, kt, F,
+                       L.capacity, TileGrid(L.tg));
 }
 
 // ---- shared neighbour sums --------------------------------------------------------------------------------------
 // v_i-weighted divergence sum_j (v_i - v_j).gradW_ij * V + sum_b V_b v_i.gradW_ib (:233-278, :483-529)
-__device__ __forceinline__ float velocity_divergence(const SceneParams& p, const KTab& k, const Fields& F,
+__device__ __forceinline__ float velocity_divergence(const SceneParams& p, const KLds& k, const Fields& F,
                                                      uint32_t cap, uint32_t i) {
     const f3 xi = ld3(F.pos[i]), vi = ld3(F.vel[i]);
     float d = 0.0f;
-    const uint32_t n = F.nbrCount[i];
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t t = nbr(F, cap, i, j);
+    each_neighbor(F.nbrList, cap, i, F.nbrCount[i], [&](uint32_t t) {
         d += dot3(vi - ld3(F.vel[t]), gradW(k, xi - ld3(F.pos[t])));
-    }
+    });
     d *= p.V;
     for (uint32_t b = 0; b < p.bodyCount; ++b) {
         const float vb = F.vb[(size_t)b * cap + i];
@@ -81,20 +83,18 @@ __device__ __forceinline__ float velocity_divergence(const SceneParams& p, const
 }
 
 // pressure acceleration from kappa (ComputePressureAccelerationKernel :342-386 and its three siblings)
-__device__ __forceinline__ f3 pressure_accel(const SceneParams& p, const KTab& k, const Fields& F, uint32_t cap,
+__device__ __forceinline__ f3 pressure_accel(const SceneParams& p, const KLds& k, const Fields& F, uint32_t cap,
                                              uint32_t i, const float* __restrict__ kap) {
     const f3 xi = ld3(F.pos[i]);
     const float ki = kap[i];
     f3 a = mk3(0.0f, 0.0f, 0.0f);
-    const uint32_t n = F.nbrCount[i];
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t t = nbr(F, cap, i, j);
+    each_neighbor(F.nbrList, cap, i, F.nbrCount[i], [&](uint32_t t) {
         const float s = ki + kap[t];
         if ((double)fabsf(s) > kEps) {
             const f3 gj = (-p.V) * gradW(k, xi - ld3(F.pos[t]));
             a = a + s * gj;
         }
-    }
+    });
     if ((double)fabsf(ki) > kEps) {
         for (uint32_t b = 0; b < p.bodyCount; ++b) {
             const float vb = F.vb[(size_t)b * cap + i];
@@ -108,15 +108,13 @@ __device__ __forceinline__ f3 pressure_accel(const SceneParams& p, const KTab& k
 }
 
 // ComputeDensityPressureForce (:280-315)
-__device__ __forceinline__ float density_pressure_force(const SceneParams& p, const KTab& k, const Fields& F,
+__device__ __forceinline__ float density_pressure_force(const SceneParams& p, const KLds& k, const Fields& F,
                                                         uint32_t cap, uint32_t i) {
     const f3 xi = ld3(F.pos[i]), ai = ld3(F.ap[i]);
     float f = 0.0f;
-    const uint32_t n = F.nbrCount[i];
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t t = nbr(F, cap, i, j);
+    each_neighbor(F.nbrList, cap, i, F.nbrCount[i], [&](uint32_t t) {
         f += dot3(ai - ld3(F.ap[t]), gradW(k, xi - ld3(F.pos[t])));
-    }
+    });
     f *= p.V;
     for (uint32_t b = 0; b < p.bodyCount; ++b) {
         const float vb = F.vb[(size_t)b * cap + i];
@@ -124,6 +122,12 @@ __device__ __forceinline__ float density_pressure_force(const SceneParams& p, co
     }
     return f;
 }
+
+// Every gradient-loop kernel stages the G table in LDS and runs the XCD-aware persistent tile loop.
+#define VFD_LDS_G_KERNEL_PROLOGUE(kh)          \
+    extern __shared__ float sG[];              \
+    load_table_g(sG, (kh).tab);                \
+    const KLds K{sG, (kh).R, (kh).invStep};
 
 // ---- Jacobi solver control (DFSPHImplementation.cu:455-461, 526-532) -------------------------------------------
 // which = 0: divergence (eta = (1/dt) tol 1e-4 rho0), 1: pressure (eta = tol 1e-4 rho0)
@@ -144,11 +148,17 @@ __global__ void k_jacobi_begin(const SceneParams* __restrict__ P, SceneState* st
     }
 }
 
-// Residual error -rho0 * sum(r) / N, iteration count and the loop condition, evaluated by the scene's last tile.
-__device__ __forceinline__ void jacobi_finish(double part, const SceneParams& p, JacobiState& J, const Fields& F,
-                                              uint32_t s, uint32_t minIt, uint32_t maxIt) {
-    double v[1] = {part}, out[1];
-    if (tile_reduce_last<1>(v, F.partials, F.counters, s, p.tileBegin, p.tileEnd, out)) {
+// Residual error -rho0 * sum(r) / N, iteration count and the loop condition (DFSPHImplementation.cu:461, 483-492,
+// 532, 554-563), one block per scene after every iteration kernel.
+__global__ void __launch_bounds__(kTile) k_jacobi_finish(const SceneParams* __restrict__ P, Fields F, int which) {
+    const uint32_t s = blockIdx.x;
+    const SceneParams& p = P[s];
+    JacobiState& J = which == 0 ? F.state[s].div : F.state[s].pres;
+    if (!J.active || p.count == 0) return;  // uniform per block
+    double out[1];
+    scene_sum<1>(F.partials, p.tileBegin, p.tileEnd, out);
+    if (threadIdx.x == 0) {
+        const uint32_t minIt = which == 0 ? p.minDiv : p.minPres, maxIt = which == 0 ? p.maxDiv : p.maxPres;
         const float err = (float)(-out[0]) / (float)p.count;
         const uint32_t it = J.it + 1u;
         J.err = err;
@@ -161,62 +171,74 @@ __device__ __forceinline__ void jacobi_finish(double part, const SceneParams& p,
 // ComputeDensityChangeKernel (DFSPHKernels.cu:483-529)
 __global__ void __launch_bounds__(kTile) k_div_init(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                    uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    const SceneParams& p = P[c.s];
-    if (!c.valid || !p.enableDiv) return;
-    const KTab k = ktab(kh);
-    float d = velocity_divergence(p, k, F, cap, c.i);
-    d = F.nbrCount[c.i] < 20u ? 0.0f : fmaxf(d, 0.0f);
-    F.densAdv[c.i] = d;
-    const float f = F.factor[c.i] * F.state[c.s].dtInv;
-    F.factor[c.i] = f;
-    F.kappaV[c.i] = d * f;
+                                                    uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneParams& p = P[c.s];
+        if (!c.valid || !p.enableDiv) return;
+        float d = velocity_divergence(p, K, F, cap, c.i);
+        d = F.nbrCount[c.i] < 20u ? 0.0f : fmaxf(d, 0.0f);
+        F.densAdv[c.i] = d;
+        const float f = F.factor[c.i] * F.state[c.s].dtInv;
+        F.factor[c.i] = f;
+        F.kappaV[c.i] = d * f;
+    });
 }
 
 // ComputePressureAccelerationAndDivergenceKernel (:388-432)
 __global__ void __launch_bounds__(kTile) k_div_accel(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                     uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    if (!F.state[c.s].div.active || !c.valid) return;
-    F.ap[c.i] = st4(pressure_accel(P[c.s], ktab(kh), F, cap, c.i, F.kappaV));
+                                                     uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        if (!F.state[c.s].div.active || !c.valid) return;
+        F.ap[c.i] = st4(pressure_accel(P[c.s], K, F, cap, c.i, F.kappaV));
+    });
 }
 
-// DivergenceSolveIterationKernel (:531-553) + residual reduction (DFSPHImplementation.cu:554-563)
+// DivergenceSolveIterationKernel (:531-553) + per-tile residual partial sums (DFSPHImplementation.cu:554-563)
 __global__ void __launch_bounds__(kTile) k_div_iter(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                    uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    SceneState& st = F.state[c.s];
-    if (!st.div.active) return;  // uniform per block
-    const SceneParams& p = P[c.s];
-    double part = 0.0;
-    if (c.valid) {
-        const float f = density_pressure_force(p, ktab(kh), F, cap, c.i) * st.dt;
-        const float r = F.nbrCount[c.i] < 20u ? 0.0f : fminf(-F.densAdv[c.i] - f, 0.0f);
-        F.residuum[c.i] = r;
-        F.kappaV[c.i] -= r * F.factor[c.i];
-        part = (double)(p.rho0 * r);
-    }
-    jacobi_finish(part, p, st.div, F, c.s, p.minDiv, p.maxDiv);
+                                                    uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneState& st = F.state[c.s];
+        if (!st.div.active) return;  // uniform per tile
+        const SceneParams& p = P[c.s];
+        double part = 0.0;
+        if (c.valid) {
+            const float f = density_pressure_force(p, K, F, cap, c.i) * st.dt;
+            const float r = F.nbrCount[c.i] < 20u ? 0.0f : fminf(-F.densAdv[c.i] - f, 0.0f);
+            F.residuum[c.i] = r;
+            F.kappaV[c.i] -= r * F.factor[c.i];
+            part = (double)(p.rho0 * r);
+        }
+        double v[1] = {part};
+        tile_partials<1>(v, F.partials, tile);
+    });
 }
 
 // ComputePressureAccelerationAndFactorKernel (:555-602) + ClearAccelerationKernel (:24-37)
 __global__ void __launch_bounds__(kTile) k_div_final(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                     uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    if (!c.valid) return;
-    const SceneParams& p = P[c.s];
-    if (p.enableDiv) {
-        const float dt = F.state[c.s].dt;
-        const f3 a = pressure_accel(p, ktab(kh), F, cap, c.i, F.kappaV);
-        F.ap[c.i] = st4(a);
-        F.vel[c.i] = st4(ld3(F.vel[c.i]) + dt * a);
-        F.factor[c.i] *= dt;
-    }
-    F.acc[c.i] = make_float4(p.gx, p.gy, p.gz, 0.0f);
+                                                     uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        if (!c.valid) return;
+        const SceneParams& p = P[c.s];
+        if (p.enableDiv) {
+            const float dt = F.state[c.s].dt;
+            const f3 a = pressure_accel(p, K, F, cap, c.i, F.kappaV);
+            F.ap[c.i] = st4(a);
+            F.vel[c.i] = st4(ld3(F.vel[c.i]) + dt * a);
+            F.factor[c.i] *= dt;
+        }
+        F.acc[c.i] = make_float4(p.gx, p.gy, p.gz, 0.0f);
+    });
 }
 
 void launch_scene_init_div(const Launch& L, const Fields& F) {
@@ -224,77 +246,95 @@ void launch_scene_init_div(const Launch& L, const Fields& F) {
                        L.sceneCount, 0);
 }
 void launch_div_init(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_init, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_div_init, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+                       TileGrid(L.tg));
 }
 void launch_div_accel(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_accel, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_div_accel, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+                       TileGrid(L.tg));
 }
 void launch_div_iter(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_iter, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_div_iter, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+                       TileGrid(L.tg));
+    hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F, 0);
 }
 void launch_div_final(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_final, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_div_final, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+                       TileGrid(L.tg));
 }
 
 // ---- pressure solve (ComputePressure, DFSPHImplementation.cu:443-504) -------------------------------------------
 // ComputeDensityAdvectionKernel (DFSPHKernels.cu:233-278)
 __global__ void __launch_bounds__(kTile) k_pres_init(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                     uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    if (!c.valid) return;
-    const SceneParams& p = P[c.s];
-    const SceneState& st = F.state[c.s];
-    const float d = velocity_divergence(p, ktab(kh), F, cap, c.i);
-    const float da = F.density[c.i] / p.rho0 + st.dt * d;
-    F.densAdv[c.i] = da;
-    const float f = F.factor[c.i] * st.dt2Inv;
-    F.factor[c.i] = f;
-    const float r = fminf(1.0f - da, 0.0f);
-    F.kappa[c.i] = -r * f;
+                                                     uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        if (!c.valid) return;
+        const SceneParams& p = P[c.s];
+        const SceneState& st = F.state[c.s];
+        const float d = velocity_divergence(p, K, F, cap, c.i);
+        const float da = F.density[c.i] / p.rho0 + st.dt * d;
+        F.densAdv[c.i] = da;
+        const float f = F.factor[c.i] * st.dt2Inv;
+        F.factor[c.i] = f;
+        const float r = fminf(1.0f - da, 0.0f);
+        F.kappa[c.i] = -r * f;
+    });
 }
 
 // ComputePressureAccelerationKernel (:342-386)
 __global__ void __launch_bounds__(kTile) k_pres_accel(const SceneParams* __restrict__ P,
                                                       const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                      uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    if (!F.state[c.s].pres.active || !c.valid) return;
-    F.ap[c.i] = st4(pressure_accel(P[c.s], ktab(kh), F, cap, c.i, F.kappa));
+                                                      uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        if (!F.state[c.s].pres.active || !c.valid) return;
+        F.ap[c.i] = st4(pressure_accel(P[c.s], K, F, cap, c.i, F.kappa));
+    });
 }
 
-// PressureSolveIterationKernel (:317-340) + residual reduction (DFSPHImplementation.cu:483-491)
+// PressureSolveIterationKernel (:317-340) + per-tile residual partial sums (DFSPHImplementation.cu:483-491)
 __global__ void __launch_bounds__(kTile) k_pres_iter(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                     uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    SceneState& st = F.state[c.s];
-    if (!st.pres.active) return;
-    const SceneParams& p = P[c.s];
-    double part = 0.0;
-    if (c.valid) {
-        const float f = density_pressure_force(p, ktab(kh), F, cap, c.i) * st.dt2;
-        const float r = fminf(1.0f - F.densAdv[c.i] - f, 0.0f);
-        F.residuum[c.i] = r;
-        F.kappa[c.i] -= r * F.factor[c.i];
-        part = (double)(p.rho0 * r);
-    }
-    jacobi_finish(part, p, st.pres, F, c.s, p.minPres, p.maxPres);
+                                                     uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneState& st = F.state[c.s];
+        if (!st.pres.active) return;
+        const SceneParams& p = P[c.s];
+        double part = 0.0;
+        if (c.valid) {
+            const float f = density_pressure_force(p, K, F, cap, c.i) * st.dt2;
+            const float r = fminf(1.0f - F.densAdv[c.i] - f, 0.0f);
+            F.residuum[c.i] = r;
+            F.kappa[c.i] -= r * F.factor[c.i];
+            part = (double)(p.rho0 * r);
+        }
+        double v[1] = {part};
+        tile_partials<1>(v, F.partials, tile);
+    });
 }
 
 // ComputePressureAccelerationAndVelocityKernel (:434-481) + ComputePositionKernel (:54-67) into posNext
 __global__ void __launch_bounds__(kTile) k_pres_final(const SceneParams* __restrict__ P,
                                                       const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                      uint32_t cap, int writeNext) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    if (!c.valid) return;
-    const SceneParams& p = P[c.s];
-    const float dt = F.state[c.s].dt;
-    const f3 a = pressure_accel(p, ktab(kh), F, cap, c.i, F.kappa);
-    F.ap[c.i] = st4(a);
-    const f3 v = ld3(F.vel[c.i]) + dt * a;
-    F.vel[c.i] = st4(v);
-    if (writeNext) F.posNext[c.i] = st4(ld3(F.pos[c.i]) + dt * v);
+                                                      uint32_t cap, TileGrid tg, int writeNext) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        if (!c.valid) return;
+        const SceneParams& p = P[c.s];
+        const float dt = F.state[c.s].dt;
+        const f3 a = pressure_accel(p, K, F, cap, c.i, F.kappa);
+        F.ap[c.i] = st4(a);
+        const f3 v = ld3(F.vel[c.i]) + dt * a;
+        F.vel[c.i] = st4(v);
+        if (writeNext) F.posNext[c.i] = st4(ld3(F.pos[c.i]) + dt * v);
+    });
 }
 
 void launch_scene_init_pres(const Launch& L, const Fields& F) {
@@ -302,30 +342,47 @@ void launch_scene_init_pres(const Launch& L, const Fields& F) {
                        L.sceneCount, 1);
 }
 void launch_pres_init(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_pres_init, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_pres_init, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+                       TileGrid(L.tg));
 }
 void launch_pres_accel(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_pres_accel, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_pres_accel, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F,
+                       L.capacity, TileGrid(L.tg));
 }
 void launch_pres_iter(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_pres_iter, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_pres_iter, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+                       TileGrid(L.tg));
+    hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F, 1);
 }
 void launch_pres_final(const Launch& L, const KTabHost& kt, const Fields& F, int writeNext) {
-    hipLaunchKernelGGL(k_pres_final, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity,
-                       writeNext);
+    hipLaunchKernelGGL(k_pres_final, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F,
+                       L.capacity, TileGrid(L.tg), writeNext);
 }
 
 // ---- time step (ComputeTimeStepSize, DFSPHImplementation.cu:395-441) --------------------------------------------
 __global__ void __launch_bounds__(kTile) k_time_step(const SceneParams* __restrict__ P,
-                                                     const uint32_t* __restrict__ tileScene, Fields F) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    SceneState& st = F.state[c.s];
+                                                     const uint32_t* __restrict__ tileScene, Fields F, TileGrid tg) {
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneState& st = F.state[c.s];
+        float m = -FLT_MAX;
+        if (c.valid) m = length2(ld3(F.vel[c.i]) + ld3(F.acc[c.i]) * st.dt);
+        m = block_max(m);
+        if (threadIdx.x == 0) F.partialsF[tile] = m;
+    });
+}
+
+// max(0.1, max |v + a dt|^2) -> CFL time step and the quantities derived from it, one block per scene
+__global__ void __launch_bounds__(kTile) k_time_step_finish(const SceneParams* __restrict__ P, Fields F) {
+    const uint32_t s = blockIdx.x;
+    const SceneParams& p = P[s];
+    if (p.count == 0) return;
     float m = -FLT_MAX;
-    if (c.valid) m = length2(ld3(F.vel[c.i]) + ld3(F.acc[c.i]) * st.dt);
-    const SceneParams& p = P[c.s];
-    float mx;
-    if (tile_max_last(m, F.partialsF, F.counters, c.s, p.tileBegin, p.tileEnd, mx)) {
-        float mv = fmaxf(0.1f, mx);
+    for (uint32_t t = p.tileBegin + threadIdx.x; t < p.tileEnd; t += kTile) m = fmaxf(m, F.partialsF[t]);
+    m = block_max(m);
+    if (threadIdx.x == 0) {
+        SceneState& st = F.state[s];
+        float mv = fmaxf(0.1f, m);
         st.maxVel = mv;
         if (mv < 1.0e-9f) mv = 1.0e-9f;
         float dt = 0.4f * (p.diam / sqrtf(mv));
@@ -336,23 +393,25 @@ __global__ void __launch_bounds__(kTile) k_time_step(const SceneParams* __restri
         st.dtInv = 1.0f / dt;
         st.dt2Inv = 1.0f / st.dt2;
         st.S = p.csdFix > 0 ? (uint32_t)p.csdFix : (uint32_t)(int32_t)((float)p.csd * dt);
-        st.frameTime += dt;
     }
 }
 
 // ComputeVelocityKernel (DFSPHKernels.cu:39-52)
 __global__ void __launch_bounds__(kTile) k_velocity(const SceneParams* __restrict__ P,
-                                                    const uint32_t* __restrict__ tileScene, Fields F) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    if (!c.valid) return;
-    F.vel[c.i] = st4(ld3(F.vel[c.i]) + F.state[c.s].dt * ld3(F.acc[c.i]));
+                                                    const uint32_t* __restrict__ tileScene, Fields F, TileGrid tg) {
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        if (!c.valid) return;
+        F.vel[c.i] = st4(ld3(F.vel[c.i]) + F.state[c.s].dt * ld3(F.acc[c.i]));
+    });
 }
 
 void launch_time_step(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_time_step, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, F);
+    hipLaunchKernelGGL(k_time_step, dim3(L.gridAll), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, TileGrid(L.tg));
+    hipLaunchKernelGGL(k_time_step_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_velocity(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_velocity, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, F);
+    hipLaunchKernelGGL(k_velocity, dim3(L.gridAll), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, TileGrid(L.tg));
 }
 
 // ---- implicit viscosity (ComputeViscosity, DFSPHImplementation.cu:577-615; Weiler et al. 2018) ------------------
@@ -402,7 +461,7 @@ struct Tangential {
     f3 gw[4];
     float vol;
 };
-__device__ __forceinline__ bool tangential(const SceneParams& p, const KTab& k, f3 xi, f3 xb, float vb, Tangential& t) {
+__device__ __forceinline__ bool tangential(const SceneParams& p, const KLds& k, f3 xi, f3 xb, float vb, Tangential& t) {
     f3 normal = -(xi - xb);
     const float nl = length(normal);
     if (!(nl > 0.0001f)) return false;
@@ -423,70 +482,68 @@ __device__ __forceinline__ bool tangential(const SceneParams& p, const KTab& k, 
 // block, right-hand side b = v (the boundary rhs term is dot(0, .) = 0) and initial guess x0 = v + dv.
 __global__ void __launch_bounds__(kTile) k_visc_precond(const SceneParams* __restrict__ P,
                                                         const uint32_t* __restrict__ tileScene, KTabHost kh,
-                                                        Fields F, uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    const SceneParams& p = P[c.s];
-    if (!c.valid || !p.enableVisc) return;
-    const KTab k = ktab(kh);
-    const f3 xi = ld3(F.pos[c.i]);
-    const float d001 = 0.01f * p.R2;
-    float M[9];
+                                                        Fields F, uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneParams& p = P[c.s];
+        if (!c.valid || !p.enableVisc) return;
+        const f3 xi = ld3(F.pos[c.i]);
+        const float d001 = 0.01f * p.R2;
+        float M[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) M[e] = 0.0f;
-    const uint32_t n = F.nbrCount[c.i];
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t t = nbr(F, cap, c.i, j);
-        const f3 xij = xi - ld3(F.pos[t]);
-        const f3 gw = gradW(k, xij);
-        const float s = 10.0f * p.mu * (p.m / F.density[t]) / (length2(xij) + d001);
-        const float a[3] = {xij.x, xij.y, xij.z}, g[3] = {gw.x, gw.y, gw.z};
+        for (int e = 0; e < 9; ++e) M[e] = 0.0f;
+        each_neighbor(F.nbrList, cap, c.i, F.nbrCount[c.i], [&](uint32_t t) {
+            const f3 xij = xi - ld3(F.pos[t]);
+            const f3 gw = gradW(K, xij);
+            const float s = 10.0f * p.mu * (p.m / F.density[t]) / (length2(xij) + d001);
+            const float a[3] = {xij.x, xij.y, xij.z}, g[3] = {gw.x, gw.y, gw.z};
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+            for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) M[3 * r + cc] += s * (a[r] * g[cc]);
-    }
-    if (p.muB != 0.0f) {
-        for (uint32_t b = 0; b < p.bodyCount; ++b) {
-            const float vb = F.vb[(size_t)b * cap + c.i];
-            if (!(vb > 0.0f)) continue;
-            Tangential t;
-            if (!tangential(p, k, xi, ld3(F.xb[(size_t)b * cap + c.i]), vb, t)) continue;
-            const float a[3] = {t.pd[0].x, t.pd[0].y, t.pd[0].z};  // the reference uses direction 1 for all four
+                for (int cc = 0; cc < 3; ++cc) M[3 * r + cc] += s * (a[r] * g[cc]);
+        });
+        if (p.muB != 0.0f) {
+            for (uint32_t b = 0; b < p.bodyCount; ++b) {
+                const float vb = F.vb[(size_t)b * cap + c.i];
+                if (!(vb > 0.0f)) continue;
+                Tangential t;
+                if (!tangential(p, K, xi, ld3(F.xb[(size_t)b * cap + c.i]), vb, t)) continue;
+                const float a[3] = {t.pd[0].x, t.pd[0].y, t.pd[0].z};  // the reference uses direction 1 for all four
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float s = 10.0f * p.muB * t.vol / (length2(t.pd[q]) + d001);
-                const float g[3] = {t.gw[q].x, t.gw[q].y, t.gw[q].z};
+                for (int q = 0; q < 4; ++q) {
+                    const float s = 10.0f * p.muB * t.vol / (length2(t.pd[q]) + d001);
+                    const float g[3] = {t.gw[q].x, t.gw[q].y, t.gw[q].z};
 #pragma unroll
-                for (int r = 0; r < 3; ++r)
+                    for (int r = 0; r < 3; ++r)
 #pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) M[3 * r + cc] += s * (a[r] * g[cc]);
+                        for (int cc = 0; cc < 3; ++cc) M[3 * r + cc] += s * (a[r] * g[cc]);
+                }
             }
         }
-    }
-    const float f = F.state[c.s].dt / F.density[c.i];
-    M3 A;
+        const float f = F.state[c.s].dt / F.density[c.i];
+        M3 A;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) A.m[e] = (e % 4 == 0 ? 1.0f : 0.0f) - f * M[e];
-    const M3 inv = inverse3(A);
+        for (int e = 0; e < 9; ++e) A.m[e] = (e % 4 == 0 ? 1.0f : 0.0f) - f * M[e];
+        const M3 inv = inverse3(A);
 #pragma unroll
-    for (int e = 0; e < 9; ++e) F.minv[(size_t)e * cap + c.i] = inv.m[e];
-    F.cgX[c.i] = st4(ld3(F.vel[c.i]) + ld3(F.dv[c.i]));
+        for (int e = 0; e < 9; ++e) F.minv[(size_t)e * cap + c.i] = inv.m[e];
+        F.cgX[c.i] = st4(ld3(F.vel[c.i]) + ld3(F.dv[c.i]));
+    });
 }
 
 // ComputeMatrixVecProdFunctionKernel (:758-842): y_i = x_i - dt/rho_i * (viscous acceleration of x)
-__device__ __forceinline__ f3 visc_matvec(const SceneParams& p, const KTab& k, const Fields& F, uint32_t cap,
+__device__ __forceinline__ f3 visc_matvec(const SceneParams& p, const KLds& k, const Fields& F, uint32_t cap,
                                           uint32_t i, const float4* __restrict__ in, float dt) {
     const f3 xi = ld3(F.pos[i]), vi = ld3(in[i]);
     const float d001 = 0.01f * p.R2;
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
-    const uint32_t n = F.nbrCount[i];
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t t = nbr(F, cap, i, j);
+    each_neighbor(F.nbrList, cap, i, F.nbrCount[i], [&](uint32_t t) {
         const f3 xij = xi - ld3(F.pos[t]);
         const f3 gw = gradW(k, xij);
         const float s = 10.0f * p.mu * (p.m / F.density[t]) * dot3(vi - ld3(in[t]), xij) / (length2(xij) + d001);
         acc = acc + s * gw;
-    }
+    });
     if (p.muB != 0.0f) {
         for (uint32_t b = 0; b < p.bodyCount; ++b) {
             const float vb = F.vb[(size_t)b * cap + i];
@@ -508,26 +565,37 @@ __device__ __forceinline__ f3 visc_matvec(const SceneParams& p, const KTab& k, c
 // SolveViscosity set-up (DFSPHImplementation.cu:627-691): r = b - A x0, ||b||^2, ||r||^2, p = M^-1 r, |r.p|
 __global__ void __launch_bounds__(kTile) k_cg_begin(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                    uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    const SceneParams& p = P[c.s];
-    if (!p.enableVisc) return;
-    SceneState& st = F.state[c.s];
-    double v[3] = {0.0, 0.0, 0.0};
-    if (c.valid) {
-        const f3 b = ld3(F.vel[c.i]);
-        const f3 ax = visc_matvec(p, ktab(kh), F, cap, c.i, F.cgX, st.dt);
-        const f3 r = b - ax;
-        const f3 z = minv_mul(F.minv, cap, c.i, r);
-        F.cgR[c.i] = st4(r);
-        F.cgP[c.i] = st4(z);
-        v[0] = (double)dot3(b, b);
-        v[1] = (double)dot3(r, r);
-        v[2] = (double)dot3(r, z);
-    }
+                                                    uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneParams& p = P[c.s];
+        if (!p.enableVisc) return;
+        const SceneState& st = F.state[c.s];
+        double v[3] = {0.0, 0.0, 0.0};
+        if (c.valid) {
+            const f3 b = ld3(F.vel[c.i]);
+            const f3 ax = visc_matvec(p, K, F, cap, c.i, F.cgX, st.dt);
+            const f3 r = b - ax;
+            const f3 z = minv_mul(F.minv, cap, c.i, r);
+            F.cgR[c.i] = st4(r);
+            F.cgP[c.i] = st4(z);
+            v[0] = (double)dot3(b, b);
+            v[1] = (double)dot3(r, r);
+            v[2] = (double)dot3(r, z);
+        }
+        tile_partials<3>(v, F.partials, tile);
+    });
+}
+
+__global__ void __launch_bounds__(kTile) k_cg_begin_finish(const SceneParams* __restrict__ P, Fields F) {
+    const uint32_t s = blockIdx.x;
+    const SceneParams& p = P[s];
+    if (!p.enableVisc || p.count == 0) return;
     double out[3];
-    if (tile_reduce_last<3>(v, F.partials, F.counters, c.s, p.tileBegin, p.tileEnd, out)) {
-        CgState& G = st.cg;
+    scene_sum<3>(F.partials, p.tileBegin, p.tileEnd, out);
+    if (threadIdx.x == 0) {
+        CgState& G = F.state[s].cg;
         G.it = 0;
         G.zero = 0;
         G.active = 0;
@@ -550,43 +618,63 @@ __global__ void __launch_bounds__(kTile) k_cg_begin(const SceneParams* __restric
 // CG matvec Ap and p.Ap -> alpha (:695-720)
 __global__ void __launch_bounds__(kTile) k_cg_matvec(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
-                                                     uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    SceneState& st = F.state[c.s];
-    if (!st.cg.active) return;
-    const SceneParams& p = P[c.s];
-    double v[1] = {0.0};
-    if (c.valid) {
-        const f3 ap = visc_matvec(p, ktab(kh), F, cap, c.i, F.cgP, st.dt);
-        F.cgAp[c.i] = st4(ap);
-        v[0] = (double)dot3(ld3(F.cgP[c.i]), ap);
-    }
+                                                     uint32_t cap, TileGrid tg) {
+    VFD_LDS_G_KERNEL_PROLOGUE(kh)
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneState& st = F.state[c.s];
+        if (!st.cg.active) return;
+        double v[1] = {0.0};
+        if (c.valid) {
+            const f3 ap = visc_matvec(P[c.s], K, F, cap, c.i, F.cgP, st.dt);
+            F.cgAp[c.i] = st4(ap);
+            v[0] = (double)dot3(ld3(F.cgP[c.i]), ap);
+        }
+        tile_partials<1>(v, F.partials, tile);
+    });
+}
+
+__global__ void __launch_bounds__(kTile) k_cg_alpha(const SceneParams* __restrict__ P, Fields F) {
+    const uint32_t s = blockIdx.x;
+    const SceneParams& p = P[s];
+    CgState& G = F.state[s].cg;
+    if (!G.active || p.count == 0) return;
     double out[1];
-    if (tile_reduce_last<1>(v, F.partials, F.counters, c.s, p.tileBegin, p.tileEnd, out))
-        st.cg.alpha = st.cg.absNew / (float)out[0];
+    scene_sum<1>(F.partials, p.tileBegin, p.tileEnd, out);
+    if (threadIdx.x == 0) G.alpha = G.absNew / (float)out[0];
 }
 
 // x += alpha p, r -= alpha Ap, ||r||^2, z = M^-1 r, |r.z| -> break test, beta (:722-804)
 __global__ void __launch_bounds__(kTile) k_cg_update(const SceneParams* __restrict__ P,
-                                                     const uint32_t* __restrict__ tileScene, Fields F, uint32_t cap) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    SceneState& st = F.state[c.s];
-    if (!st.cg.active) return;
-    const SceneParams& p = P[c.s];
-    double v[2] = {0.0, 0.0};
-    if (c.valid) {
-        const float alpha = st.cg.alpha;
-        F.cgX[c.i] = st4(ld3(F.cgX[c.i]) + ld3(F.cgP[c.i]) * alpha);
-        const f3 r = ld3(F.cgR[c.i]) - ld3(F.cgAp[c.i]) * alpha;
-        F.cgR[c.i] = st4(r);
-        const f3 z = minv_mul(F.minv, cap, c.i, r);
-        F.cgZ[c.i] = st4(z);
-        v[0] = (double)dot3(r, r);
-        v[1] = (double)dot3(r, z);
-    }
+                                                     const uint32_t* __restrict__ tileScene, Fields F, uint32_t cap,
+                                                     TileGrid tg) {
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneState& st = F.state[c.s];
+        if (!st.cg.active) return;
+        double v[2] = {0.0, 0.0};
+        if (c.valid) {
+            const float alpha = st.cg.alpha;
+            F.cgX[c.i] = st4(ld3(F.cgX[c.i]) + ld3(F.cgP[c.i]) * alpha);
+            const f3 r = ld3(F.cgR[c.i]) - ld3(F.cgAp[c.i]) * alpha;
+            F.cgR[c.i] = st4(r);
+            const f3 z = minv_mul(F.minv, cap, c.i, r);
+            F.cgZ[c.i] = st4(z);
+            v[0] = (double)dot3(r, r);
+            v[1] = (double)dot3(r, z);
+        }
+        tile_partials<2>(v, F.partials, tile);
+    });
+}
+
+__global__ void __launch_bounds__(kTile) k_cg_update_finish(const SceneParams* __restrict__ P, Fields F) {
+    const uint32_t s = blockIdx.x;
+    const SceneParams& p = P[s];
+    CgState& G = F.state[s].cg;
+    if (!G.active || p.count == 0) return;
     double out[2];
-    if (tile_reduce_last<2>(v, F.partials, F.counters, c.s, p.tileBegin, p.tileEnd, out)) {
-        CgState& G = st.cg;
+    scene_sum<2>(F.partials, p.tileBegin, p.tileEnd, out);
+    if (threadIdx.x == 0) {
         G.res2 = (float)out[0];
         G.err = sqrtf(G.res2 / G.rhs2);
         if (G.res2 < G.thr) {
@@ -603,47 +691,58 @@ __global__ void __launch_bounds__(kTile) k_cg_update(const SceneParams* __restri
 
 // p = p * beta + z (:788-802)
 __global__ void __launch_bounds__(kTile) k_cg_direction(const SceneParams* __restrict__ P,
-                                                        const uint32_t* __restrict__ tileScene, Fields F) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    const SceneState& st = F.state[c.s];
-    if (!st.cg.active || !c.valid) return;
-    F.cgP[c.i] = st4(ld3(F.cgP[c.i]) * st.cg.beta + ld3(F.cgZ[c.i]));
+                                                        const uint32_t* __restrict__ tileScene, Fields F,
+                                                        TileGrid tg) {
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneState& st = F.state[c.s];
+        if (!st.cg.active || !c.valid) return;
+        F.cgP[c.i] = st4(ld3(F.cgP[c.i]) * st.cg.beta + ld3(F.cgZ[c.i]));
+    });
 }
 
 // ApplyViscosityForceKernel (:844-862); a zero right-hand side zeroes the solution first (:654-659)
 __global__ void __launch_bounds__(kTile) k_visc_apply(const SceneParams* __restrict__ P,
-                                                      const uint32_t* __restrict__ tileScene, Fields F) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    const SceneParams& p = P[c.s];
-    if (!c.valid || !p.enableVisc) return;
-    const SceneState& st = F.state[c.s];
-    f3 x = ld3(F.cgX[c.i]);
-    if (st.cg.zero) {
-        x = mk3(0.0f, 0.0f, 0.0f);
-        F.cgX[c.i] = st4(x);
-    }
-    const f3 d = x - ld3(F.vel[c.i]);
-    F.acc[c.i] = st4(ld3(F.acc[c.i]) + (1.0f / st.dt) * d);
-    F.dv[c.i] = st4(d);
+                                                      const uint32_t* __restrict__ tileScene, Fields F, TileGrid tg) {
+    tile_loop(tg, [&](uint32_t tile) {
+        const TileCtx c = tile_ctx(tile, tileScene, P);
+        const SceneParams& p = P[c.s];
+        if (!c.valid || !p.enableVisc) return;
+        const SceneState& st = F.state[c.s];
+        f3 x = ld3(F.cgX[c.i]);
+        if (st.cg.zero) {
+            x = mk3(0.0f, 0.0f, 0.0f);
+            F.cgX[c.i] = st4(x);
+        }
+        const f3 d = x - ld3(F.vel[c.i]);
+        F.acc[c.i] = st4(ld3(F.acc[c.i]) + (1.0f / st.dt) * d);
+        F.dv[c.i] = st4(d);
+    });
 }
 
 void launch_visc_precond(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_visc_precond, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_visc_precond, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F,
+                       L.capacity, TileGrid(L.tg));
 }
 void launch_cg_begin(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_begin, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_cg_begin, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+                       TileGrid(L.tg));
+    hipLaunchKernelGGL(k_cg_begin_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_cg_matvec(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_matvec, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, kt, F, L.capacity);
+    hipLaunchKernelGGL(k_cg_matvec, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F,
+                       L.capacity, TileGrid(L.tg));
+    hipLaunchKernelGGL(k_cg_alpha, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_cg_update(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_update, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, L.capacity);
+    hipLaunchKernelGGL(k_cg_update, dim3(L.gridAll), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, L.capacity, TileGrid(L.tg));
+    hipLaunchKernelGGL(k_cg_update_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_cg_direction(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_direction, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, F);
+    hipLaunchKernelGGL(k_cg_direction, dim3(L.gridAll), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, TileGrid(L.tg));
 }
 void launch_visc_apply(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_visc_apply, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, F);
+    hipLaunchKernelGGL(k_visc_apply, dim3(L.gridAll), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, TileGrid(L.tg));
 }
 
 // ---- Monte-Carlo surface tension (SolveSurfaceTension, DFSPHImplementation.cu:810-839; Zorilla et al. 2020) -----
@@ -764,4 +863,12 @@ void launch_surface_tension(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_st_blend, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, F);
 }
 
+}  // namespace vfd
+
+namespace vfd {
+int configure_kernels() {
+    hipError_t e = hipFuncSetAttribute((const void*)k_density_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kLdsWG);
+    return e == hipSuccess ? 0 : 1;
+}
 }  // namespace vfd

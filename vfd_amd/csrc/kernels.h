@@ -7,7 +7,9 @@
 
 namespace vfd {
 
-// Launch geometry shared by per-particle kernels: one block of kTile threads per tile.
+struct TileGrid;
+
+// Launch geometry shared by per-particle kernels: kTile threads per block, one tile (kTile particles) at a time.
 struct Launch {
     hipStream_t stream;
     const SceneParams* P;        // device [sceneCount]
@@ -15,7 +17,15 @@ struct Launch {
     uint32_t tiles;
     uint32_t sceneCount;
     uint32_t capacity;           // tiles * kTile
+    uint32_t tilesPerXcd;        // ceil(tiles / 8)
+    uint32_t gridAll;            // 8 * tilesPerXcd: one block per tile, XCD-aware order
+    uint32_t grid4;              // persistent grid, 4 blocks per CU (40 KB LDS table)
+    uint32_t grid2;              // persistent grid, 2 blocks per CU (80 KB LDS table)
+    struct { uint32_t tiles, per; } tg;
 };
+
+// one-time kernel attributes (dynamic LDS above 64 KB)
+int configure_kernels();
 
 // Analytic shape + volume-map geometry of one boundary body (device copies live in an array indexed by
 // scene * kMaxBodies + body).
@@ -42,7 +52,7 @@ struct KTabHost {
 // ---- search.hip -----------------------------------------------------------------------------------------------
 uint32_t scan_scratch_words(uint32_t n);
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch, hipStream_t st);
-void launch_minmax(const Launch& L, const float4* pos, int* mm);
+void launch_minmax(const Launch& L, const float4* pos, int* part, int* mm);
 void launch_sort(const Launch& L, const SceneGrid* G, uint32_t totalCells, const float4* posIn, const float4* velIn,
                  const float4* dvIn, const float* dkIn, const uint32_t* idIn, float4* posOut, float4* velOut,
                  float4* dvOut, float* dkOut, uint32_t* idOut, uint32_t* cellOf, uint32_t* rank, uint32_t* tmpId,

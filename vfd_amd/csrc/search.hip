@@ -29,38 +29,62 @@ __device__ __forceinline__ uint32_t cell_key(const SceneGrid& g, uint32_t cx, ui
     return g.cellBase + lin * kMetaBlock + m;
 }
 
-__global__ void k_minmax_init(int* mm, uint32_t sceneCount) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= sceneCount) return;
+// ComputeMinMaxKernel (ParticleSearchKernels.cu:40-62): integer cell bounds floor(x / R). Each tile writes its
+// bounds to part[tile * 6 + a]; k_minmax_finish reduces a scene's tiles (no atomics on a single address).
+__global__ void __launch_bounds__(kTile) k_minmax(const float4* __restrict__ pos, const SceneParams* __restrict__ P,
+                                                  const uint32_t* __restrict__ tileScene, int* part) {
+    __shared__ int sh[kTile / kWave][6];
+    const TileCtx c = tile_ctx(tileScene, P);
+    const float R = P[c.s].R;
+    int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    if (c.valid) {
+        const float4 x = pos[c.i];
+        v[0] = v[3] = (int)floorf(x.x / R);
+        v[1] = v[4] = (int)floorf(x.y / R);
+        v[2] = v[5] = (int)floorf(x.z / R);
+    }
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
-        mm[6 * s + a] = INT_MAX;
-        mm[6 * s + 3 + a] = INT_MIN;
+        v[a] = wave_min_i(v[a]);
+        v[3 + a] = wave_max_i(v[3 + a]);
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int a = 0; a < 6; ++a) sh[w][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        int m = sh[0][a];
+        for (int j = 1; j < (int)(kTile / kWave); ++j) m = a < 3 ? min(m, sh[j][a]) : max(m, sh[j][a]);
+        part[(size_t)blockIdx.x * 6 + a] = m;
     }
 }
 
-// ComputeMinMaxKernel (ParticleSearchKernels.cu:40-62): integer cell bounds floor(x / R), one atomic per wave.
-__global__ void __launch_bounds__(kTile) k_minmax(const float4* __restrict__ pos, const SceneParams* __restrict__ P,
-                                                  const uint32_t* __restrict__ tileScene, int* mm) {
-    const TileCtx c = tile_ctx(tileScene, P);
-    const float R = P[c.s].R;
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    if (c.valid) {
-        const float4 x = pos[c.i];
-        lo[0] = hi[0] = (int)floorf(x.x / R);
-        lo[1] = hi[1] = (int)floorf(x.y / R);
-        lo[2] = hi[2] = (int)floorf(x.z / R);
-    }
+__global__ void __launch_bounds__(kTile) k_minmax_finish(const SceneParams* __restrict__ P,
+                                                         const int* __restrict__ part, int* mm) {
+    __shared__ int sh[kTile / kWave][6];
+    const uint32_t s = blockIdx.x;
+    const SceneParams& p = P[s];
+    int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    for (uint32_t t = p.tileBegin + threadIdx.x; t < p.tileEnd; t += kTile)
+#pragma unroll
+        for (int a = 0; a < 6; ++a) v[a] = a < 3 ? min(v[a], part[(size_t)t * 6 + a]) : max(v[a], part[(size_t)t * 6 + a]);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        lo[a] = wave_min_i(lo[a]);
-        hi[a] = wave_max_i(hi[a]);
+        v[a] = wave_min_i(v[a]);
+        v[3 + a] = wave_max_i(v[3 + a]);
     }
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    if (lane == 0)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (lo[a] != INT_MAX) atomicMin(&mm[6 * c.s + a], lo[a]);
-            if (hi[a] != INT_MIN) atomicMax(&mm[6 * c.s + 3 + a], hi[a]);
-        }
+        for (int a = 0; a < 6; ++a) sh[w][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        int m = sh[0][a];
+        for (int j = 1; j < (int)(kTile / kWave); ++j) m = a < 3 ? min(m, sh[j][a]) : max(m, sh[j][a]);
+        mm[6 * s + a] = m;
     }
 }
 
@@ -273,9 +297,9 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scr
 }
 
 // ---- host launchers ---------------------------------------------------------------------------------------------
-void launch_minmax(const Launch& L, const float4* pos, int* mm) {
-    hipLaunchKernelGGL(k_minmax_init, dim3((L.sceneCount + 63) / 64), dim3(64), 0, L.stream, mm, L.sceneCount);
-    hipLaunchKernelGGL(k_minmax, dim3(L.tiles), dim3(kTile), 0, L.stream, pos, L.P, L.tileScene, mm);
+void launch_minmax(const Launch& L, const float4* pos, int* part, int* mm) {
+    hipLaunchKernelGGL(k_minmax, dim3(L.tiles), dim3(kTile), 0, L.stream, pos, L.P, L.tileScene, part);
+    hipLaunchKernelGGL(k_minmax_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, (const int*)part, mm);
 }
 
 void launch_sort(const Launch& L, const SceneGrid* G, uint32_t totalCells, const float4* posIn, const float4* velIn,

@@ -316,7 +316,7 @@ public:
     DevBuf<SceneState> dState;
     DevBuf<SceneGrid> dGrid;
     DevBuf<MapDev> dMaps;
-    DevBuf<int> dMM;
+    DevBuf<int> dMM, dMMPart;
     DevBuf<float4> pos, posNext, vel[2], dv[2], acc, ap, nMC, nS, xb, cgX, cgR, cgP, cgZ, cgAp;
     DevBuf<float> dk[2], density, factor, densAdv, kappa, kappaV, residuum, kMC, kS, vb, minv;
     DevBuf<uint32_t> id[2], nbrCount, nbrList, cellOf, rank, tmpId, cellCount, cellStart, scanScratch, counters;
@@ -393,8 +393,11 @@ public:
             if (descs[i].ParticleRadius != descs[0].ParticleRadius)
                 return fail(VFD_ERROR_NOT_SUPPORTED, "all scenes of a batch must use the same ParticleRadius");
         }
-        int dev = 0;
+        int dev = 0, cus = 0;
         VFD_HIP(hipGetDevice(&dev));
+        VFD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        numCU = cus > 0 ? (uint32_t)cus : 256u;
+        if (configure_kernels() != 0) return fail(VFD_ERROR_HIP, "hipFuncSetAttribute (dynamic LDS) failed");
         for (uint32_t i = 0; i < n; ++i) {
             Scene* s = new Scene();
             s->desc = descs[i];
@@ -584,6 +587,7 @@ public:
         VFD_TRY(dGrid.alloc(S));
         VFD_TRY(dMaps.alloc((size_t)S * kMaxBodies));
         VFD_TRY(dMM.alloc((size_t)S * 6));
+        VFD_TRY(dMMPart.alloc((size_t)std::max<uint32_t>(tiles, 1) * 6));
         VFD_TRY(counters.alloc(S));
         VFD_TRY(dTileScene.alloc(std::max<uint32_t>(tiles, 1)));
         DevBuf<float4>* f4[] = {&pos, &posNext, &vel[0], &vel[1], &dv[0], &dv[1], &acc, &ap, &nMC, &nS,
@@ -661,8 +665,24 @@ public:
         return VFD_OK;
     }
 
+    uint32_t numCU = 256;
+
     Launch launch() const {
-        return Launch{stream, dP.p, dTileScene.p, tiles, (uint32_t)sc.size(), cap};
+        Launch L{};
+        L.stream = stream;
+        L.P = dP.p;
+        L.tileScene = dTileScene.p;
+        L.tiles = tiles;
+        L.sceneCount = (uint32_t)sc.size();
+        L.capacity = cap;
+        L.tilesPerXcd = (tiles + 7) / 8;
+        L.gridAll = 8 * L.tilesPerXcd;
+        const uint32_t p4 = std::max<uint32_t>(8, (numCU * 4) / 8 * 8), p2 = std::max<uint32_t>(8, (numCU * 2) / 8 * 8);
+        L.grid4 = std::min(L.gridAll, p4);
+        L.grid2 = std::min(L.gridAll, p2);
+        L.tg.tiles = tiles;
+        L.tg.per = L.tilesPerXcd;
+        return L;
     }
     Fields fields() {
         Fields F{};
@@ -706,7 +726,7 @@ public:
     int search() {
         const Launch L = launch();
         const uint32_t S = (uint32_t)sc.size();
-        VFD_TRY(timed("minmax", [&] { launch_minmax(L, posNext.p, dMM.p); }));
+        VFD_TRY(timed("minmax", [&] { launch_minmax(L, posNext.p, dMMPart.p, dMM.p); }));
         std::vector<int> mm((size_t)S * 6);
         VFD_HIP(hipMemcpyAsync(mm.data(), dMM.p, mm.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
         VFD_HIP(hipStreamSynchronize(stream));

@@ -9,6 +9,9 @@ and analytic boundary shapes instead; every choice the configuration text leaves
   - "dynamic viscosity mu" maps to Viscosity = mu / rho0 (the code multiplies by rho0, :347) and
     BoundaryViscosity = Viscosity;
   - a CG tolerance of 1e-3 is MaxViscositySolverError = 0.1 (threshold tol^2 * 1e-4 * |b|^2, :661);
+  - Jacobi tolerances are the description's own units (eta = tol * 1e-4 * rho0, :455, :526). Config 1 states
+    them (divergence 0.1, density 0.01); the other configurations state none and use the reference defaults (10 and
+    10, i.e. a mean density error below 0.1 %);
   - fluid blocks start one support radius (4r) away from walls: the reference's single-point boundary term
     V_b W(x - x_b) (DFSPHKernels.cu:176-184) gives a lattice layer at distance 2r from a wall about 2.5 rho0, which
     the first pressure solve turns into velocities of metres per second.
@@ -82,7 +85,7 @@ def viscous_coil_250k(seed=2, n=250_000):
         ParticleRadius=r, Viscosity=50.0 / RHO0, BoundaryViscosity=50.0 / RHO0, MaxViscositySolverError=0.1,
         EnableSurfaceTensionSolver=1, SurfaceTension=0.05, CSD=10000,
         MinDivergenceSolverIterations=2, MinPressureSolverIterations=2,
-        MaxDivergenceSolverError=0.1, MaxPressureSolverError=0.01, FrameCount=500)
+        MaxDivergenceSolverError=10.0, MaxPressureSolverError=10.0, FrameCount=500)
     plane = RigidBodyDescription("plane", center=(0, 0, 0), normal=(0, 1, 0), resolution=(24, 6, 24),
                                  domain=((-0.6, -0.12, -0.6), (0.6, 0.12, 0.6)))
     walls = RigidBodyDescription("box", center=(0, top / 2 - 0.05, 0), half_extents=(0.5, top / 2 + 0.05, 0.5),
@@ -98,7 +101,7 @@ def dam_break_2m(seed=3):
     r = 0.004
     desc = DFSPHSimulationDescription(
         ParticleRadius=r, Viscosity=0.01, BoundaryViscosity=0.01, MaxViscositySolverError=0.1,
-        EnableSurfaceTensionSolver=0, MaxDivergenceSolverError=0.1, MaxPressureSolverError=0.01,
+        EnableSurfaceTensionSolver=0, MaxDivergenceSolverError=10.0, MaxPressureSolverError=10.0,
         MinDivergenceSolverIterations=2, MinPressureSolverIterations=2, FrameCount=100)
     x = lattice_block((4 * r, 4 * r, 4 * r), (200, 100, 100), 2 * r, 0.01 * r, rng)
     v = np.zeros_like(x)
@@ -126,8 +129,8 @@ def batched_uneven(seed=4, scenes=64, r=0.01):
         x = lattice_block((2 * d, 2 * d, 2 * d), counts, d, 0.01 * r, rng)[:n]
         descs.append(DFSPHSimulationDescription(
             ParticleRadius=r, Viscosity=float(mus[s]) / RHO0, BoundaryViscosity=float(mus[s]) / RHO0,
-            EnableSurfaceTensionSolver=0, MaxViscositySolverError=0.1, MaxDivergenceSolverError=0.1,
-            MaxPressureSolverError=0.01, MinDivergenceSolverIterations=2, MinPressureSolverIterations=2,
+            EnableSurfaceTensionSolver=0, MaxViscositySolverError=0.1, MaxDivergenceSolverError=10.0,
+            MaxPressureSolverError=10.0, MinDivergenceSolverIterations=2, MinPressureSolverIterations=2,
             FrameCount=200))
         fluids.append((x, np.zeros_like(x)))
         w = side * d + 4 * d
@@ -152,7 +155,7 @@ def falling_viscous_block(seed=5, n_side=(50, 40, 50), mu=20.0):
     v *= 0.5 / max(np.abs(v).max(), 1e-12)
     desc = DFSPHSimulationDescription(
         ParticleRadius=r, Viscosity=mu / RHO0, BoundaryViscosity=mu / RHO0, EnableSurfaceTensionSolver=0,
-        MaxViscositySolverError=0.1, MaxDivergenceSolverError=0.1, MaxPressureSolverError=0.01,
+        MaxViscositySolverError=0.1, MaxDivergenceSolverError=10.0, MaxPressureSolverError=10.0,
         MinDivergenceSolverIterations=2, MinPressureSolverIterations=2, FrameCount=60)
     box = RigidBodyDescription("box", center=(1, 1, 1), half_extents=(1, 1, 1), inverted=True, resolution=(20, 20, 20))
     return [desc], [(x, v.astype(np.float32))], [[box]]
