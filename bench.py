@@ -2,18 +2,27 @@
 """Benchmark: DFSPH particle-updates/s on one MI355X (2M-particle dam break with a sphere obstacle).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config dam_break_2m] [--no-cpu-baseline]
+    python bench.py --copy-probe                      # plain 512 MiB device copies, for counter calibration
+    python bench.py --pmc-summary FETCH WRITE COPY_FETCH COPY_WRITE   # rocprofv3 --pmc output dirs -> traffic table
 
 Prints one JSON line. A step is one full DFSPH step (neighbour search, boundary, density/factor, divergence solve,
 viscosity solve, time step, pressure solve, integration) of every particle; particles stay resident in HBM and frame
 capture is off. `value` = particles * K / seconds over K timed steps (synchronised on both sides). `roofline` is
 computed for the kernel with the largest share of GPU time, from its algorithmic bytes per launch (bytes_per_particle
 below, with the measured mean neighbour count) and its mean duration measured with HIP events on the simulation's
-stream during the timed steps. `cpu_baseline` times the CPU oracle (FP32, OpenMP over the host cores) on the same
-scene for a few steps.
+stream during a replay of the timed steps from a snapshot of the state (the replay is deterministic and is checked to
+end in the identical state, so both passes do the same solver iterations). `roofline.traffic` is the HBM traffic per
+launch of that kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, when one exists for it.
+`cpu_baseline` times the CPU oracle (FP32, OpenMP over the host cores) from the same snapshot for as many of the
+timed steps as fit in --cpu-seconds, and reports the GPU rate over exactly those steps beside it; the GPU's first
+step from the snapshot is compared with the oracle's for a full-size parity check. The default K = 95 after W = 5
+warm-up steps covers the 100-step job.
 """
 import argparse
+import csv
 import json
 import os
+import re
 import sys
 import time
 
@@ -21,11 +30,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor peak
+COPY_PROBE_BYTES = 512 * 1024 * 1024
 
 
 def bytes_per_particle(kernel, nbar, bodies):
     """Compulsory HBM bytes per particle per launch (12 B per vec3, 4 B per scalar, 4 B per list entry + 8 B per
-    list header, 16 B per boundary body), following the per-kernel data each kernel must read and write."""
+    list header, 16 B per boundary body), following the per-kernel data each kernel must read and write. Neighbour
+    data re-read through the caches is not counted."""
     L = 4.0 * nbar + 8.0
     B = 16.0 * bodies
     table = {
@@ -53,9 +64,28 @@ def bytes_per_particle(kernel, nbar, bodies):
     return table.get(kernel)
 
 
-def cpu_baseline(cfg, steps, snap):
-    """The CPU oracle (FP32 mirror) advanced `steps` steps from the GPU's state `snap`, so both sides do the same
-    solver iterations; returns its throughput and its positions after the first step."""
+def snapshot(sim):
+    """Everything a step reads from the previous one: positions, velocities, the viscosity warm start, the surface
+    tension curvature blend and the time step."""
+    return {"x": sim.field("position"), "v": sim.field("velocity"), "dv": sim.field("velocity_difference"),
+            "dk": sim.field("delta_final_curvature"), "dt": sim.GetDebugInfo().TimeStepSize}
+
+
+def restore(sim, snap):
+    """Put a snapshot back; SetDescription resets the time step to desc.TimeStepSize as the reference's does."""
+    d = sim.GetDescription()
+    d.TimeStepSize = snap["dt"]
+    sim.SetDescription(d)
+    sim.set_field("position", snap["x"])
+    sim.set_field("velocity", snap["v"])
+    sim.set_field("velocity_difference", snap["dv"])
+    sim.set_field("delta_final_curvature", snap["dk"])
+
+
+def cpu_baseline(cfg, snap, max_steps, budget_s):
+    """The CPU oracle (FP32 mirror) advanced from the GPU's state `snap` until `budget_s` seconds have passed or
+    `max_steps` steps are done, so it meets the same solver iterations as the GPU's first timed steps; returns its
+    throughput and its positions after the first step."""
     from oracle.oracle import OracleSim, build_oracle
     from vfd_amd import scenes
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
@@ -71,34 +101,99 @@ def cpu_baseline(cfg, steps, snap):
     t0 = time.perf_counter()
     its = []
     x1 = None
-    for k in range(steps):
+    while len(its) < max_steps:
         o.step()
-        if k == 0:
+        if not its:
             x1 = o.field("position")
         d = o.debug_info()
         its.append((d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount))
+        if time.perf_counter() - t0 >= budget_s:
+            break
     dt = time.perf_counter() - t0
     n = len(snap["x"])
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n * steps / dt, "unit": "particle-updates/s", "cores": cores, "kind": "port",
-            "sample": "%s, %d particles, %d step(s) from the GPU state after the warm-up steps (setup excluded); "
-                      "solver iterations div/pres/visc per step %s" % (cfg, n, steps, its),
-            "seconds": dt, "x1": x1, "iterations": its}
+    return {"value": n * len(its) / dt, "unit": "particle-updates/s", "cores": cores, "kind": "port",
+            "sample": "%s, %d particles, the first %d timed step(s) from the GPU's state after the warm-up (setup "
+                      "excluded); solver iterations div/pres/visc per step %s" % (cfg, n, len(its), its),
+            "seconds": dt, "x1": x1, "iterations": its, "steps": len(its)}
+
+
+def _pmc_table(path):
+    """{kernel: {counter: mean value}} from a rocprofv3 --pmc counter_collection.csv (or the directory holding it)."""
+    if os.path.isdir(path):
+        found = [os.path.join(dp, f) for dp, _, fs in os.walk(path) for f in fs if f.endswith("counter_collection.csv")]
+        path = found[0]
+    acc = {}
+    for r in csv.DictReader(open(path)):
+        k = re.sub(r"^(?:void )?(?:vfd::)?", "", r["Kernel_Name"]).split("(")[0]
+        d = acc.setdefault(k, {}).setdefault(r["Counter_Name"], [])
+        d.append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in acc.items()}
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
+    """Markdown table of HBM traffic per launch. FETCH_SIZE/WRITE_SIZE (KiB per dispatch) are scaled by what the
+    same counters read for a device-to-device copy of COPY_PROBE_BYTES (`--copy-probe`)."""
+    cf, cw = _pmc_table(copy_fetch_dir), _pmc_table(copy_write_dir)
+    copy_k = [k for k in cf if "copyBuffer" in k][0]
+    f_scale = COPY_PROBE_BYTES / (cf[copy_k]["FETCH_SIZE"] * 1024.0)
+    w_scale = COPY_PROBE_BYTES / (cw[copy_k]["WRITE_SIZE"] * 1024.0)
+    fe, wr = _pmc_table(fetch_dir), _pmc_table(write_dir)
+    lines = ["calibration: %s of %d bytes reads FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB -> scale fetch x%.4f, "
+             "write x%.4f" % (copy_k, COPY_PROBE_BYTES, cf[copy_k]["FETCH_SIZE"], cw[copy_k]["WRITE_SIZE"], f_scale,
+                              w_scale), "",
+             "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes per launch (scaled) |", "|---|---:|---:|---:|"]
+    for k in sorted(fe):
+        if k in wr and k.startswith("k_"):
+            t = fe[k]["FETCH_SIZE"] * 1024.0 * f_scale + wr[k]["WRITE_SIZE"] * 1024.0 * w_scale
+            lines.append("| %s | %.1f | %.1f | %d |" % (k[2:], fe[k]["FETCH_SIZE"], wr[k]["WRITE_SIZE"], t))
+    return "\n".join(lines)
+
+
+def committed_traffic(config, kernel):
+    """Traffic bytes per launch of `kernel` from profiles/<config>_traffic.md (written from pmc_summary)."""
+    path = os.path.join(ROOT, "profiles", "%s_traffic.md" % config)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\|\s*%s\s*\|[^|]*\|[^|]*\|\s*(\d+)\s*\|" % re.escape(kernel), line)
+        if m:
+            return int(m.group(1))
+    return None
+
+
+def copy_probe():
+    import torch
+    n = COPY_PROBE_BYTES // 4
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("copy probe: 3 device-to-device copies of %d bytes" % COPY_PROBE_BYTES, flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=95)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="dam_break_2m")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the DFSPH step runs on a single MI355X (multi-GPU sharding is a design, "
               "not a measured path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_summary:
+        print(pmc_summary(*args.pmc_summary))
+        return
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import numpy as np
     from vfd_amd import DFSPHSimulation, scenes
@@ -111,14 +206,21 @@ def main():
     n = sim.GetParticleCount()
     print("[bench] %s: %d particles, warmup %d steps" % (args.config, n, args.warmup), file=sys.stderr, flush=True)
     sim.advance(args.warmup)
-    # state after warm-up (for the CPU baseline) and one untimed check step
-    snap = {"x": sim.field("position"), "v": sim.field("velocity"), "dv": sim.field("velocity_difference"),
-            "dk": sim.field("delta_final_curvature"), "dt": sim.GetDebugInfo().TimeStepSize}
-    sim.OnUpdate()
-    d = sim.GetDebugInfo()
-    check_its = (d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount)
-    x1_gpu = sim.field("position")
+    snap = snapshot(sim)
 
+    # headline: K plain steps (no per-kernel instrumentation); every step ends with a host synchronisation
+    step_s = []
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        sim.OnUpdate()
+        step_s.append(time.perf_counter() - ts)
+    elapsed_plain = time.perf_counter() - t1
+    x_end = sim.field("position")
+    print("[bench] timed %d steps: %.3f s" % (args.steps, elapsed_plain), file=sys.stderr, flush=True)
+
+    # the same K steps again from the same state, with HIP-event timing around every kernel launch
+    restore(sim, snap)
     sim.set_profiling(True)
     its = []
     nb = []
@@ -131,12 +233,16 @@ def main():
     elapsed = time.perf_counter() - t0
     prof = sim.profile()
     sim.set_profiling(False)
-    # the same steps once more without event instrumentation, for the headline number
-    t1 = time.perf_counter()
-    sim.advance(args.steps)
-    elapsed_plain = time.perf_counter() - t1
-    print("[bench] timed %d steps: %.3f s (profiled %.3f s)" % (args.steps, elapsed_plain, elapsed), file=sys.stderr,
+    replay_identical = bool(np.array_equal(x_end, sim.field("position")))
+    print("[bench] profiled replay %.3f s, identical end state: %s" % (elapsed, replay_identical), file=sys.stderr,
           flush=True)
+
+    # one untimed check step from the snapshot, compared with the CPU oracle's step from the same state below
+    restore(sim, snap)
+    sim.OnUpdate()
+    d = sim.GetDebugInfo()
+    check_its = (d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount)
+    x1_gpu = sim.field("position")
 
     nbar = float(np.mean(nb))
     nbody = len(bodies[0])
@@ -172,14 +278,18 @@ def main():
                    "mean_iterations": {"divergence": round(it[0], 2), "pressure": round(it[1], 2),
                                        "viscosity_cg": round(it[2], 2)}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": committed_traffic(args.config, dom),
+                     "bytes_per_launch": single[dom]["bytes_per_launch"], "mean_us": single[dom]["mean_us"]},
         "kernels": kernels,
+        "profiled_replay_identical": replay_identical,
         "gpu_kernel_ms_per_step": round(total_gpu_ms / args.steps, 3),
+        "iterations_per_step": its,
     }
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline: oracle, %d steps" % args.cpu_steps, file=sys.stderr, flush=True)
-        cb = cpu_baseline(args.config, args.cpu_steps, snap)
+        print("[bench] cpu baseline: oracle, up to %.0f s" % args.cpu_seconds, file=sys.stderr, flush=True)
+        cb = cpu_baseline(args.config, snap, args.steps, args.cpu_seconds)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"]["gpu_value_same_steps"] = n * cb["steps"] / sum(step_s[:cb["steps"]])
         r = descs[0].ParticleRadius
         out["parity_full_size"] = {
             "check": "one step from the same state on GPU and CPU oracle",

@@ -133,6 +133,9 @@ typedef struct vfd_debug_info {
     float MaxVelocityMagnitude;
     uint32_t MaxNeighborCount;
     uint64_t TotalNeighborCount;
+    uint32_t StagedGroupCount;   /* neighbour groups of the whole simulation whose halo was staged in LDS */
+    uint32_t GatheredGroupCount; /* groups whose halo exceeded the limit (neighbour data gathered from HBM) */
+    uint32_t MaxHaloSize;        /* distinct neighbours of the largest staged group */
 } vfd_debug_info;
 
 /* Boundary shapes. The reference builds its volume map from a triangle mesh (RigidBodyDescription,
@@ -293,6 +296,16 @@ typedef struct vfd_kernel_stat {
 } vfd_kernel_stat;
 int vfd_set_profiling(vfd_sim* sim, int32_t enabled);
 int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint32_t* count);
+
+/* Launch and solver-control knobs; no counterpart in the reference (which synchronises after every iteration).
+ * Results do not depend on them: a converged scene's extra iterations are skipped on the device, and a staged halo
+ * hands the same values to the same arithmetic as a gather from HBM. */
+typedef struct vfd_tuning {
+    uint32_t SolverCheckInterval; /* Jacobi / CG iterations enqueued between two convergence checks, >= 1 */
+    uint32_t HaloLimit;           /* largest group halo staged in LDS (<= 3808); 0: every neighbour from HBM */
+} vfd_tuning;
+int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out);
+int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* tuning);
 
 /* PrecomputedDFSPHCubicKernel (Kernel/DFSPHKernels.h:7-143): W[10000], GradientW[10001] and
  * {Radius, Radius2, InvStepSize, WZero, K, L}. */

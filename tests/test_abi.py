@@ -83,3 +83,9 @@ def test_product_never_imports_the_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 text = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "oracle" not in re.sub(r"(#|//).*", "", text).lower() or f == "__init__.py", f
+
+
+def test_tuning_struct_and_defaults_without_gpu(vfd_lib):
+    t = _abi.Tuning()
+    assert vfd_lib.vfd_get_tuning(None, ctypes.byref(t)) == 1  # NULL handle rejected before any GPU call
+    assert b"NULL" in vfd_lib.vfd_last_error()

@@ -211,3 +211,32 @@ def test_determinism():
         b.OnUpdate()
     assert np.array_equal(a.field("position"), b.field("position"))
     assert np.array_equal(a.field("velocity"), b.field("velocity"))
+
+
+@pytest.mark.parametrize("kind", ["dam", "all"])
+def test_tuning_never_changes_results(kind):
+    """Halo staging (all groups, a limit that splits them, none) and the convergence-check interval change how the
+    step runs, not what it computes: every combination ends bit-identical after three steps."""
+    scene = small_scene(kind)
+    ref = make_gpu(*scene)
+    for _ in range(3):
+        ref.OnUpdate()
+    d = ref.GetDebugInfo()
+    assert d.StagedGroupCount > 0 and d.GatheredGroupCount == 0
+    split = max(1, d.MaxHaloSize - 1)
+    for knobs in [dict(HaloLimit=0), dict(HaloLimit=split), dict(SolverCheckInterval=1),
+                  dict(SolverCheckInterval=7, HaloLimit=0)]:
+        sim = make_gpu(*scene)
+        sim.set_tuning(**knobs)
+        for _ in range(3):
+            sim.OnUpdate()
+        e = sim.GetDebugInfo()
+        if knobs.get("HaloLimit") == 0:
+            assert e.StagedGroupCount == 0 and e.GatheredGroupCount > 0
+        if knobs.get("HaloLimit") == split:
+            assert e.StagedGroupCount > 0 and e.GatheredGroupCount > 0
+        for f in ["position", "velocity", "density", "factor", "pressure_rho2", "viscosity_solution"]:
+            assert np.array_equal(sim.field(f), ref.field(f)), (knobs, f)
+        assert (e.DivergenceSolverIterationCount, e.PressureSolverIterationCount, e.ViscositySolverIterationCount) \
+            == (d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount)
+        print(kind, knobs, "staged", e.StagedGroupCount, "gathered", e.GatheredGroupCount, flush=True)

@@ -108,7 +108,7 @@ class Info(ctypes.Structure):
 
 
 class DebugInfo(ctypes.Structure):
-    """vfd_debug_info == DFSPHDebugInfo (Structures/DFSPHDebugInfo.h:8-28) plus four additions."""
+    """vfd_debug_info == DFSPHDebugInfo (Structures/DFSPHDebugInfo.h:8-28) plus seven additions."""
     _fields_ = [
         ("IterationCount", ctypes.c_uint32), ("DivergenceSolverIterationCount", ctypes.c_uint32),
         ("PressureSolverIterationCount", ctypes.c_uint32), ("ViscositySolverIterationCount", ctypes.c_uint32),
@@ -119,6 +119,8 @@ class DebugInfo(ctypes.Structure):
         ("ViscositySolverTime", ctypes.c_float), ("PressureSolverTime", ctypes.c_float),
         ("TimeStepSize", ctypes.c_float), ("MaxVelocityMagnitude", ctypes.c_float),
         ("MaxNeighborCount", ctypes.c_uint32), ("TotalNeighborCount", ctypes.c_uint64),
+        ("StagedGroupCount", ctypes.c_uint32), ("GatheredGroupCount", ctypes.c_uint32),
+        ("MaxHaloSize", ctypes.c_uint32),
     ]
 
     def as_dict(self):
@@ -158,6 +160,11 @@ class KernelStat(ctypes.Structure):
     """vfd_kernel_stat: per-kernel launch count and summed HIP-event time."""
     _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint32), ("pad", ctypes.c_uint32),
                 ("total_ms", ctypes.c_double)]
+
+
+class Tuning(ctypes.Structure):
+    """vfd_tuning: launch and solver-control knobs (no reference counterpart)."""
+    _fields_ = [("SolverCheckInterval", ctypes.c_uint32), ("HaloLimit", ctypes.c_uint32)]
 
 
 _P = ctypes.c_void_p
@@ -201,6 +208,8 @@ _PROTOS = {
     "vfd_get_kernel_table": (ctypes.c_int, [_P, _P, _P, _P]),
     "vfd_set_profiling": (ctypes.c_int, [_P, _I32]),
     "vfd_get_profile": (ctypes.c_int, [_P, ctypes.POINTER(KernelStat), _U32, ctypes.POINTER(_U32)]),
+    "vfd_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "vfd_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
 }
 
 EXPORTED = sorted(_PROTOS)

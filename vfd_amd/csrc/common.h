@@ -5,7 +5,9 @@
 //   positions / velocities / vectors: float4 per particle (w unused), one 16-byte load per neighbour gather
 //   scalars: float per particle
 //   neighbour lists: uint32 slot indices, transposed list[j * capacity + i] so that the j-th neighbour of 64
-//   consecutive particles is one coalesced 256-byte read; at most kMaxNeighbors per particle.
+//   consecutive particles is one coalesced 256-byte read; at most kMaxNeighbors per particle; for the particles of a
+//   staged group also as uint16 halo slots in the same transposed layout;
+//   halos: per group, haloCount[g] and the halo's particle slots haloIds[g * kHaloMax + k].
 //
 // All HIP sources are compiled with -ffp-contract=off: every fused multiply-add is an explicit fmaf, so the
 // arithmetic that decides discrete outcomes (neighbour membership, kernel-table index, cell index) is reproducible on
@@ -31,6 +33,15 @@ constexpr unsigned kTableEntries = kTableResolution - 1;  // averaged entries 0.
 constexpr unsigned kHaltonFloats = 49152;    // HaltonVec323.cuh
 constexpr unsigned kMaxBodies = 4;           // boundary bodies per scene
 constexpr double kEps = 1.0e-5;              // Core/Math/Math.h:6 (double literal)
+
+// Neighbour groups: kGroupTiles consecutive tiles (one 1024-thread block) whose distinct neighbours - the halo - are
+// staged in LDS by the neighbour-loop kernels. A group's halo holds at most kHaloMax particles; a group with more
+// keeps gathering from the global arrays (haloCount = kHaloNone). The lists of a staged group are also stored as
+// 16-bit halo slots.
+constexpr unsigned kGroupTiles = 4;
+constexpr unsigned kGroupSize = kTile * kGroupTiles;
+constexpr unsigned kHaloMax = 3808;
+constexpr uint32_t kHaloNone = 0xFFFFFFFFu;
 
 // Constant per-scene parameters (DFSPHSimulationInfo + DFSPHSimulationDescription, DFSPHImplementation.cu:318-368).
 struct SceneParams {
@@ -69,7 +80,7 @@ struct JacobiState {
 
 // PCG control (DFSPHImplementation.cu:617-808).
 struct CgState {
-    uint32_t active, it, zero, pad0;
+    uint32_t active, it, zero, loops;  // loops: iterations that ran (profiling attribution)
     float rhs2, thr, res2, absNew;
     float alpha, beta, err, pad1;
 };
@@ -85,7 +96,7 @@ struct SceneState {
     CgState cg;
 };
 
-// Reduction counters / partials: per-tile partial sums (double), per-scene tile counters.
+// Reduction partials: per-tile partial sums (double), kPartialSlots values per tile.
 constexpr unsigned kPartialSlots = 4;  // values reduced at once
 
 }  // namespace vfd

@@ -194,6 +194,40 @@ __device__ __forceinline__ void tile_loop(const TileGrid& tg, Body&& body) {
     }
 }
 
+// ---- neighbour groups (kGroupTiles tiles per kGroupSize-thread block) ---------------------------------------------
+// Thread group sub = threadIdx.x / kTile of a block works on tile kGroupTiles * g + sub of group g.
+__device__ __forceinline__ TileCtx tile_ctx4(uint32_t tile, bool ok, const uint32_t* tileScene,
+                                             const SceneParams* P) {
+    TileCtx c;
+    c.s = ok ? tileScene[tile] : 0u;
+    c.i = tile * kTile + (threadIdx.x & (kTile - 1));
+    c.valid = ok && c.i < P[c.s].base + P[c.s].count;
+    return c;
+}
+// Per-tile sums inside a kGroupSize-thread block: wave sums, then the tile's four waves in order (the same association
+// as block_sum), stored by the tile's first thread when `store`. Called by every thread.
+template <int NV>
+__device__ __forceinline__ void tile_partials4(double (&v)[NV], double* partials, uint32_t tile, bool store) {
+    __shared__ double sh[kGroupSize / kWave][NV];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sh[w][i] = v[i];
+    __syncthreads();
+    if ((threadIdx.x & (kTile - 1)) == 0 && store) {
+        const int w0 = (threadIdx.x / kTile) * (kTile / kWave);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = sh[w0][i];
+            for (int j = 1; j < (int)(kTile / kWave); ++j) s += sh[w0 + j][i];
+            partials[(size_t)tile * kPartialSlots + i] = s;
+        }
+    }
+    __syncthreads();
+}
+
 // Neighbour j of particle i in the transposed lists; f(t) is called in list order, four list entries fetched per
 // trip so that their gathers can be in flight together.
 template <class Fn>
@@ -211,9 +245,10 @@ __device__ __forceinline__ void each_neighbor(const uint32_t* __restrict__ list,
     for (; j < n; ++j) f(list[(size_t)j * cap + i]);
 }
 
-// Kernel table staged in LDS: tab[p] (float2 {W, G}) or G only, loaded once per block.
-constexpr unsigned kLdsG = kTableEntries * sizeof(float);
-constexpr unsigned kLdsWG = kTableEntries * sizeof(float2);
+// Kernel table staged in LDS: tab[p] (float2 {W, G}) or G only, loaded once per block. The sizes are rounded up to
+// 16 bytes so that the halo records can follow the table.
+constexpr unsigned kLdsG = (kTableEntries * sizeof(float) + 15u) / 16u * 16u;
+constexpr unsigned kLdsWG = (kTableEntries * sizeof(float2) + 15u) / 16u * 16u;
 __device__ __forceinline__ void load_table_g(float* sG, const float2* __restrict__ tab) {
     for (uint32_t p = threadIdx.x; p < kTableEntries; p += blockDim.x) sG[p] = tab[p].y;
     __syncthreads();
@@ -235,6 +270,138 @@ __device__ __forceinline__ f3 gradW(const KLds& k, f3 r) {
         return k.sG[p] * r;
     }
     return mk3(0.0f, 0.0f, 0.0f);
+}
+
+// ---- neighbour sources: halo records in LDS or gathers from the global arrays -----------------------------------
+// A neighbour-loop kernel sees each neighbour t of particle i as one or two float4 records {lo, hi} that a loader
+// assembles from the global arrays (e.g. lo = {x_t, kappa_t}, hi = {a_t}). For a staged group the records of its
+// halo are copied to LDS once and the lists are read as 16-bit halo slots; otherwise the loader gathers them per
+// list entry. Both paths hand the same values to the same arithmetic in the same list order.
+struct NbrSrc {
+    const float4* rec;                     // LDS records, R per halo slot; nullptr: global gathers
+    const uint16_t* __restrict__ local;    // [j * cap + i] halo slots (staged groups)
+    const uint32_t* __restrict__ list;     // [j * cap + i] particle slots
+    const uint32_t* __restrict__ count;    // [i]
+    uint32_t cap;
+};
+
+struct LoadX {  // lo = hi = {x_t, -}
+    const float4* __restrict__ pos;
+    __device__ __forceinline__ void operator()(uint32_t t, float4& lo, float4& hi) const {
+        lo = pos[t];
+        hi = lo;
+    }
+};
+struct LoadXS {  // lo = hi = {x_t, s_t}
+    const float4* __restrict__ pos;
+    const float* __restrict__ s;
+    __device__ __forceinline__ void operator()(uint32_t t, float4& lo, float4& hi) const {
+        const float4 x = pos[t];
+        lo = make_float4(x.x, x.y, x.z, s[t]);
+        hi = lo;
+    }
+};
+struct LoadXV {  // lo = {x_t, -}, hi = {v_t, -}
+    const float4* __restrict__ pos;
+    const float4* __restrict__ v;
+    __device__ __forceinline__ void operator()(uint32_t t, float4& lo, float4& hi) const {
+        lo = pos[t];
+        hi = v[t];
+    }
+};
+struct LoadXSV {  // lo = {x_t, s_t}, hi = {v_t, -}
+    const float4* __restrict__ pos;
+    const float* __restrict__ s;
+    const float4* __restrict__ v;
+    __device__ __forceinline__ void operator()(uint32_t t, float4& lo, float4& hi) const {
+        const float4 x = pos[t];
+        lo = make_float4(x.x, x.y, x.z, s[t]);
+        hi = v[t];
+    }
+};
+
+// f(lo, hi) for every neighbour of particle i in list order, four list entries fetched per trip.
+template <int R, class Load, class Fn>
+__device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i, const Load& load, Fn&& f) {
+    const uint32_t n = src.count[i];
+    const size_t cap = src.cap;
+    uint32_t j = 0;
+    if (src.rec) {
+        const float4* rec = src.rec;
+        for (; j + 4 <= n; j += 4) {
+            uint32_t s[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s[u] = src.local[(j + u) * cap + i];
+            float4 lo[4], hi[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                lo[u] = rec[R * s[u]];
+                hi[u] = R == 2 ? rec[R * s[u] + 1] : lo[u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) f(lo[u], hi[u]);
+        }
+        for (; j < n; ++j) {
+            const uint32_t s = src.local[j * cap + i];
+            const float4 lo = rec[R * s];
+            const float4 hi = R == 2 ? rec[R * s + 1] : lo;
+            f(lo, hi);
+        }
+    } else {
+        for (; j + 4 <= n; j += 4) {
+            uint32_t t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) t[u] = src.list[(j + u) * cap + i];
+            float4 lo[4], hi[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) load(t[u], lo[u], hi[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) f(lo[u], hi[u]);
+        }
+        for (; j < n; ++j) {
+            float4 lo, hi;
+            load(src.list[j * cap + i], lo, hi);
+            f(lo, hi);
+        }
+    }
+}
+
+struct HaloView {
+    const uint32_t* __restrict__ count;  // [groups]
+    const uint32_t* __restrict__ ids;    // [groups][kHaloMax]
+};
+
+// Persistent, XCD-aware loop over groups: XCD x (= block index mod 8) takes groups [x * per/4, (x + 1) * per/4) of
+// the tiles it owns, so that its L2 holds the data of one contiguous stretch of the Morton order. For a staged group
+// every thread first copies halo records to LDS; body(tile, ok, src) is then called by every thread of the block
+// (ok = false past the last tile) and may contain block barriers.
+template <int R, class Load, class Body>
+__device__ __forceinline__ void group_loop(const TileGrid& tg, const HaloView& H, float4* rec, const NbrSrc& global,
+                                           const Load& load, Body&& body) {
+    const uint32_t sub = threadIdx.x / kTile;
+    const uint32_t perG = tg.per / kGroupTiles;
+    for (uint32_t q = blockIdx.x; q < 8u * perG; q += gridDim.x) {
+        const uint32_t g = (q & 7u) * perG + (q >> 3);
+        if (g * kGroupTiles >= tg.tiles) continue;  // block-uniform
+        const uint32_t tile = g * kGroupTiles + sub;
+        const uint32_t m = H.count[g];
+        if (m <= kHaloMax) {
+            const uint32_t* ids = H.ids + (size_t)g * kHaloMax;
+            for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+                float4 lo, hi;
+                load(ids[k], lo, hi);
+                rec[R * k] = lo;
+                if (R == 2) rec[R * k + 1] = hi;
+            }
+            __syncthreads();
+            NbrSrc src = global;
+            src.rec = rec;
+            body(tile, tile < tg.tiles, src);
+            __syncthreads();
+        } else {
+            body(tile, tile < tg.tiles, global);
+        }
+    }
 }
 
 }  // namespace vfd

@@ -19,17 +19,23 @@ __device__ __forceinline__ KTab ktab(const KTabHost& h) { return KTab{h.tab, h.R
 __device__ __forceinline__ uint32_t nbr(const Fields& F, uint32_t cap, uint32_t i, uint32_t j) {
     return F.nbrList[(size_t)j * cap + i];
 }
+__device__ __forceinline__ HaloView halo(const Fields& F) { return HaloView{F.haloCount, F.haloIds}; }
+__device__ __forceinline__ NbrSrc global_src(const Fields& F, uint32_t cap) {
+    return NbrSrc{nullptr, F.nbrLocal, F.nbrList, F.nbrCount, cap};
+}
 
 // ---- ComputeDensityKernel (DFSPHKernels.cu:152-187) + ComputeDFSPHFactorKernel (:189-231) ----------------------
 // One table fetch per pair gives W (cutoff r^2 <= R^2) and gradW (cutoff |r| <= R); the {W, G} table sits in LDS.
-__global__ void __launch_bounds__(kTile) k_density_factor(const SceneParams* __restrict__ P,
-                                                          const uint32_t* __restrict__ tileScene, KTabHost kh,
-                                                          Fields F, uint32_t cap, TileGrid tg) {
-    extern __shared__ float2 sWG[];
+__global__ void __launch_bounds__(kGroupSize) k_density_factor(const SceneParams* __restrict__ P,
+                                                           const uint32_t* __restrict__ tileScene, KTabHost kh,
+                                                           Fields F, uint32_t cap, TileGrid tg) {
+    extern __shared__ float4 lds[];
+    float2* sWG = reinterpret_cast<float2*>(lds);
     load_table_wg(sWG, kh.tab);
     const KTab k{sWG, kh.R, kh.R2, kh.invStep, kh.W0};
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    float4* rec = lds + kLdsWG / sizeof(float4);
+    group_loop<1>(tg, halo(F), rec, global_src(F, cap), LoadX{F.pos}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         if (!c.valid) return;
         const SceneParams& p = P[c.s];
         const f3 xi = ld3(F.pos[c.i]);
@@ -37,8 +43,8 @@ __global__ void __launch_bounds__(kTile) k_density_factor(const SceneParams* __r
         float dens = V * k.W0;
         f3 gi = mk3(0.0f, 0.0f, 0.0f);
         float sum = 0.0f;
-        each_neighbor(F.nbrList, cap, c.i, F.nbrCount[c.i], [&](uint32_t t) {
-            const f3 r = xi - ld3(F.pos[t]);
+        each_neighbor_rec<1>(src, c.i, LoadX{F.pos}, [&](float4 lo, float4) {
+            const f3 r = xi - ld3(lo);
             const Pair q = pair_lookup(k, r);
             dens += V * q.W;
             const f3 gj = (-V) * (q.G * r);
@@ -61,18 +67,19 @@ __global__ void __launch_bounds__(kTile) k_density_factor(const SceneParams* __r
 }
 
 void launch_density_factor(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_density_factor, dim3(L.grid2), dim3(kTile), kLdsWG, L.stream, L.P, L.tileScene, kt, F,
-                       L.capacity, TileGrid(L.tg));
+    hipLaunchKernelGGL(k_density_factor, dim3(L.gridGroups), dim3(kGroupSize), kLdsWG + kHaloMax * sizeof(float4),
+                       L.stream, L.P, L.tileScene, kt, F, L.capacity, TileGrid(L.tg));
 }
 
 // ---- shared neighbour sums --------------------------------------------------------------------------------------
-// v_i-weighted divergence sum_j (v_i - v_j).gradW_ij * V + sum_b V_b v_i.gradW_ib (:233-278, :483-529)
+// v_i-weighted divergence sum_j (v_i - v_j).gradW_ij * V + sum_b V_b v_i.gradW_ib (:233-278, :483-529);
+// records lo = {x_j}, hi = {v_j}
 __device__ __forceinline__ float velocity_divergence(const SceneParams& p, const KLds& k, const Fields& F,
-                                                     uint32_t cap, uint32_t i) {
+                                                     uint32_t cap, uint32_t i, const NbrSrc& src) {
     const f3 xi = ld3(F.pos[i]), vi = ld3(F.vel[i]);
     float d = 0.0f;
-    each_neighbor(F.nbrList, cap, i, F.nbrCount[i], [&](uint32_t t) {
-        d += dot3(vi - ld3(F.vel[t]), gradW(k, xi - ld3(F.pos[t])));
+    each_neighbor_rec<2>(src, i, LoadXV{F.pos, F.vel}, [&](float4 lo, float4 hi) {
+        d += dot3(vi - ld3(hi), gradW(k, xi - ld3(lo)));
     });
     d *= p.V;
     for (uint32_t b = 0; b < p.bodyCount; ++b) {
@@ -82,16 +89,17 @@ __device__ __forceinline__ float velocity_divergence(const SceneParams& p, const
     return d;
 }
 
-// pressure acceleration from kappa (ComputePressureAccelerationKernel :342-386 and its three siblings)
+// pressure acceleration from kappa (ComputePressureAccelerationKernel :342-386 and its three siblings);
+// records lo = {x_j, kappa_j}
 __device__ __forceinline__ f3 pressure_accel(const SceneParams& p, const KLds& k, const Fields& F, uint32_t cap,
-                                             uint32_t i, const float* __restrict__ kap) {
+                                             uint32_t i, const float* __restrict__ kap, const NbrSrc& src) {
     const f3 xi = ld3(F.pos[i]);
     const float ki = kap[i];
     f3 a = mk3(0.0f, 0.0f, 0.0f);
-    each_neighbor(F.nbrList, cap, i, F.nbrCount[i], [&](uint32_t t) {
-        const float s = ki + kap[t];
+    each_neighbor_rec<1>(src, i, LoadXS{F.pos, kap}, [&](float4 lo, float4) {
+        const float s = ki + lo.w;
         if ((double)fabsf(s) > kEps) {
-            const f3 gj = (-p.V) * gradW(k, xi - ld3(F.pos[t]));
+            const f3 gj = (-p.V) * gradW(k, xi - ld3(lo));
             a = a + s * gj;
         }
     });
@@ -107,13 +115,13 @@ __device__ __forceinline__ f3 pressure_accel(const SceneParams& p, const KLds& k
     return a;
 }
 
-// ComputeDensityPressureForce (:280-315)
+// ComputeDensityPressureForce (:280-315); records lo = {x_j}, hi = {a_j}
 __device__ __forceinline__ float density_pressure_force(const SceneParams& p, const KLds& k, const Fields& F,
-                                                        uint32_t cap, uint32_t i) {
+                                                        uint32_t cap, uint32_t i, const NbrSrc& src) {
     const f3 xi = ld3(F.pos[i]), ai = ld3(F.ap[i]);
     float f = 0.0f;
-    each_neighbor(F.nbrList, cap, i, F.nbrCount[i], [&](uint32_t t) {
-        f += dot3(ai - ld3(F.ap[t]), gradW(k, xi - ld3(F.pos[t])));
+    each_neighbor_rec<2>(src, i, LoadXV{F.pos, F.ap}, [&](float4 lo, float4 hi) {
+        f += dot3(ai - ld3(hi), gradW(k, xi - ld3(lo)));
     });
     f *= p.V;
     for (uint32_t b = 0; b < p.bodyCount; ++b) {
@@ -123,28 +131,42 @@ __device__ __forceinline__ float density_pressure_force(const SceneParams& p, co
     return f;
 }
 
-// Every gradient-loop kernel stages the G table in LDS and runs the XCD-aware persistent tile loop.
-#define VFD_LDS_G_KERNEL_PROLOGUE(kh)          \
-    extern __shared__ float sG[];              \
-    load_table_g(sG, (kh).tab);                \
-    const KLds K{sG, (kh).R, (kh).invStep};
+// Every gradient-loop kernel stages the G table in LDS, followed by the halo records of the current group.
+#define VFD_HALO_KERNEL_PROLOGUE(kh)                                   \
+    extern __shared__ float4 lds[];                                    \
+    float* sG = reinterpret_cast<float*>(lds);                         \
+    load_table_g(sG, (kh).tab);                                        \
+    const KLds K{sG, (kh).R, (kh).invStep};                            \
+    float4* rec = lds + kLdsG / sizeof(float4);
 
 // ---- Jacobi solver control (DFSPHImplementation.cu:455-461, 526-532) -------------------------------------------
 // which = 0: divergence (eta = (1/dt) tol 1e-4 rho0), 1: pressure (eta = tol 1e-4 rho0)
-__global__ void k_jacobi_begin(const SceneParams* __restrict__ P, SceneState* state, uint32_t sceneCount, int which) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= sceneCount) return;
-    const SceneParams& p = P[s];
-    SceneState& st = state[s];
-    JacobiState& J = which == 0 ? st.div : st.pres;
-    J.it = 0;
-    J.err = 0.0f;
-    if (which == 0) {
-        J.eta = st.dtInv * p.divTol * 0.0001f * p.rho0;
-        J.active = p.enableDiv && ((J.err > J.eta || 0u < p.minDiv) && 0u < p.maxDiv) ? 1u : 0u;
-    } else {
-        J.eta = p.presTol * 0.0001f * p.rho0;
-        J.active = ((J.err > J.eta || 0u < p.minPres) && 0u < p.maxPres) ? 1u : 0u;
+// One block; also counts the scenes that iterate into live[which] (and clears live[2], the viscosity count, at the
+// start of a step).
+__global__ void __launch_bounds__(kTile) k_jacobi_begin(const SceneParams* __restrict__ P, SceneState* state,
+                                                        uint32_t* live, uint32_t sceneCount, int which) {
+    __shared__ uint32_t count;
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < sceneCount; s += blockDim.x) {
+        const SceneParams& p = P[s];
+        SceneState& st = state[s];
+        JacobiState& J = which == 0 ? st.div : st.pres;
+        J.it = 0;
+        J.err = 0.0f;
+        if (which == 0) {
+            J.eta = st.dtInv * p.divTol * 0.0001f * p.rho0;
+            J.active = p.enableDiv && ((J.err > J.eta || 0u < p.minDiv) && 0u < p.maxDiv) ? 1u : 0u;
+        } else {
+            J.eta = p.presTol * 0.0001f * p.rho0;
+            J.active = ((J.err > J.eta || 0u < p.minPres) && 0u < p.maxPres) ? 1u : 0u;
+        }
+        if (J.active && p.count) atomicAdd(&count, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        live[which] = count;
+        if (which == 0) live[2] = 0u;
     }
 }
 
@@ -164,20 +186,21 @@ __global__ void __launch_bounds__(kTile) k_jacobi_finish(const SceneParams* __re
         J.err = err;
         J.it = it;
         J.active = ((err > J.eta || it < minIt) && it < maxIt) ? 1u : 0u;
+        if (!J.active) atomicSub(&F.live[which], 1u);
     }
 }
 
 // ---- divergence solve (ComputeDivergence, DFSPHImplementation.cu:506-575) ---------------------------------------
 // ComputeDensityChangeKernel (DFSPHKernels.cu:483-529)
-__global__ void __launch_bounds__(kTile) k_div_init(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_div_init(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                     uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<2>(tg, halo(F), rec, global_src(F, cap), LoadXV{F.pos, F.vel}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         const SceneParams& p = P[c.s];
         if (!c.valid || !p.enableDiv) return;
-        float d = velocity_divergence(p, K, F, cap, c.i);
+        float d = velocity_divergence(p, K, F, cap, c.i, src);
         d = F.nbrCount[c.i] < 20u ? 0.0f : fmaxf(d, 0.0f);
         F.densAdv[c.i] = d;
         const float f = F.factor[c.i] * F.state[c.s].dtInv;
@@ -187,52 +210,54 @@ __global__ void __launch_bounds__(kTile) k_div_init(const SceneParams* __restric
 }
 
 // ComputePressureAccelerationAndDivergenceKernel (:388-432)
-__global__ void __launch_bounds__(kTile) k_div_accel(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_div_accel(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    if (F.live[0] == 0u) return;  // every scene has left this solve (block-uniform)
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<1>(tg, halo(F), rec, global_src(F, cap), LoadXS{F.pos, F.kappaV}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         if (!F.state[c.s].div.active || !c.valid) return;
-        F.ap[c.i] = st4(pressure_accel(P[c.s], K, F, cap, c.i, F.kappaV));
+        F.ap[c.i] = st4(pressure_accel(P[c.s], K, F, cap, c.i, F.kappaV, src));
     });
 }
 
 // DivergenceSolveIterationKernel (:531-553) + per-tile residual partial sums (DFSPHImplementation.cu:554-563)
-__global__ void __launch_bounds__(kTile) k_div_iter(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_div_iter(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                     uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    if (F.live[0] == 0u) return;  // every scene has left this solve (block-uniform)
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<2>(tg, halo(F), rec, global_src(F, cap), LoadXV{F.pos, F.ap}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         const SceneState& st = F.state[c.s];
-        if (!st.div.active) return;  // uniform per tile
+        const bool act = ok && st.div.active;
         const SceneParams& p = P[c.s];
         double part = 0.0;
-        if (c.valid) {
-            const float f = density_pressure_force(p, K, F, cap, c.i) * st.dt;
+        if (act && c.valid) {
+            const float f = density_pressure_force(p, K, F, cap, c.i, src) * st.dt;
             const float r = F.nbrCount[c.i] < 20u ? 0.0f : fminf(-F.densAdv[c.i] - f, 0.0f);
             F.residuum[c.i] = r;
             F.kappaV[c.i] -= r * F.factor[c.i];
             part = (double)(p.rho0 * r);
         }
         double v[1] = {part};
-        tile_partials<1>(v, F.partials, tile);
+        tile_partials4<1>(v, F.partials, tile, act);
     });
 }
 
 // ComputePressureAccelerationAndFactorKernel (:555-602) + ClearAccelerationKernel (:24-37)
-__global__ void __launch_bounds__(kTile) k_div_final(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_div_final(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<1>(tg, halo(F), rec, global_src(F, cap), LoadXS{F.pos, F.kappaV}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         if (!c.valid) return;
         const SceneParams& p = P[c.s];
         if (p.enableDiv) {
             const float dt = F.state[c.s].dt;
-            const f3 a = pressure_accel(p, K, F, cap, c.i, F.kappaV);
+            const f3 a = pressure_accel(p, K, F, cap, c.i, F.kappaV, src);
             F.ap[c.i] = st4(a);
             F.vel[c.i] = st4(ld3(F.vel[c.i]) + dt * a);
             F.factor[c.i] *= dt;
@@ -242,39 +267,38 @@ __global__ void __launch_bounds__(kTile) k_div_final(const SceneParams* __restri
 }
 
 void launch_scene_init_div(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_jacobi_begin, dim3((L.sceneCount + 63) / 64), dim3(64), 0, L.stream, L.P, F.state,
-                       L.sceneCount, 0);
+    hipLaunchKernelGGL(k_jacobi_begin, dim3(1), dim3(kTile), 0, L.stream, L.P, F.state, F.live, L.sceneCount, 0);
 }
 void launch_div_init(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_init, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_div_init, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 void launch_div_accel(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_accel, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_div_accel, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 void launch_div_iter(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_iter, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_div_iter, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
     hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F, 0);
 }
 void launch_div_final(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_final, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_div_final, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 
 // ---- pressure solve (ComputePressure, DFSPHImplementation.cu:443-504) -------------------------------------------
 // ComputeDensityAdvectionKernel (DFSPHKernels.cu:233-278)
-__global__ void __launch_bounds__(kTile) k_pres_init(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_pres_init(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<2>(tg, halo(F), rec, global_src(F, cap), LoadXV{F.pos, F.vel}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         if (!c.valid) return;
         const SceneParams& p = P[c.s];
         const SceneState& st = F.state[c.s];
-        const float d = velocity_divergence(p, K, F, cap, c.i);
+        const float d = velocity_divergence(p, K, F, cap, c.i, src);
         const float da = F.density[c.i] / p.rho0 + st.dt * d;
         F.densAdv[c.i] = da;
         const float f = F.factor[c.i] * st.dt2Inv;
@@ -285,51 +309,53 @@ __global__ void __launch_bounds__(kTile) k_pres_init(const SceneParams* __restri
 }
 
 // ComputePressureAccelerationKernel (:342-386)
-__global__ void __launch_bounds__(kTile) k_pres_accel(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_pres_accel(const SceneParams* __restrict__ P,
                                                       const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                       uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    if (F.live[1] == 0u) return;  // every scene has left this solve (block-uniform)
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<1>(tg, halo(F), rec, global_src(F, cap), LoadXS{F.pos, F.kappa}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         if (!F.state[c.s].pres.active || !c.valid) return;
-        F.ap[c.i] = st4(pressure_accel(P[c.s], K, F, cap, c.i, F.kappa));
+        F.ap[c.i] = st4(pressure_accel(P[c.s], K, F, cap, c.i, F.kappa, src));
     });
 }
 
 // PressureSolveIterationKernel (:317-340) + per-tile residual partial sums (DFSPHImplementation.cu:483-491)
-__global__ void __launch_bounds__(kTile) k_pres_iter(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_pres_iter(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    if (F.live[1] == 0u) return;  // every scene has left this solve (block-uniform)
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<2>(tg, halo(F), rec, global_src(F, cap), LoadXV{F.pos, F.ap}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         const SceneState& st = F.state[c.s];
-        if (!st.pres.active) return;
+        const bool act = ok && st.pres.active;
         const SceneParams& p = P[c.s];
         double part = 0.0;
-        if (c.valid) {
-            const float f = density_pressure_force(p, K, F, cap, c.i) * st.dt2;
+        if (act && c.valid) {
+            const float f = density_pressure_force(p, K, F, cap, c.i, src) * st.dt2;
             const float r = fminf(1.0f - F.densAdv[c.i] - f, 0.0f);
             F.residuum[c.i] = r;
             F.kappa[c.i] -= r * F.factor[c.i];
             part = (double)(p.rho0 * r);
         }
         double v[1] = {part};
-        tile_partials<1>(v, F.partials, tile);
+        tile_partials4<1>(v, F.partials, tile, act);
     });
 }
 
 // ComputePressureAccelerationAndVelocityKernel (:434-481) + ComputePositionKernel (:54-67) into posNext
-__global__ void __launch_bounds__(kTile) k_pres_final(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_pres_final(const SceneParams* __restrict__ P,
                                                       const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                       uint32_t cap, TileGrid tg, int writeNext) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<1>(tg, halo(F), rec, global_src(F, cap), LoadXS{F.pos, F.kappa}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         if (!c.valid) return;
         const SceneParams& p = P[c.s];
         const float dt = F.state[c.s].dt;
-        const f3 a = pressure_accel(p, K, F, cap, c.i, F.kappa);
+        const f3 a = pressure_accel(p, K, F, cap, c.i, F.kappa, src);
         F.ap[c.i] = st4(a);
         const f3 v = ld3(F.vel[c.i]) + dt * a;
         F.vel[c.i] = st4(v);
@@ -338,24 +364,23 @@ __global__ void __launch_bounds__(kTile) k_pres_final(const SceneParams* __restr
 }
 
 void launch_scene_init_pres(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_jacobi_begin, dim3((L.sceneCount + 63) / 64), dim3(64), 0, L.stream, L.P, F.state,
-                       L.sceneCount, 1);
+    hipLaunchKernelGGL(k_jacobi_begin, dim3(1), dim3(kTile), 0, L.stream, L.P, F.state, F.live, L.sceneCount, 1);
 }
 void launch_pres_init(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_pres_init, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_pres_init, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 void launch_pres_accel(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_pres_accel, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F,
+    hipLaunchKernelGGL(k_pres_accel, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg));
 }
 void launch_pres_iter(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_pres_iter, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_pres_iter, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
     hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F, 1);
 }
 void launch_pres_final(const Launch& L, const KTabHost& kt, const Fields& F, int writeNext) {
-    hipLaunchKernelGGL(k_pres_final, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F,
+    hipLaunchKernelGGL(k_pres_final, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg), writeNext);
 }
 
@@ -480,12 +505,12 @@ __device__ __forceinline__ bool tangential(const SceneParams& p, const KLds& k, 
 
 // ComputeViscosityPreconditionerKernel (:604-684) + ComputeViscosityGradientKernel (:686-756): inverse diagonal
 // block, right-hand side b = v (the boundary rhs term is dot(0, .) = 0) and initial guess x0 = v + dv.
-__global__ void __launch_bounds__(kTile) k_visc_precond(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_visc_precond(const SceneParams* __restrict__ P,
                                                         const uint32_t* __restrict__ tileScene, KTabHost kh,
                                                         Fields F, uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<1>(tg, halo(F), rec, global_src(F, cap), LoadXS{F.pos, F.density}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         const SceneParams& p = P[c.s];
         if (!c.valid || !p.enableVisc) return;
         const f3 xi = ld3(F.pos[c.i]);
@@ -493,10 +518,10 @@ __global__ void __launch_bounds__(kTile) k_visc_precond(const SceneParams* __res
         float M[9];
 #pragma unroll
         for (int e = 0; e < 9; ++e) M[e] = 0.0f;
-        each_neighbor(F.nbrList, cap, c.i, F.nbrCount[c.i], [&](uint32_t t) {
-            const f3 xij = xi - ld3(F.pos[t]);
+        each_neighbor_rec<1>(src, c.i, LoadXS{F.pos, F.density}, [&](float4 lo, float4) {
+            const f3 xij = xi - ld3(lo);
             const f3 gw = gradW(K, xij);
-            const float s = 10.0f * p.mu * (p.m / F.density[t]) / (length2(xij) + d001);
+            const float s = 10.0f * p.mu * (p.m / lo.w) / (length2(xij) + d001);
             const float a[3] = {xij.x, xij.y, xij.z}, g[3] = {gw.x, gw.y, gw.z};
 #pragma unroll
             for (int r = 0; r < 3; ++r)
@@ -533,15 +558,16 @@ __global__ void __launch_bounds__(kTile) k_visc_precond(const SceneParams* __res
 }
 
 // ComputeMatrixVecProdFunctionKernel (:758-842): y_i = x_i - dt/rho_i * (viscous acceleration of x)
+// records lo = {x_j, rho_j}, hi = {in_j}
 __device__ __forceinline__ f3 visc_matvec(const SceneParams& p, const KLds& k, const Fields& F, uint32_t cap,
-                                          uint32_t i, const float4* __restrict__ in, float dt) {
+                                          uint32_t i, const float4* __restrict__ in, float dt, const NbrSrc& src) {
     const f3 xi = ld3(F.pos[i]), vi = ld3(in[i]);
     const float d001 = 0.01f * p.R2;
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
-    each_neighbor(F.nbrList, cap, i, F.nbrCount[i], [&](uint32_t t) {
-        const f3 xij = xi - ld3(F.pos[t]);
+    each_neighbor_rec<2>(src, i, LoadXSV{F.pos, F.density, in}, [&](float4 lo, float4 hi) {
+        const f3 xij = xi - ld3(lo);
         const f3 gw = gradW(k, xij);
-        const float s = 10.0f * p.mu * (p.m / F.density[t]) * dot3(vi - ld3(in[t]), xij) / (length2(xij) + d001);
+        const float s = 10.0f * p.mu * (p.m / lo.w) * dot3(vi - ld3(hi), xij) / (length2(xij) + d001);
         acc = acc + s * gw;
     });
     if (p.muB != 0.0f) {
@@ -563,19 +589,19 @@ __device__ __forceinline__ f3 visc_matvec(const SceneParams& p, const KLds& k, c
 }
 
 // SolveViscosity set-up (DFSPHImplementation.cu:627-691): r = b - A x0, ||b||^2, ||r||^2, p = M^-1 r, |r.p|
-__global__ void __launch_bounds__(kTile) k_cg_begin(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_cg_begin(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                     uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<2>(tg, halo(F), rec, global_src(F, cap), LoadXSV{F.pos, F.density, F.cgX}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         const SceneParams& p = P[c.s];
-        if (!p.enableVisc) return;
+        const bool act = ok && p.enableVisc;
         const SceneState& st = F.state[c.s];
         double v[3] = {0.0, 0.0, 0.0};
-        if (c.valid) {
+        if (act && c.valid) {
             const f3 b = ld3(F.vel[c.i]);
-            const f3 ax = visc_matvec(p, K, F, cap, c.i, F.cgX, st.dt);
+            const f3 ax = visc_matvec(p, K, F, cap, c.i, F.cgX, st.dt, src);
             const f3 r = b - ax;
             const f3 z = minv_mul(F.minv, cap, c.i, r);
             F.cgR[c.i] = st4(r);
@@ -584,7 +610,7 @@ __global__ void __launch_bounds__(kTile) k_cg_begin(const SceneParams* __restric
             v[1] = (double)dot3(r, r);
             v[2] = (double)dot3(r, z);
         }
-        tile_partials<3>(v, F.partials, tile);
+        tile_partials4<3>(v, F.partials, tile, act);
     });
 }
 
@@ -599,6 +625,7 @@ __global__ void __launch_bounds__(kTile) k_cg_begin_finish(const SceneParams* __
         G.it = 0;
         G.zero = 0;
         G.active = 0;
+        G.loops = 0;
         const float rhs2 = (float)out[0];
         G.rhs2 = rhs2;
         if (rhs2 == 0.0f) {
@@ -612,29 +639,32 @@ __global__ void __launch_bounds__(kTile) k_cg_begin_finish(const SceneParams* __
         if (G.res2 < G.thr) return;
         G.absNew = fabsf((float)out[2]);
         G.active = (0u >= p.minVisc && 0u < p.maxVisc) ? 1u : 0u;
+        if (G.active) atomicAdd(&F.live[2], 1u);
     }
 }
 
 // CG matvec Ap and p.Ap -> alpha (:695-720)
-__global__ void __launch_bounds__(kTile) k_cg_matvec(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize) k_cg_matvec(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
-    VFD_LDS_G_KERNEL_PROLOGUE(kh)
-    tile_loop(tg, [&](uint32_t tile) {
-        const TileCtx c = tile_ctx(tile, tileScene, P);
+    if (F.live[2] == 0u) return;  // every scene has left this solve (block-uniform)
+    VFD_HALO_KERNEL_PROLOGUE(kh)
+    group_loop<2>(tg, halo(F), rec, global_src(F, cap), LoadXSV{F.pos, F.density, F.cgP}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         const SceneState& st = F.state[c.s];
-        if (!st.cg.active) return;
+        const bool act = ok && st.cg.active;
         double v[1] = {0.0};
-        if (c.valid) {
-            const f3 ap = visc_matvec(P[c.s], K, F, cap, c.i, F.cgP, st.dt);
+        if (act && c.valid) {
+            const f3 ap = visc_matvec(P[c.s], K, F, cap, c.i, F.cgP, st.dt, src);
             F.cgAp[c.i] = st4(ap);
             v[0] = (double)dot3(ld3(F.cgP[c.i]), ap);
         }
-        tile_partials<1>(v, F.partials, tile);
+        tile_partials4<1>(v, F.partials, tile, act);
     });
 }
 
 __global__ void __launch_bounds__(kTile) k_cg_alpha(const SceneParams* __restrict__ P, Fields F) {
+    if (F.live[2] == 0u) return;  // every scene has left this solve (block-uniform)
     const uint32_t s = blockIdx.x;
     const SceneParams& p = P[s];
     CgState& G = F.state[s].cg;
@@ -648,6 +678,7 @@ __global__ void __launch_bounds__(kTile) k_cg_alpha(const SceneParams* __restric
 __global__ void __launch_bounds__(kTile) k_cg_update(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, Fields F, uint32_t cap,
                                                      TileGrid tg) {
+    if (F.live[2] == 0u) return;  // every scene has left this solve (block-uniform)
     tile_loop(tg, [&](uint32_t tile) {
         const TileCtx c = tile_ctx(tile, tileScene, P);
         const SceneState& st = F.state[c.s];
@@ -675,10 +706,12 @@ __global__ void __launch_bounds__(kTile) k_cg_update_finish(const SceneParams* _
     double out[2];
     scene_sum<2>(F.partials, p.tileBegin, p.tileEnd, out);
     if (threadIdx.x == 0) {
+        G.loops += 1u;
         G.res2 = (float)out[0];
         G.err = sqrtf(G.res2 / G.rhs2);
         if (G.res2 < G.thr) {
             G.active = 0;
+            atomicSub(&F.live[2], 1u);
             return;
         }
         const float absOld = G.absNew;
@@ -686,6 +719,7 @@ __global__ void __launch_bounds__(kTile) k_cg_update_finish(const SceneParams* _
         G.beta = G.absNew / absOld;
         G.it += 1u;
         G.active = (G.it >= p.minVisc && G.it < p.maxVisc) ? 1u : 0u;
+        if (!G.active) atomicSub(&F.live[2], 1u);
     }
 }
 
@@ -693,6 +727,7 @@ __global__ void __launch_bounds__(kTile) k_cg_update_finish(const SceneParams* _
 __global__ void __launch_bounds__(kTile) k_cg_direction(const SceneParams* __restrict__ P,
                                                         const uint32_t* __restrict__ tileScene, Fields F,
                                                         TileGrid tg) {
+    if (F.live[2] == 0u) return;  // every scene has left this solve (block-uniform)
     tile_loop(tg, [&](uint32_t tile) {
         const TileCtx c = tile_ctx(tile, tileScene, P);
         const SceneState& st = F.state[c.s];
@@ -721,16 +756,16 @@ __global__ void __launch_bounds__(kTile) k_visc_apply(const SceneParams* __restr
 }
 
 void launch_visc_precond(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_visc_precond, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F,
+    hipLaunchKernelGGL(k_visc_precond, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg));
 }
 void launch_cg_begin(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_begin, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_cg_begin, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
     hipLaunchKernelGGL(k_cg_begin_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_cg_matvec(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_matvec, dim3(L.grid4), dim3(kTile), kLdsG, L.stream, L.P, L.tileScene, kt, F,
+    hipLaunchKernelGGL(k_cg_matvec, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg));
     hipLaunchKernelGGL(k_cg_alpha, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
@@ -866,9 +901,27 @@ void launch_surface_tension(const Launch& L, const Fields& F) {
 }  // namespace vfd
 
 namespace vfd {
+// Dynamic LDS above the 64 KB default: kernel table plus one group's halo records.
 int configure_kernels() {
-    hipError_t e = hipFuncSetAttribute((const void*)k_density_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)kLdsWG);
-    return e == hipSuccess ? 0 : 1;
+    struct {
+        const void* fn;
+        size_t bytes;
+    } const k[] = {
+        {(const void*)k_density_factor, kLdsWG + kHaloMax * sizeof(float4)},
+        {(const void*)k_div_init, kLdsG + 2 * kHaloMax * sizeof(float4)},
+        {(const void*)k_div_accel, kLdsG + kHaloMax * sizeof(float4)},
+        {(const void*)k_div_iter, kLdsG + 2 * kHaloMax * sizeof(float4)},
+        {(const void*)k_div_final, kLdsG + kHaloMax * sizeof(float4)},
+        {(const void*)k_pres_init, kLdsG + 2 * kHaloMax * sizeof(float4)},
+        {(const void*)k_pres_accel, kLdsG + kHaloMax * sizeof(float4)},
+        {(const void*)k_pres_iter, kLdsG + 2 * kHaloMax * sizeof(float4)},
+        {(const void*)k_pres_final, kLdsG + kHaloMax * sizeof(float4)},
+        {(const void*)k_visc_precond, kLdsG + kHaloMax * sizeof(float4)},
+        {(const void*)k_cg_begin, kLdsG + 2 * kHaloMax * sizeof(float4)},
+        {(const void*)k_cg_matvec, kLdsG + 2 * kHaloMax * sizeof(float4)},
+    };
+    for (const auto& e : k)
+        if (hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.bytes) != hipSuccess) return 1;
+    return 0;
 }
 }  // namespace vfd

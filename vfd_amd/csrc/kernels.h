@@ -21,7 +21,9 @@ struct Launch {
     uint32_t gridAll;            // 8 * tilesPerXcd: one block per tile, XCD-aware order
     uint32_t grid4;              // persistent grid, 4 blocks per CU (40 KB LDS table)
     uint32_t grid2;              // persistent grid, 2 blocks per CU (80 KB LDS table)
-    struct { uint32_t tiles, per; } tg;
+    uint32_t gridGroups;         // persistent grid of kGroupSize-thread blocks (one group at a time), 1 per CU
+    uint32_t groups;             // 8 * per / kGroupTiles
+    struct { uint32_t tiles, per; } tg;  // per = tiles of one XCD chunk, a multiple of 4
 };
 
 // one-time kernel attributes (dynamic LDS above 64 KB)
@@ -57,6 +59,8 @@ void launch_sort(const Launch& L, const SceneGrid* G, uint32_t totalCells, const
                  const float4* dvIn, const float* dkIn, const uint32_t* idIn, float4* posOut, float4* velOut,
                  float4* dvOut, float* dkOut, uint32_t* idOut, uint32_t* cellOf, uint32_t* rank, uint32_t* tmpId,
                  uint32_t* cellCount, uint32_t* cellStart, uint32_t* scanScratch);
+void launch_build_halo(const Launch& L, const uint32_t* nbrCount, const uint32_t* nbrList, uint32_t haloLimit,
+                       uint32_t* haloCount, uint32_t* haloIds, uint16_t* nbrLocal, uint32_t* haloStats);
 void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, const uint32_t* cellStart,
                         const uint32_t* cellCount, uint32_t* nbrCount, uint32_t* nbrList,
                         unsigned long long* nbStats);
@@ -90,6 +94,9 @@ struct Fields {
     float* vb;        // [kMaxBodies][capacity]
     const uint32_t* nbrCount;
     const uint32_t* nbrList;
+    const uint16_t* nbrLocal;    // [j][capacity] halo slots (staged groups)
+    const uint32_t* haloCount;   // [groups] kHaloNone: global gathers
+    const uint32_t* haloIds;     // [groups][kHaloMax]
     // viscosity solver
     float* minv;      // [9][capacity]
     float4* cgX;      // solution (ViscosityGradientG)
@@ -100,7 +107,7 @@ struct Fields {
     // reductions
     double* partials;      // [tiles][kPartialSlots]
     float* partialsF;      // [tiles]
-    uint32_t* counters;    // [sceneCount]
+    uint32_t* live;        // [3] scenes still iterating in the divergence, pressure and viscosity solves
     SceneState* state;     // [sceneCount]
     const float* halton;   // [49152]
 };

@@ -322,6 +322,139 @@ __global__ void k_fill_u64(unsigned long long* p, uint32_t n) {
     if (i < n) p[i] = 0ull;
 }
 
+// ---- halos of the neighbour groups ------------------------------------------------------------------------------
+// One block per group. The distinct particle slots among the lists of its kGroupSize particles are collected in an
+// LDS hash set of 32-slot buckets (bucket = slot >> 5, Fibonacci-hashed, linear probing) with a presence mask each;
+// halo numbers are handed out bucket by bucket, ascending inside a bucket, so runs of consecutive particle slots stay
+// consecutive in the halo. Every list entry is then rewritten as its halo number. A group with more than `limit`
+// distinct neighbours (or more than kBucketLimit buckets) gets kHaloNone and is served by the global lists.
+// haloStats = {groups staged, groups gathered, largest halo}.
+constexpr uint32_t kBucketSlots = 2048;
+constexpr uint32_t kBucketLimit = 896;  // + one bucket per thread in flight < kBucketSlots: probing always ends
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+static_assert(kBucketLimit + kGroupSize < kBucketSlots, "bucket table must never fill");
+
+__device__ __forceinline__ uint32_t bucket_hash(uint32_t b) { return (b * 0x9E3779B1u) >> 21; }  // 11 bits
+
+__global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __restrict__ P,
+                                                           const uint32_t* __restrict__ tileScene, uint32_t tiles,
+                                                           const uint32_t* __restrict__ nbrCount,
+                                                           const uint32_t* __restrict__ nbrList, uint32_t cap,
+                                                           uint32_t limit, uint32_t* haloCount, uint32_t* haloIds,
+                                                           uint16_t* nbrLocal, uint32_t* haloStats) {
+    __shared__ uint32_t bkey[kBucketSlots];
+    __shared__ uint32_t bmask[kBucketSlots];
+    __shared__ uint16_t bbase[kBucketSlots];
+    __shared__ uint32_t waveTotal[kGroupSize / kWave];
+    __shared__ uint32_t used, rows;
+    const uint32_t g = blockIdx.x;
+    const uint32_t tile = g * kGroupTiles + threadIdx.x / kTile;
+    const uint32_t i = g * kGroupSize + threadIdx.x;
+    uint32_t n = 0;
+    if (tile < tiles) {
+        const SceneParams& p = P[tileScene[tile]];
+        if (i < p.base + p.count) n = nbrCount[i];
+    }
+    for (uint32_t k = threadIdx.x; k < kBucketSlots; k += blockDim.x) {
+        bkey[k] = kHashEmpty;
+        bmask[k] = 0u;
+    }
+    if (threadIdx.x == 0) {
+        used = 0;
+        rows = 0;
+    }
+    __syncthreads();
+    if (n) atomicMax(&rows, n);
+    __syncthreads();
+    const uint32_t nr = rows;
+    volatile uint32_t* vkey = bkey;
+    volatile uint32_t* vmask = bmask;
+    volatile uint32_t* vused = &used;
+    for (uint32_t j = 0; j < nr; ++j) {
+        if (j >= n || *vused > kBucketLimit) continue;
+        const uint32_t t = nbrList[(size_t)j * cap + i];
+        const uint32_t b = t >> 5, bit = 1u << (t & 31u);
+        uint32_t h = bucket_hash(b);
+        for (;;) {
+            uint32_t cur = vkey[h];
+            if (cur == kHashEmpty) {
+                cur = atomicCAS(&bkey[h], kHashEmpty, b);
+                if (cur == kHashEmpty) {
+                    atomicAdd(&used, 1u);
+                    cur = b;
+                }
+            }
+            if (cur == b) {
+                if (!(vmask[h] & bit)) atomicOr(&bmask[h], bit);
+                break;
+            }
+            h = (h + 1) & (kBucketSlots - 1);
+        }
+    }
+    __syncthreads();
+    // halo numbers: thread t owns bucket slots [2t, 2t + 2)
+    constexpr uint32_t kPer = kBucketSlots / kGroupSize;
+    uint32_t own = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) own += __popc(bmask[kPer * threadIdx.x + k]);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    uint32_t incl = own;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += v;
+    }
+    if (lane == kWave - 1) waveTotal[w] = incl;
+    __syncthreads();
+    uint32_t m = 0;
+    for (int k = 0; k < (int)(kGroupSize / kWave); ++k) m += waveTotal[k];
+    if (used > kBucketLimit || m > limit) {  // block-uniform
+        if (threadIdx.x == 0) {
+            haloCount[g] = kHaloNone;
+            atomicAdd(&haloStats[1], 1u);
+        }
+        return;
+    }
+    uint32_t base = incl - own;
+    for (int k = 0; k < w; ++k) base += waveTotal[k];
+    uint32_t* ids = haloIds + (size_t)g * kHaloMax;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t slot = kPer * threadIdx.x + k;
+        uint32_t mask = bmask[slot];
+        bbase[slot] = (uint16_t)base;
+        const uint32_t b0 = bkey[slot] << 5;
+        while (mask) {
+            const uint32_t bit = (uint32_t)__ffs(mask) - 1u;
+            ids[base++] = b0 + bit;
+            mask &= mask - 1u;
+        }
+    }
+    if (threadIdx.x == 0) {
+        haloCount[g] = m;
+        if (m) atomicAdd(&haloStats[0], 1u);
+        atomicMax(&haloStats[2], m);
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < nr; ++j) {
+        if (j >= n) continue;
+        const size_t e = (size_t)j * cap + i;
+        const uint32_t t = nbrList[e];
+        const uint32_t b = t >> 5, below = (1u << (t & 31u)) - 1u;
+        uint32_t h = bucket_hash(b);
+        while (bkey[h] != b) h = (h + 1) & (kBucketSlots - 1);
+        nbrLocal[e] = (uint16_t)(bbase[h] + __popc(bmask[h] & below));
+    }
+}
+
+void launch_build_halo(const Launch& L, const uint32_t* nbrCount, const uint32_t* nbrList, uint32_t haloLimit,
+                       uint32_t* haloCount, uint32_t* haloIds, uint16_t* nbrLocal, uint32_t* haloStats) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, L.stream, haloStats, 0u, 3u);
+    hipLaunchKernelGGL(k_build_halo, dim3(L.groups), dim3(kGroupSize), 0, L.stream, L.P, L.tileScene, L.tiles,
+                       nbrCount, nbrList, L.capacity, haloLimit < kHaloMax ? haloLimit : kHaloMax, haloCount, haloIds,
+                       nbrLocal, haloStats);
+}
+
 void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, const uint32_t* cellStart,
                         const uint32_t* cellCount, uint32_t* nbrCount, uint32_t* nbrList,
                         unsigned long long* nbStats) {

@@ -319,13 +319,20 @@ public:
     DevBuf<int> dMM, dMMPart;
     DevBuf<float4> pos, posNext, vel[2], dv[2], acc, ap, nMC, nS, xb, cgX, cgR, cgP, cgZ, cgAp;
     DevBuf<float> dk[2], density, factor, densAdv, kappa, kappaV, residuum, kMC, kS, vb, minv;
-    DevBuf<uint32_t> id[2], nbrCount, nbrList, cellOf, rank, tmpId, cellCount, cellStart, scanScratch, counters;
+    DevBuf<uint32_t> id[2], nbrCount, nbrList, cellOf, rank, tmpId, cellCount, cellStart, scanScratch, live;
     DevBuf<double> partials;
     DevBuf<float> partialsF;
     DevBuf<unsigned long long> nbStats;
+    DevBuf<uint32_t> haloCount, haloIds, haloStats;
+    DevBuf<uint16_t> nbrLocal;
+    uint32_t haloLimit = kHaloMax;  // largest halo staged in LDS (tuning)
     uint32_t cellCap = 0;
     hipEvent_t ev[8] = {};
     bool eventsMade = false;
+    // solver polling: live[which] is copied into one of two pinned slots after each chunk of iterations
+    uint32_t* hostLive = nullptr;  // [2]
+    hipEvent_t pollEv[2] = {};
+    uint32_t pollChunk = 4;  // iterations enqueued between convergence checks
 
     // per-kernel timing: an event pair around each launch, resolved at the end of a step
     struct KStat {
@@ -336,12 +343,20 @@ public:
     bool profiling = false;
     std::vector<KStat> kstats;
     std::vector<hipEvent_t> evPool;
-    std::vector<std::pair<size_t, size_t>> pending;  // (stat index, first event of the pair)
+    struct Pending {
+        size_t stat, event;  // stat index, first event of the pair
+        int solver;          // -1, or the solver loop (0 divergence, 1 pressure, 2 viscosity) the launch belongs to
+        uint32_t iter;       // its iteration index within that loop
+    };
+    std::vector<Pending> pending;
     size_t evUsed = 0;
 
     ~Simulation() {
-        if (eventsMade)
+        if (eventsMade) {
             for (auto& e : ev) (void)hipEventDestroy(e);
+            for (auto& e : pollEv) (void)hipEventDestroy(e);
+        }
+        if (hostLive) (void)hipHostFree(hostLive);
         for (auto& e : evPool) (void)hipEventDestroy(e);
         for (Scene* s : sc) delete s;
     }
@@ -352,8 +367,10 @@ public:
         kstats.push_back(KStat{name, 0, 0.0});
         return kstats.size() - 1;
     }
-    // Runs `fn` (which enqueues launches on the stream) between two recorded events when profiling.
-    template <class Fn> int timed(const char* name, Fn fn) {
+    // Runs `fn` (which enqueues launches on the stream) between two recorded events when profiling. Launches of a
+    // solver loop are attributed at the end of the step: those past the last iteration any scene ran (issued ahead of
+    // the convergence check and skipped on the device) are counted under "<name>_idle".
+    template <class Fn> int timed(const char* name, Fn fn, int solver = -1, uint32_t iter = 0) {
         if (!profiling) {
             fn();
             return VFD_OK;
@@ -368,17 +385,26 @@ public:
         VFD_HIP(hipEventRecord(evPool[first], stream));
         fn();
         VFD_HIP(hipEventRecord(evPool[first + 1], stream));
-        pending.emplace_back(stat_index(name), first);
+        pending.push_back(Pending{stat_index(name), first, solver, iter});
         return VFD_OK;
     }
     int resolve_profile() {
         if (pending.empty()) return VFD_OK;
         VFD_HIP(hipStreamSynchronize(stream));
+        uint32_t ran[3] = {0, 0, 0};
+        for (Scene* s : sc) {
+            if (!s->count) continue;
+            ran[0] = std::max(ran[0], s->st.div.it);
+            ran[1] = std::max(ran[1], s->st.pres.it);
+            ran[2] = std::max(ran[2], s->st.cg.loops);
+        }
         for (const auto& pr : pending) {
             float ms = 0.0f;
-            VFD_HIP(hipEventElapsedTime(&ms, evPool[pr.second], evPool[pr.second + 1]));
-            kstats[pr.first].launches++;
-            kstats[pr.first].ms += ms;
+            VFD_HIP(hipEventElapsedTime(&ms, evPool[pr.event], evPool[pr.event + 1]));
+            size_t k = pr.stat;
+            if (pr.solver >= 0 && pr.iter >= ran[pr.solver]) k = stat_index((kstats[k].name + "_idle").c_str());
+            kstats[k].launches++;
+            kstats[k].ms += ms;
         }
         pending.clear();
         evUsed = 0;
@@ -407,7 +433,9 @@ public:
         }
         VFD_TRY(build_tables(descs[0].ParticleRadius));
         for (auto& e : ev) VFD_HIP(hipEventCreate(&e));
+        for (auto& e : pollEv) VFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         eventsMade = true;
+        VFD_HIP(hipHostMalloc((void**)&hostLive, 2 * sizeof(uint32_t), hipHostMallocDefault));
         return VFD_OK;
     }
 
@@ -588,7 +616,7 @@ public:
         VFD_TRY(dMaps.alloc((size_t)S * kMaxBodies));
         VFD_TRY(dMM.alloc((size_t)S * 6));
         VFD_TRY(dMMPart.alloc((size_t)std::max<uint32_t>(tiles, 1) * 6));
-        VFD_TRY(counters.alloc(S));
+        VFD_TRY(live.alloc(4));
         VFD_TRY(dTileScene.alloc(std::max<uint32_t>(tiles, 1)));
         DevBuf<float4>* f4[] = {&pos, &posNext, &vel[0], &vel[1], &dv[0], &dv[1], &acc, &ap, &nMC, &nS,
                                 &cgX, &cgR, &cgP, &cgZ, &cgAp};
@@ -604,13 +632,20 @@ public:
         VFD_TRY(partials.alloc((size_t)std::max<uint32_t>(tiles, 1) * kPartialSlots));
         VFD_TRY(partialsF.alloc(std::max<uint32_t>(tiles, 1)));
         VFD_TRY(nbStats.alloc((size_t)2 * S));
+        {
+            const Launch L = launch();
+            VFD_TRY(haloCount.alloc(std::max<uint32_t>(L.groups, 1)));
+            VFD_TRY(haloIds.alloc((size_t)std::max<uint32_t>(L.groups, 1) * kHaloMax));
+            VFD_TRY(haloStats.alloc(3));
+            VFD_TRY(nbrLocal.alloc(c * kMaxNeighbors));
+        }
         if (tiles) VFD_HIP(hipMemcpy(dTileScene.p, ts.data(), tiles * sizeof(uint32_t), hipMemcpyHostToDevice));
         std::vector<MapDev> maps((size_t)S * kMaxBodies);
         for (uint32_t k = 0; k < S; ++k)
             for (size_t b = 0; b < sc[k]->bodies.size(); ++b) maps[(size_t)k * kMaxBodies + b] = sc[k]->bodies[b]->map;
         VFD_HIP(hipMemcpy(dMaps.p, maps.data(), maps.size() * sizeof(MapDev), hipMemcpyHostToDevice));
         // every particle field starts at zero, as SetFluidObjects initialises DFSPHParticle (:198-222)
-        VFD_HIP(hipMemset(counters.p, 0, S * sizeof(uint32_t)));
+        VFD_HIP(hipMemset(live.p, 0, 4 * sizeof(uint32_t)));
         VFD_HIP(hipMemset(xb.p, 0, c * kMaxBodies * sizeof(float4)));
         VFD_HIP(hipMemset(vb.p, 0, c * kMaxBodies * sizeof(float)));
         VFD_HIP(hipMemset(minv.p, 0, c * 9 * sizeof(float)));
@@ -676,12 +711,15 @@ public:
         L.sceneCount = (uint32_t)sc.size();
         L.capacity = cap;
         L.tilesPerXcd = (tiles + 7) / 8;
-        L.gridAll = 8 * L.tilesPerXcd;
+        const uint32_t per = (L.tilesPerXcd + 3) / 4 * 4;
+        L.gridAll = 8 * per;
         const uint32_t p4 = std::max<uint32_t>(8, (numCU * 4) / 8 * 8), p2 = std::max<uint32_t>(8, (numCU * 2) / 8 * 8);
         L.grid4 = std::min(L.gridAll, p4);
         L.grid2 = std::min(L.gridAll, p2);
+        L.groups = 8 * (per / kGroupTiles);
+        L.gridGroups = std::min(L.groups, std::max<uint32_t>(8, numCU / 8 * 8));
         L.tg.tiles = tiles;
-        L.tg.per = L.tilesPerXcd;
+        L.tg.per = per;
         return L;
     }
     Fields fields() {
@@ -716,7 +754,10 @@ public:
         F.cgAp = cgAp.p;
         F.partials = partials.p;
         F.partialsF = partialsF.p;
-        F.counters = counters.p;
+        F.live = live.p;
+        F.nbrLocal = nbrLocal.p;
+        F.haloCount = haloCount.p;
+        F.haloIds = haloIds.p;
         F.state = dState.p;
         F.halton = dHalton.p;
         return F;
@@ -782,24 +823,33 @@ public:
         VFD_TRY(timed("build_lists", [&] {
             launch_build_lists(L, dGrid.p, pos.p, cellStart.p, cellCount.p, nbrCount.p, nbrList.p, nbStats.p);
         }));
+        VFD_TRY(timed("build_halo", [&] {
+            launch_build_halo(L, nbrCount.p, nbrList.p, haloLimit, haloCount.p, haloIds.p, nbrLocal.p, haloStats.p);
+        }));
         VFD_HIP(hipGetLastError());
         return VFD_OK;
     }
 
+    // Runs a solver loop whose iterations are no-ops on the device once a scene has converged: `first` iterations,
+    // then chunks of pollChunk. Each chunk ends with an asynchronous copy of live[which] into a pinned slot; the next
+    // chunk is enqueued before the host waits for the previous slot, so the GPU never idles on a convergence check
+    // (at most one chunk of empty launches after the last scene converges).
     template <class Fn>
     int poll_loop(Fn enqueue, uint32_t first, int which) {
-        uint32_t chunk = std::max<uint32_t>(first, 1);
+        uint32_t iter = 0;
+        auto issue = [&](uint32_t n, int slot) -> int {
+            for (uint32_t c = 0; c < n; ++c) enqueue(iter++);
+            VFD_HIP(hipMemcpyAsync(hostLive + slot, live.p + which, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            VFD_HIP(hipEventRecord(pollEv[slot], stream));
+            return VFD_OK;
+        };
+        int cur = 0;
+        VFD_TRY(issue(std::max<uint32_t>(first, 1), cur));
         for (;;) {
-            for (uint32_t c = 0; c < chunk; ++c) enqueue();
-            VFD_HIP(hipGetLastError());
-            VFD_TRY(pull_state());
-            bool any = false;
-            for (Scene* s : sc) {
-                const uint32_t a = which == 0 ? s->st.div.active : which == 1 ? s->st.pres.active : s->st.cg.active;
-                any = any || (a && s->count);
-            }
-            if (!any) return VFD_OK;
-            chunk = 4;
+            VFD_TRY(issue(pollChunk, cur ^ 1));
+            VFD_HIP(hipEventSynchronize(pollEv[cur]));
+            if (hostLive[cur] == 0) return VFD_OK;
+            cur ^= 1;
         }
     }
 
@@ -831,9 +881,9 @@ public:
         launch_scene_init_div(L, F);
         VFD_TRY(timed("div_init", [&] { launch_div_init(L, K, F); }));
         VFD_TRY(poll_loop(
-            [&] {
-                (void)timed("div_accel", [&] { launch_div_accel(L, K, F); });
-                (void)timed("div_iter", [&] { launch_div_iter(L, K, F); });
+            [&](uint32_t k) {
+                (void)timed("div_accel", [&] { launch_div_accel(L, K, F); }, 0, k);
+                (void)timed("div_iter", [&] { launch_div_iter(L, K, F); }, 0, k);
             },
             minDiv, 0));
         VFD_TRY(timed("div_final", [&] { launch_div_final(L, K, F); }));  // also ClearAccelerationKernel
@@ -846,10 +896,10 @@ public:
             VFD_TRY(timed("visc_precond", [&] { launch_visc_precond(L, K, F); }));
             VFD_TRY(timed("cg_begin", [&] { launch_cg_begin(L, K, F); }));
             VFD_TRY(poll_loop(
-                [&] {
-                    (void)timed("cg_matvec", [&] { launch_cg_matvec(L, K, F); });
-                    (void)timed("cg_update", [&] { launch_cg_update(L, F); });
-                    (void)timed("cg_direction", [&] { launch_cg_direction(L, F); });
+                [&](uint32_t k) {
+                    (void)timed("cg_matvec", [&] { launch_cg_matvec(L, K, F); }, 2, k);
+                    (void)timed("cg_update", [&] { launch_cg_update(L, F); }, 2, k);
+                    (void)timed("cg_direction", [&] { launch_cg_direction(L, F); }, 2, k);
                 },
                 minVisc, 2));
             VFD_TRY(timed("visc_apply", [&] { launch_visc_apply(L, F); }));
@@ -863,9 +913,9 @@ public:
         launch_scene_init_pres(L, F);
         VFD_TRY(timed("pres_init", [&] { launch_pres_init(L, K, F); }));
         VFD_TRY(poll_loop(
-            [&] {
-                (void)timed("pres_accel", [&] { launch_pres_accel(L, K, F); });
-                (void)timed("pres_iter", [&] { launch_pres_iter(L, K, F); });
+            [&](uint32_t k) {
+                (void)timed("pres_accel", [&] { launch_pres_accel(L, K, F); }, 1, k);
+                (void)timed("pres_iter", [&] { launch_pres_iter(L, K, F); }, 1, k);
             },
             minPres, 1));
         VFD_TRY(timed("pres_final", [&] { launch_pres_final(L, K, F, last >= VFD_PHASE_POSITION ? 1 : 0); }));
@@ -930,9 +980,15 @@ public:
         VFD_HIP(hipMemcpyAsync(nb.data(), nbStats.p, nb.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                stream));
         VFD_HIP(hipStreamSynchronize(stream));
+        uint32_t hs[3];
+        VFD_HIP(hipMemcpyAsync(hs, haloStats.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
+        VFD_HIP(hipStreamSynchronize(stream));
         for (size_t k = 0; k < sc.size(); ++k) {
             sc[k]->dbg.TotalNeighborCount = nb[2 * k];
             sc[k]->dbg.MaxNeighborCount = (uint32_t)nb[2 * k + 1];
+            sc[k]->dbg.StagedGroupCount = hs[0];
+            sc[k]->dbg.GatheredGroupCount = hs[1];
+            sc[k]->dbg.MaxHaloSize = hs[2];
         }
         return VFD_OK;
     }
@@ -1407,6 +1463,24 @@ int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint3
             out[i].launches = ks[i].launches;
             out[i].total_ms = ks[i].ms;
         }
+    return VFD_OK;
+}
+int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out) {
+    CHECK_SIM(sim);
+    if (!out) return fail(VFD_ERROR_INVALID_ARGUMENT, "out must not be NULL");
+    out->SolverCheckInterval = sim->impl.pollChunk;
+    out->HaloLimit = sim->impl.haloLimit;
+    return VFD_OK;
+}
+int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* t) {
+    CHECK_SIM(sim);
+    if (!t) return fail(VFD_ERROR_INVALID_ARGUMENT, "tuning must not be NULL");
+    if (t->SolverCheckInterval < 1 || t->SolverCheckInterval > 1024)
+        return fail(VFD_ERROR_INVALID_ARGUMENT, "SolverCheckInterval must be in [1, 1024]");
+    if (t->HaloLimit > vfd::kHaloMax)
+        return fail(VFD_ERROR_INVALID_ARGUMENT, "HaloLimit must be at most " + std::to_string(vfd::kHaloMax));
+    sim->impl.pollChunk = t->SolverCheckInterval;
+    sim->impl.haloLimit = t->HaloLimit;
     return VFD_OK;
 }
 int vfd_get_kernel_table(vfd_sim* sim, float* w, float* gradient_w, float* scalars) {

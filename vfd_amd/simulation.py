@@ -234,6 +234,16 @@ class DFSPHSimulation:
         check(self._lib.vfd_get_profile(self._h, arr, n.value, _abi.ctypes.byref(n)), "vfd_get_profile")
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
 
+    def tuning(self):
+        """{SolverCheckInterval, HaloLimit}: launch and solver-control knobs (results do not depend on them)."""
+        t = _abi.Tuning()
+        check(self._lib.vfd_get_tuning(self._h, _abi.ctypes.byref(t)), "vfd_get_tuning")
+        return {name: getattr(t, name) for name, _ in _abi.Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = _abi.Tuning(**{**self.tuning(), **kw})
+        check(self._lib.vfd_set_tuning(self._h, _abi.ctypes.byref(t)), "vfd_set_tuning")
+
     def GetParticleFrameBuffer(self):
         cnt = _abi.ctypes.c_uint32()
         check(self._lib.vfd_get_frame_count(self._h, self._scene, _abi.ctypes.byref(cnt)), "vfd_get_frame_count")
@@ -274,6 +284,12 @@ class DFSPHBatch:
 
     def profile(self):
         return self.scenes[0].profile()
+
+    def tuning(self):
+        return self.scenes[0].tuning()
+
+    def set_tuning(self, **kw):
+        self.scenes[0].set_tuning(**kw)
 
     def close(self):
         if getattr(self, "_h", None):
