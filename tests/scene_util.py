@@ -7,7 +7,8 @@ from vfd_amd.simulation import DFSPHSimulationDescription, RigidBodyDescription
 
 def small_scene(kind="dam", seed=7):
     """Small scenes (<= ~8k particles) that exercise every solver: kind = dam (C1 settings), visc (viscosity +
-    boundary viscosity), st (surface tension + viscosity), all (everything, temporal smoothing, 2 blend passes)."""
+    boundary viscosity), st (surface tension + viscosity), all (everything, temporal smoothing, 2 blend passes),
+    push (visc with the obstacle inside the block, so the boundary moves particles)."""
     if kind == "dam":
         return scenes.dam_break_8k(seed=seed)
     rng = np.random.default_rng(seed)
@@ -26,6 +27,11 @@ def small_scene(kind="dam", seed=7):
     desc = DFSPHSimulationDescription(**kw)
     box = RigidBodyDescription("box", center=(0, 0, 0), half_extents=(1, 1, 1), inverted=True, resolution=(16, 16, 16))
     ball = RigidBodyDescription("sphere", center=(0.5, -0.8, 0.3), radius=0.15, padding=2 * r, resolution=(10, 10, 10))
+    if kind == "push":
+        # the obstacle sits inside the fluid block: the boundary pass pushes the particles it contains out before
+        # the density of the first step
+        ball = RigidBodyDescription("sphere", center=(0.02, -0.56, 0.12), radius=0.09, padding=2 * r,
+                                    resolution=(10, 10, 10))
     return [desc], [(x, v.astype(np.float32))], [[box, ball]]
 
 

@@ -213,7 +213,7 @@ def test_determinism():
     assert np.array_equal(a.field("velocity"), b.field("velocity"))
 
 
-@pytest.mark.parametrize("kind", ["dam", "all"])
+@pytest.mark.parametrize("kind", ["dam", "all", "push"])
 def test_tuning_never_changes_results(kind):
     """Halo staging (all groups, a limit that splits them, none) and the convergence-check interval change how the
     step runs, not what it computes: every combination ends bit-identical after three steps."""
@@ -240,3 +240,24 @@ def test_tuning_never_changes_results(kind):
         assert (e.DivergenceSolverIterationCount, e.PressureSolverIterationCount, e.ViscositySolverIterationCount) \
             == (d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount)
         print(kind, knobs, "staged", e.StagedGroupCount, "gathered", e.GatheredGroupCount, flush=True)
+
+
+def test_boundary_push_out_step_bit_exact():
+    """Particles that start inside the obstacle are pushed out by the boundary pass before the density; every later
+    neighbour sum (staged halos included) must use the pushed positions. Two steps against the oracle."""
+    scene = small_scene("push")
+    gpu, orc = make_gpu(*scene), make_oracle(*scene)
+    x0 = gpu.field("position")
+    r = scene[0][0].ParticleRadius
+    for k in range(2):
+        gpu.OnUpdate()
+        orc.step()
+        if k == 0:
+            moved = np.abs(gpu.field("position") - x0).max(1) > 0.5 * r
+            assert moved.sum() > 10  # the push-out happened
+        for f in ["density", "factor", "position", "velocity"]:
+            e = rel_inf(gpu.field(f), orc.field(f))
+            print("push step", k, f, "%.2e" % e, flush=True)
+            assert e <= REL, (k, f, e)
+        dx = np.abs(gpu.field("position") - orc.field("position")).max()
+        assert dx <= 1e-4 * r

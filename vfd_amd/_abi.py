@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvfd_hip.so")
+# VFD_LIBRARY selects another build of the library (the autotuner points its trial processes at candidate builds)
+LIB_PATH = os.environ.get("VFD_LIBRARY") or os.path.join(HERE, "libvfd_hip.so")
 
 c_float3 = ctypes.c_float * 3
 c_uint3 = ctypes.c_uint32 * 3

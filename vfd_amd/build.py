@@ -45,27 +45,33 @@ def _run(cmd):
         raise subprocess.CalledProcessError(p.returncode, cmd)
 
 
-def build(verbose=False):
+def build(verbose=False, out=None, defines=()):
+    """Builds LIB (or `out`); `defines` are compile-time launch-shape choices such as "VFD_LIST_CHUNK=16" (see
+    vfd_amd.autotune)."""
     cc = hipcc()
-    objdir = os.path.join(HERE, "_build")
+    if out is None:
+        from vfd_amd.autotune import chosen_defines
+        defines = list(chosen_defines()) + list(defines)
+    out = out or LIB
+    objdir = os.path.join(os.path.dirname(out), "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [cc] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc] + flags() + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".hip"):
             cmd[1:1] = ["-x", "hip"]
         if verbose:
             print(" ".join(cmd), flush=True)
         _run(cmd)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     _run(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -38,6 +38,11 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
 }
 __device__ __forceinline__ f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot3(a, a))); }
 
+// |x| > EPS for a float x, where the reference compares in double against the double literal 1e-5
+// (Core/Math/Math.h:6): (double)x > 1e-5 holds exactly when x >= the smallest float above 1e-5.
+constexpr float kEpsAbove = 1.00000008e-05f;  // 0x3727c5ad: nextafterf((float)1e-5, inf); (float)1e-5 < 1e-5
+__device__ __forceinline__ bool above_eps(float ax) { return ax >= kEpsAbove; }
+
 // ---- kernel table ----------------------------------------------------------------------------------------------
 // tab[p] = {0.5*(W[p]+W[p+1]), 0.5*(G[p]+G[p+1])}, p in [0, 9998], exactly the values GetW / GetGradientW form
 // (Kernel/DFSPHKernels.h:43-80). G stores gradW(x)_x / x, so gradW(r) = G * r.
@@ -251,6 +256,7 @@ constexpr unsigned kLdsG = (kTableEntries * sizeof(float) + 15u) / 16u * 16u;
 constexpr unsigned kLdsWG = (kTableEntries * sizeof(float2) + 15u) / 16u * 16u;
 __device__ __forceinline__ void load_table_g(float* sG, const float2* __restrict__ tab) {
     for (uint32_t p = threadIdx.x; p < kTableEntries; p += blockDim.x) sG[p] = tab[p].y;
+    if (threadIdx.x == 0) sG[kTableEntries] = 0.0f;  // code kGCodeZero: beyond the gradient cutoff
     __syncthreads();
 }
 __device__ __forceinline__ void load_table_wg(float2* sWG, const float2* __restrict__ tab) {
@@ -259,9 +265,18 @@ __device__ __forceinline__ void load_table_wg(float2* sWG, const float2* __restr
 }
 // gradW(r) with the G table in LDS (cutoff |r| <= R), same values as gradW(KTab, r)
 struct KLds {
-    const float* sG;
+    const float* sG;  // kTableEntries + 1 entries, the last one 0
     float R, invStep;
 };
+// Table code of a pair offset r for the gradient: the table index GetGradientW reads, or kGCodeZero beyond its
+// cutoff |r| <= R (Kernel/DFSPHKernels.h:67-80). G[code] * r is gradW(r), up to the sign of a zero component.
+constexpr uint32_t kGCodeZero = kTableEntries;
+__device__ __forceinline__ uint32_t gcode(float R, float invStep, f3 r) {
+    const float rl = sqrtf(dot3(r, r));
+    if (!(rl <= R)) return kGCodeZero;
+    const unsigned p0 = (unsigned)(rl * invStep);
+    return p0 < kTableEntries - 1u ? p0 : kTableEntries - 1u;
+}
 __device__ __forceinline__ f3 gradW(const KLds& k, f3 r) {
     const float rl = sqrtf(dot3(r, r));
     if (rl <= k.R) {
@@ -279,7 +294,7 @@ __device__ __forceinline__ f3 gradW(const KLds& k, f3 r) {
 // list entry. Both paths hand the same values to the same arithmetic in the same list order.
 struct NbrSrc {
     const float4* rec;                     // LDS records, R per halo slot; nullptr: global gathers
-    const uint16_t* __restrict__ local;    // [j * cap + i] halo slots (staged groups)
+    const uint32_t* __restrict__ local;    // [j * cap + i] halo slot | gradient table code << 16 (staged groups)
     const uint32_t* __restrict__ list;     // [j * cap + i] particle slots
     const uint32_t* __restrict__ count;    // [i]
     uint32_t cap;
@@ -320,33 +335,59 @@ struct LoadXSV {  // lo = {x_t, s_t}, hi = {v_t, -}
     }
 };
 
-// f(lo, hi) for every neighbour of particle i in list order, four list entries fetched per trip.
+// f(lo, hi, code) for every neighbour of particle i in list order, code = gcode(xi - x_j). Staged groups read
+// packed list entries (halo slot | code << 16, the code computed once per step when the halo is built) in chunks of
+// kListChunk independent loads, and (VFD_LIST_PREFETCH) request the next chunk before consuming the current one, so
+// list latency overlaps the LDS work; the global path fetches four list entries per trip so that their gathers can
+// be in flight together, and computes the code itself.
+#ifndef VFD_LIST_CHUNK
+#define VFD_LIST_CHUNK 8
+#endif
+#ifndef VFD_LIST_PREFETCH
+#define VFD_LIST_PREFETCH 1
+#endif
+constexpr uint32_t kListChunk = VFD_LIST_CHUNK;
 template <int R, class Load, class Fn>
-__device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i, const Load& load, Fn&& f) {
+__device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i, float cutR, float invStep, f3 xi,
+                                                  const Load& load, Fn&& f) {
     const uint32_t n = src.count[i];
     const size_t cap = src.cap;
     uint32_t j = 0;
     if (src.rec) {
         const float4* rec = src.rec;
-        for (; j + 4 <= n; j += 4) {
-            uint32_t s[4];
+        const uint32_t* __restrict__ li = src.local + i;
+        auto fetch = [&](uint32_t (&s)[kListChunk], uint32_t b) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) s[u] = src.local[(j + u) * cap + i];
-            float4 lo[4], hi[4];
+            for (uint32_t u = 0; u < kListChunk; ++u) s[u] = b + u < n ? li[(b + u) * cap] : 0u;
+        };
+        auto consume = [&](const uint32_t (&s)[kListChunk], uint32_t b) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                lo[u] = rec[R * s[u]];
-                hi[u] = R == 2 ? rec[R * s[u] + 1] : lo[u];
+            for (uint32_t u = 0; u < kListChunk; ++u) {
+                if (b + u < n) {
+                    const uint32_t slot = s[u] & 0xFFFFu;
+                    const float4 lo = rec[R * slot];
+                    const float4 hi = R == 2 ? rec[R * slot + 1] : lo;
+                    f(lo, hi, s[u] >> 16);
+                }
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) f(lo[u], hi[u]);
+        };
+#if VFD_LIST_PREFETCH
+        // two chunks in flight: the next chunk's entries are requested before the current chunk is consumed
+        uint32_t s0[kListChunk], s1[kListChunk];
+        fetch(s0, 0);
+        for (; j < n; j += 2 * kListChunk) {
+            if (j + kListChunk < n) fetch(s1, j + kListChunk);
+            consume(s0, j);
+            if (j + 2 * kListChunk < n) fetch(s0, j + 2 * kListChunk);
+            if (j + kListChunk < n) consume(s1, j + kListChunk);
         }
-        for (; j < n; ++j) {
-            const uint32_t s = src.local[j * cap + i];
-            const float4 lo = rec[R * s];
-            const float4 hi = R == 2 ? rec[R * s + 1] : lo;
-            f(lo, hi);
+#else
+        for (; j < n; j += kListChunk) {
+            uint32_t s0[kListChunk];
+            fetch(s0, j);
+            consume(s0, j);
         }
+#endif
     } else {
         for (; j + 4 <= n; j += 4) {
             uint32_t t[4];
@@ -356,12 +397,12 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
 #pragma unroll
             for (int u = 0; u < 4; ++u) load(t[u], lo[u], hi[u]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) f(lo[u], hi[u]);
+            for (int u = 0; u < 4; ++u) f(lo[u], hi[u], gcode(cutR, invStep, xi - ld3(lo[u])));
         }
         for (; j < n; ++j) {
             float4 lo, hi;
             load(src.list[j * cap + i], lo, hi);
-            f(lo, hi);
+            f(lo, hi, gcode(cutR, invStep, xi - ld3(lo)));
         }
     }
 }
@@ -386,13 +427,22 @@ __device__ __forceinline__ void group_loop(const TileGrid& tg, const HaloView& H
         const uint32_t tile = g * kGroupTiles + sub;
         const uint32_t m = H.count[g];
         if (m <= kHaloMax) {
+            // every thread copies up to four records; the four slot loads are issued before any gather
+            static_assert(kHaloMax <= 4 * kGroupSize, "fill covers four records per thread");
             const uint32_t* ids = H.ids + (size_t)g * kHaloMax;
-            for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+            const uint32_t k0 = threadIdx.x, k1 = k0 + kGroupSize, k2 = k1 + kGroupSize, k3 = k2 + kGroupSize;
+            const uint32_t t0 = k0 < m ? ids[k0] : 0u, t1 = k1 < m ? ids[k1] : 0u;
+            const uint32_t t2 = k2 < m ? ids[k2] : 0u, t3 = k3 < m ? ids[k3] : 0u;
+            auto copy = [&](uint32_t k, uint32_t t) {
                 float4 lo, hi;
-                load(ids[k], lo, hi);
+                load(t, lo, hi);
                 rec[R * k] = lo;
                 if (R == 2) rec[R * k + 1] = hi;
-            }
+            };
+            if (k0 < m) copy(k0, t0);
+            if (k1 < m) copy(k1, t1);
+            if (k2 < m) copy(k2, t2);
+            if (k3 < m) copy(k3, t3);
             __syncthreads();
             NbrSrc src = global;
             src.rec = rec;

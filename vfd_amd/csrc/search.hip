@@ -154,11 +154,25 @@ __global__ void __launch_bounds__(kTile) k_reorder(const SceneParams* __restrict
     idOut[dst] = me;
 }
 
+#ifndef VFD_SEARCH_UNROLL
+#define VFD_SEARCH_UNROLL 8
+#endif
+#ifndef VFD_STAGE_LISTS
+#define VFD_STAGE_LISTS 0
+#endif
+#ifndef VFD_SEARCH_PREFETCH
+#define VFD_SEARCH_PREFETCH 1
+#endif
+constexpr uint32_t kSearchUnroll = VFD_SEARCH_UNROLL;
+
+// FindNeighbors (ParticleSearch.cu:95-195; kernels ParticleSearchKernels.cu:93-193) in one traversal: the 27 cells
+// in z, y, x order, each cell's particles in slot order (= ascending original id), strict 0 < d^2 < R^2, at most
+// kMaxNeighbors. out(cnt, t) receives the cnt-th neighbour.
+template <class Out>
 __device__ __forceinline__ uint32_t neighbor_list(const TileCtx& c, const SceneParams* __restrict__ P,
                                                   const SceneGrid* __restrict__ G, const float4* __restrict__ pos,
                                                   const uint32_t* __restrict__ cellStart,
-                                                  const uint32_t* __restrict__ cellCount, uint32_t* nbrList,
-                                                  uint32_t capacity) {
+                                                  const uint32_t* __restrict__ cellCount, Out&& out) {
     const SceneGrid& g = G[c.s];
     const float R2 = P[c.s].R * P[c.s].R;
     const float4 xi = pos[c.i];
@@ -166,6 +180,25 @@ __device__ __forceinline__ uint32_t neighbor_list(const TileCtx& c, const SceneP
     const int gy = (int)floorf((xi.y - g.gridMin[1]) * g.gridDelta[1]);
     const int gz = (int)floorf((xi.z - g.gridMin[2]) * g.gridDelta[2]);
     uint32_t cnt = 0;
+#if VFD_SEARCH_PREFETCH
+    // the ranges of all 27 cells are requested before the first candidate is loaded (one latency, not 27)
+    uint32_t cs[27], cn[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const int cx = gx + k % 3 - 1, cy = gy + (k / 3) % 3 - 1, cz = gz + k / 9 - 1;
+        const bool in = cx >= 0 && cx < (int)g.dims[0] && cy >= 0 && cy < (int)g.dims[1] && cz >= 0 &&
+                        cz < (int)g.dims[2];
+        const uint32_t key = in ? cell_key(g, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz) : 0u;
+        cn[k] = in ? cellCount[key] : 0u;
+        cs[k] = in ? (uint32_t)((int32_t)cellStart[key] + g.startCorr) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        {
+            {
+                const uint32_t start = cs[k];
+                const uint32_t end = start + cn[k];
+#else
     for (int dz = -1; dz < 2; ++dz) {
         const int cz = gz + dz;
         if (cz < 0 || cz >= (int)g.dims[2]) continue;
@@ -178,13 +211,23 @@ __device__ __forceinline__ uint32_t neighbor_list(const TileCtx& c, const SceneP
                 const uint32_t key = cell_key(g, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
                 const uint32_t n = cellCount[key];
                 const uint32_t start = (uint32_t)((int32_t)cellStart[key] + g.startCorr);
-                for (uint32_t t = start; t < start + n; ++t) {
-                    const float4 xj = pos[t];
-                    const float ddx = xj.x - xi.x, ddy = xj.y - xi.y, ddz = xj.z - xi.z;
-                    const float d2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-                    if (d2 < R2 && d2 > 0.0f) {
-                        nbrList[(size_t)cnt * capacity + c.i] = t;
-                        if (++cnt == kMaxNeighbors) goto done;
+                const uint32_t end = start + n;
+#endif
+                // kSearchUnroll candidate positions are loaded before any is tested (one latency per batch)
+                for (uint32_t t0 = start; t0 < end; t0 += kSearchUnroll) {
+                    float4 xj[kSearchUnroll];
+#pragma unroll
+                    for (uint32_t u = 0; u < kSearchUnroll; ++u)
+                        if (t0 + u < end) xj[u] = pos[t0 + u];
+#pragma unroll
+                    for (uint32_t u = 0; u < kSearchUnroll; ++u) {
+                        if (t0 + u >= end) break;
+                        const float ddx = xj[u].x - xi.x, ddy = xj[u].y - xi.y, ddz = xj[u].z - xi.z;
+                        const float d2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+                        if (d2 < R2 && d2 > 0.0f) {
+                            out(cnt, t0 + u);
+                            if (++cnt == kMaxNeighbors) goto done;
+                        }
                     }
                 }
             }
@@ -194,8 +237,10 @@ done:
     return cnt;
 }
 
-// ComputeCountsKernel + NeighborhoodQueryWithCountsKernel (ParticleSearchKernels.cu:93-193) in one pass: lists are
-// written to a fixed capacity of 70 per particle, transposed.
+// With VFD_STAGE_LISTS the lists are staged in LDS ([kMaxNeighbors][kTile] per block) while the traversals run,
+// then written row by row, so every global store of a row is one coalesced 1 KB write of the tile; otherwise each
+// entry is stored as it is found.
+constexpr unsigned kLdsListStage = VFD_STAGE_LISTS ? kMaxNeighbors * kTile * sizeof(uint32_t) : 0u;
 __global__ void __launch_bounds__(kTile) k_build_lists(const float4* __restrict__ pos, const SceneParams* __restrict__ P,
                                                        const SceneGrid* __restrict__ G,
                                                        const uint32_t* __restrict__ tileScene,
@@ -203,10 +248,28 @@ __global__ void __launch_bounds__(kTile) k_build_lists(const float4* __restrict_
                                                        const uint32_t* __restrict__ cellCount, uint32_t* nbrCount,
                                                        uint32_t* nbrList, uint32_t capacity,
                                                        unsigned long long* nbStats) {
+    extern __shared__ uint32_t stage[];  // [kMaxNeighbors][kTile]
+    __shared__ uint32_t rows;
     const TileCtx c = tile_ctx(tileScene, P);
+    if (threadIdx.x == 0) rows = 0;
+    __syncthreads();
     uint32_t cnt = 0;
-    if (c.valid) cnt = neighbor_list(c, P, G, pos, cellStart, cellCount, nbrList, capacity);
-    if (c.valid) nbrCount[c.i] = cnt;
+    if (c.valid) {
+        cnt = neighbor_list(c, P, G, pos, cellStart, cellCount, [&](uint32_t j, uint32_t t) {
+            if (VFD_STAGE_LISTS)
+                stage[j * kTile + threadIdx.x] = t;
+            else
+                nbrList[(size_t)j * capacity + c.i] = t;
+        });
+        nbrCount[c.i] = cnt;
+        if (cnt) atomicMax(&rows, cnt);
+    }
+    if (VFD_STAGE_LISTS) {
+        __syncthreads();
+        const uint32_t nr = rows;
+        for (uint32_t j = 0; j < nr; ++j)
+            if (j < cnt) nbrList[(size_t)j * capacity + c.i] = stage[j * kTile + threadIdx.x];
+    }
     // per-scene total and maximum neighbour count (debug statistics)
     int tot = (int)cnt, mx = (int)cnt;
 #pragma unroll
@@ -338,10 +401,11 @@ __device__ __forceinline__ uint32_t bucket_hash(uint32_t b) { return (b * 0x9E37
 
 __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __restrict__ P,
                                                            const uint32_t* __restrict__ tileScene, uint32_t tiles,
+                                                           float cutR, float invStep, const float4* __restrict__ pos,
                                                            const uint32_t* __restrict__ nbrCount,
                                                            const uint32_t* __restrict__ nbrList, uint32_t cap,
                                                            uint32_t limit, uint32_t* haloCount, uint32_t* haloIds,
-                                                           uint16_t* nbrLocal, uint32_t* haloStats) {
+                                                           uint32_t* nbrLocal, uint32_t* haloStats) {
     __shared__ uint32_t bkey[kBucketSlots];
     __shared__ uint32_t bmask[kBucketSlots];
     __shared__ uint16_t bbase[kBucketSlots];
@@ -436,6 +500,10 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
         atomicMax(&haloStats[2], m);
     }
     __syncthreads();
+    // packed entry: halo number | gradient table code of the pair << 16 (the code the neighbour loops would compute
+    // from x_i - x_j)
+    f3 xi = mk3(0.0f, 0.0f, 0.0f);
+    if (n) xi = ld3(pos[i]);
     for (uint32_t j = 0; j < nr; ++j) {
         if (j >= n) continue;
         const size_t e = (size_t)j * cap + i;
@@ -443,16 +511,18 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
         const uint32_t b = t >> 5, below = (1u << (t & 31u)) - 1u;
         uint32_t h = bucket_hash(b);
         while (bkey[h] != b) h = (h + 1) & (kBucketSlots - 1);
-        nbrLocal[e] = (uint16_t)(bbase[h] + __popc(bmask[h] & below));
+        const uint32_t code = gcode(cutR, invStep, xi - ld3(pos[t]));
+        nbrLocal[e] = (bbase[h] + __popc(bmask[h] & below)) | (code << 16);
     }
 }
 
-void launch_build_halo(const Launch& L, const uint32_t* nbrCount, const uint32_t* nbrList, uint32_t haloLimit,
-                       uint32_t* haloCount, uint32_t* haloIds, uint16_t* nbrLocal, uint32_t* haloStats) {
+void launch_build_halo(const Launch& L, const KTabHost& kt, const float4* pos, const uint32_t* nbrCount,
+                       const uint32_t* nbrList, uint32_t haloLimit, uint32_t* haloCount, uint32_t* haloIds,
+                       uint32_t* nbrLocal, uint32_t* haloStats) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, L.stream, haloStats, 0u, 3u);
-    hipLaunchKernelGGL(k_build_halo, dim3(L.groups), dim3(kGroupSize), 0, L.stream, L.P, L.tileScene, L.tiles,
-                       nbrCount, nbrList, L.capacity, haloLimit < kHaloMax ? haloLimit : kHaloMax, haloCount, haloIds,
-                       nbrLocal, haloStats);
+    hipLaunchKernelGGL(k_build_halo, dim3(L.groups), dim3(kGroupSize), 0, L.stream, L.P, L.tileScene, L.tiles, kt.R,
+                       kt.invStep, pos, nbrCount, nbrList, L.capacity, haloLimit < kHaloMax ? haloLimit : kHaloMax,
+                       haloCount, haloIds, nbrLocal, haloStats);
 }
 
 void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, const uint32_t* cellStart,
@@ -460,8 +530,14 @@ void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, 
                         unsigned long long* nbStats) {
     hipLaunchKernelGGL(k_fill_u64, dim3((2 * L.sceneCount + 63) / 64), dim3(64), 0, L.stream, nbStats,
                        2 * L.sceneCount);
-    hipLaunchKernelGGL(k_build_lists, dim3(L.tiles), dim3(kTile), 0, L.stream, pos, L.P, G, L.tileScene, cellStart,
-                       cellCount, nbrCount, nbrList, L.capacity, nbStats);
+    hipLaunchKernelGGL(k_build_lists, dim3(L.tiles), dim3(kTile), kLdsListStage, L.stream, pos, L.P, G, L.tileScene,
+                       cellStart, cellCount, nbrCount, nbrList, L.capacity, nbStats);
+}
+
+int configure_search_kernels() {
+    if (kLdsListStage == 0) return 0;
+    return hipFuncSetAttribute((const void*)k_build_lists, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kLdsListStage) == hipSuccess ? 0 : 1;
 }
 
 }  // namespace vfd

@@ -24,6 +24,14 @@
 #include "common.h"
 #include "kernels.h"
 
+// run-time knob defaults (vfd_set_tuning); vfd_amd/build.py passes the values chosen by vfd_amd/autotune.py
+#ifndef VFD_DEFAULT_CHECK_INTERVAL
+#define VFD_DEFAULT_CHECK_INTERVAL 4
+#endif
+#ifndef VFD_DEFAULT_HALO_LIMIT
+#define VFD_DEFAULT_HALO_LIMIT vfd::kHaloMax
+#endif
+
 namespace vfd {
 
 static thread_local std::string g_lastError;
@@ -324,15 +332,15 @@ public:
     DevBuf<float> partialsF;
     DevBuf<unsigned long long> nbStats;
     DevBuf<uint32_t> haloCount, haloIds, haloStats;
-    DevBuf<uint16_t> nbrLocal;
-    uint32_t haloLimit = kHaloMax;  // largest halo staged in LDS (tuning)
+    DevBuf<uint32_t> nbrLocal;
+    uint32_t haloLimit = VFD_DEFAULT_HALO_LIMIT;  // largest halo staged in LDS (tuning)
     uint32_t cellCap = 0;
     hipEvent_t ev[8] = {};
     bool eventsMade = false;
     // solver polling: live[which] is copied into one of two pinned slots after each chunk of iterations
     uint32_t* hostLive = nullptr;  // [2]
     hipEvent_t pollEv[2] = {};
-    uint32_t pollChunk = 4;  // iterations enqueued between convergence checks
+    uint32_t pollChunk = VFD_DEFAULT_CHECK_INTERVAL;  // iterations enqueued between convergence checks
 
     // per-kernel timing: an event pair around each launch, resolved at the end of a step
     struct KStat {
@@ -823,9 +831,6 @@ public:
         VFD_TRY(timed("build_lists", [&] {
             launch_build_lists(L, dGrid.p, pos.p, cellStart.p, cellCount.p, nbrCount.p, nbrList.p, nbStats.p);
         }));
-        VFD_TRY(timed("build_halo", [&] {
-            launch_build_halo(L, nbrCount.p, nbrList.p, haloLimit, haloCount.p, haloIds.p, nbrLocal.p, haloStats.p);
-        }));
         VFD_HIP(hipGetLastError());
         return VFD_OK;
     }
@@ -873,6 +878,12 @@ public:
         if (last == VFD_PHASE_SEARCH) return finish_partial();
         Fields F = fields();
         VFD_TRY(timed("boundary", [&] { launch_boundary(L, dMaps.p, pos.p, vel[cur].p, xb.p, vb.p); }));
+        // halos and the gradient codes of the staged lists need the positions the solvers will use, i.e. after the
+        // boundary has pushed penetrating particles out (the lists themselves come from the positions before)
+        VFD_TRY(timed("build_halo", [&] {
+            launch_build_halo(L, K, pos.p, nbrCount.p, nbrList.p, haloLimit, haloCount.p, haloIds.p, nbrLocal.p,
+                              haloStats.p);
+        }));
         if (last == VFD_PHASE_BOUNDARY) return finish_partial();
         VFD_TRY(timed("density_factor", [&] { launch_density_factor(L, K, F); }));
         VFD_HIP(hipEventRecord(ev[2], stream));
