@@ -4,6 +4,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config dam_break_2m] [--no-cpu-baseline]
     python bench.py --copy-probe                      # plain 512 MiB device copies, for counter calibration
     python bench.py --pmc-summary FETCH WRITE COPY_FETCH COPY_WRITE   # rocprofv3 --pmc output dirs -> traffic table
+    python bench.py --trace-summary DIR               # rocprofv3 --kernel-trace output -> per-kernel means
 
 Prints one JSON line. A step is one full DFSPH step (neighbour search, boundary, density/factor, divergence solve,
 viscosity solve, time step, pressure solve, integration) of every particle; particles stay resident in HBM and frame
@@ -118,17 +119,49 @@ def cpu_baseline(cfg, snap, max_steps, budget_s):
             "seconds": dt, "x1": x1, "iterations": its, "steps": len(its)}
 
 
-def _pmc_table(path):
-    """{kernel: {counter: mean value}} from a rocprofv3 --pmc counter_collection.csv (or the directory holding it)."""
+LIVE_US = 20.0  # loop-kernel dispatches shorter than this exited at once (issued after convergence)
+LOOP_KERNELS = {"k_div_accel", "k_div_iter", "k_pres_accel", "k_pres_iter", "k_cg_matvec", "k_cg_update",
+                "k_cg_direction"}
+
+
+def _find(path, suffix):
     if os.path.isdir(path):
-        found = [os.path.join(dp, f) for dp, _, fs in os.walk(path) for f in fs if f.endswith("counter_collection.csv")]
-        path = found[0]
+        return [os.path.join(dp, f) for dp, _, fs in os.walk(path) for f in fs if f.endswith(suffix)][0]
+    return path
+
+
+def _kname(raw):
+    return re.sub(r"^(?:void )?(?:vfd::)?", "", raw).split("(")[0]
+
+
+def _pmc_table(path):
+    """{kernel: {counter: mean value}} from a rocprofv3 --pmc counter_collection.csv (or the directory holding it),
+    over the live dispatches (>= LIVE_US) of the solver-loop kernels and every dispatch of the others."""
     acc = {}
-    for r in csv.DictReader(open(path)):
-        k = re.sub(r"^(?:void )?(?:vfd::)?", "", r["Kernel_Name"]).split("(")[0]
+    for r in csv.DictReader(open(_find(path, "counter_collection.csv"))):
+        us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0
+        k = _kname(r["Kernel_Name"])
+        if us < LIVE_US and k in LOOP_KERNELS:
+            continue
         d = acc.setdefault(k, {}).setdefault(r["Counter_Name"], [])
         d.append(float(r["Counter_Value"]))
     return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in acc.items()}
+
+
+def trace_summary(path):
+    """Markdown table from a rocprofv3 --kernel-trace kernel_trace.csv: every dispatch, and the live ones."""
+    acc = {}
+    for r in csv.DictReader(open(_find(path, "kernel_trace.csv"))):
+        us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0
+        acc.setdefault(_kname(r["Kernel_Name"]), []).append(us)
+    lines = ["| kernel | dispatches | mean us (all) | live dispatches (>= %g us) | mean us (live) | total ms |" % LIVE_US,
+             "|---|---:|---:|---:|---:|---:|"]
+    for k, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        live = [x for x in v if x >= LIVE_US]
+        lines.append("| %s | %d | %.2f | %d | %s | %.3f |" % (k, len(v), sum(v) / len(v), len(live),
+                                                           "%.2f" % (sum(live) / len(live)) if live else "-",
+                                                           sum(v) / 1000.0))
+    return "\n".join(lines)
 
 
 def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
@@ -183,6 +216,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"))
+    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_DIR")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the DFSPH step runs on a single MI355X (multi-GPU sharding is a design, "
@@ -190,6 +224,9 @@ def main():
         sys.exit(2)
     if args.pmc_summary:
         print(pmc_summary(*args.pmc_summary))
+        return
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
         return
     if args.copy_probe:
         copy_probe()
