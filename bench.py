@@ -17,7 +17,9 @@ launch of that kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
 `cpu_baseline` times the CPU oracle (FP32, OpenMP over the host cores) from the same snapshot for as many of the
 timed steps as fit in --cpu-seconds, and reports the GPU rate over exactly those steps beside it; the GPU's first
 step from the snapshot is compared with the oracle's for a full-size parity check. The default K = 95 after W = 5
-warm-up steps covers the 100-step job.
+warm-up steps covers the 100-step job. `batched_gain` (second figure of merit) runs config C4, 64 uneven scenes, as
+one batch and as one simulation per scene on the same GPU: value = batched throughput / one-scene-per-launch
+throughput over the same steps.
 """
 import argparse
 import csv
@@ -195,6 +197,43 @@ def committed_traffic(config, kernel):
     return None
 
 
+def batched_gain(steps, warmup, config="batched_uneven_64"):
+    """Config C4 both ways on the same GPU: one DFSPHBatch advancing every scene with one set of launches, and one
+    DFSPHSimulation per scene stepped in turn (one scene per launch). Both run the same `steps` steps after the same
+    warm-up and must end in the same state, bit for bit."""
+    import numpy as np
+    from vfd_amd import DFSPHBatch, DFSPHSimulation, scenes
+    descs, fluids, bodies = scenes.CONFIGS[config]()
+    n = sum(len(f[0]) for f in fluids)
+    batch = DFSPHBatch(descs)
+    for s, (f, b) in enumerate(zip(fluids, bodies)):
+        batch.scenes[s].set_fluid(*f)
+        batch.scenes[s].SetRigidBodies(b)
+    batch.set_frame_capture(False)
+    batch.advance(warmup)
+    t0 = time.perf_counter()
+    batch.advance(steps)
+    tb = time.perf_counter() - t0
+    singles = []
+    for d, f, b in zip(descs, fluids, bodies):
+        sim = DFSPHSimulation(d)
+        sim.set_fluid(*f)
+        sim.SetRigidBodies(b)
+        sim.set_frame_capture(False)
+        sim.advance(warmup)
+        singles.append(sim)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sim in singles:
+            sim.OnUpdate()
+    ts = time.perf_counter() - t0
+    same = all(np.array_equal(batch.scenes[s].field("position"), singles[s].field("position"))
+               for s in range(len(singles)))
+    return {"scenes": len(descs), "particles": n, "steps": steps, "warmup": warmup,
+            "batched_updates_per_s": n * steps / tb, "one_scene_per_launch_updates_per_s": n * steps / ts,
+            "value": ts / tb, "identical_states": bool(same)}
+
+
 def copy_probe():
     import torch
     n = COPY_PROBE_BYTES // 4
@@ -215,6 +254,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-gain measurement (config C4)")
+    ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"))
     ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_DIR")
     args = ap.parse_args()
@@ -334,6 +375,10 @@ def main():
             "iterations_gpu": check_its, "iterations_cpu": cb["iterations"][0]}
     else:
         out["cpu_baseline"] = None
+    if not args.no_batched:
+        del sim
+        print("[bench] batched gain: C4, %d steps" % args.batched_steps, file=sys.stderr, flush=True)
+        out["batched_gain"] = batched_gain(args.batched_steps, 3)
     print(json.dumps(out), flush=True)
 
 

@@ -12,7 +12,7 @@
  *     library keeps particles internally in Morton (cell) order and maps back on every read.
  *   - A vfd_sim holds one or more scenes ("batch"). The plain functions act on scene 0; vfd_batch_* take a scene index.
  *   - One vfd_sim must not be used from two threads at once. Work is enqueued on the stream set with vfd_set_stream
- *     (default: the null stream of the current device).
+ *     (default: a non-blocking stream the simulation creates on the current device).
  */
 #ifndef VFD_H
 #define VFD_H
@@ -224,7 +224,9 @@ int vfd_create(const vfd_desc* desc, vfd_sim** out);
 int vfd_batch_create(const vfd_desc* descs, uint32_t scene_count, vfd_sim** out);
 /* ~DFSPHImplementation (DFSPHImplementation.cu:23-31). */
 int vfd_destroy(vfd_sim* sim);
-/* hipStream_t passed as void*. No reference counterpart (the reference uses the default stream). */
+/* hipStream_t passed as void*; NULL selects the simulation's own stream again. No reference counterpart (the
+ * reference uses the default stream). Solver-loop graphs are captured on this stream, so it must not be the legacy
+ * null stream for them to be used. */
 int vfd_set_stream(vfd_sim* sim, void* stream);
 int vfd_scene_count(const vfd_sim* sim, uint32_t* count);
 
@@ -303,6 +305,7 @@ int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint3
 typedef struct vfd_tuning {
     uint32_t SolverCheckInterval; /* Jacobi / CG iterations enqueued between two convergence checks, >= 1 */
     uint32_t HaloLimit;           /* largest group halo staged in LDS (<= 3808); 0: every neighbour from HBM */
+    uint32_t UseGraphs;           /* 1: each chunk of iterations is one launch of a captured HIP graph */
 } vfd_tuning;
 int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out);
 int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* tuning);

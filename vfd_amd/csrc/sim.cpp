@@ -25,6 +25,9 @@
 #include "kernels.h"
 
 // run-time knob defaults (vfd_set_tuning); vfd_amd/build.py passes the values chosen by vfd_amd/autotune.py
+#ifndef VFD_DEFAULT_USE_GRAPHS
+#define VFD_DEFAULT_USE_GRAPHS 1
+#endif
 #ifndef VFD_DEFAULT_CHECK_INTERVAL
 #define VFD_DEFAULT_CHECK_INTERVAL 4
 #endif
@@ -341,6 +344,20 @@ public:
     uint32_t* hostLive = nullptr;  // [2]
     hipEvent_t pollEv[2] = {};
     uint32_t pollChunk = VFD_DEFAULT_CHECK_INTERVAL;  // iterations enqueued between convergence checks
+    // One chunk of pollChunk iterations of each solver loop, captured as a HIP graph on first use. The launches'
+    // arguments differ only with the double-buffered particle arrays (index cur), so there is one graph per
+    // (solver, cur); they are dropped whenever buffers are reallocated or the chunk length changes.
+    bool useGraphs = VFD_DEFAULT_USE_GRAPHS != 0;
+    hipGraphExec_t loopGraph[3][2] = {};
+    hipStream_t ownStream = nullptr;  // default stream (capture needs a stream other than the legacy null stream)
+    void drop_graphs() {
+        for (auto& row : loopGraph)
+            for (auto& g : row)
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
+    }
 
     // per-kernel timing: an event pair around each launch, resolved at the end of a step
     struct KStat {
@@ -365,6 +382,8 @@ public:
             for (auto& e : pollEv) (void)hipEventDestroy(e);
         }
         if (hostLive) (void)hipHostFree(hostLive);
+        drop_graphs();
+        if (ownStream) (void)hipStreamDestroy(ownStream);
         for (auto& e : evPool) (void)hipEventDestroy(e);
         for (Scene* s : sc) delete s;
     }
@@ -444,10 +463,13 @@ public:
         for (auto& e : pollEv) VFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         eventsMade = true;
         VFD_HIP(hipHostMalloc((void**)&hostLive, 2 * sizeof(uint32_t), hipHostMallocDefault));
+        VFD_HIP(hipStreamCreateWithFlags(&ownStream, hipStreamNonBlocking));
+        stream = ownStream;
         return VFD_OK;
     }
 
     int build_tables(float r) {
+        drop_graphs();  // the table address is a captured launch argument
         kt.set_radius(4.0f * r);
         const std::vector<float2> avg = kt.averaged();
         VFD_TRY(dTab.alloc(avg.size()));
@@ -600,6 +622,7 @@ public:
 
     // Allocates the packed particle arrays for the current scenes and uploads the baked particles.
     int commit() {
+        drop_graphs();
         tiles = 0;
         for (Scene* s : sc) {
             s->base = tiles * kTile;
@@ -843,7 +866,22 @@ public:
     int poll_loop(Fn enqueue, uint32_t first, int which) {
         uint32_t iter = 0;
         auto issue = [&](uint32_t n, int slot) -> int {
-            for (uint32_t c = 0; c < n; ++c) enqueue(iter++);
+            if (useGraphs && !profiling && n == pollChunk && stream != nullptr) {
+                hipGraphExec_t& g = loopGraph[which][cur];
+                if (!g) {
+                    hipGraph_t graph = nullptr;
+                    VFD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                    for (uint32_t c = 0; c < n; ++c) enqueue(iter + c);
+                    VFD_HIP(hipStreamEndCapture(stream, &graph));
+                    const hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    VFD_HIP(e);
+                }
+                VFD_HIP(hipGraphLaunch(g, stream));
+                iter += n;
+            } else {
+                for (uint32_t c = 0; c < n; ++c) enqueue(iter++);
+            }
             VFD_HIP(hipMemcpyAsync(hostLive + slot, live.p + which, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
             VFD_HIP(hipEventRecord(pollEv[slot], stream));
             return VFD_OK;
@@ -1257,7 +1295,8 @@ int vfd_destroy(vfd_sim* sim) {
 }
 int vfd_set_stream(vfd_sim* sim, void* stream) {
     CHECK_SIM(sim);
-    sim->impl.stream = (hipStream_t)stream;
+    sim->impl.drop_graphs();
+    sim->impl.stream = stream ? (hipStream_t)stream : sim->impl.ownStream;
     return VFD_OK;
 }
 int vfd_scene_count(const vfd_sim* sim, uint32_t* count) {
@@ -1481,6 +1520,7 @@ int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out) {
     if (!out) return fail(VFD_ERROR_INVALID_ARGUMENT, "out must not be NULL");
     out->SolverCheckInterval = sim->impl.pollChunk;
     out->HaloLimit = sim->impl.haloLimit;
+    out->UseGraphs = sim->impl.useGraphs ? 1u : 0u;
     return VFD_OK;
 }
 int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* t) {
@@ -1490,8 +1530,10 @@ int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* t) {
         return fail(VFD_ERROR_INVALID_ARGUMENT, "SolverCheckInterval must be in [1, 1024]");
     if (t->HaloLimit > vfd::kHaloMax)
         return fail(VFD_ERROR_INVALID_ARGUMENT, "HaloLimit must be at most " + std::to_string(vfd::kHaloMax));
+    if (t->SolverCheckInterval != sim->impl.pollChunk) sim->impl.drop_graphs();
     sim->impl.pollChunk = t->SolverCheckInterval;
     sim->impl.haloLimit = t->HaloLimit;
+    sim->impl.useGraphs = t->UseGraphs != 0;
     return VFD_OK;
 }
 int vfd_get_kernel_table(vfd_sim* sim, float* w, float* gradient_w, float* scalars) {
