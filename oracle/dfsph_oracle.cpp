@@ -20,6 +20,12 @@
 //               index (the reference's order there follows atomicAdd, ParticleSearchKernels.cu:77)
 //
 // Parity status of each function is stated in the function comments ("pinned by ..." names the test that pins it).
+// The reference adjoint (Sim::reverse_step) is pinned by central finite differences of the smooth-kernel forward
+// (tests/test_adjoint_oracle.py).
+//
+// Parity unpinned (pinnable only by running the reference's CUDA build, which cannot be built here): the order of
+// thrust's `minus` reduction for the solver errors, fast-math sqrt/division rounding, and the atomicAdd order inside a
+// cell that decides which neighbours fall past the cap of 70. This oracle fixes all three deterministically.
 
 #include <algorithm>
 #include <cfloat>
@@ -30,6 +36,7 @@
 #include <limits>
 #include <string>
 #include <vector>
+#include <type_traits>
 
 #include "../include/vfd.h"
 
@@ -180,6 +187,7 @@ template <class T> struct CubicKernel {
         return T(0);
     }
     T getW(V3<T> r) const {  // vector overload, cutoff |r|^2 <= R^2
+        if (smooth) return calcW(std::sqrt(dot3(r, r)));
         const T r2 = dot3(r, r);
         if (r2 <= R2) {
             const unsigned p = index(std::sqrt(r2));
@@ -188,6 +196,7 @@ template <class T> struct CubicKernel {
         return T(0);
     }
     V3<T> gradW(V3<T> r) const {  // cutoff |r| <= R
+        if (smooth) return calcGradW(r);
         const T rl = std::sqrt(dot3(r, r));
         if (rl <= R) {
             const unsigned p = index(rl);
@@ -195,6 +204,32 @@ template <class T> struct CubicKernel {
             return s * r;
         }
         return v3<T>(0, 0, 0);
+    }
+
+    // ---- adjoint support -------------------------------------------------------------------------------------
+    // smooth = true evaluates W and gradW from the spline itself instead of the table (a differentiable forward, used
+    // only to pin the adjoint against finite differences).
+    bool smooth = false;
+    // gradW(r) / r: the table value the forward uses (or the spline's value when smooth)
+    T gval(T rl) const {
+        if (smooth) return rl > T(1.0e-5) ? calcGradW(v3<T>(rl, 0, 0)).x / rl : T(0);
+        if (!(rl <= R)) return T(0);
+        const unsigned p = index(rl);
+        return T(0.5) * (G[p] + G[p + 1]);
+    }
+    // derivatives of the spline (the table's smooth counterpart): dW/drho and d(gradW/r)/drho
+    T dW(T rl) const {
+        const T q = rl / R;
+        if (!(q <= T(1))) return T(0);
+        if (q <= T(0.5)) return K * (T(18) * q * q - T(12) * q) / R;
+        return -T(6) * K * (T(1) - q) * (T(1) - q) / R;
+    }
+    T dG(T rl) const {
+        const T q = rl / R;
+        if (!(q <= T(1)) || !(rl > T(1.0e-5))) return T(0);
+        if (q <= T(0.5)) return T(3) * L / (R * R * R);
+        const T f = T(1) - q;
+        return (L / R) * (T(2) * f / (R * rl) + f * f / (rl * rl));
     }
 };
 
@@ -753,6 +788,21 @@ public:
     T maxVel = 0;
     bool kernelSet = false;
 
+    // Per-step record of everything the adjoint needs (frozen lists, clamp masks, solver iterates, time steps).
+    struct StepTape {
+        NeighborLists nb;
+        std::vector<V3<T>> x, v0, v1, y, v2, v3;  // positions; velocity at entry, after divergence, viscosity solution,
+                                                  // before and after the pressure solve
+        std::vector<T> alpha, D, rhoadv;
+        std::vector<uint8_t> m0, mp0;
+        std::vector<std::vector<T>> kv, kp;            // kappa iterates (K + 1 each)
+        std::vector<std::vector<uint8_t>> mv, mp;      // residual masks (K each)
+        T h = 0, H = 0;
+        bool visc = false;
+    };
+    std::vector<StepTape> tapes;
+    bool taping = false;
+
     explicit Sim(const vfd_desc& d) {
         halton = halton_table<T>();
         set_description(d);
@@ -1005,6 +1055,18 @@ public:
             p[i].factor *= info.dtInv;
             p[i].kappaV = d * p[i].factor;
         }
+        if (taping) {
+            StepTape& t = tapes.back();
+            t.h = info.dt;
+            t.D.resize(n);
+            t.m0.resize(n);
+            t.kv.assign(1, std::vector<T>(n));
+            for (long i = 0; i < n; ++i) {
+                t.D[i] = p[i].densityAdv;
+                t.m0[i] = cnt(i) >= 20u && velocity_divergence(i) > T(0);
+                t.kv[0][i] = p[i].kappaV;
+            }
+        }
         const T eta = info.dtInv * (T)desc.MaxDivergenceSolverError * T(0.0001) * info.rho0;
         unsigned it = 0;
         T err = 0;
@@ -1016,6 +1078,15 @@ public:
                 const T f = density_pressure_force(i) * info.dt;
                 p[i].residuum = cnt(i) < 20u ? T(0) : std::min(-p[i].densityAdv - f, T(0));
                 p[i].kappaV -= p[i].residuum * p[i].factor;
+            }
+            if (taping) {
+                StepTape& t = tapes.back();
+                t.mv.emplace_back(n);
+                t.kv.emplace_back(n);
+                for (long i = 0; i < n; ++i) {
+                    t.mv.back()[i] = p[i].residuum < T(0);
+                    t.kv.back()[i] = p[i].kappaV;
+                }
             }
             err = residual_error();
             ++it;
@@ -1041,6 +1112,17 @@ public:
             const T r = std::min(si, T(0));
             p[i].kappa = -r * p[i].factor;
         }
+        if (taping) {
+            StepTape& t = tapes.back();
+            t.rhoadv.resize(n);
+            t.mp0.resize(n);
+            t.kp.assign(1, std::vector<T>(n));
+            for (long i = 0; i < n; ++i) {
+                t.rhoadv[i] = p[i].densityAdv;
+                t.mp0[i] = T(1) - p[i].densityAdv < T(0);
+                t.kp[0][i] = p[i].kappa;
+            }
+        }
         const T eta = (T)desc.MaxPressureSolverError * T(0.0001) * info.rho0;
         unsigned it = 0;
         T err = 0;
@@ -1052,6 +1134,15 @@ public:
                 const T f = density_pressure_force(i) * info.dt2;
                 p[i].residuum = std::min(T(1) - p[i].densityAdv - f, T(0));
                 p[i].kappa -= p[i].residuum * p[i].factor;
+            }
+            if (taping) {
+                StepTape& t = tapes.back();
+                t.mp.emplace_back(n);
+                t.kp.emplace_back(n);
+                for (long i = 0; i < n; ++i) {
+                    t.mp.back()[i] = p[i].residuum < T(0);
+                    t.kp.back()[i] = p[i].kappa;
+                }
             }
             err = residual_error();
             ++it;
@@ -1342,23 +1433,38 @@ public:
         if (last == VFD_PHASE_SEARCH) return;
         boundary();
         if (last == VFD_PHASE_BOUNDARY) return;
+        if (taping) tape_begin();
         density();
         if (last == VFD_PHASE_DENSITY) return;
         factor();
+        if (taping) {
+            tapes.back().alpha.resize(n);
+            for (long i = 0; i < n; ++i) tapes.back().alpha[i] = p[i].factor;
+        }
         if (last == VFD_PHASE_FACTOR) return;
         divergence_solve();
+        if (taping) tape_velocity(tapes.back().v1);
         if (last == VFD_PHASE_DIVERGENCE) return;
         for (auto& q : p) q.a = info.g;  // ClearAccelerationKernel :24-37
         if (last == VFD_PHASE_CLEAR_ACCELERATION) return;
         surface_tension();
         if (last == VFD_PHASE_SURFACE_TENSION) return;
         viscosity_solve();
+        if (taping) {
+            tapes.back().visc = desc.EnableViscositySolver != 0;
+            tapes.back().y = vb_g;
+        }
         if (last == VFD_PHASE_VISCOSITY) return;
         time_step();
         if (last == VFD_PHASE_TIME_STEP) return;
         for (auto& q : p) q.v = q.v + info.dt * q.a;  // ComputeVelocityKernel :39-52
+        if (taping) {
+            tapes.back().H = info.dt;
+            tape_velocity(tapes.back().v2);
+        }
         if (last == VFD_PHASE_VELOCITY) return;
         pressure_solve();
+        if (taping) tape_velocity(tapes.back().v3);
         if (last == VFD_PHASE_PRESSURE) return;
         for (auto& q : p) q.x = q.x + info.dt * q.v;  // ComputePositionKernel :54-67
         if (dbg.FrameIndex < desc.FrameCount && dbg.FrameTime >= desc.FrameLength) {
@@ -1366,6 +1472,316 @@ public:
             dbg.FrameIndex++;
         }
         dbg.IterationCount++;
+    }
+
+    // ==========================================================================================================
+    // Adjoint of the step (reference gradient for the HIP adjoint; TEST INFRASTRUCTURE like the rest of this file).
+    //
+    // Differentiated map: one step (x, v; mu) -> (x', v') with the neighbour lists, the time steps, the solver
+    // iteration counts and the Jacobi clamp masks of the forward held fixed. Kernel values are the forward's (table
+    // or spline); their derivatives are the spline's (dW/drho, d(gradW/r)/drho), so with `smooth` kernels this is
+    // the exact derivative of the computed step and with the table it is the derivative of the step the table
+    // approximates. The Jacobi solves are reversed iteration by iteration (their transposed iteration), the viscosity
+    // solve by one transposed linear solve. Scope: no boundary contact, no surface tension, symmetric lists (no
+    // particle at the 70-neighbour cap); tape_begin reports anything outside it.
+    // ==========================================================================================================
+    std::string tape_error;
+    void tape_begin() {
+        tapes.emplace_back();
+        StepTape& t = tapes.back();
+        const long n = (long)p.size();
+        t.nb = nb;
+        t.x.resize(n);
+        for (long i = 0; i < n; ++i) t.x[i] = p[i].x;
+        tape_velocity(t.v0);
+        if (desc.EnableSurfaceTensionSolver) tape_error = "surface tension is outside the adjoint's scope";
+        for (auto& b : bodies)
+            for (long i = 0; i < n; ++i)
+                if (b.vb[i] > T(0)) tape_error = "boundary contact is outside the adjoint's scope";
+        for (long i = 0; i < n; ++i)
+            if (cnt(i) >= kMaxNeighbors) tape_error = "a particle reached the neighbour cap (lists not symmetric)";
+    }
+    void tape_velocity(std::vector<V3<T>>& out) const {
+        out.resize(p.size());
+        for (size_t i = 0; i < p.size(); ++i) out[i] = p[i].v;
+    }
+
+    // gradient-of-gradW contraction: d(c . G(|r|) r)/dr = G c + G'(|r|) (c . r)/|r| r
+    V3<T> vjp_grad(V3<T> r, V3<T> c) const {
+        const T rl = std::sqrt(dot3(r, r));
+        const T g = kernel.gval(rl);
+        V3<T> o = g * c;
+        if (rl > T(0)) o = o + (kernel.dG(rl) * dot3(c, r) / rl) * r;
+        return o;
+    }
+    // A(k)_i = -V sum_j (k_i + k_j) gradW_ij   (pressure acceleration of a kappa field)
+    void op_A(const StepTape& t, const std::vector<T>& k, std::vector<V3<T>>& out) const {
+        const long n = (long)t.x.size();
+        out.assign(n, v3<T>(0, 0, 0));
+        for (long i = 0; i < n; ++i) {
+            V3<T> a = v3<T>(0, 0, 0);
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                const V3<T> r = t.x[i] - t.x[j];
+                a = a + (-info.V * (k[i] + k[j]) * kernel.gval(std::sqrt(dot3(r, r)))) * r;
+            }
+            out[i] = a;
+        }
+    }
+    // D(u)_i = V sum_j (u_i - u_j) . gradW_ij   (divergence of a vector field)
+    void op_D(const StepTape& t, const std::vector<V3<T>>& u, std::vector<T>& out) const {
+        const long n = (long)t.x.size();
+        out.assign(n, T(0));
+        for (long i = 0; i < n; ++i) {
+            T d = 0;
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                const V3<T> r = t.x[i] - t.x[j];
+                d += dot3(u[i] - u[j], kernel.gval(std::sqrt(dot3(r, r))) * r);
+            }
+            out[i] = info.V * d;
+        }
+    }
+    // position cotangents of A(k) against la, and of D(u) against lf
+    void vjp_x_A(const StepTape& t, const std::vector<T>& k, const std::vector<V3<T>>& la,
+                 std::vector<V3<T>>& lx) const {
+        const long n = (long)t.x.size();
+        for (long i = 0; i < n; ++i)
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                const V3<T> g = vjp_grad(t.x[i] - t.x[j], (-info.V * (k[i] + k[j])) * la[i]);
+                lx[i] = lx[i] + g;
+                lx[j] = lx[j] - g;
+            }
+    }
+    void vjp_x_D(const StepTape& t, const std::vector<V3<T>>& u, const std::vector<T>& lf,
+                 std::vector<V3<T>>& lx) const {
+        const long n = (long)t.x.size();
+        for (long i = 0; i < n; ++i)
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                const V3<T> g = vjp_grad(t.x[i] - t.x[j], (info.V * lf[i]) * (u[i] - u[j]));
+                lx[i] = lx[i] + g;
+                lx[j] = lx[j] - g;
+            }
+    }
+    // reverse of one Jacobi solve (divergence: sign = -1, source D; pressure: sign = +1, source 1 - rhoadv):
+    // kappa^{k+1} = kappa^k - s o r^k, r^k = M^k o (src - c f^k), f^k = D(A(kappa^k)).
+    // Accumulates lk0 (cotangent of kappa^0), lsrc (of the source) and ls (of the scale s) and positions.
+    void reverse_jacobi(const StepTape& t, const std::vector<std::vector<T>>& K, const std::vector<std::vector<uint8_t>>& M,
+                        const std::vector<T>& scale, const std::vector<T>& src, T c, std::vector<T>& lk,
+                        std::vector<T>& lsrc, std::vector<T>& ls, std::vector<V3<T>>& lx) const {
+        const long n = (long)t.x.size();
+        std::vector<V3<T>> a, la;
+        std::vector<T> f, lf(n), tmp;
+        for (long k = (long)M.size() - 1; k >= 0; --k) {
+            op_A(t, K[k], a);
+            op_D(t, a, f);
+            for (long i = 0; i < n; ++i) {
+                const T r = M[k][i] ? src[i] - c * f[i] : T(0);
+                const T lr = -scale[i] * lk[i];
+                ls[i] += -r * lk[i];
+                lsrc[i] += M[k][i] ? lr : T(0);
+                lf[i] = M[k][i] ? -c * lr : T(0);
+            }
+            // f = D(a): la = D^T lf = -A(lf)
+            op_A(t, lf, la);
+            for (auto& q : la) q = -q;
+            vjp_x_D(t, a, lf, lx);
+            // a = A(kappa): lk += A^T la = -D(la)
+            op_D(t, la, tmp);
+            for (long i = 0; i < n; ++i) lk[i] -= tmp[i];
+            vjp_x_A(t, K[k], la, lx);
+        }
+    }
+    // viscosity system of the tape: (A y)_i = y_i - sum_j T_ij,
+    // T_ij = (h / rho_i) 10 mu m / (rho_j (|r|^2 + 0.01 R^2)) G(|r|) ((y_i - y_j) . r) r
+    T visc_k(const StepTape& t, const std::vector<T>& rho, long i, unsigned j, V3<T> r) const {
+        return (t.h / rho[i]) * (T(10) * info.mu * info.m / rho[j]) * kernel.gval(std::sqrt(dot3(r, r))) /
+               (length2(r) + T(0.01) * info.R2);
+    }
+    void visc_apply_T(const StepTape& t, const std::vector<T>& rho, const std::vector<V3<T>>& w,
+                      std::vector<V3<T>>& out) const {  // A^T w
+        const long n = (long)t.x.size();
+        out = w;
+        for (long i = 0; i < n; ++i)
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                const V3<T> r = t.x[i] - t.x[j];
+                // (A^T w)_i -= r r^T (k_ij w_i - k_ji w_j)
+                const V3<T> u = visc_k(t, rho, i, j, r) * w[i] - visc_k(t, rho, (long)j, (unsigned)i, -r) * w[j];
+                out[i] = out[i] - dot3(r, u) * r;
+            }
+    }
+    // BiCGSTAB for A^T x = b (double precision, relative residual 1e-12)
+    void solve_visc_T(const StepTape& t, const std::vector<T>& rho, const std::vector<V3<T>>& b,
+                      std::vector<V3<T>>& x) const {
+        const long n = (long)b.size();
+        auto dot = [&](const std::vector<V3<T>>& a, const std::vector<V3<T>>& c) {
+            double s = 0;
+            for (long i = 0; i < n; ++i) s += (double)dot3(a[i], c[i]);
+            return s;
+        };
+        x = b;
+        std::vector<V3<T>> r(n), rh, pv(n, v3<T>(0, 0, 0)), v(n, v3<T>(0, 0, 0)), sv(n), tv(n), ax;
+        visc_apply_T(t, rho, x, ax);
+        for (long i = 0; i < n; ++i) r[i] = b[i] - ax[i];
+        rh = r;
+        double rho_old = 1, alpha = 1, omega = 1;
+        const double bn = std::sqrt(dot(b, b));
+        if (bn == 0) return;
+        for (int it = 0; it < 1000 && std::sqrt(dot(r, r)) > 1e-12 * bn; ++it) {
+            const double rho_new = dot(rh, r);
+            const double beta = (rho_new / rho_old) * (alpha / omega);
+            for (long i = 0; i < n; ++i) pv[i] = r[i] + (T)beta * (pv[i] - (T)omega * v[i]);
+            visc_apply_T(t, rho, pv, v);
+            alpha = rho_new / dot(rh, v);
+            for (long i = 0; i < n; ++i) sv[i] = r[i] - (T)alpha * v[i];
+            visc_apply_T(t, rho, sv, tv);
+            omega = dot(tv, sv) / dot(tv, tv);
+            for (long i = 0; i < n; ++i) {
+                x[i] = x[i] + (T)alpha * pv[i] + (T)omega * sv[i];
+                r[i] = sv[i] - (T)omega * tv[i];
+            }
+            rho_old = rho_new;
+        }
+    }
+    // Reverse of one taped step: (lx, lv) are the cotangents of the step's output (x', v') on entry and of its input
+    // (x, v) on return; lmu accumulates d/dmu.
+    void reverse_step(const StepTape& t, std::vector<V3<T>>& lx, std::vector<V3<T>>& lv, double& lmu) const {
+        const long n = (long)t.x.size();
+        const T h = t.h, H = t.H;
+        std::vector<T> lrho(n, T(0)), lalpha(n, T(0)), tmp;
+        std::vector<V3<T>> la(n), tmpv;
+        // x' = x + H v3
+        std::vector<V3<T>> lv3(n);
+        for (long i = 0; i < n; ++i) lv3[i] = lv[i] + H * lx[i];  // lx keeps flowing to x
+        // pressure final: v3 = v2 + H A(kp^K)
+        std::vector<V3<T>> lv2 = lv3;
+        for (long i = 0; i < n; ++i) la[i] = H * lv3[i];
+        std::vector<T> lk(n);
+        op_D(t, la, tmp);
+        for (long i = 0; i < n; ++i) lk[i] = -tmp[i];
+        vjp_x_A(t, t.kp.back(), la, lx);
+        // pressure iterations: scale gamma = alpha / H^2, source 1 - rhoadv, c = H^2
+        std::vector<T> gamma(n), srcp(n), lsrc(n, T(0)), lgamma(n, T(0));
+        for (long i = 0; i < n; ++i) {
+            gamma[i] = t.alpha[i] / (H * H);
+            srcp[i] = T(1) - t.rhoadv[i];
+        }
+        reverse_jacobi(t, t.kp, t.mp, gamma, srcp, H * H, lk, lsrc, lgamma, lx);
+        // pressure init: kp^0 = mp0 (rhoadv - 1) gamma
+        std::vector<T> lrhoadv(n);
+        for (long i = 0; i < n; ++i) {
+            lrhoadv[i] = -lsrc[i] + (t.mp0[i] ? gamma[i] * lk[i] : T(0));
+            lgamma[i] += t.mp0[i] ? (t.rhoadv[i] - T(1)) * lk[i] : T(0);
+            lalpha[i] += lgamma[i] / (H * H);
+        }
+        // rhoadv = rho / rho0 + H D(v2)
+        std::vector<T> ld2(n);
+        for (long i = 0; i < n; ++i) {
+            lrho[i] += lrhoadv[i] / info.rho0;
+            ld2[i] = H * lrhoadv[i];
+        }
+        op_A(t, ld2, tmpv);  // D^T ld2 = -A(ld2)
+        for (long i = 0; i < n; ++i) lv2[i] = lv2[i] - tmpv[i];
+        vjp_x_D(t, t.v2, ld2, lx);
+        // v2 = v1 + H g + (H / h)(y - v1)
+        std::vector<V3<T>> ly(n), lv1(n);
+        for (long i = 0; i < n; ++i) {
+            ly[i] = t.visc ? (H / h) * lv2[i] : v3<T>(0, 0, 0);
+            lv1[i] = t.visc ? (T(1) - H / h) * lv2[i] : lv2[i];
+        }
+        // densities (for the viscosity terms)
+        std::vector<T> rho(n);
+        for (long i = 0; i < n; ++i) {
+            T d = info.V * kernel.W0;
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj)
+                d += info.V * kernel.getW(t.x[i] - t.x[t.nb.ids[t.nb.offsets[i] + jj]]);
+            rho[i] = d * info.rho0;
+        }
+        if (t.visc) {
+            // A y = v1: lb = A^-T ly; parameters through -lb^T (dA) y = + lb . dT
+            std::vector<V3<T>> lb;
+            solve_visc_T(t, rho, ly, lb);
+            for (long i = 0; i < n; ++i) lv1[i] = lv1[i] + lb[i];
+            const T e = T(0.01) * info.R2;
+            for (long i = 0; i < n; ++i)
+                for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                    const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                    const V3<T> r = t.x[i] - t.x[j];
+                    const V3<T> u = t.y[i] - t.y[j];
+                    const T k = visc_k(t, rho, i, j, r);
+                    const T ur = dot3(u, r), sr = dot3(lb[i], r);
+                    const T lT = k * ur * sr;  // lb_i . T_ij
+                    lmu += (double)(lT / info.mu);
+                    lrho[i] -= lT / rho[i];
+                    lrho[j] -= lT / rho[(long)j];
+                    const T rl = std::sqrt(dot3(r, r));
+                    const T C = (h / rho[i]) * (T(10) * info.mu * info.m / rho[j]);
+                    const T g = kernel.gval(rl), d2 = rl * rl + e;
+                    const T dk = C * (kernel.dG(rl) * d2 - g * T(2) * rl) / (d2 * d2);
+                    V3<T> gr = k * (ur * lb[i] + sr * u);
+                    if (rl > T(0)) gr = gr + (ur * sr * dk / rl) * r;
+                    lx[i] = lx[i] + gr;
+                    lx[j] = lx[j] - gr;
+                }
+        }
+        // divergence final: v1 = v0 + h A(kv^K)
+        for (long i = 0; i < n; ++i) {
+            lv[i] = lv1[i];
+            la[i] = h * lv1[i];
+        }
+        op_D(t, la, tmp);
+        for (long i = 0; i < n; ++i) lk[i] = -tmp[i];
+        vjp_x_A(t, t.kv.back(), la, lx);
+        // divergence iterations: scale beta = alpha / h, source -D, c = h
+        std::vector<T> beta(n), srcd(n), lsrcd(n, T(0)), lbeta(n, T(0));
+        for (long i = 0; i < n; ++i) {
+            beta[i] = t.alpha[i] / h;
+            srcd[i] = -t.D[i];
+        }
+        reverse_jacobi(t, t.kv, t.mv, beta, srcd, h, lk, lsrcd, lbeta, lx);
+        // divergence init: kv^0 = D beta, D = m0 D(v0)
+        std::vector<T> ld(n);
+        for (long i = 0; i < n; ++i) {
+            const T lD = -lsrcd[i] + beta[i] * lk[i];
+            lbeta[i] += t.D[i] * lk[i];
+            ld[i] = t.m0[i] ? lD : T(0);
+            lalpha[i] += lbeta[i] / h;
+        }
+        op_A(t, ld, tmpv);
+        for (long i = 0; i < n; ++i) lv[i] = lv[i] - tmpv[i];
+        vjp_x_D(t, t.v0, ld, lx);
+        // factor: alpha = 1 / S, S_i = V^2 sum |gradW_ij|^2 + |V sum_j gradW_ij|^2
+        for (long i = 0; i < n; ++i) {
+            if (t.alpha[i] == T(0)) continue;
+            const T lS = -t.alpha[i] * t.alpha[i] * lalpha[i];
+            V3<T> gs = v3<T>(0, 0, 0);
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const V3<T> r = t.x[i] - t.x[t.nb.ids[t.nb.offsets[i] + jj]];
+                gs = gs + (info.V * kernel.gval(std::sqrt(dot3(r, r)))) * r;
+            }
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                const V3<T> r = t.x[i] - t.x[j];
+                const V3<T> gw = kernel.gval(std::sqrt(dot3(r, r))) * r;
+                const V3<T> g = vjp_grad(r, (T(2) * info.V * lS) * (info.V * gw + gs));
+                lx[i] = lx[i] + g;
+                lx[j] = lx[j] - g;
+            }
+        }
+        // density: rho_i = rho0 V (W0 + sum_j W_ij)
+        for (long i = 0; i < n; ++i)
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                const V3<T> r = t.x[i] - t.x[j];
+                const T rl = std::sqrt(dot3(r, r));
+                if (!(rl > T(0))) continue;
+                const V3<T> g = (info.rho0 * info.V * lrho[i] * kernel.dW(rl) / rl) * r;
+                lx[i] = lx[i] + g;
+                lx[j] = lx[j] - g;
+            }
     }
 
     // Dense viscosity system of the current state: A (3n x 3n, row-major) built by applying the matvec to unit
@@ -1514,6 +1930,48 @@ int orc_get_field(void* h, int field, uint32_t body, double* out) {
 int orc_set_field(void* h, int field, const double* in) {
     DISPATCH(h, return s->set_field(field, in));
     return VFD_OK;
+}
+// ---- adjoint (reference gradient) -------------------------------------------------------------------------------
+int orc_set_smooth_kernel(void* h, int on) { DISPATCH(h, s->kernel.smooth = on != 0); return VFD_OK; }
+int orc_tape(void* h, int on) {
+    DISPATCH(h, {
+        s->taping = on != 0;
+        if (on) {
+            s->tapes.clear();
+            s->tape_error.clear();
+        }
+    });
+    return VFD_OK;
+}
+// cotangents of the last taped step's output (lx_out, lv_out; 3n each) -> of the first taped step's input
+// (lx_in, lv_in) and d/dmu; returns the number of reversed steps, or -1 with a message in err (256 bytes)
+int orc_adjoint(void* h, const double* lx_out, const double* lv_out, double* lx_in, double* lv_in, double* lmu,
+                char* err) {
+    int steps = -1;
+    DISPATCH(h, {
+        if (!s->tape_error.empty()) {
+            std::strncpy(err, s->tape_error.c_str(), 255);
+            err[255] = 0;
+        } else {
+            const size_t n = s->p.size();
+            using VT = typename std::remove_reference<decltype(s->p[0].x)>::type;
+            using TT = decltype(VT{}.x);
+            std::vector<VT> lx(n), lv(n);
+            for (size_t i = 0; i < n; ++i) {
+                lx[i] = VT{(TT)lx_out[3 * i], (TT)lx_out[3 * i + 1], (TT)lx_out[3 * i + 2]};
+                lv[i] = VT{(TT)lv_out[3 * i], (TT)lv_out[3 * i + 1], (TT)lv_out[3 * i + 2]};
+            }
+            double m = 0;
+            for (long k = (long)s->tapes.size() - 1; k >= 0; --k) s->reverse_step(s->tapes[k], lx, lv, m);
+            for (size_t i = 0; i < n; ++i) {
+                lx_in[3 * i] = lx[i].x, lx_in[3 * i + 1] = lx[i].y, lx_in[3 * i + 2] = lx[i].z;
+                lv_in[3 * i] = lv[i].x, lv_in[3 * i + 1] = lv[i].y, lv_in[3 * i + 2] = lv[i].z;
+            }
+            *lmu = m;
+            steps = (int)s->tapes.size();
+        }
+    });
+    return steps;
 }
 int orc_particle_count(void* h, uint32_t* n) { DISPATCH(h, *n = (uint32_t)s->p.size()); return VFD_OK; }
 int orc_get_neighbors(void* h, uint32_t* counts, uint32_t* offsets, uint32_t* ids, uint64_t* total) {

@@ -53,6 +53,8 @@ def lib():
             "orc_brute_force_neighbors": [P, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, P, P],
             "orc_search_grid": [P, ctypes.c_uint32, ctypes.c_float, P, P],
             "orc_viscosity_system": [P, P, P],
+            "orc_set_smooth_kernel": [P, ctypes.c_int], "orc_tape": [P, ctypes.c_int],
+            "orc_adjoint": [P, P, P, P, P, P, P],
         }.items():
             fn = getattr(L, name)
             fn.restype = ctypes.c_int
@@ -123,6 +125,28 @@ class OracleSim:
 
     def step(self):
         self._L.orc_step(self._h)
+
+    # ---- reference adjoint ----------------------------------------------------------------------------------
+    def set_smooth_kernel(self, on):
+        """Evaluate W and gradW from the spline instead of the table (a differentiable forward for adjoint pins)."""
+        self._L.orc_set_smooth_kernel(self._h, 1 if on else 0)
+
+    def tape(self, on):
+        """Start (clearing earlier records) or stop recording steps for adjoint()."""
+        self._L.orc_tape(self._h, 1 if on else 0)
+
+    def adjoint(self, lx_out, lv_out=None):
+        """Cotangents of the last recorded step's (x, v) -> those of the first recorded step's input and d/dmu
+        (mu = dynamic viscosity, Viscosity * rho0)."""
+        n = self.n
+        lx = np.ascontiguousarray(lx_out, np.float64).reshape(n, 3)
+        lv = np.zeros((n, 3)) if lv_out is None else np.ascontiguousarray(lv_out, np.float64).reshape(n, 3)
+        ox, ov, mu = np.zeros((n, 3)), np.zeros((n, 3)), ctypes.c_double(0)
+        err = ctypes.create_string_buffer(256)
+        k = self._L.orc_adjoint(self._h, _p(lx), _p(lv), _p(ox), _p(ov), ctypes.byref(mu), err)
+        if k < 0:
+            raise RuntimeError("adjoint: " + err.value.decode())
+        return ox, ov, mu.value
 
     def step_until(self, phase):
         self._L.orc_step_until(self._h, PHASES.index(phase) if isinstance(phase, str) else int(phase))
