@@ -299,6 +299,17 @@ typedef struct vfd_kernel_stat {
 int vfd_set_profiling(vfd_sim* sim, int32_t enabled);
 int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint32_t* count);
 
+/* Adjoint of the step (no reference counterpart). vfd_tape(sim, 1) starts recording the following steps (clearing
+ * earlier records), vfd_tape(sim, 0) stops. vfd_adjoint maps cotangents of the last recorded step's positions and
+ * velocities (lx_out, lv_out: 3n floats in original particle order; lv_out may be NULL for zero) to those of the first
+ * recorded step's input (lx_in, lv_in) and d/dmu (mu = dynamic viscosity = Viscosity * rho0). Lists, time steps,
+ * solver iteration counts and clamp masks are held at their recorded values; kernel derivatives are the spline's.
+ * Scope: one scene, no boundary contact, no surface tension, no particle at the neighbour cap (else
+ * VFD_ERROR_NOT_SUPPORTED). */
+int vfd_tape(vfd_sim* sim, int32_t on);
+int vfd_tape_steps(vfd_sim* sim, uint32_t* steps);
+int vfd_adjoint(vfd_sim* sim, const float* lx_out, const float* lv_out, float* lx_in, float* lv_in, double* lmu);
+
 /* Launch and solver-control knobs; no counterpart in the reference (which synchronises after every iteration).
  * Results do not depend on them: a converged scene's extra iterations are skipped on the device, and a staged halo
  * hands the same values to the same arithmetic as a gather from HBM. */

@@ -210,6 +210,9 @@ _PROTOS = {
     "vfd_get_kernel_table": (ctypes.c_int, [_P, _P, _P, _P]),
     "vfd_set_profiling": (ctypes.c_int, [_P, _I32]),
     "vfd_get_profile": (ctypes.c_int, [_P, ctypes.POINTER(KernelStat), _U32, ctypes.POINTER(_U32)]),
+    "vfd_tape": (ctypes.c_int, [_P, _I32]),
+    "vfd_tape_steps": (ctypes.c_int, [_P, ctypes.POINTER(_U32)]),
+    "vfd_adjoint": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
     "vfd_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "vfd_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
 }

@@ -1,0 +1,359 @@
+// adjoint.hip - reverse of a recorded DFSPH step (hand-written adjoint; no reference counterpart).
+//
+// Differentiated map: one step (x, v; mu) -> (x', v') with the step's neighbour lists, time steps, Jacobi iteration
+// counts and clamp masks held fixed. Kernel values are the forward's table values; their derivatives are those of
+// the cubic spline the table samples (Kernel/DFSPHKernels.h:86-131): dW/drho and d(gradW/r)/drho. The Jacobi solves
+// are reversed iteration by iteration (their transposed iteration: the same neighbour sums, in reverse order); the
+// viscosity solve by one transposed linear solve (BiCGSTAB). Lists are symmetric (no particle at the cap, checked
+// when recording), so every position cotangent is gathered per particle: pair (i, j) contributes to both ends from
+// i's list and from j's, without atomics.
+//
+// Per-step arrays are in that step's sorted slot order; the host maps cotangents through the recorded ids.
+#include "device.h"
+#include "kernels.h"
+
+namespace vfd {
+
+struct AdjK {
+    const float2* __restrict__ tab;
+    float R, invStep, K, L, V, rho0, m, mu, R2;
+    uint32_t n, cap;
+    const uint32_t* __restrict__ count;
+    const uint32_t* __restrict__ list;  // [j * cap + i]
+    const float4* __restrict__ x;
+};
+
+__device__ __forceinline__ float adj_g(const AdjK& a, float rl) {  // gradW / r as the forward's table gives it
+    if (!(rl <= a.R)) return 0.0f;
+    const unsigned p0 = (unsigned)(rl * a.invStep);
+    return a.tab[p0 < kTableEntries - 1u ? p0 : kTableEntries - 1u].y;
+}
+__device__ __forceinline__ float adj_w(const AdjK& a, float r2) {  // W as the forward's table gives it
+    if (!(r2 <= a.R2)) return 0.0f;
+    const unsigned p0 = (unsigned)(sqrtf(r2) * a.invStep);
+    return a.tab[p0 < kTableEntries - 1u ? p0 : kTableEntries - 1u].x;
+}
+__device__ __forceinline__ float adj_dW(const AdjK& a, float rl) {  // spline dW/drho
+    const float q = rl / a.R;
+    if (!(q <= 1.0f)) return 0.0f;
+    if (q <= 0.5f) return a.K * (18.0f * q * q - 12.0f * q) / a.R;
+    return -6.0f * a.K * (1.0f - q) * (1.0f - q) / a.R;
+}
+__device__ __forceinline__ float adj_dG(const AdjK& a, float rl) {  // spline d(gradW/r)/drho
+    const float q = rl / a.R;
+    if (!(q <= 1.0f) || !(rl > 1.0e-5f)) return 0.0f;
+    if (q <= 0.5f) return 3.0f * a.L / (a.R * a.R * a.R);
+    const float f = 1.0f - q;
+    return (a.L / a.R) * (2.0f * f / (a.R * rl) + f * f / (rl * rl));
+}
+// d(c . G(|r|) r)/dr = G c + G'(|r|) (c . r) / |r| r
+__device__ __forceinline__ f3 vjp_grad(const AdjK& a, f3 r, float rl, float g, f3 c) {
+    f3 o = g * c;
+    if (rl > 0.0f) o = o + (adj_dG(a, rl) * dot3(c, r) / rl) * r;
+    return o;
+}
+__device__ __forceinline__ uint32_t adj_nbr(const AdjK& a, uint32_t i, uint32_t j) {
+    return a.list[(size_t)j * a.cap + i];
+}
+__device__ __forceinline__ uint32_t pid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// lk_i -= V sum_j (la_i - la_j) . gradW_ij            (A^T la = -D(la))
+// lx_i += sum_j vjp(r_ij, -V (k_i + k_j)(la_i - la_j))  (positions through A(k))
+__global__ void __launch_bounds__(kTile) k_adj_accel_T(AdjK a, const float* __restrict__ kap,
+                                                       const float4* __restrict__ la, float* lk, float4* lx) {
+    const uint32_t i = pid();
+    if (i >= a.n) return;
+    const f3 xi = ld3(a.x[i]), lai = ld3(la[i]);
+    const float ki = kap[i];
+    float d = 0.0f;
+    f3 gx = mk3(0.0f, 0.0f, 0.0f);
+    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
+        const uint32_t j = adj_nbr(a, i, jj);
+        const f3 r = xi - ld3(a.x[j]);
+        const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
+        const f3 dl = lai - ld3(la[j]);
+        d += dot3(dl, g * r);
+        gx = gx + vjp_grad(a, r, rl, g, (-a.V * (ki + kap[j])) * dl);
+    }
+    lk[i] -= a.V * d;
+    lx[i] = st4(ld3(lx[i]) + gx);
+}
+
+// lu_i += V sum_j (lf_i + lf_j) gradW_ij               (D^T lf = -A(lf))
+// lx_i += sum_j vjp(r_ij, V (lf_i + lf_j)(u_i - u_j))   (positions through D(u))
+__global__ void __launch_bounds__(kTile) k_adj_div_T(AdjK a, const float* __restrict__ lf,
+                                                     const float4* __restrict__ u, float4* lu, float4* lx) {
+    const uint32_t i = pid();
+    if (i >= a.n) return;
+    const f3 xi = ld3(a.x[i]), ui = ld3(u[i]);
+    const float lfi = lf[i];
+    f3 s = mk3(0.0f, 0.0f, 0.0f), gx = mk3(0.0f, 0.0f, 0.0f);
+    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
+        const uint32_t j = adj_nbr(a, i, jj);
+        const f3 r = xi - ld3(a.x[j]);
+        const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
+        const float l2 = lfi + lf[j];
+        s = s + (l2 * g) * r;
+        gx = gx + vjp_grad(a, r, rl, g, (a.V * l2) * (ui - ld3(u[j])));
+    }
+    lu[i] = st4(ld3(lu[i]) + a.V * s);
+    lx[i] = st4(ld3(lx[i]) + gx);
+}
+
+// one reversed Jacobi iteration, pointwise part: kappa^{k+1} = kappa^k - s r^k, r^k = M (src - c f^k)
+//   ls += -r lk, lsrc += M lr, lf = -c M lr, lr = -s lk, s = alpha / div
+__global__ void __launch_bounds__(kTile) k_adj_jacobi_point(uint32_t n, const float* __restrict__ lk,
+                                                            const float* __restrict__ alpha, float div,
+                                                            const float* __restrict__ res, float c, float* lsrc,
+                                                            float* ls, float* lf) {
+    const uint32_t i = pid();
+    if (i >= n) return;
+    const float s = alpha[i] / div, r = res[i], l = lk[i];
+    const bool m = r < 0.0f;
+    const float lr = -s * l;
+    ls[i] += -r * l;
+    lsrc[i] += m ? lr : 0.0f;
+    lf[i] = m ? -c * lr : 0.0f;
+}
+
+// pressure initialisation reversed: kp0 = mp0 (rhoadv - 1) gamma, rhoadv = rho / rho0 + H D(v2), gamma = alpha / H^2
+__global__ void __launch_bounds__(kTile) k_adj_pres_init(uint32_t n, float H, float rho0,
+                                                         const float* __restrict__ alpha,
+                                                         const float* __restrict__ rhoadv, const float* __restrict__ lk,
+                                                         const float* __restrict__ lsrc, const float* __restrict__ lgam,
+                                                         float* lalpha, float* lrho, float* ld2) {
+    const uint32_t i = pid();
+    if (i >= n) return;
+    const float gam = alpha[i] / (H * H), ra = rhoadv[i];
+    const bool m = 1.0f - ra < 0.0f;
+    const float lra = -lsrc[i] + (m ? gam * lk[i] : 0.0f);
+    const float lg = lgam[i] + (m ? (ra - 1.0f) * lk[i] : 0.0f);
+    lalpha[i] += lg / (H * H);
+    lrho[i] += lra / rho0;
+    ld2[i] = H * lra;
+}
+
+// divergence initialisation reversed: kv0 = D beta, D = max(D(v0), 0) (0 below 20 neighbours), beta = alpha / h
+__global__ void __launch_bounds__(kTile) k_adj_div_init(uint32_t n, float h, const float* __restrict__ alpha,
+                                                        const float* __restrict__ D, const float* __restrict__ lk,
+                                                        const float* __restrict__ lsrc, const float* __restrict__ lbet,
+                                                        float* lalpha, float* ld) {
+    const uint32_t i = pid();
+    if (i >= n) return;
+    const float beta = alpha[i] / h, Di = D[i];
+    const float lD = -lsrc[i] + beta * lk[i];
+    const float lb = lbet[i] + Di * lk[i];
+    ld[i] = Di > 0.0f ? lD : 0.0f;
+    lalpha[i] += lb / h;
+}
+
+// la = s * lv (s = h or H), lk = 0 is set by the caller; generic helpers
+__global__ void k_adj_scale4(uint32_t n, const float4* __restrict__ in, float s, float4* out) {
+    const uint32_t i = pid();
+    if (i < n) out[i] = st4(s * ld3(in[i]));
+}
+__global__ void k_adj_axpy4(uint32_t n, float s, const float4* __restrict__ in, float4* out) {  // out += s in
+    const uint32_t i = pid();
+    if (i < n) out[i] = st4(ld3(out[i]) + s * ld3(in[i]));
+}
+__global__ void k_adj_fill(float* p, float v, uint32_t n) {
+    const uint32_t i = pid();
+    if (i < n) p[i] = v;
+}
+
+// viscosity system T_ij = k_ij (r r^T)(y_i - y_j), k_ij = (h / rho_i) 10 mu m / (rho_j (|r|^2 + 0.01 R^2)) G(|r|)
+__device__ __forceinline__ float visc_k(const AdjK& a, float h, float rhoi, float rhoj, float rl, float g) {
+    return (h / rhoi) * (10.0f * a.mu * a.m / rhoj) * g / (rl * rl + 0.01f * a.R2);
+}
+// out = A^T w: out_i = w_i - sum_j r r^T (k_ij w_i - k_ji w_j)
+__global__ void __launch_bounds__(kTile) k_adj_visc_T(AdjK a, float h, const float* __restrict__ rho,
+                                                      const float4* __restrict__ w, float4* out) {
+    const uint32_t i = pid();
+    if (i >= a.n) return;
+    const f3 xi = ld3(a.x[i]), wi = ld3(w[i]);
+    f3 o = wi;
+    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
+        const uint32_t j = adj_nbr(a, i, jj);
+        const f3 r = xi - ld3(a.x[j]);
+        const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
+        const f3 u = visc_k(a, h, rho[i], rho[j], rl, g) * wi - visc_k(a, h, rho[j], rho[i], rl, g) * ld3(w[j]);
+        o = o - dot3(r, u) * r;
+    }
+    out[i] = st4(o);
+}
+// parameter cotangents of -lb^T (dA) y = lb . dT: positions, densities, and per-tile partial sums of lb . T / mu
+__global__ void __launch_bounds__(kTile) k_adj_visc_vjp(AdjK a, float h, const float* __restrict__ rho,
+                                                        const float4* __restrict__ y, const float4* __restrict__ lb,
+                                                        float4* lx, float* lrho, double* muPart) {
+    const uint32_t i = pid();
+    double part = 0.0;
+    if (i < a.n) {
+        const f3 xi = ld3(a.x[i]), yi = ld3(y[i]), lbi = ld3(lb[i]);
+        const float e = 0.01f * a.R2;
+        f3 gx = mk3(0.0f, 0.0f, 0.0f);
+        float lr = 0.0f;
+        for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
+            const uint32_t j = adj_nbr(a, i, jj);
+            const f3 r = xi - ld3(a.x[j]);
+            const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
+            const f3 u = yi - ld3(y[j]), lbj = ld3(lb[j]);
+            const float ur = dot3(u, r), si = dot3(lbi, r), sj = dot3(lbj, r);
+            const float kij = visc_k(a, h, rho[i], rho[j], rl, g), kji = visc_k(a, h, rho[j], rho[i], rl, g);
+            const float lTij = kij * ur * si, lTji = -kji * ur * sj;  // lb_i . T_ij, lb_j . T_ji
+            part += (double)lTij;
+            lr -= (lTij + lTji) / rho[i];
+            const float d2 = rl * rl + e, dg = adj_dG(a, rl);
+            const float Ci = (h / rho[i]) * (10.0f * a.mu * a.m / rho[j]);
+            const float Cj = (h / rho[j]) * (10.0f * a.mu * a.m / rho[i]);
+            const float dki = Ci * (dg * d2 - g * 2.0f * rl) / (d2 * d2);
+            const float dkj = Cj * (dg * d2 - g * 2.0f * rl) / (d2 * d2);
+            f3 t = kij * (ur * lbi + si * u) - kji * (ur * lbj + sj * u);
+            if (rl > 0.0f) t = t + (ur * (si * dki - sj * dkj) / rl) * r;
+            gx = gx + t;
+        }
+        lx[i] = st4(ld3(lx[i]) + gx);
+        lrho[i] += lr;
+    }
+    double v[1] = {part};
+    block_sum<1>(v);
+    if (threadIdx.x == 0) muPart[blockIdx.x] = v[0] / (double)a.mu;
+}
+
+// factor: S_i = V^2 sum_j |gradW_ij|^2 + |gs_i|^2, gs_i = V sum_j gradW_ij, alpha = 1/S
+__global__ void __launch_bounds__(kTile) k_adj_gsum(AdjK a, float4* gs) {
+    const uint32_t i = pid();
+    if (i >= a.n) return;
+    const f3 xi = ld3(a.x[i]);
+    f3 s = mk3(0.0f, 0.0f, 0.0f);
+    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
+        const f3 r = xi - ld3(a.x[adj_nbr(a, i, jj)]);
+        s = s + adj_g(a, sqrtf(dot3(r, r))) * r;
+    }
+    gs[i] = st4(a.V * s);
+}
+__global__ void __launch_bounds__(kTile) k_adj_factor_lS(uint32_t n, const float* __restrict__ alpha,
+                                                         const float* __restrict__ lalpha, float* lS) {
+    const uint32_t i = pid();
+    if (i < n) lS[i] = alpha[i] == 0.0f ? 0.0f : -alpha[i] * alpha[i] * lalpha[i];
+}
+// lx_i += sum_j vjp(r_ij, 2V [V gradW_ij (lS_i + lS_j) + lS_i gs_i - lS_j gs_j])
+//       + sum_j rho0 V (lrho_i + lrho_j) W'(|r|) r / |r|                     (densities)
+__global__ void __launch_bounds__(kTile) k_adj_factor_density(AdjK a, const float* __restrict__ lS,
+                                                              const float4* __restrict__ gs,
+                                                              const float* __restrict__ lrho, float4* lx) {
+    const uint32_t i = pid();
+    if (i >= a.n) return;
+    const f3 xi = ld3(a.x[i]), gsi = ld3(gs[i]);
+    const float lSi = lS[i], lri = lrho[i];
+    f3 gx = mk3(0.0f, 0.0f, 0.0f);
+    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
+        const uint32_t j = adj_nbr(a, i, jj);
+        const f3 r = xi - ld3(a.x[j]);
+        const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
+        const f3 c = (2.0f * a.V) * ((a.V * (lSi + lS[j])) * (g * r) + lSi * gsi - lS[j] * ld3(gs[j]));
+        gx = gx + vjp_grad(a, r, rl, g, c);
+        if (rl > 0.0f) gx = gx + (a.rho0 * a.V * (lri + lrho[j]) * adj_dW(a, rl) / rl) * r;
+    }
+    lx[i] = st4(ld3(lx[i]) + gx);
+}
+
+// BiCGSTAB helpers: per-block FP64 partial sums of a . b
+__global__ void __launch_bounds__(kTile) k_adj_dot(uint32_t n, const float4* __restrict__ p,
+                                                   const float4* __restrict__ q, double* part) {
+    const uint32_t i = pid();
+    double v[1] = {i < n ? (double)dot3(ld3(p[i]), ld3(q[i])) : 0.0};
+    block_sum<1>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+// p = r + beta (p - omega v)
+__global__ void k_adj_bicg_p(uint32_t n, const float4* __restrict__ r, float beta, float omega,
+                             const float4* __restrict__ v, float4* p) {
+    const uint32_t i = pid();
+    if (i < n) p[i] = st4(ld3(r[i]) + beta * (ld3(p[i]) - omega * ld3(v[i])));
+}
+
+// cotangents between the step's slot order and the original particle order
+__global__ void k_adj_to_slots(uint32_t n, const uint32_t* __restrict__ id, const float* __restrict__ orig,
+                               float4* slot) {
+    const uint32_t i = pid();
+    if (i < n) {
+        const uint32_t o = id[i];
+        slot[i] = make_float4(orig[3 * o], orig[3 * o + 1], orig[3 * o + 2], 0.0f);
+    }
+}
+__global__ void k_adj_to_orig(uint32_t n, const uint32_t* __restrict__ id, const float4* __restrict__ slot,
+                              float* orig) {
+    const uint32_t i = pid();
+    if (i < n) {
+        const uint32_t o = id[i];
+        const float4 v = slot[i];
+        orig[3 * o] = v.x;
+        orig[3 * o + 1] = v.y;
+        orig[3 * o + 2] = v.z;
+    }
+}
+
+// ---- host side ---------------------------------------------------------------------------------------------------
+static inline dim3 grid_of(uint32_t n) { return dim3((n + kTile - 1) / kTile); }
+
+void adj_launch_accel_T(hipStream_t s, const AdjKHost& h, const float* kap, const float4* la, float* lk, float4* lx) {
+    hipLaunchKernelGGL(k_adj_accel_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), kap, la, lk, lx);
+}
+void adj_launch_div_T(hipStream_t s, const AdjKHost& h, const float* lf, const float4* u, float4* lu, float4* lx) {
+    hipLaunchKernelGGL(k_adj_div_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), lf, u, lu, lx);
+}
+void adj_launch_jacobi_point(hipStream_t s, uint32_t n, const float* lk, const float* alpha, float div,
+                             const float* res, float c, float* lsrc, float* ls, float* lf) {
+    hipLaunchKernelGGL(k_adj_jacobi_point, grid_of(n), dim3(kTile), 0, s, n, lk, alpha, div, res, c, lsrc, ls, lf);
+}
+void adj_launch_pres_init(hipStream_t s, uint32_t n, float H, float rho0, const float* alpha, const float* rhoadv,
+                          const float* lk, const float* lsrc, const float* lgam, float* lalpha, float* lrho,
+                          float* ld2) {
+    hipLaunchKernelGGL(k_adj_pres_init, grid_of(n), dim3(kTile), 0, s, n, H, rho0, alpha, rhoadv, lk, lsrc, lgam,
+                       lalpha, lrho, ld2);
+}
+void adj_launch_div_init(hipStream_t s, uint32_t n, float h, const float* alpha, const float* D, const float* lk,
+                         const float* lsrc, const float* lbet, float* lalpha, float* ld) {
+    hipLaunchKernelGGL(k_adj_div_init, grid_of(n), dim3(kTile), 0, s, n, h, alpha, D, lk, lsrc, lbet, lalpha, ld);
+}
+void adj_launch_scale4(hipStream_t s, uint32_t n, const float4* in, float f, float4* out) {
+    hipLaunchKernelGGL(k_adj_scale4, grid_of(n), dim3(kTile), 0, s, n, in, f, out);
+}
+void adj_launch_axpy4(hipStream_t s, uint32_t n, float f, const float4* in, float4* out) {
+    hipLaunchKernelGGL(k_adj_axpy4, grid_of(n), dim3(kTile), 0, s, n, f, in, out);
+}
+void adj_launch_fill(hipStream_t s, float* p, float v, uint32_t n) {
+    hipLaunchKernelGGL(k_adj_fill, grid_of(n), dim3(kTile), 0, s, p, v, n);
+}
+void adj_launch_visc_T(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* w, float4* out) {
+    hipLaunchKernelGGL(k_adj_visc_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), dt, rho, w, out);
+}
+void adj_launch_visc_vjp(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* y,
+                         const float4* lb, float4* lx, float* lrho, double* muPart) {
+    hipLaunchKernelGGL(k_adj_visc_vjp, grid_of(h.n), dim3(kTile), 0, s, h.dev(), dt, rho, y, lb, lx, lrho, muPart);
+}
+void adj_launch_factor_density(hipStream_t s, const AdjKHost& h, const float* alpha, const float* lalpha, float* lS,
+                               float4* gs, const float* lrho, float4* lx) {
+    hipLaunchKernelGGL(k_adj_gsum, grid_of(h.n), dim3(kTile), 0, s, h.dev(), gs);
+    hipLaunchKernelGGL(k_adj_factor_lS, grid_of(h.n), dim3(kTile), 0, s, h.n, alpha, lalpha, lS);
+    hipLaunchKernelGGL(k_adj_factor_density, grid_of(h.n), dim3(kTile), 0, s, h.dev(), lS, gs, lrho, lx);
+}
+void adj_launch_dot(hipStream_t s, uint32_t n, const float4* p, const float4* q, double* part) {
+    hipLaunchKernelGGL(k_adj_dot, grid_of(n), dim3(kTile), 0, s, n, p, q, part);
+}
+void adj_launch_bicg_p(hipStream_t s, uint32_t n, const float4* r, float beta, float omega, const float4* v,
+                       float4* p) {
+    hipLaunchKernelGGL(k_adj_bicg_p, grid_of(n), dim3(kTile), 0, s, n, r, beta, omega, v, p);
+}
+void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float4* slot) {
+    hipLaunchKernelGGL(k_adj_to_slots, grid_of(n), dim3(kTile), 0, s, n, id, orig, slot);
+}
+void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const float4* slot, float* orig) {
+    hipLaunchKernelGGL(k_adj_to_orig, grid_of(n), dim3(kTile), 0, s, n, id, slot, orig);
+}
+
+AdjK AdjKHost::dev() const {
+    return AdjK{tab, R, invStep, K, L, V, rho0, m, mu, R2, n, cap, count, list, x};
+}
+
+}  // namespace vfd

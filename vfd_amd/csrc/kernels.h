@@ -26,6 +26,42 @@ struct Launch {
     struct { uint32_t tiles, per; } tg;  // per = tiles of one XCD chunk, a multiple of 4
 };
 
+// ---- adjoint.hip -----------------------------------------------------------------------------------------------
+struct AdjK;
+// Geometry and constants of one recorded step for the adjoint kernels (spline constants K = 8/(pi R^3),
+// L = 48/(pi R^3)); arrays in the step's slot order.
+struct AdjKHost {
+    const float2* tab;
+    float R, invStep, K, L, V, rho0, m, mu, R2;
+    uint32_t n, cap;
+    const uint32_t* count;
+    const uint32_t* list;
+    const float4* x;
+    AdjK dev() const;
+};
+void adj_launch_accel_T(hipStream_t s, const AdjKHost& h, const float* kap, const float4* la, float* lk, float4* lx);
+void adj_launch_div_T(hipStream_t s, const AdjKHost& h, const float* lf, const float4* u, float4* lu, float4* lx);
+void adj_launch_jacobi_point(hipStream_t s, uint32_t n, const float* lk, const float* alpha, float div,
+                             const float* res, float c, float* lsrc, float* ls, float* lf);
+void adj_launch_pres_init(hipStream_t s, uint32_t n, float H, float rho0, const float* alpha, const float* rhoadv,
+                          const float* lk, const float* lsrc, const float* lgam, float* lalpha, float* lrho,
+                          float* ld2);
+void adj_launch_div_init(hipStream_t s, uint32_t n, float h, const float* alpha, const float* D, const float* lk,
+                         const float* lsrc, const float* lbet, float* lalpha, float* ld);
+void adj_launch_scale4(hipStream_t s, uint32_t n, const float4* in, float f, float4* out);
+void adj_launch_axpy4(hipStream_t s, uint32_t n, float f, const float4* in, float4* out);
+void adj_launch_fill(hipStream_t s, float* p, float v, uint32_t n);
+void adj_launch_visc_T(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* w, float4* out);
+void adj_launch_visc_vjp(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* y,
+                         const float4* lb, float4* lx, float* lrho, double* muPart);
+void adj_launch_factor_density(hipStream_t s, const AdjKHost& h, const float* alpha, const float* lalpha, float* lS,
+                               float4* gs, const float* lrho, float4* lx);
+void adj_launch_dot(hipStream_t s, uint32_t n, const float4* p, const float4* q, double* part);
+void adj_launch_bicg_p(hipStream_t s, uint32_t n, const float4* r, float beta, float omega, const float4* v,
+                       float4* p);
+void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float4* slot);
+void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const float4* slot, float* orig);
+
 // one-time kernel attributes (dynamic LDS above 64 KB)
 int configure_kernels();
 int configure_search_kernels();

@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -336,6 +338,30 @@ public:
     DevBuf<unsigned long long> nbStats;
     DevBuf<uint32_t> haloCount, haloIds, haloStats;
     DevBuf<uint32_t> nbrLocal;
+
+    // ---- adjoint recording (one scene): per step, everything the reverse needs, in that step's slot order -----
+    struct DevTape {
+        DevBuf<float4> x, v0, v1, y, v2;
+        DevBuf<float> alpha, D, rho, rhoadv, kvEnd, kpEnd;
+        DevBuf<uint32_t> id, count, list;
+        std::deque<DevBuf<float>> kv, rv, kp, rp;  // per Jacobi iteration: kappa^k, residual^k
+        std::deque<DevBuf<float4>> av, ap;         // per Jacobi iteration: pressure acceleration a^k
+        float h = 0, H = 0;
+        uint32_t n = 0, Kd = 0, Kp = 0;
+        bool visc = false;
+    };
+    std::deque<std::unique_ptr<DevTape>> tapes;
+    bool taping = false;
+    std::string tapeError;
+    template <class U> int tape_copy(DevBuf<U>& dst, const U* src, size_t count) {
+        VFD_TRY(dst.alloc(count));
+        VFD_HIP(hipMemcpyAsync(dst.p, src, count * sizeof(U), hipMemcpyDeviceToDevice, stream));
+        return VFD_OK;
+    }
+    template <class U> int tape_slot(std::deque<DevBuf<U>>& q, uint32_t k, const U* src, size_t count) {
+        while (q.size() <= k) q.emplace_back();
+        return tape_copy(q[k], src, count);
+    }
     uint32_t haloLimit = VFD_DEFAULT_HALO_LIMIT;  // largest halo staged in LDS (tuning)
     uint32_t cellCap = 0;
     hipEvent_t ev[8] = {};
@@ -866,7 +892,7 @@ public:
     int poll_loop(Fn enqueue, uint32_t first, int which) {
         uint32_t iter = 0;
         auto issue = [&](uint32_t n, int slot) -> int {
-            if (useGraphs && !profiling && n == pollChunk && stream != nullptr) {
+            if (useGraphs && !profiling && !taping && n == pollChunk && stream != nullptr) {
                 hipGraphExec_t& g = loopGraph[which][cur];
                 if (!g) {
                     hipGraph_t graph = nullptr;
@@ -923,19 +949,51 @@ public:
                               haloStats.p);
         }));
         if (last == VFD_PHASE_BOUNDARY) return finish_partial();
+        DevTape* tp = nullptr;
+        if (taping) {
+            if (sc.size() != 1) return fail(VFD_ERROR_NOT_SUPPORTED, "adjoint recording needs a single scene");
+            tapes.emplace_back(new DevTape());
+            tp = tapes.back().get();
+            const size_t n = sc[0]->count;
+            tp->n = (uint32_t)n;
+            tp->h = sc[0]->st.dt;
+            VFD_TRY(tape_copy(tp->x, pos.p, n));
+            VFD_TRY(tape_copy(tp->v0, vel[cur].p, n));
+            VFD_TRY(tape_copy(tp->id, id[cur].p, n));
+            VFD_TRY(tape_copy(tp->count, nbrCount.p, n));
+            VFD_TRY(tape_copy(tp->list, nbrList.p, (size_t)cap * kMaxNeighbors));
+            // scope: no boundary contact (every Vb = 0 after the boundary pass)
+            std::vector<float> vbh((size_t)cap * kMaxBodies);
+            VFD_HIP(hipMemcpyAsync(vbh.data(), vb.p, vbh.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+            VFD_HIP(hipStreamSynchronize(stream));
+            for (size_t b = 0; b < sc[0]->bodies.size(); ++b)
+                for (size_t i = 0; i < n; ++i)
+                    if (vbh[b * cap + i] > 0.0f) tapeError = "boundary contact is outside the adjoint's scope";
+            if (anyST) tapeError = "surface tension is outside the adjoint's scope";
+        }
         VFD_TRY(timed("density_factor", [&] { launch_density_factor(L, K, F); }));
+        if (tp) {
+            VFD_TRY(tape_copy(tp->rho, density.p, tp->n));
+            VFD_TRY(tape_copy(tp->alpha, factor.p, tp->n));
+        }
         VFD_HIP(hipEventRecord(ev[2], stream));
         if (last == VFD_PHASE_DENSITY || last == VFD_PHASE_FACTOR) return finish_partial();
         // divergence solve (ComputeDivergence :506-575)
         launch_scene_init_div(L, F);
         VFD_TRY(timed("div_init", [&] { launch_div_init(L, K, F); }));
+        if (tp) VFD_TRY(tape_copy(tp->D, densAdv.p, tp->n));
         VFD_TRY(poll_loop(
             [&](uint32_t k) {
+                if (tp) (void)tape_slot(tp->kv, k, kappaV.p, tp->n);
                 (void)timed("div_accel", [&] { launch_div_accel(L, K, F); }, 0, k);
+                if (tp) (void)tape_slot(tp->av, k, ap.p, tp->n);
                 (void)timed("div_iter", [&] { launch_div_iter(L, K, F); }, 0, k);
+                if (tp) (void)tape_slot(tp->rv, k, residuum.p, tp->n);
             },
             minDiv, 0));
+        if (tp) VFD_TRY(tape_copy(tp->kvEnd, kappaV.p, tp->n));
         VFD_TRY(timed("div_final", [&] { launch_div_final(L, K, F); }));  // also ClearAccelerationKernel
+        if (tp) VFD_TRY(tape_copy(tp->v1, vel[cur].p, tp->n));
         VFD_HIP(hipEventRecord(ev[3], stream));
         if (last == VFD_PHASE_DIVERGENCE || last == VFD_PHASE_CLEAR_ACCELERATION) return finish_partial();
         if (anyST) VFD_TRY(timed("surface_tension", [&] { launch_surface_tension(L, F); }));
@@ -952,21 +1010,31 @@ public:
                 },
                 minVisc, 2));
             VFD_TRY(timed("visc_apply", [&] { launch_visc_apply(L, F); }));
+            if (tp) {
+                VFD_TRY(tape_copy(tp->y, cgX.p, tp->n));
+                tp->visc = true;
+            }
         }
         VFD_HIP(hipEventRecord(ev[5], stream));
         if (last == VFD_PHASE_VISCOSITY) return finish_partial();
         VFD_TRY(timed("time_step", [&] { launch_time_step(L, F); }));
         if (last == VFD_PHASE_TIME_STEP) return finish_partial();
         VFD_TRY(timed("velocity", [&] { launch_velocity(L, F); }));
+        if (tp) VFD_TRY(tape_copy(tp->v2, vel[cur].p, tp->n));
         if (last == VFD_PHASE_VELOCITY) return finish_partial();
         launch_scene_init_pres(L, F);
         VFD_TRY(timed("pres_init", [&] { launch_pres_init(L, K, F); }));
+        if (tp) VFD_TRY(tape_copy(tp->rhoadv, densAdv.p, tp->n));
         VFD_TRY(poll_loop(
             [&](uint32_t k) {
+                if (tp) (void)tape_slot(tp->kp, k, kappa.p, tp->n);
                 (void)timed("pres_accel", [&] { launch_pres_accel(L, K, F); }, 1, k);
+                if (tp) (void)tape_slot(tp->ap, k, ap.p, tp->n);
                 (void)timed("pres_iter", [&] { launch_pres_iter(L, K, F); }, 1, k);
+                if (tp) (void)tape_slot(tp->rp, k, residuum.p, tp->n);
             },
             minPres, 1));
+        if (tp) VFD_TRY(tape_copy(tp->kpEnd, kappa.p, tp->n));
         VFD_TRY(timed("pres_final", [&] { launch_pres_final(L, K, F, last >= VFD_PHASE_POSITION ? 1 : 0); }));
         VFD_HIP(hipEventRecord(ev[6], stream));
         if (last == VFD_PHASE_PRESSURE) return finish_partial();
@@ -975,6 +1043,13 @@ public:
         solver_debug();
         VFD_TRY(neighbor_stats());
         VFD_TRY(resolve_profile());
+        if (tp) {
+            tp->H = sc[0]->st.dt;
+            tp->Kd = sc[0]->desc.EnableDivergenceSolverError ? sc[0]->st.div.it : 0;
+            tp->Kp = sc[0]->st.pres.it;
+            if (sc[0]->dbg.MaxNeighborCount >= kMaxNeighbors)
+                tapeError = "a particle reached the neighbour cap (lists not symmetric)";
+        }
         float t[6] = {0, 0, 0, 0, 0, 0};
         VFD_HIP(hipEventElapsedTime(&t[0], ev[0], ev[1]));
         VFD_HIP(hipEventElapsedTime(&t[1], ev[1], ev[2]));
@@ -1011,6 +1086,155 @@ public:
             }
             d.IterationCount++;
         }
+        return VFD_OK;
+    }
+
+    // ---- adjoint: reverse of the recorded steps (kernels in adjoint.hip; the FP64 reference is the oracle's) -------
+    DevBuf<float4> aLx, aLv, aLa, aTmp4, aGs, aLy, aLb, bR, bRh, bP, bV, bS, bT;
+    DevBuf<float> aLk, aLsrc, aLs, aLf, aLrho, aLalpha, aLd, aLS, aOrigX, aOrigV;
+    DevBuf<double> aPart;
+    int adj_dot(const float4* p, const float4* q, uint32_t n, double& out) {
+        const uint32_t blocks = (n + kTile - 1) / kTile;
+        VFD_TRY(aPart.alloc(blocks));
+        adj_launch_dot(stream, n, p, q, aPart.p);
+        std::vector<double> h(blocks);
+        VFD_HIP(hipMemcpyAsync(h.data(), aPart.p, blocks * sizeof(double), hipMemcpyDeviceToHost, stream));
+        VFD_HIP(hipStreamSynchronize(stream));
+        out = 0;
+        for (double v : h) out += v;
+        return VFD_OK;
+    }
+    // BiCGSTAB for A^T x = b (the transposed viscosity system of one recorded step), relative residual 1e-6
+    int adj_visc_solve(const AdjKHost& A, float h, const float* rho, const float4* b, float4* x) {
+        const uint32_t n = A.n;
+        for (auto* q : {&bR, &bRh, &bP, &bV, &bS, &bT}) VFD_TRY(q->alloc(n));
+        VFD_HIP(hipMemcpyAsync(x, b, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        adj_launch_visc_T(stream, A, h, rho, x, bT.p);
+        VFD_HIP(hipMemcpyAsync(bR.p, b, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        adj_launch_axpy4(stream, n, -1.0f, bT.p, bR.p);
+        VFD_HIP(hipMemcpyAsync(bRh.p, bR.p, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        VFD_HIP(hipMemsetAsync(bP.p, 0, n * sizeof(float4), stream));
+        VFD_HIP(hipMemsetAsync(bV.p, 0, n * sizeof(float4), stream));
+        double bn2 = 0, rr = 0, rhoOld = 1, alpha = 1, omega = 1;
+        VFD_TRY(adj_dot(b, b, n, bn2));
+        if (bn2 == 0) return VFD_OK;
+        VFD_TRY(adj_dot(bR.p, bR.p, n, rr));
+        for (int it = 0; it < 300 && rr > 1e-12 * bn2; ++it) {
+            double rhoNew = 0, rv = 0, ts = 0, tt = 0;
+            VFD_TRY(adj_dot(bRh.p, bR.p, n, rhoNew));
+            const double beta = (rhoNew / rhoOld) * (alpha / omega);
+            adj_launch_bicg_p(stream, n, bR.p, (float)beta, (float)omega, bV.p, bP.p);
+            adj_launch_visc_T(stream, A, h, rho, bP.p, bV.p);
+            VFD_TRY(adj_dot(bRh.p, bV.p, n, rv));
+            alpha = rhoNew / rv;
+            VFD_HIP(hipMemcpyAsync(bS.p, bR.p, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+            adj_launch_axpy4(stream, n, (float)-alpha, bV.p, bS.p);
+            adj_launch_visc_T(stream, A, h, rho, bS.p, bT.p);
+            VFD_TRY(adj_dot(bT.p, bS.p, n, ts));
+            VFD_TRY(adj_dot(bT.p, bT.p, n, tt));
+            omega = tt > 0 ? ts / tt : 0.0;
+            adj_launch_axpy4(stream, n, (float)alpha, bP.p, x);
+            adj_launch_axpy4(stream, n, (float)omega, bS.p, x);
+            VFD_HIP(hipMemcpyAsync(bR.p, bS.p, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+            adj_launch_axpy4(stream, n, (float)-omega, bT.p, bR.p);
+            VFD_TRY(adj_dot(bR.p, bR.p, n, rr));
+            rhoOld = rhoNew;
+            if (omega == 0.0) break;
+        }
+        return VFD_OK;
+    }
+    // reverse of one recorded step: aLx, aLv hold the cotangents of its output (x', v') in its slot order on entry
+    // and those of its input (x, v) on return; lmu accumulates d/dmu
+    int adj_step(const DevTape& t, double& lmu) {
+        const uint32_t n = t.n;
+        const SceneParams& pr = sc[0]->prm;
+        AdjKHost A{dTab.p, kt.R, kt.invStep, kt.K, kt.L, pr.V, pr.rho0, pr.m, pr.mu, kt.R2, n, cap,
+                   t.count.p, t.list.p, t.x.p};
+        const float h = t.h, H = t.H;
+        for (auto* q : {&aLa, &aTmp4, &aGs, &aLy, &aLb}) VFD_TRY(q->alloc(n));
+        for (auto* q : {&aLk, &aLsrc, &aLs, &aLf, &aLrho, &aLalpha, &aLd, &aLS}) VFD_TRY(q->alloc(n));
+        adj_launch_fill(stream, aLrho.p, 0.0f, n);
+        adj_launch_fill(stream, aLalpha.p, 0.0f, n);
+        // x' = x + H v3  ->  lv3 = lv + H lx (aLv becomes lv3, then lv2)
+        adj_launch_axpy4(stream, n, H, aLx.p, aLv.p);
+        // pressure final: v3 = v2 + H A(kpEnd)
+        adj_launch_scale4(stream, n, aLv.p, H, aLa.p);
+        adj_launch_fill(stream, aLk.p, 0.0f, n);
+        adj_launch_accel_T(stream, A, t.kpEnd.p, aLa.p, aLk.p, aLx.p);
+        // pressure iterations, gamma = alpha / H^2, source 1 - rhoadv, c = H^2
+        adj_launch_fill(stream, aLsrc.p, 0.0f, n);
+        adj_launch_fill(stream, aLs.p, 0.0f, n);
+        for (int k = (int)t.Kp - 1; k >= 0; --k) {
+            adj_launch_jacobi_point(stream, n, aLk.p, t.alpha.p, H * H, t.rp[k].p, H * H, aLsrc.p, aLs.p, aLf.p);
+            VFD_HIP(hipMemsetAsync(aLa.p, 0, n * sizeof(float4), stream));
+            adj_launch_div_T(stream, A, aLf.p, t.ap[k].p, aLa.p, aLx.p);
+            adj_launch_accel_T(stream, A, t.kp[k].p, aLa.p, aLk.p, aLx.p);
+        }
+        adj_launch_pres_init(stream, n, H, pr.rho0, t.alpha.p, t.rhoadv.p, aLk.p, aLsrc.p, aLs.p, aLalpha.p, aLrho.p,
+                             aLd.p);
+        adj_launch_div_T(stream, A, aLd.p, t.v2.p, aLv.p, aLx.p);  // rhoadv's H D(v2)
+        // v2 = v1 + H g + (H / h)(y - v1)
+        if (t.visc) {
+            adj_launch_scale4(stream, n, aLv.p, H / h, aLy.p);
+            adj_launch_scale4(stream, n, aLv.p, 1.0f - H / h, aLv.p);
+            // A y = v1: lb = A^-T ly; lv1 += lb; positions, densities and mu through lb . dT
+            VFD_TRY(adj_visc_solve(A, h, t.rho.p, aLy.p, aLb.p));
+            adj_launch_axpy4(stream, n, 1.0f, aLb.p, aLv.p);
+            const uint32_t blocks = (n + kTile - 1) / kTile;
+            VFD_TRY(aPart.alloc(blocks));
+            adj_launch_visc_vjp(stream, A, h, t.rho.p, t.y.p, aLb.p, aLx.p, aLrho.p, aPart.p);
+            std::vector<double> hp(blocks);
+            VFD_HIP(hipMemcpyAsync(hp.data(), aPart.p, blocks * sizeof(double), hipMemcpyDeviceToHost, stream));
+            VFD_HIP(hipStreamSynchronize(stream));
+            for (double v : hp) lmu += v;
+        }
+        // divergence final: v1 = v0 + h A(kvEnd)
+        adj_launch_scale4(stream, n, aLv.p, h, aLa.p);
+        adj_launch_fill(stream, aLk.p, 0.0f, n);
+        adj_launch_accel_T(stream, A, t.kvEnd.p, aLa.p, aLk.p, aLx.p);
+        // divergence iterations, beta = alpha / h, source -D, c = h
+        adj_launch_fill(stream, aLsrc.p, 0.0f, n);
+        adj_launch_fill(stream, aLs.p, 0.0f, n);
+        for (int k = (int)t.Kd - 1; k >= 0; --k) {
+            adj_launch_jacobi_point(stream, n, aLk.p, t.alpha.p, h, t.rv[k].p, h, aLsrc.p, aLs.p, aLf.p);
+            VFD_HIP(hipMemsetAsync(aLa.p, 0, n * sizeof(float4), stream));
+            adj_launch_div_T(stream, A, aLf.p, t.av[k].p, aLa.p, aLx.p);
+            adj_launch_accel_T(stream, A, t.kv[k].p, aLa.p, aLk.p, aLx.p);
+        }
+        adj_launch_div_init(stream, n, h, t.alpha.p, t.D.p, aLk.p, aLsrc.p, aLs.p, aLalpha.p, aLd.p);
+        adj_launch_div_T(stream, A, aLd.p, t.v0.p, aLv.p, aLx.p);
+        // factor and density
+        adj_launch_factor_density(stream, A, t.alpha.p, aLalpha.p, aLS.p, aGs.p, aLrho.p, aLx.p);
+        VFD_HIP(hipGetLastError());
+        return VFD_OK;
+    }
+    // cotangents of the last recorded step's output -> of the first recorded step's input (original order)
+    int adjoint(const float* lxOut, const float* lvOut, float* lxIn, float* lvIn, double* lmu) {
+        if (!tapeError.empty()) return fail(VFD_ERROR_NOT_SUPPORTED, tapeError);
+        if (tapes.empty()) return fail(VFD_ERROR_STATE, "no recorded steps (vfd_tape)");
+        const uint32_t n = tapes.back()->n;
+        VFD_TRY(aOrigX.alloc((size_t)3 * n));
+        VFD_TRY(aOrigV.alloc((size_t)3 * n));
+        VFD_TRY(aLx.alloc(n));
+        VFD_TRY(aLv.alloc(n));
+        VFD_HIP(hipMemcpyAsync(aOrigX.p, lxOut, (size_t)3 * n * sizeof(float), hipMemcpyHostToDevice, stream));
+        if (lvOut)
+            VFD_HIP(hipMemcpyAsync(aOrigV.p, lvOut, (size_t)3 * n * sizeof(float), hipMemcpyHostToDevice, stream));
+        else
+            VFD_HIP(hipMemsetAsync(aOrigV.p, 0, (size_t)3 * n * sizeof(float), stream));
+        double m = 0;
+        for (auto it = tapes.rbegin(); it != tapes.rend(); ++it) {
+            const DevTape& t = **it;
+            adj_launch_to_slots(stream, n, t.id.p, aOrigX.p, aLx.p);
+            adj_launch_to_slots(stream, n, t.id.p, aOrigV.p, aLv.p);
+            VFD_TRY(adj_step(t, m));
+            adj_launch_to_orig(stream, n, t.id.p, aLx.p, aOrigX.p);
+            adj_launch_to_orig(stream, n, t.id.p, aLv.p, aOrigV.p);
+        }
+        VFD_HIP(hipMemcpyAsync(lxIn, aOrigX.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, stream));
+        VFD_HIP(hipMemcpyAsync(lvIn, aOrigV.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, stream));
+        VFD_HIP(hipStreamSynchronize(stream));
+        if (lmu) *lmu = m;
         return VFD_OK;
     }
 
@@ -1514,6 +1738,28 @@ int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint3
             out[i].total_ms = ks[i].ms;
         }
     return VFD_OK;
+}
+int vfd_tape(vfd_sim* sim, int32_t on) {
+    CHECK_SIM(sim);
+    auto& S = sim->impl;
+    S.drop_graphs();
+    S.taping = on != 0;
+    if (on) {
+        S.tapes.clear();
+        S.tapeError.clear();
+    }
+    return VFD_OK;
+}
+int vfd_tape_steps(vfd_sim* sim, uint32_t* steps) {
+    CHECK_SIM(sim);
+    if (!steps) return fail(VFD_ERROR_INVALID_ARGUMENT, "steps must not be NULL");
+    *steps = (uint32_t)sim->impl.tapes.size();
+    return VFD_OK;
+}
+int vfd_adjoint(vfd_sim* sim, const float* lx_out, const float* lv_out, float* lx_in, float* lv_in, double* lmu) {
+    CHECK_SIM(sim);
+    if (!lx_out || !lx_in || !lv_in) return fail(VFD_ERROR_INVALID_ARGUMENT, "cotangent buffers must not be NULL");
+    return sim->impl.adjoint(lx_out, lv_out, lx_in, lv_in, lmu);
 }
 int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out) {
     CHECK_SIM(sim);

@@ -234,6 +234,28 @@ class DFSPHSimulation:
         check(self._lib.vfd_get_profile(self._h, arr, n.value, _abi.ctypes.byref(n)), "vfd_get_profile")
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
 
+    # ---- adjoint (no reference counterpart) ------------------------------------------------------------------
+    def tape(self, on):
+        """Start (clearing earlier records) or stop recording steps for adjoint()."""
+        check(self._lib.vfd_tape(self._h, 1 if on else 0), "vfd_tape")
+
+    def taped_steps(self):
+        n = _abi.ctypes.c_uint32()
+        check(self._lib.vfd_tape_steps(self._h, _abi.ctypes.byref(n)), "vfd_tape_steps")
+        return n.value
+
+    def adjoint(self, lx_out, lv_out=None):
+        """Cotangents of the last recorded step's positions/velocities (original order) -> those of the first
+        recorded step's input, and d/dmu (mu = dynamic viscosity)."""
+        n = self.GetParticleCount()
+        lx = np.ascontiguousarray(lx_out, np.float32).reshape(n, 3)
+        lv = None if lv_out is None else np.ascontiguousarray(lv_out, np.float32).reshape(n, 3)
+        ox, ov = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        mu = _abi.ctypes.c_double()
+        check(self._lib.vfd_adjoint(self._h, ptr(lx), None if lv is None else ptr(lv), ptr(ox), ptr(ov),
+                                    _abi.ctypes.byref(mu)), "vfd_adjoint")
+        return ox, ov, mu.value
+
     def tuning(self):
         """{SolverCheckInterval, HaloLimit}: launch and solver-control knobs (results do not depend on them)."""
         t = _abi.Tuning()
