@@ -19,7 +19,9 @@ timed steps as fit in --cpu-seconds, and reports the GPU rate over exactly those
 step from the snapshot is compared with the oracle's for a full-size parity check. The default K = 95 after W = 5
 warm-up steps covers the 100-step job. `batched_gain` (second figure of merit) runs config C4, 64 uneven scenes, as
 one batch and as one simulation per scene on the same GPU: value = batched throughput / one-scene-per-launch
-throughput over the same steps.
+throughput over the same steps. `adjoint_over_forward` (third) runs config C5, 100k particles over 60 steps with
+checkpoints every 10: value = time of the backward pass (segment re-runs with recording, plus the reverse sweeps)
+/ time of the forward rollout.
 """
 import argparse
 import csv
@@ -65,24 +67,6 @@ def bytes_per_particle(kernel, nbar, bodies):
         "minmax": 12,
     }
     return table.get(kernel)
-
-
-def snapshot(sim):
-    """Everything a step reads from the previous one: positions, velocities, the viscosity warm start, the surface
-    tension curvature blend and the time step."""
-    return {"x": sim.field("position"), "v": sim.field("velocity"), "dv": sim.field("velocity_difference"),
-            "dk": sim.field("delta_final_curvature"), "dt": sim.GetDebugInfo().TimeStepSize}
-
-
-def restore(sim, snap):
-    """Put a snapshot back; SetDescription resets the time step to desc.TimeStepSize as the reference's does."""
-    d = sim.GetDescription()
-    d.TimeStepSize = snap["dt"]
-    sim.SetDescription(d)
-    sim.set_field("position", snap["x"])
-    sim.set_field("velocity", snap["v"])
-    sim.set_field("velocity_difference", snap["dv"])
-    sim.set_field("delta_final_curvature", snap["dk"])
 
 
 def cpu_baseline(cfg, snap, max_steps, budget_s):
@@ -234,6 +218,32 @@ def batched_gain(steps, warmup, config="batched_uneven_64"):
             "value": ts / tb, "identical_states": bool(same)}
 
 
+def adjoint_ratio(reps=3):
+    """Config C5, the viscosity / initial-velocity inversion: 100k particles, 60 steps, checkpoints every 10. Times
+    the forward rollout (with its checkpoints) and the backward pass (re-running each segment with recording on, then
+    vfd_adjoint) through the torch.autograd.Function, after one untimed pass; reports the median of `reps`."""
+    import numpy as np
+    import torch
+    from vfd_amd import adjoint
+    ro, v0, loss = adjoint.viscosity_inversion()
+    v = v0.clone().requires_grad_(True)
+    mu = torch.tensor(5.0, dtype=torch.float64, requires_grad=True)
+    rows = []
+    for rep in range(reps + 1):
+        v.grad, mu.grad = None, None
+        x, _ = ro(v, mu)
+        loss(x).backward()
+        if rep:
+            rows.append(dict(ro.timings))
+    fwd = float(np.median([r["forward_s"] for r in rows]))
+    bwd = float(np.median([r["backward_s"] for r in rows]))
+    rev = float(np.median([r["reverse_s"] for r in rows]))
+    return {"workload": "falling_viscous_block_100k", "particles": len(v0), "steps": ro.steps,
+            "checkpoint_every": ro.every, "forward_ms": round(1000 * fwd, 2), "backward_ms": round(1000 * bwd, 2),
+            "backward_reverse_ms": round(1000 * rev, 2), "value": round(bwd / fwd, 3),
+            "dloss_dmu": float(mu.grad)}
+
+
 def copy_probe():
     import torch
     n = COPY_PROBE_BYTES // 4
@@ -256,6 +266,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-gain measurement (config C4)")
     ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement (config C5)")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"))
     ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_DIR")
     args = ap.parse_args()
@@ -284,7 +295,7 @@ def main():
     n = sim.GetParticleCount()
     print("[bench] %s: %d particles, warmup %d steps" % (args.config, n, args.warmup), file=sys.stderr, flush=True)
     sim.advance(args.warmup)
-    snap = snapshot(sim)
+    snap = sim.checkpoint()
 
     # headline: K plain steps (no per-kernel instrumentation); every step ends with a host synchronisation
     step_s = []
@@ -298,7 +309,7 @@ def main():
     print("[bench] timed %d steps: %.3f s" % (args.steps, elapsed_plain), file=sys.stderr, flush=True)
 
     # the same K steps again from the same state, with HIP-event timing around every kernel launch
-    restore(sim, snap)
+    sim.restore(snap)
     sim.set_profiling(True)
     its = []
     nb = []
@@ -316,7 +327,7 @@ def main():
           flush=True)
 
     # one untimed check step from the snapshot, compared with the CPU oracle's step from the same state below
-    restore(sim, snap)
+    sim.restore(snap)
     sim.OnUpdate()
     d = sim.GetDebugInfo()
     check_its = (d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount)
@@ -379,6 +390,9 @@ def main():
         del sim
         print("[bench] batched gain: C4, %d steps" % args.batched_steps, file=sys.stderr, flush=True)
         out["batched_gain"] = batched_gain(args.batched_steps, 3)
+    if not args.no_adjoint:
+        print("[bench] adjoint/forward: C5", file=sys.stderr, flush=True)
+        out["adjoint_over_forward"] = adjoint_ratio()
     print(json.dumps(out), flush=True)
 
 

@@ -294,6 +294,16 @@ __global__ void k_adj_to_orig(uint32_t n, const uint32_t* __restrict__ id, const
 }
 
 // ---- host side ---------------------------------------------------------------------------------------------------
+// flag = 1 when any particle of a recorded step has a boundary volume (contact is outside the adjoint's scope)
+__global__ void __launch_bounds__(kTile) k_adj_contact(uint32_t n, uint32_t cap, uint32_t bodies,
+                                                       const float* __restrict__ vb, uint32_t* flag) {
+    const uint32_t i = blockIdx.x * kTile + threadIdx.x;
+    if (i >= n) return;
+    bool hit = false;
+    for (uint32_t b = 0; b < bodies; ++b) hit |= vb[(size_t)b * cap + i] > 0.0f;
+    if (hit) *flag = 1u;
+}
+
 static inline dim3 grid_of(uint32_t n) { return dim3((n + kTile - 1) / kTile); }
 
 void adj_launch_accel_T(hipStream_t s, const AdjKHost& h, const float* kap, const float4* la, float* lk, float4* lx) {
@@ -354,6 +364,10 @@ void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const flo
 
 AdjK AdjKHost::dev() const {
     return AdjK{tab, R, invStep, K, L, V, rho0, m, mu, R2, n, cap, count, list, x};
+}
+
+void adj_launch_contact(hipStream_t s, uint32_t n, uint32_t cap, uint32_t bodies, const float* vb, uint32_t* flag) {
+    if (n) hipLaunchKernelGGL(k_adj_contact, grid_of(n), dim3(kTile), 0, s, n, cap, bodies, vb, flag);
 }
 
 }  // namespace vfd

@@ -61,6 +61,7 @@ void adj_launch_bicg_p(hipStream_t s, uint32_t n, const float4* r, float beta, f
                        float4* p);
 void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float4* slot);
 void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const float4* slot, float* orig);
+void adj_launch_contact(hipStream_t s, uint32_t n, uint32_t cap, uint32_t bodies, const float* vb, uint32_t* flag);
 
 // one-time kernel attributes (dynamic LDS above 64 KB)
 int configure_kernels();

@@ -344,13 +344,16 @@ public:
         DevBuf<float4> x, v0, v1, y, v2;
         DevBuf<float> alpha, D, rho, rhoadv, kvEnd, kpEnd;
         DevBuf<uint32_t> id, count, list;
+        DevBuf<uint32_t> contact;  // [1]: set when a particle touched a boundary during the step
         std::deque<DevBuf<float>> kv, rv, kp, rp;  // per Jacobi iteration: kappa^k, residual^k
         std::deque<DevBuf<float4>> av, ap;         // per Jacobi iteration: pressure acceleration a^k
         float h = 0, H = 0;
         uint32_t n = 0, Kd = 0, Kp = 0;
         bool visc = false;
     };
+    // recorded steps: the first tapeCount entries of tapes; the buffers are kept and reused by later recordings
     std::deque<std::unique_ptr<DevTape>> tapes;
+    size_t tapeCount = 0;
     bool taping = false;
     std::string tapeError;
     template <class U> int tape_copy(DevBuf<U>& dst, const U* src, size_t count) {
@@ -556,6 +559,7 @@ public:
         S.frames.clear();
         S.dbg = vfd_debug_info{};
         committed = false;
+        tapeCount = 0;  // recorded steps belong to the particles they were recorded with
         return VFD_OK;
     }
 
@@ -952,23 +956,21 @@ public:
         DevTape* tp = nullptr;
         if (taping) {
             if (sc.size() != 1) return fail(VFD_ERROR_NOT_SUPPORTED, "adjoint recording needs a single scene");
-            tapes.emplace_back(new DevTape());
-            tp = tapes.back().get();
+            if (tapeCount == tapes.size()) tapes.emplace_back(new DevTape());
+            tp = tapes[tapeCount++].get();
             const size_t n = sc[0]->count;
             tp->n = (uint32_t)n;
+            tp->visc = false;
             tp->h = sc[0]->st.dt;
             VFD_TRY(tape_copy(tp->x, pos.p, n));
             VFD_TRY(tape_copy(tp->v0, vel[cur].p, n));
             VFD_TRY(tape_copy(tp->id, id[cur].p, n));
             VFD_TRY(tape_copy(tp->count, nbrCount.p, n));
             VFD_TRY(tape_copy(tp->list, nbrList.p, (size_t)cap * kMaxNeighbors));
-            // scope: no boundary contact (every Vb = 0 after the boundary pass)
-            std::vector<float> vbh((size_t)cap * kMaxBodies);
-            VFD_HIP(hipMemcpyAsync(vbh.data(), vb.p, vbh.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
-            VFD_HIP(hipStreamSynchronize(stream));
-            for (size_t b = 0; b < sc[0]->bodies.size(); ++b)
-                for (size_t i = 0; i < n; ++i)
-                    if (vbh[b * cap + i] > 0.0f) tapeError = "boundary contact is outside the adjoint's scope";
+            // scope: no boundary contact (every Vb = 0 after the boundary pass); checked by vfd_adjoint
+            VFD_TRY(tp->contact.alloc(1));
+            VFD_HIP(hipMemsetAsync(tp->contact.p, 0, sizeof(uint32_t), stream));
+            adj_launch_contact(stream, (uint32_t)n, cap, (uint32_t)sc[0]->bodies.size(), vb.p, tp->contact.p);
             if (anyST) tapeError = "surface tension is outside the adjoint's scope";
         }
         VFD_TRY(timed("density_factor", [&] { launch_density_factor(L, K, F); }));
@@ -1210,9 +1212,15 @@ public:
     }
     // cotangents of the last recorded step's output -> of the first recorded step's input (original order)
     int adjoint(const float* lxOut, const float* lvOut, float* lxIn, float* lvIn, double* lmu) {
+        if (tapeCount == 0) return fail(VFD_ERROR_STATE, "no recorded steps (vfd_tape)");
+        std::vector<uint32_t> contact(tapeCount);
+        for (size_t k = 0; k < tapeCount; ++k)
+            VFD_HIP(hipMemcpyAsync(&contact[k], tapes[k]->contact.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        VFD_HIP(hipStreamSynchronize(stream));
+        for (uint32_t c : contact)
+            if (c) tapeError = "boundary contact is outside the adjoint's scope";
         if (!tapeError.empty()) return fail(VFD_ERROR_NOT_SUPPORTED, tapeError);
-        if (tapes.empty()) return fail(VFD_ERROR_STATE, "no recorded steps (vfd_tape)");
-        const uint32_t n = tapes.back()->n;
+        const uint32_t n = tapes[tapeCount - 1]->n;
         VFD_TRY(aOrigX.alloc((size_t)3 * n));
         VFD_TRY(aOrigV.alloc((size_t)3 * n));
         VFD_TRY(aLx.alloc(n));
@@ -1223,8 +1231,8 @@ public:
         else
             VFD_HIP(hipMemsetAsync(aOrigV.p, 0, (size_t)3 * n * sizeof(float), stream));
         double m = 0;
-        for (auto it = tapes.rbegin(); it != tapes.rend(); ++it) {
-            const DevTape& t = **it;
+        for (size_t k = tapeCount; k-- > 0;) {
+            const DevTape& t = *tapes[k];
             adj_launch_to_slots(stream, n, t.id.p, aOrigX.p, aLx.p);
             adj_launch_to_slots(stream, n, t.id.p, aOrigV.p, aLv.p);
             VFD_TRY(adj_step(t, m));
@@ -1742,10 +1750,9 @@ int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint3
 int vfd_tape(vfd_sim* sim, int32_t on) {
     CHECK_SIM(sim);
     auto& S = sim->impl;
-    S.drop_graphs();
-    S.taping = on != 0;
+    S.taping = on != 0;  // (solver chunks run without their graphs while recording)
     if (on) {
-        S.tapes.clear();
+        S.tapeCount = 0;
         S.tapeError.clear();
     }
     return VFD_OK;
@@ -1753,7 +1760,7 @@ int vfd_tape(vfd_sim* sim, int32_t on) {
 int vfd_tape_steps(vfd_sim* sim, uint32_t* steps) {
     CHECK_SIM(sim);
     if (!steps) return fail(VFD_ERROR_INVALID_ARGUMENT, "steps must not be NULL");
-    *steps = (uint32_t)sim->impl.tapes.size();
+    *steps = (uint32_t)sim->impl.tapeCount;
     return VFD_OK;
 }
 int vfd_adjoint(vfd_sim* sim, const float* lx_out, const float* lv_out, float* lx_in, float* lv_in, double* lmu) {

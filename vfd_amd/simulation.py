@@ -234,6 +234,25 @@ class DFSPHSimulation:
         check(self._lib.vfd_get_profile(self._h, arr, n.value, _abi.ctypes.byref(n)), "vfd_get_profile")
         return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms) for i in range(n.value)}
 
+    # ---- checkpoints (no reference counterpart: the reference only keeps per-frame snapshots for playback,
+    #      DFSPHParticleBuffer.h:13-45, and Simulate restarts from the baked particles, DFSPHImplementation.cu:36) ---
+    def checkpoint(self):
+        """Everything a step reads from the previous one, in original particle order: positions, velocities, the
+        viscosity warm start (velocity_difference), the surface-tension curvature blend and the time step."""
+        return {"x": self.field("position"), "v": self.field("velocity"), "dv": self.field("velocity_difference"),
+                "dk": self.field("delta_final_curvature"), "dt": self.GetDebugInfo().TimeStepSize}
+
+    def restore(self, cp):
+        """Put a checkpoint back. The next step from it repeats, bit for bit, the step that followed it when it was
+        taken (same description and boundaries)."""
+        d = self.GetDescription()
+        d.TimeStepSize = cp["dt"]
+        self.SetDescription(d)  # SetDescription resets the time step to desc.TimeStepSize (:352-356)
+        self.set_field("position", cp["x"])
+        self.set_field("velocity", cp["v"])
+        self.set_field("velocity_difference", cp["dv"])
+        self.set_field("delta_final_curvature", cp["dk"])
+
     # ---- adjoint (no reference counterpart) ------------------------------------------------------------------
     def tape(self, on):
         """Start (clearing earlier records) or stop recording steps for adjoint()."""
