@@ -305,7 +305,8 @@ int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint3
  * recorded step's input (lx_in, lv_in) and d/dmu (mu = dynamic viscosity = Viscosity * rho0). Lists, time steps,
  * solver iteration counts and clamp masks are held at their recorded values; kernel derivatives are the spline's.
  * Scope: one scene, no boundary contact, no surface tension, no particle at the neighbour cap (else
- * VFD_ERROR_NOT_SUPPORTED). */
+ * VFD_ERROR_NOT_SUPPORTED, from vfd_adjoint). vfd_set_fluid discards the records; their device buffers are kept for
+ * the next recording until vfd_destroy. */
 int vfd_tape(vfd_sim* sim, int32_t on);
 int vfd_tape_steps(vfd_sim* sim, uint32_t* steps);
 int vfd_adjoint(vfd_sim* sim, const float* lx_out, const float* lv_out, float* lx_in, float* lv_in, double* lmu);
