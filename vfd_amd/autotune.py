@@ -1,21 +1,23 @@
 """Autotuner for the launch shapes and solver control of the DFSPH step.
 
 Compile-time shapes (each candidate is a separate build of the library, vfd_amd.build with defines):
-  VFD_LIST_CHUNK     halo-slot list entries fetched per batch in the staged neighbour loops
+  VFD_LIST_CHUNK     halo-slot list entries fetched per batch in the one-record staged neighbour loops
   VFD_LIST_PREFETCH  the next batch is requested before the current one is consumed (1) or not (0)
+  VFD_LIST_CHUNK2, VFD_LIST_PREFETCH2  the same for the two-record loops
   VFD_SEARCH_UNROLL  candidate positions loaded per batch in the neighbour search
   VFD_SEARCH_PREFETCH  the ranges of all 27 cells are loaded before the traversal (1) or cell by cell (0)
   VFD_STAGE_LISTS    neighbour lists staged in LDS and written row by row (1) or stored as found (0)
 Run-time knobs (vfd_set_tuning; their defaults are compiled in as VFD_DEFAULT_*):
   SolverCheckInterval  Jacobi / CG iterations enqueued between two convergence checks
 
-    python -m vfd_amd.autotune build                     # CPU: compile every candidate into vfd_amd/_tune/
+    python -m vfd_amd.autotune build                     # CPU: compile the candidates into vfd_amd/_tune/
     python -m vfd_amd.autotune run --config dam_break_2m # GPU: time them, write vfd_amd/tuning.json
 
-`run` searches one axis at a time from the defaults (coordinate descent). Every candidate runs in a process of its own
-(VFD_LIBRARY points it at its build) over the same deterministic steps; its final positions must equal the default
-build's bit for bit, or the candidate is rejected. tuning.json records every timing and the choice; vfd_amd.build
-compiles the chosen values in.
+The candidates are the defaults and every shape that differs from them in one axis (a star around the defaults; the
+full grid would be hundreds of builds). `run` times each of them, keeps the fastest whose end state is identical, and
+then sweeps the check interval on it. Every candidate runs in a process of its own (VFD_LIBRARY points it at its
+build) over the same deterministic steps; its final positions must equal the default build's bit for bit, or the
+candidate is rejected. tuning.json records every timing and the choice; vfd_amd.build compiles the chosen values in.
 """
 import argparse
 import hashlib
@@ -33,12 +35,14 @@ TUNING_JSON = os.path.join(HERE, "tuning.json")
 AXES = {
     "VFD_LIST_CHUNK": [4, 8],
     "VFD_LIST_PREFETCH": [0, 1],
+    "VFD_LIST_CHUNK2": [2, 4, 8],
+    "VFD_LIST_PREFETCH2": [0, 1],
     "VFD_SEARCH_UNROLL": [4, 8],
     "VFD_SEARCH_PREFETCH": [0, 1],
     "VFD_STAGE_LISTS": [0, 1],
 }
-DEFAULTS = {"VFD_LIST_CHUNK": 8, "VFD_LIST_PREFETCH": 1, "VFD_SEARCH_UNROLL": 8, "VFD_SEARCH_PREFETCH": 1,
-            "VFD_STAGE_LISTS": 0}
+DEFAULTS = {"VFD_LIST_CHUNK": 8, "VFD_LIST_PREFETCH": 1, "VFD_LIST_CHUNK2": 4, "VFD_LIST_PREFETCH2": 1,
+            "VFD_SEARCH_UNROLL": 4, "VFD_SEARCH_PREFETCH": 1, "VFD_STAGE_LISTS": 0}
 INTERVALS = [2, 4, 8, 16]
 DEFAULT_INTERVAL = 4
 
@@ -58,10 +62,10 @@ def _name(shape):
 
 
 def _candidates():
-    """Every shape of the grid (coordinate descent may visit any of them)."""
-    shapes = [{}]
+    """The defaults and every shape one axis away from them."""
+    shapes = [dict(DEFAULTS)]
     for k, vals in AXES.items():
-        shapes = [dict(s, **{k: v}) for s in shapes for v in vals]
+        shapes += [dict(DEFAULTS, **{k: v}) for v in vals if v != DEFAULTS[k]]
     return shapes
 
 
@@ -122,17 +126,12 @@ def run(config, steps, warmup, timeout):
     ref = measure(best_shape, best_interval)
     best_ms = ref["ms_per_step"]
     log.append({"shape": dict(best_shape), "interval": best_interval, **ref})
-    for axis, vals in AXES.items():
-        for v in vals:
-            if v == best_shape[axis]:
-                continue
-            s = dict(best_shape)
-            s[axis] = v
-            r = measure(s, best_interval)
-            ok = r["digest"] == ref["digest"]
-            log.append({"shape": s, "interval": best_interval, "identical": ok, **r})
-            if ok and r["ms_per_step"] < best_ms:
-                best_shape, best_ms = s, r["ms_per_step"]
+    for s in _candidates()[1:]:
+        r = measure(s, DEFAULT_INTERVAL)
+        ok = r["digest"] == ref["digest"]
+        log.append({"shape": s, "interval": DEFAULT_INTERVAL, "identical": ok, **r})
+        if ok and r["ms_per_step"] < best_ms:
+            best_shape, best_ms = s, r["ms_per_step"]
     for iv in INTERVALS:
         if iv == best_interval:
             continue

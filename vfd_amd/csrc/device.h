@@ -252,15 +252,18 @@ __device__ __forceinline__ void each_neighbor(const uint32_t* __restrict__ list,
 
 // Kernel table staged in LDS: tab[p] (float2 {W, G}) or G only, loaded once per block. The sizes are rounded up to
 // 16 bytes so that the halo records can follow the table.
-constexpr unsigned kLdsG = (kTableEntries * sizeof(float) + 15u) / 16u * 16u;
-constexpr unsigned kLdsWG = (kTableEntries * sizeof(float2) + 15u) / 16u * 16u;
+// both LDS tables hold one entry past the kernel table, for code kGCodeZero (see gcode)
+constexpr unsigned kLdsG = ((kTableEntries + 1u) * sizeof(float) + 15u) / 16u * 16u;
+constexpr unsigned kLdsWG = ((kTableEntries + 1u) * sizeof(float2) + 15u) / 16u * 16u;
 __device__ __forceinline__ void load_table_g(float* sG, const float2* __restrict__ tab) {
     for (uint32_t p = threadIdx.x; p < kTableEntries; p += blockDim.x) sG[p] = tab[p].y;
     if (threadIdx.x == 0) sG[kTableEntries] = 0.0f;  // code kGCodeZero: beyond the gradient cutoff
     __syncthreads();
 }
+// {W, G} pairs; entry kGCodeZero = {W of the last entry (what GetW reads beyond R), 0}
 __device__ __forceinline__ void load_table_wg(float2* sWG, const float2* __restrict__ tab) {
     for (uint32_t p = threadIdx.x; p < kTableEntries; p += blockDim.x) sWG[p] = tab[p];
+    if (threadIdx.x == 0) sWG[kTableEntries] = make_float2(tab[kTableEntries - 1u].x, 0.0f);
     __syncthreads();
 }
 // gradW(r) with the G table in LDS (cutoff |r| <= R), same values as gradW(KTab, r)
@@ -335,59 +338,90 @@ struct LoadXSV {  // lo = {x_t, s_t}, hi = {v_t, -}
     }
 };
 
-// f(lo, hi, code) for every neighbour of particle i in list order, code = gcode(xi - x_j). Staged groups read
-// packed list entries (halo slot | code << 16, the code computed once per step when the halo is built) in chunks of
-// kListChunk independent loads, and (VFD_LIST_PREFETCH) request the next chunk before consuming the current one, so
-// list latency overlaps the LDS work; the global path fetches four list entries per trip so that their gathers can
-// be in flight together, and computes the code itself.
+// f(lo, hi, t) for every neighbour of particle i in list order, t = tab[code] and code = gcode(xi - x_j) (the
+// kernel-table entry of the pair). Staged groups read packed list entries (halo slot | code << 16, the code computed
+// once per step when the halo is built) in chunks of ListShape<R>::chunk entries: the chunk's list loads are issued together (rows
+// past the end repeat the last entry, so every load and LDS read stays in bounds), then all of its LDS record and
+// table reads, and only then its f calls, so that neither the list latency nor the LDS latency is paid once per
+// neighbour. With VFD_LIST_PREFETCH the next chunk's list loads are issued before the current chunk is consumed.
+// The global path fetches four list entries per trip so that their gathers can be in flight together, and computes
+// the code itself.
+// Chunk length and prefetch are chosen per record count R: one-record loops (16 B per neighbour) profit from long
+// chunks with the next one in flight; two-record loops hold twice the registers per entry and take short chunks.
 #ifndef VFD_LIST_CHUNK
 #define VFD_LIST_CHUNK 8
 #endif
 #ifndef VFD_LIST_PREFETCH
 #define VFD_LIST_PREFETCH 1
 #endif
-constexpr uint32_t kListChunk = VFD_LIST_CHUNK;
-template <int R, class Load, class Fn>
+#ifndef VFD_LIST_CHUNK2
+#define VFD_LIST_CHUNK2 4
+#endif
+#ifndef VFD_LIST_PREFETCH2
+#define VFD_LIST_PREFETCH2 0
+#endif
+template <int R> struct ListShape {
+    static constexpr uint32_t chunk = R == 1 ? VFD_LIST_CHUNK : VFD_LIST_CHUNK2;
+    static constexpr bool prefetch = (R == 1 ? VFD_LIST_PREFETCH : VFD_LIST_PREFETCH2) != 0;
+};
+// Halo records in LDS: R = 1 is one float4 per slot; R = 2 keeps the two halves in two arrays of kHaloMax float4
+// (lo at [slot], hi at [kHaloMax + slot]), so that both reads of a neighbour spread over the banks as R = 1 reads do.
+template <int R> __device__ __forceinline__ float4 rec_lo(const float4* rec, uint32_t slot) { return rec[slot]; }
+template <int R> __device__ __forceinline__ float4 rec_hi(const float4* rec, uint32_t slot) {
+    return R == 2 ? rec[kHaloMax + slot] : rec[slot];
+}
+template <int R, class TV, class Load, class Fn>
 __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i, float cutR, float invStep, f3 xi,
-                                                  const Load& load, Fn&& f) {
+                                                  const TV* tab, const Load& load, Fn&& f) {
+    constexpr uint32_t C = ListShape<R>::chunk;
     const uint32_t n = src.count[i];
     const size_t cap = src.cap;
     uint32_t j = 0;
     if (src.rec) {
+        if (n == 0) return;
         const float4* rec = src.rec;
         const uint32_t* __restrict__ li = src.local + i;
-        auto fetch = [&](uint32_t (&s)[kListChunk], uint32_t b) {
+        const uint32_t last = n - 1u;
+        auto fetch = [&](uint32_t (&e)[C], uint32_t b) {
 #pragma unroll
-            for (uint32_t u = 0; u < kListChunk; ++u) s[u] = b + u < n ? li[(b + u) * cap] : 0u;
+            for (uint32_t u = 0; u < C; ++u) e[u] = li[(size_t)min(b + u, last) * cap];
         };
-        auto consume = [&](const uint32_t (&s)[kListChunk], uint32_t b) {
+        auto consume = [&](const uint32_t (&e)[C], uint32_t b) {
+            float4 lo[C], hi[C];
+            TV tv[C];
 #pragma unroll
-            for (uint32_t u = 0; u < kListChunk; ++u) {
-                if (b + u < n) {
-                    const uint32_t slot = s[u] & 0xFFFFu;
-                    const float4 lo = rec[R * slot];
-                    const float4 hi = R == 2 ? rec[R * slot + 1] : lo;
-                    f(lo, hi, s[u] >> 16);
-                }
+            for (uint32_t u = 0; u < C; ++u) {
+                const uint32_t slot = e[u] & 0xFFFFu;
+                lo[u] = rec_lo<R>(rec, slot);
+                hi[u] = R == 2 ? rec_hi<R>(rec, slot) : lo[u];
+                tv[u] = tab[e[u] >> 16];
+            }
+            if (b + C <= n) {
+#pragma unroll
+                for (uint32_t u = 0; u < C; ++u) f(lo[u], hi[u], tv[u]);
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < C; ++u)
+                    if (b + u < n) f(lo[u], hi[u], tv[u]);
             }
         };
-#if VFD_LIST_PREFETCH
-        // two chunks in flight: the next chunk's entries are requested before the current chunk is consumed
-        uint32_t s0[kListChunk], s1[kListChunk];
-        fetch(s0, 0);
-        for (; j < n; j += 2 * kListChunk) {
-            if (j + kListChunk < n) fetch(s1, j + kListChunk);
-            consume(s0, j);
-            if (j + 2 * kListChunk < n) fetch(s0, j + 2 * kListChunk);
-            if (j + kListChunk < n) consume(s1, j + kListChunk);
+        uint32_t e0[C];
+        fetch(e0, 0);
+        if (ListShape<R>::prefetch) {
+            for (; j < n; j += 2 * C) {
+                uint32_t e1[C];
+                const bool more = j + C < n;
+                if (more) fetch(e1, j + C);
+                consume(e0, j);
+                if (j + 2 * C < n) fetch(e0, j + 2 * C);
+                if (more) consume(e1, j + C);
+            }
+        } else {
+            for (; j < n; j += C) {
+                if (j) fetch(e0, j);
+                consume(e0, j);
+            }
         }
-#else
-        for (; j < n; j += kListChunk) {
-            uint32_t s0[kListChunk];
-            fetch(s0, j);
-            consume(s0, j);
-        }
-#endif
     } else {
         for (; j + 4 <= n; j += 4) {
             uint32_t t[4];
@@ -397,12 +431,12 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
 #pragma unroll
             for (int u = 0; u < 4; ++u) load(t[u], lo[u], hi[u]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) f(lo[u], hi[u], gcode(cutR, invStep, xi - ld3(lo[u])));
+            for (int u = 0; u < 4; ++u) f(lo[u], hi[u], tab[gcode(cutR, invStep, xi - ld3(lo[u]))]);
         }
         for (; j < n; ++j) {
             float4 lo, hi;
             load(src.list[j * cap + i], lo, hi);
-            f(lo, hi, gcode(cutR, invStep, xi - ld3(lo)));
+            f(lo, hi, tab[gcode(cutR, invStep, xi - ld3(lo))]);
         }
     }
 }
@@ -427,22 +461,29 @@ __device__ __forceinline__ void group_loop(const TileGrid& tg, const HaloView& H
         const uint32_t tile = g * kGroupTiles + sub;
         const uint32_t m = H.count[g];
         if (m <= kHaloMax) {
-            // every thread copies up to four records; the four slot loads are issued before any gather
+            // every thread copies up to four records: the four slot loads first (indices past the halo repeat its
+            // last entry, so they stay in bounds), then the four record gathers, then the LDS stores
             static_assert(kHaloMax <= 4 * kGroupSize, "fill covers four records per thread");
-            const uint32_t* ids = H.ids + (size_t)g * kHaloMax;
-            const uint32_t k0 = threadIdx.x, k1 = k0 + kGroupSize, k2 = k1 + kGroupSize, k3 = k2 + kGroupSize;
-            const uint32_t t0 = k0 < m ? ids[k0] : 0u, t1 = k1 < m ? ids[k1] : 0u;
-            const uint32_t t2 = k2 < m ? ids[k2] : 0u, t3 = k3 < m ? ids[k3] : 0u;
-            auto copy = [&](uint32_t k, uint32_t t) {
-                float4 lo, hi;
-                load(t, lo, hi);
-                rec[R * k] = lo;
-                if (R == 2) rec[R * k + 1] = hi;
-            };
-            if (k0 < m) copy(k0, t0);
-            if (k1 < m) copy(k1, t1);
-            if (k2 < m) copy(k2, t2);
-            if (k3 < m) copy(k3, t3);
+            if (m) {
+                const uint32_t* ids = H.ids + (size_t)g * kHaloMax;
+                const uint32_t k0 = threadIdx.x, k1 = k0 + kGroupSize, k2 = k1 + kGroupSize, k3 = k2 + kGroupSize;
+                const uint32_t mm = m - 1u;
+                const uint32_t t0 = ids[min(k0, mm)], t1 = ids[min(k1, mm)];
+                const uint32_t t2 = ids[min(k2, mm)], t3 = ids[min(k3, mm)];
+                float4 lo0, hi0, lo1, hi1, lo2, hi2, lo3, hi3;
+                load(t0, lo0, hi0);
+                if (k1 < m) load(t1, lo1, hi1);
+                if (k2 < m) load(t2, lo2, hi2);
+                if (k3 < m) load(t3, lo3, hi3);
+                auto put = [&](uint32_t k, const float4& lo, const float4& hi) {
+                    rec[k] = lo;
+                    if (R == 2) rec[kHaloMax + k] = hi;
+                };
+                if (k0 < m) put(k0, lo0, hi0);
+                if (k1 < m) put(k1, lo1, hi1);
+                if (k2 < m) put(k2, lo2, hi2);
+                if (k3 < m) put(k3, lo3, hi3);
+            }
             __syncthreads();
             NbrSrc src = global;
             src.rec = rec;

@@ -43,13 +43,13 @@ __global__ void __launch_bounds__(kGroupSize) k_density_factor(const SceneParams
         float dens = V * k.W0;
         f3 gi = mk3(0.0f, 0.0f, 0.0f);
         float sum = 0.0f;
-        each_neighbor_rec<1>(src, c.i, k.R, k.invStep, xi, LoadX{F.pos}, [&](float4 lo, float4, uint32_t code) {
+        each_neighbor_rec<1>(src, c.i, k.R, k.invStep, xi, k.tab, LoadX{F.pos}, [&](float4 lo, float4, float2 t) {
             // W: cutoff r^2 <= R^2 always holds for a listed pair (0 < d^2 < R^2); beyond the gradient cutoff the
-            // index GetW reads is the last entry (|r| > R gives |r| / step > 9998)
+            // index GetW reads is the last entry (|r| > R gives |r| / step > 9998), which the table's entry
+            // kGCodeZero repeats with G = 0
             const f3 r = xi - ld3(lo);
-            const float2 t = k.tab[code < kGCodeZero ? code : kTableEntries - 1u];
             dens += V * t.x;
-            const f3 gj = (-V) * ((code < kGCodeZero ? t.y : 0.0f) * r);
+            const f3 gj = (-V) * (t.y * r);
             sum += dot3(gj, gj);
             gi = gi - gj;
         });
@@ -80,8 +80,8 @@ __device__ __forceinline__ float velocity_divergence(const SceneParams& p, const
                                                      uint32_t cap, uint32_t i, const NbrSrc& src) {
     const f3 xi = ld3(F.pos[i]), vi = ld3(F.vel[i]);
     float d = 0.0f;
-    each_neighbor_rec<2>(src, i, k.R, k.invStep, xi, LoadXV{F.pos, F.vel}, [&](float4 lo, float4 hi, uint32_t code) {
-        d += dot3(vi - ld3(hi), k.sG[code] * (xi - ld3(lo)));
+    each_neighbor_rec<2>(src, i, k.R, k.invStep, xi, k.sG, LoadXV{F.pos, F.vel}, [&](float4 lo, float4 hi, float g) {
+        d += dot3(vi - ld3(hi), g * (xi - ld3(lo)));
     });
     d *= p.V;
     for (uint32_t b = 0; b < p.bodyCount; ++b) {
@@ -98,10 +98,10 @@ __device__ __forceinline__ f3 pressure_accel(const SceneParams& p, const KLds& k
     const f3 xi = ld3(F.pos[i]);
     const float ki = kap[i];
     f3 a = mk3(0.0f, 0.0f, 0.0f);
-    each_neighbor_rec<1>(src, i, k.R, k.invStep, xi, LoadXS{F.pos, kap}, [&](float4 lo, float4, uint32_t code) {
+    each_neighbor_rec<1>(src, i, k.R, k.invStep, xi, k.sG, LoadXS{F.pos, kap}, [&](float4 lo, float4, float g) {
         const float s = ki + lo.w;
         if (above_eps(fabsf(s))) {
-            const f3 gj = (-p.V) * (k.sG[code] * (xi - ld3(lo)));
+            const f3 gj = (-p.V) * (g * (xi - ld3(lo)));
             a = a + s * gj;
         }
     });
@@ -122,8 +122,8 @@ __device__ __forceinline__ float density_pressure_force(const SceneParams& p, co
                                                         uint32_t cap, uint32_t i, const NbrSrc& src) {
     const f3 xi = ld3(F.pos[i]), ai = ld3(F.ap[i]);
     float f = 0.0f;
-    each_neighbor_rec<2>(src, i, k.R, k.invStep, xi, LoadXV{F.pos, F.ap}, [&](float4 lo, float4 hi, uint32_t code) {
-        f += dot3(ai - ld3(hi), k.sG[code] * (xi - ld3(lo)));
+    each_neighbor_rec<2>(src, i, k.R, k.invStep, xi, k.sG, LoadXV{F.pos, F.ap}, [&](float4 lo, float4 hi, float g) {
+        f += dot3(ai - ld3(hi), g * (xi - ld3(lo)));
     });
     f *= p.V;
     for (uint32_t b = 0; b < p.bodyCount; ++b) {
@@ -520,10 +520,10 @@ __global__ void __launch_bounds__(kGroupSize) k_visc_precond(const SceneParams* 
         float M[9];
 #pragma unroll
         for (int e = 0; e < 9; ++e) M[e] = 0.0f;
-        each_neighbor_rec<1>(src, c.i, K.R, K.invStep, xi, LoadXS{F.pos, F.density},
-                             [&](float4 lo, float4, uint32_t code) {
+        each_neighbor_rec<1>(src, c.i, K.R, K.invStep, xi, K.sG, LoadXS{F.pos, F.density},
+                             [&](float4 lo, float4, float gt) {
             const f3 xij = xi - ld3(lo);
-            const f3 gw = K.sG[code] * xij;
+            const f3 gw = gt * xij;
             const float s = 10.0f * p.mu * (p.m / lo.w) / (length2(xij) + d001);
             const float a[3] = {xij.x, xij.y, xij.z}, g[3] = {gw.x, gw.y, gw.z};
 #pragma unroll
@@ -567,10 +567,10 @@ __device__ __forceinline__ f3 visc_matvec(const SceneParams& p, const KLds& k, c
     const f3 xi = ld3(F.pos[i]), vi = ld3(in[i]);
     const float d001 = 0.01f * p.R2;
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
-    each_neighbor_rec<2>(src, i, k.R, k.invStep, xi, LoadXSV{F.pos, F.density, in},
-                         [&](float4 lo, float4 hi, uint32_t code) {
+    each_neighbor_rec<2>(src, i, k.R, k.invStep, xi, k.sG, LoadXSV{F.pos, F.density, in},
+                         [&](float4 lo, float4 hi, float g) {
         const f3 xij = xi - ld3(lo);
-        const f3 gw = k.sG[code] * xij;
+        const f3 gw = g * xij;
         const float s = 10.0f * p.mu * (p.m / lo.w) * dot3(vi - ld3(hi), xij) / (length2(xij) + d001);
         acc = acc + s * gw;
     });
