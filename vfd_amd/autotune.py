@@ -4,6 +4,7 @@ Compile-time shapes (each candidate is a separate build of the library, vfd_amd.
   VFD_LIST_CHUNK     halo-slot list entries fetched per batch in the one-record staged neighbour loops
   VFD_LIST_PREFETCH  the next batch is requested before the current one is consumed (1) or not (0)
   VFD_LIST_CHUNK2, VFD_LIST_PREFETCH2  the same for the two-record loops
+  VFD_PACK_RECORDS   halo record halves read as three floats stored in 12 bytes (1) or as 16-byte float4 (0)
   VFD_SEARCH_UNROLL  candidate positions loaded per batch in the neighbour search
   VFD_SEARCH_PREFETCH  the ranges of all 27 cells are loaded before the traversal (1) or cell by cell (0)
   VFD_STAGE_LISTS    neighbour lists staged in LDS and written row by row (1) or stored as found (0)
@@ -40,9 +41,10 @@ AXES = {
     "VFD_SEARCH_UNROLL": [4, 8],
     "VFD_SEARCH_PREFETCH": [0, 1],
     "VFD_STAGE_LISTS": [0, 1],
+    "VFD_PACK_RECORDS": [0, 1],
 }
 DEFAULTS = {"VFD_LIST_CHUNK": 8, "VFD_LIST_PREFETCH": 1, "VFD_LIST_CHUNK2": 4, "VFD_LIST_PREFETCH2": 1,
-            "VFD_SEARCH_UNROLL": 4, "VFD_SEARCH_PREFETCH": 1, "VFD_STAGE_LISTS": 0}
+            "VFD_SEARCH_UNROLL": 8, "VFD_SEARCH_PREFETCH": 1, "VFD_STAGE_LISTS": 0, "VFD_PACK_RECORDS": 1}
 INTERVALS = [2, 4, 8, 16]
 DEFAULT_INTERVAL = 4
 

@@ -296,7 +296,7 @@ __device__ __forceinline__ f3 gradW(const KLds& k, f3 r) {
 // halo are copied to LDS once and the lists are read as 16-bit halo slots; otherwise the loader gathers them per
 // list entry. Both paths hand the same values to the same arithmetic in the same list order.
 struct NbrSrc {
-    const float4* rec;                     // LDS records, R per halo slot; nullptr: global gathers
+    const float* rec;                      // LDS records of the halo (layout: HaloRec); nullptr: global gathers
     const uint32_t* __restrict__ local;    // [j * cap + i] halo slot | gradient table code << 16 (staged groups)
     const uint32_t* __restrict__ list;     // [j * cap + i] particle slots
     const uint32_t* __restrict__ count;    // [i]
@@ -304,6 +304,7 @@ struct NbrSrc {
 };
 
 struct LoadX {  // lo = hi = {x_t, -}
+    static constexpr int kLo = 3, kHi = 0;  // floats of lo and hi that the neighbour terms read
     const float4* __restrict__ pos;
     __device__ __forceinline__ void operator()(uint32_t t, float4& lo, float4& hi) const {
         lo = pos[t];
@@ -311,6 +312,7 @@ struct LoadX {  // lo = hi = {x_t, -}
     }
 };
 struct LoadXS {  // lo = hi = {x_t, s_t}
+    static constexpr int kLo = 4, kHi = 0;
     const float4* __restrict__ pos;
     const float* __restrict__ s;
     __device__ __forceinline__ void operator()(uint32_t t, float4& lo, float4& hi) const {
@@ -320,6 +322,7 @@ struct LoadXS {  // lo = hi = {x_t, s_t}
     }
 };
 struct LoadXV {  // lo = {x_t, -}, hi = {v_t, -}
+    static constexpr int kLo = 3, kHi = 3;
     const float4* __restrict__ pos;
     const float4* __restrict__ v;
     __device__ __forceinline__ void operator()(uint32_t t, float4& lo, float4& hi) const {
@@ -328,6 +331,7 @@ struct LoadXV {  // lo = {x_t, -}, hi = {v_t, -}
     }
 };
 struct LoadXSV {  // lo = {x_t, s_t}, hi = {v_t, -}
+    static constexpr int kLo = 4, kHi = 3;
     const float4* __restrict__ pos;
     const float* __restrict__ s;
     const float4* __restrict__ v;
@@ -346,8 +350,10 @@ struct LoadXSV {  // lo = {x_t, s_t}, hi = {v_t, -}
 // neighbour. With VFD_LIST_PREFETCH the next chunk's list loads are issued before the current chunk is consumed.
 // The global path fetches four list entries per trip so that their gathers can be in flight together, and computes
 // the code itself.
-// Chunk length and prefetch are chosen per record count R: one-record loops (16 B per neighbour) profit from long
-// chunks with the next one in flight; two-record loops hold twice the registers per entry and take short chunks.
+// Chunk length and lookahead are chosen per record count R: one-record loops (16 B per neighbour) profit from long
+// chunks; two-record loops hold twice the registers per entry and take short ones. VFD_LIST_PREFETCH(2) is the
+// number of further chunks whose list loads are in flight while a chunk is consumed (0: none), since the list
+// latency, not the LDS, bounds these loops at one 1024-thread block per CU.
 #ifndef VFD_LIST_CHUNK
 #define VFD_LIST_CHUNK 8
 #endif
@@ -358,18 +364,44 @@ struct LoadXSV {  // lo = {x_t, s_t}, hi = {v_t, -}
 #define VFD_LIST_CHUNK2 4
 #endif
 #ifndef VFD_LIST_PREFETCH2
-#define VFD_LIST_PREFETCH2 0
+#define VFD_LIST_PREFETCH2 1
 #endif
 template <int R> struct ListShape {
     static constexpr uint32_t chunk = R == 1 ? VFD_LIST_CHUNK : VFD_LIST_CHUNK2;
-    static constexpr bool prefetch = (R == 1 ? VFD_LIST_PREFETCH : VFD_LIST_PREFETCH2) != 0;
+    static constexpr uint32_t depth = 1u + (R == 1 ? VFD_LIST_PREFETCH : VFD_LIST_PREFETCH2);  // chunks in flight
 };
-// Halo records in LDS: R = 1 is one float4 per slot; R = 2 keeps the two halves in two arrays of kHaloMax float4
-// (lo at [slot], hi at [kHaloMax + slot]), so that both reads of a neighbour spread over the banks as R = 1 reads do.
-template <int R> __device__ __forceinline__ float4 rec_lo(const float4* rec, uint32_t slot) { return rec[slot]; }
-template <int R> __device__ __forceinline__ float4 rec_hi(const float4* rec, uint32_t slot) {
-    return R == 2 ? rec[kHaloMax + slot] : rec[slot];
-}
+// Halo records in LDS, as two arrays of kHaloMax records: lo of slot s at rec[L * s], hi at rec[L * kHaloMax + H * s]
+// (H = 0 for one-record loops). With VFD_PACK_RECORDS a half that the neighbour terms read as three floats is stored
+// in 12 bytes (L or H = 3), which spreads a wave's random reads over all 32 banks instead of 8 groups of 4; otherwise
+// every half is a 16-byte float4.
+#ifndef VFD_PACK_RECORDS
+#define VFD_PACK_RECORDS 1
+#endif
+template <class Load> struct HaloRec {
+    static constexpr int L = VFD_PACK_RECORDS ? Load::kLo : 4;
+    static constexpr int H = Load::kHi == 0 ? 0 : (VFD_PACK_RECORDS ? Load::kHi : 4);
+    static __device__ __forceinline__ float4 get(const float* p, int n) {
+        if (n == 4) return *reinterpret_cast<const float4*>(p);
+        return make_float4(p[0], p[1], p[2], 0.0f);
+    }
+    static __device__ __forceinline__ void put(float* p, int n, const float4& v) {
+        if (n == 4) {
+            *reinterpret_cast<float4*>(p) = v;
+        } else {
+            p[0] = v.x;
+            p[1] = v.y;
+            p[2] = v.z;
+        }
+    }
+    static __device__ __forceinline__ float4 lo(const float* rec, uint32_t s) { return get(rec + L * s, L); }
+    static __device__ __forceinline__ float4 hi(const float* rec, uint32_t s) {
+        return get(rec + L * kHaloMax + H * s, H);
+    }
+    static __device__ __forceinline__ void store(float* rec, uint32_t s, const float4& l, const float4& h) {
+        put(rec + L * s, L, l);
+        if (H) put(rec + L * kHaloMax + H * s, H, h);
+    }
+};
 template <int R, class TV, class Load, class Fn>
 __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i, float cutR, float invStep, f3 xi,
                                                   const TV* tab, const Load& load, Fn&& f) {
@@ -379,7 +411,8 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
     uint32_t j = 0;
     if (src.rec) {
         if (n == 0) return;
-        const float4* rec = src.rec;
+        const float* rec = src.rec;
+        using HR = HaloRec<Load>;
         const uint32_t* __restrict__ li = src.local + i;
         const uint32_t last = n - 1u;
         auto fetch = [&](uint32_t (&e)[C], uint32_t b) {
@@ -392,8 +425,8 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
 #pragma unroll
             for (uint32_t u = 0; u < C; ++u) {
                 const uint32_t slot = e[u] & 0xFFFFu;
-                lo[u] = rec_lo<R>(rec, slot);
-                hi[u] = R == 2 ? rec_hi<R>(rec, slot) : lo[u];
+                lo[u] = HR::lo(rec, slot);
+                hi[u] = HR::H ? HR::hi(rec, slot) : lo[u];
                 tv[u] = tab[e[u] >> 16];
             }
             if (b + C <= n) {
@@ -405,21 +438,18 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
                     if (b + u < n) f(lo[u], hi[u], tv[u]);
             }
         };
-        uint32_t e0[C];
-        fetch(e0, 0);
-        if (ListShape<R>::prefetch) {
-            for (; j < n; j += 2 * C) {
-                uint32_t e1[C];
-                const bool more = j + C < n;
-                if (more) fetch(e1, j + C);
-                consume(e0, j);
-                if (j + 2 * C < n) fetch(e0, j + 2 * C);
-                if (more) consume(e1, j + C);
-            }
-        } else {
-            for (; j < n; j += C) {
-                if (j) fetch(e0, j);
-                consume(e0, j);
+        // ring of D chunk buffers: chunk c lives in e[c % D]; chunk c + D - 1 is requested before chunk c is consumed
+        constexpr uint32_t D = ListShape<R>::depth;
+        uint32_t e[D][C];
+#pragma unroll
+        for (uint32_t k = 0; k + 1 < D; ++k)
+            if (k * C < n) fetch(e[k], k * C);
+        for (; j < n; j += D * C) {
+#pragma unroll
+            for (uint32_t k = 0; k < D; ++k) {
+                const uint32_t b = j + k * C;
+                if (b + (D - 1) * C < n) fetch(e[(k + D - 1) % D], b + (D - 1) * C);
+                if (b < n) consume(e[k], b);
             }
         }
     } else {
@@ -451,7 +481,7 @@ struct HaloView {
 // every thread first copies halo records to LDS; body(tile, ok, src) is then called by every thread of the block
 // (ok = false past the last tile) and may contain block barriers.
 template <int R, class Load, class Body>
-__device__ __forceinline__ void group_loop(const TileGrid& tg, const HaloView& H, float4* rec, const NbrSrc& global,
+__device__ __forceinline__ void group_loop(const TileGrid& tg, const HaloView& H, float* rec, const NbrSrc& global,
                                            const Load& load, Body&& body) {
     const uint32_t sub = threadIdx.x / kTile;
     const uint32_t perG = tg.per / kGroupTiles;
@@ -475,10 +505,7 @@ __device__ __forceinline__ void group_loop(const TileGrid& tg, const HaloView& H
                 if (k1 < m) load(t1, lo1, hi1);
                 if (k2 < m) load(t2, lo2, hi2);
                 if (k3 < m) load(t3, lo3, hi3);
-                auto put = [&](uint32_t k, const float4& lo, const float4& hi) {
-                    rec[k] = lo;
-                    if (R == 2) rec[kHaloMax + k] = hi;
-                };
+                auto put = [&](uint32_t k, const float4& lo, const float4& hi) { HaloRec<Load>::store(rec, k, lo, hi); };
                 if (k0 < m) put(k0, lo0, hi0);
                 if (k1 < m) put(k1, lo1, hi1);
                 if (k2 < m) put(k2, lo2, hi2);

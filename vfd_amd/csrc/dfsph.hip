@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(kGroupSize) k_density_factor(const SceneParams
     float2* sWG = reinterpret_cast<float2*>(lds);
     load_table_wg(sWG, kh.tab);
     const KTab k{sWG, kh.R, kh.R2, kh.invStep, kh.W0};
-    float4* rec = lds + kLdsWG / sizeof(float4);
+    float* rec = reinterpret_cast<float*>(lds + kLdsWG / sizeof(float4));
     group_loop<1>(tg, halo(F), rec, global_src(F, cap), LoadX{F.pos}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
         const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         if (!c.valid) return;
@@ -139,7 +139,7 @@ __device__ __forceinline__ float density_pressure_force(const SceneParams& p, co
     float* sG = reinterpret_cast<float*>(lds);                         \
     load_table_g(sG, (kh).tab);                                        \
     const KLds K{sG, (kh).R, (kh).invStep};                            \
-    float4* rec = lds + kLdsG / sizeof(float4);
+    float* rec = reinterpret_cast<float*>(lds + kLdsG / sizeof(float4));
 
 // ---- Jacobi solver control (DFSPHImplementation.cu:455-461, 526-532) -------------------------------------------
 // which = 0: divergence (eta = (1/dt) tol 1e-4 rho0), 1: pressure (eta = tol 1e-4 rho0)
