@@ -253,7 +253,9 @@ int vfd_clear_boundaries(vfd_sim* sim, uint32_t scene);
 int vfd_get_boundary_map(vfd_sim* sim, uint32_t scene, uint32_t body, uint32_t field, float* nodes,
                          uint32_t* node_count);
 
-/* DFSPHImplementation::OnUpdate (DFSPHImplementation.cu:63-170): one step of every scene. */
+/* DFSPHImplementation::OnUpdate (DFSPHImplementation.cu:63-170): one step of every scene. Returns
+ * VFD_ERROR_NONFINITE (after completing the step; the state keeps its values) when a particle's new position or
+ * velocity is not finite; the reference has no such check and lets NaN propagate. */
 int vfd_step(vfd_sim* sim);
 /* Runs `steps` steps. */
 int vfd_advance(vfd_sim* sim, uint32_t steps);

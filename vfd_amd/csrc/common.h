@@ -91,7 +91,7 @@ struct SceneState {
     uint32_t S;  // SurfaceTensionSampleCount
     float maxVel;
     float frameTime;
-    uint32_t pad0;
+    uint32_t nonfinite;  // set by the step's last kernel when a particle's new position or velocity is not finite
     JacobiState div, pres;
     CgState cg;
 };

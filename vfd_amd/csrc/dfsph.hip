@@ -361,7 +361,11 @@ __global__ void __launch_bounds__(kGroupSize) k_pres_final(const SceneParams* __
         F.ap[c.i] = st4(a);
         const f3 v = ld3(F.vel[c.i]) + dt * a;
         F.vel[c.i] = st4(v);
-        if (writeNext) F.posNext[c.i] = st4(ld3(F.pos[c.i]) + dt * v);
+        const f3 x = ld3(F.pos[c.i]) + dt * v;
+        if (writeNext) F.posNext[c.i] = st4(x);
+        // per-step non-finite check (no reference counterpart): any lane of the scene may raise the flag
+        if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(x.x) && isfinite(x.y) && isfinite(x.z)))
+            F.state[c.s].nonfinite = 1u;
     });
 }
 
@@ -409,6 +413,7 @@ __global__ void __launch_bounds__(kTile) k_time_step_finish(const SceneParams* _
     m = block_max(m);
     if (threadIdx.x == 0) {
         SceneState& st = F.state[s];
+        st.nonfinite = 0u;  // raised again by k_pres_final if this step ends with a non-finite particle
         float mv = fmaxf(0.1f, m);
         st.maxVel = mv;
         if (mv < 1.0e-9f) mv = 1.0e-9f;

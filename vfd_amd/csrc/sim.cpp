@@ -1045,6 +1045,9 @@ public:
         solver_debug();
         VFD_TRY(neighbor_stats());
         VFD_TRY(resolve_profile());
+        int nonfiniteScene = -1;  // per-step non-finite check (no reference counterpart)
+        for (size_t k = 0; k < sc.size() && nonfiniteScene < 0; ++k)
+            if (sc[k]->count && sc[k]->st.nonfinite) nonfiniteScene = (int)k;
         if (tp) {
             tp->H = sc[0]->st.dt;
             tp->Kd = sc[0]->desc.EnableDivergenceSolverError ? sc[0]->st.div.it : 0;
@@ -1088,6 +1091,10 @@ public:
             }
             d.IterationCount++;
         }
+        if (nonfiniteScene >= 0)
+            return fail(VFD_ERROR_NONFINITE, "scene " + std::to_string(nonfiniteScene) +
+                                                 ": a particle's position or velocity is not finite after step " +
+                                                 std::to_string(sc[nonfiniteScene]->dbg.IterationCount));
         return VFD_OK;
     }
 
