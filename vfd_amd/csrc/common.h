@@ -6,7 +6,7 @@
 //   scalars: float per particle
 //   neighbour lists: uint32 slot indices, transposed list[j * capacity + i] so that the j-th neighbour of 64
 //   consecutive particles is one coalesced 256-byte read; at most kMaxNeighbors per particle; for the particles of a
-//   staged group also as uint16 halo slots in the same transposed layout;
+//   staged group also as uint16 halo slots and uint16 kernel-table codes of the pairs, in the same transposed layout;
 //   halos: per group, haloCount[g] and the halo's particle slots haloIds[g * kHaloMax + k].
 //
 // All HIP sources are compiled with -ffp-contract=off: every fused multiply-add is an explicit fmaf, so the

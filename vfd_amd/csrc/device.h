@@ -297,7 +297,7 @@ __device__ __forceinline__ f3 gradW(const KLds& k, f3 r) {
 // list entry. Both paths hand the same values to the same arithmetic in the same list order.
 struct NbrSrc {
     const float* rec;                      // LDS records of the halo (layout: HaloRec); nullptr: global gathers
-    const uint32_t* __restrict__ local;    // [j * cap + i] halo slot | gradient table code << 16 (staged groups)
+    const uint16_t* __restrict__ local;    // [j * cap + i] halo slots (staged groups)
     const uint32_t* __restrict__ list;     // [j * cap + i] particle slots
     const uint32_t* __restrict__ count;    // [i]
     uint32_t cap;
@@ -343,13 +343,13 @@ struct LoadXSV {  // lo = {x_t, s_t}, hi = {v_t, -}
 };
 
 // f(lo, hi, t) for every neighbour of particle i in list order, t = tab[code] and code = gcode(xi - x_j) (the
-// kernel-table entry of the pair). Staged groups read packed list entries (halo slot | code << 16, the code computed
-// once per step when the halo is built) in chunks of ListShape<R>::chunk entries: the chunk's list loads are issued together (rows
-// past the end repeat the last entry, so every load and LDS read stays in bounds), then all of its LDS record and
-// table reads, and only then its f calls, so that neither the list latency nor the LDS latency is paid once per
-// neighbour. With VFD_LIST_PREFETCH the next chunk's list loads are issued before the current chunk is consumed.
-// The global path fetches four list entries per trip so that their gathers can be in flight together, and computes
-// the code itself.
+// kernel-table entry of the pair). Staged groups read their list as uint16 halo slots (2 bytes per neighbour; the
+// code is computed from the record's position) in chunks of ListShape<R>::chunk entries: the chunk's
+// list loads are issued together (rows past the end repeat the last entry, so every load and LDS read stays in
+// bounds), then all of its LDS record and table reads, and only then its f calls, so that neither the list latency
+// nor the LDS latency is paid once per neighbour. With a lookahead the next chunks' list loads are issued before
+// the current chunk is consumed. The global path fetches four list entries per trip so that their gathers can be
+// in flight together, and computes the code itself.
 // Chunk length and lookahead are chosen per record count R: one-record loops (16 B per neighbour) profit from long
 // chunks; two-record loops hold twice the registers per entry and take short ones. VFD_LIST_PREFETCH(2) is the
 // number of further chunks whose list loads are in flight while a chunk is consumed (0: none), since the list
@@ -413,7 +413,7 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
         if (n == 0) return;
         const float* rec = src.rec;
         using HR = HaloRec<Load>;
-        const uint32_t* __restrict__ li = src.local + i;
+        const uint16_t* __restrict__ li = src.local + i;
         const uint32_t last = n - 1u;
         auto fetch = [&](uint32_t (&e)[C], uint32_t b) {
 #pragma unroll
@@ -424,10 +424,9 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
             TV tv[C];
 #pragma unroll
             for (uint32_t u = 0; u < C; ++u) {
-                const uint32_t slot = e[u] & 0xFFFFu;
-                lo[u] = HR::lo(rec, slot);
-                hi[u] = HR::H ? HR::hi(rec, slot) : lo[u];
-                tv[u] = tab[e[u] >> 16];
+                lo[u] = HR::lo(rec, e[u]);
+                hi[u] = HR::H ? HR::hi(rec, e[u]) : lo[u];
+                tv[u] = tab[gcode(cutR, invStep, xi - ld3(lo[u]))];
             }
             if (b + C <= n) {
 #pragma unroll

@@ -401,11 +401,10 @@ __device__ __forceinline__ uint32_t bucket_hash(uint32_t b) { return (b * 0x9E37
 
 __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __restrict__ P,
                                                            const uint32_t* __restrict__ tileScene, uint32_t tiles,
-                                                           float cutR, float invStep, const float4* __restrict__ pos,
                                                            const uint32_t* __restrict__ nbrCount,
                                                            const uint32_t* __restrict__ nbrList, uint32_t cap,
                                                            uint32_t limit, uint32_t* haloCount, uint32_t* haloIds,
-                                                           uint32_t* nbrLocal, uint32_t* haloStats) {
+                                                           uint16_t* nbrLocal, uint32_t* haloStats) {
     __shared__ uint32_t bkey[kBucketSlots];
     __shared__ uint32_t bmask[kBucketSlots];
     __shared__ uint16_t bbase[kBucketSlots];
@@ -500,10 +499,7 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
         atomicMax(&haloStats[2], m);
     }
     __syncthreads();
-    // packed entry: halo number | gradient table code of the pair << 16 (the code the neighbour loops would compute
-    // from x_i - x_j)
-    f3 xi = mk3(0.0f, 0.0f, 0.0f);
-    if (n) xi = ld3(pos[i]);
+    // staged entries: the halo number of each neighbour, in list order
     for (uint32_t j = 0; j < nr; ++j) {
         if (j >= n) continue;
         const size_t e = (size_t)j * cap + i;
@@ -511,18 +507,16 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
         const uint32_t b = t >> 5, below = (1u << (t & 31u)) - 1u;
         uint32_t h = bucket_hash(b);
         while (bkey[h] != b) h = (h + 1) & (kBucketSlots - 1);
-        const uint32_t code = gcode(cutR, invStep, xi - ld3(pos[t]));
-        nbrLocal[e] = (bbase[h] + __popc(bmask[h] & below)) | (code << 16);
+        nbrLocal[e] = (uint16_t)(bbase[h] + __popc(bmask[h] & below));
     }
 }
 
-void launch_build_halo(const Launch& L, const KTabHost& kt, const float4* pos, const uint32_t* nbrCount,
-                       const uint32_t* nbrList, uint32_t haloLimit, uint32_t* haloCount, uint32_t* haloIds,
-                       uint32_t* nbrLocal, uint32_t* haloStats) {
+void launch_build_halo(const Launch& L, const uint32_t* nbrCount, const uint32_t* nbrList, uint32_t haloLimit,
+                       uint32_t* haloCount, uint32_t* haloIds, uint16_t* nbrLocal, uint32_t* haloStats) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, L.stream, haloStats, 0u, 3u);
-    hipLaunchKernelGGL(k_build_halo, dim3(L.groups), dim3(kGroupSize), 0, L.stream, L.P, L.tileScene, L.tiles, kt.R,
-                       kt.invStep, pos, nbrCount, nbrList, L.capacity, haloLimit < kHaloMax ? haloLimit : kHaloMax,
-                       haloCount, haloIds, nbrLocal, haloStats);
+    hipLaunchKernelGGL(k_build_halo, dim3(L.groups), dim3(kGroupSize), 0, L.stream, L.P, L.tileScene, L.tiles,
+                       nbrCount, nbrList, L.capacity, haloLimit < kHaloMax ? haloLimit : kHaloMax, haloCount, haloIds,
+                       nbrLocal, haloStats);
 }
 
 void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, const uint32_t* cellStart,

@@ -337,7 +337,7 @@ public:
     DevBuf<float> partialsF;
     DevBuf<unsigned long long> nbStats;
     DevBuf<uint32_t> haloCount, haloIds, haloStats;
-    DevBuf<uint32_t> nbrLocal;
+    DevBuf<uint16_t> nbrLocal;
 
     // ---- adjoint recording (one scene): per step, everything the reverse needs, in that step's slot order -----
     struct DevTape {
@@ -946,11 +946,10 @@ public:
         if (last == VFD_PHASE_SEARCH) return finish_partial();
         Fields F = fields();
         VFD_TRY(timed("boundary", [&] { launch_boundary(L, dMaps.p, pos.p, vel[cur].p, xb.p, vb.p); }));
-        // halos and the gradient codes of the staged lists need the positions the solvers will use, i.e. after the
-        // boundary has pushed penetrating particles out (the lists themselves come from the positions before)
+        // halos are built after the boundary has pushed penetrating particles out (the lists themselves come from the
+        // positions before)
         VFD_TRY(timed("build_halo", [&] {
-            launch_build_halo(L, K, pos.p, nbrCount.p, nbrList.p, haloLimit, haloCount.p, haloIds.p, nbrLocal.p,
-                              haloStats.p);
+            launch_build_halo(L, nbrCount.p, nbrList.p, haloLimit, haloCount.p, haloIds.p, nbrLocal.p, haloStats.p);
         }));
         if (last == VFD_PHASE_BOUNDARY) return finish_partial();
         DevTape* tp = nullptr;
