@@ -835,8 +835,7 @@ __global__ void __launch_bounds__(kTile) k_st_classify(const SceneParams* __rest
     if (count > 0) {
         if (length2(nsum) > 0.0f) nsum = normalize(nsum);
         F.nMC[c.i] = st4(nsum);
-        F.kMC[c.i] = 1.0f / R * -2.0f * sqrtf(1.0f - p.NR * p.NR / (p.r * p.r)) *
-                     cosf(acosf(1.0f - 2.0f * ((float)count / (float)S)) + p.mcFactor);
+        F.kMC[c.i] = 1.0f / R * -2.0f * sqrtf(1.0f - p.NR * p.NR / (p.r * p.r)) * F.mcTab[F.mcOff[c.s] + count];
     } else {
         F.kMC[c.i] = 0.0f;
     }

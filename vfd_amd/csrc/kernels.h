@@ -148,6 +148,10 @@ struct Fields {
     uint32_t* live;        // [3] scenes still iterating in the divergence, pressure and viscosity solves
     SceneState* state;     // [sceneCount]
     const float* halton;   // [49152]
+    // per scene s, mcTab[mcOff[s] + count] = cos(acos(1 - 2 count / S) + MonteCarloFactor) for count = 0..S (the
+    // curvature factor of the surface-tension classification, evaluated on the host for this step's S)
+    const float* mcTab;
+    const uint32_t* mcOff;
 };
 
 void launch_density_factor(const Launch& L, const KTabHost& kt, const Fields& F);

@@ -321,6 +321,34 @@ public:
     KernelTable kt;
     DevBuf<float2> dTab;
     DevBuf<float> dHalton, dGlx, dGlw;
+    DevBuf<float> dMcTab;
+    DevBuf<uint32_t> dMcOff;
+    std::vector<float> mcTab;     // host copies of this step's tables (kept until the next step's upload)
+    std::vector<uint32_t> mcOff;
+    // cos(acos(1 - 2 count / S) + MonteCarloFactor) for count = 0..S of every surface-tension scene, in float with the
+    // host's libm, the expression of ComputeSurfaceTensionClassificationKernel (DFSPHKernels.cu:950-953); the
+    // classification kernel reads it instead of evaluating cos and acos on the device
+    int upload_mc_tables(Fields& F) {
+        mcOff.assign(sc.size(), 0u);
+        mcTab.clear();
+        for (size_t k = 0; k < sc.size(); ++k) {
+            const Scene& s = *sc[k];
+            mcOff[k] = (uint32_t)mcTab.size();
+            if (!s.desc.EnableSurfaceTensionSolver || s.count == 0) continue;
+            const uint32_t S = s.st.S;
+            for (uint32_t count = 0; count <= S; ++count)
+                mcTab.push_back(std::cos(std::acos(1.0f - 2.0f * ((float)count / (float)S)) + s.prm.mcFactor));
+        }
+        if (mcTab.empty()) mcTab.push_back(0.0f);
+        VFD_TRY(dMcTab.alloc(mcTab.size()));
+        VFD_TRY(dMcOff.alloc(mcOff.size()));
+        VFD_HIP(hipMemcpyAsync(dMcTab.p, mcTab.data(), mcTab.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        VFD_HIP(hipMemcpyAsync(dMcOff.p, mcOff.data(), mcOff.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               stream));
+        F.mcTab = dMcTab.p;
+        F.mcOff = dMcOff.p;
+        return VFD_OK;
+    }
 
     uint32_t tiles = 0, cap = 0;
     int cur = 0;
@@ -997,7 +1025,10 @@ public:
         if (tp) VFD_TRY(tape_copy(tp->v1, vel[cur].p, tp->n));
         VFD_HIP(hipEventRecord(ev[3], stream));
         if (last == VFD_PHASE_DIVERGENCE || last == VFD_PHASE_CLEAR_ACCELERATION) return finish_partial();
-        if (anyST) VFD_TRY(timed("surface_tension", [&] { launch_surface_tension(L, F); }));
+        if (anyST) {
+            VFD_TRY(upload_mc_tables(F));
+            VFD_TRY(timed("surface_tension", [&] { launch_surface_tension(L, F); }));
+        }
         VFD_HIP(hipEventRecord(ev[4], stream));
         if (last == VFD_PHASE_SURFACE_TENSION) return finish_partial();
         if (anyVisc) {
