@@ -282,7 +282,10 @@ void launch_div_accel(const Launch& L, const KTabHost& kt, const Fields& F) {
 void launch_div_iter(const Launch& L, const KTabHost& kt, const Fields& F) {
     hipLaunchKernelGGL(k_div_iter, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
-    hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F, 0);
+}
+// the residual reduction and loop condition after a Jacobi iteration; which = 0 divergence, 1 pressure
+void launch_jacobi_finish(const Launch& L, const Fields& F, int which) {
+    hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F, which);
 }
 void launch_div_final(const Launch& L, const KTabHost& kt, const Fields& F) {
     hipLaunchKernelGGL(k_div_final, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
@@ -383,7 +386,6 @@ void launch_pres_accel(const Launch& L, const KTabHost& kt, const Fields& F) {
 void launch_pres_iter(const Launch& L, const KTabHost& kt, const Fields& F) {
     hipLaunchKernelGGL(k_pres_iter, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
-    hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F, 1);
 }
 void launch_pres_final(const Launch& L, const KTabHost& kt, const Fields& F, int writeNext) {
     hipLaunchKernelGGL(k_pres_final, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
@@ -440,6 +442,8 @@ __global__ void __launch_bounds__(kTile) k_velocity(const SceneParams* __restric
 
 void launch_time_step(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_time_step, dim3(L.gridAll), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, TileGrid(L.tg));
+}
+void launch_time_step_finish(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_time_step_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_velocity(const Launch& L, const Fields& F) {
@@ -771,15 +775,21 @@ void launch_visc_precond(const Launch& L, const KTabHost& kt, const Fields& F) {
 void launch_cg_begin(const Launch& L, const KTabHost& kt, const Fields& F) {
     hipLaunchKernelGGL(k_cg_begin, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
+}
+void launch_cg_begin_finish(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_cg_begin_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_cg_matvec(const Launch& L, const KTabHost& kt, const Fields& F) {
     hipLaunchKernelGGL(k_cg_matvec, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg));
+}
+void launch_cg_alpha(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_cg_alpha, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_cg_update(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_cg_update, dim3(L.gridAll), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, L.capacity, TileGrid(L.tg));
+}
+void launch_cg_update_finish(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_cg_update_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_cg_direction(const Launch& L, const Fields& F) {

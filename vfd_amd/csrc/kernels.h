@@ -159,15 +159,20 @@ void launch_scene_init_div(const Launch& L, const Fields& F);
 void launch_div_init(const Launch& L, const KTabHost& kt, const Fields& F);
 void launch_div_accel(const Launch& L, const KTabHost& kt, const Fields& F);
 void launch_div_iter(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_jacobi_finish(const Launch& L, const Fields& F, int which);
 void launch_div_final(const Launch& L, const KTabHost& kt, const Fields& F);
 void launch_surface_tension(const Launch& L, const Fields& F);
 void launch_visc_precond(const Launch& L, const KTabHost& kt, const Fields& F);
 void launch_cg_begin(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_cg_begin_finish(const Launch& L, const Fields& F);
 void launch_cg_matvec(const Launch& L, const KTabHost& kt, const Fields& F);
+void launch_cg_alpha(const Launch& L, const Fields& F);
 void launch_cg_update(const Launch& L, const Fields& F);
+void launch_cg_update_finish(const Launch& L, const Fields& F);
 void launch_cg_direction(const Launch& L, const Fields& F);
 void launch_visc_apply(const Launch& L, const Fields& F);
 void launch_time_step(const Launch& L, const Fields& F);
+void launch_time_step_finish(const Launch& L, const Fields& F);
 void launch_velocity(const Launch& L, const Fields& F);
 void launch_scene_init_pres(const Launch& L, const Fields& F);
 void launch_pres_init(const Launch& L, const KTabHost& kt, const Fields& F);

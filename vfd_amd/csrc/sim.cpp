@@ -1017,6 +1017,7 @@ public:
                 (void)timed("div_accel", [&] { launch_div_accel(L, K, F); }, 0, k);
                 if (tp) (void)tape_slot(tp->av, k, ap.p, tp->n);
                 (void)timed("div_iter", [&] { launch_div_iter(L, K, F); }, 0, k);
+                (void)timed("jacobi_finish", [&] { launch_jacobi_finish(L, F, 0); }, 0, k);
                 if (tp) (void)tape_slot(tp->rv, k, residuum.p, tp->n);
             },
             minDiv, 0));
@@ -1034,10 +1035,13 @@ public:
         if (anyVisc) {
             VFD_TRY(timed("visc_precond", [&] { launch_visc_precond(L, K, F); }));
             VFD_TRY(timed("cg_begin", [&] { launch_cg_begin(L, K, F); }));
+            VFD_TRY(timed("cg_begin_finish", [&] { launch_cg_begin_finish(L, F); }));
             VFD_TRY(poll_loop(
                 [&](uint32_t k) {
                     (void)timed("cg_matvec", [&] { launch_cg_matvec(L, K, F); }, 2, k);
+                    (void)timed("cg_alpha", [&] { launch_cg_alpha(L, F); }, 2, k);
                     (void)timed("cg_update", [&] { launch_cg_update(L, F); }, 2, k);
+                    (void)timed("cg_update_finish", [&] { launch_cg_update_finish(L, F); }, 2, k);
                     (void)timed("cg_direction", [&] { launch_cg_direction(L, F); }, 2, k);
                 },
                 minVisc, 2));
@@ -1050,6 +1054,7 @@ public:
         VFD_HIP(hipEventRecord(ev[5], stream));
         if (last == VFD_PHASE_VISCOSITY) return finish_partial();
         VFD_TRY(timed("time_step", [&] { launch_time_step(L, F); }));
+        VFD_TRY(timed("time_step_finish", [&] { launch_time_step_finish(L, F); }));
         if (last == VFD_PHASE_TIME_STEP) return finish_partial();
         VFD_TRY(timed("velocity", [&] { launch_velocity(L, F); }));
         if (tp) VFD_TRY(tape_copy(tp->v2, vel[cur].p, tp->n));
@@ -1063,6 +1068,7 @@ public:
                 (void)timed("pres_accel", [&] { launch_pres_accel(L, K, F); }, 1, k);
                 if (tp) (void)tape_slot(tp->ap, k, ap.p, tp->n);
                 (void)timed("pres_iter", [&] { launch_pres_iter(L, K, F); }, 1, k);
+                (void)timed("jacobi_finish", [&] { launch_jacobi_finish(L, F, 1); }, 1, k);
                 if (tp) (void)tape_slot(tp->rp, k, residuum.p, tp->n);
             },
             minPres, 1));
