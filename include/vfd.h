@@ -17,6 +17,7 @@
 #ifndef VFD_H
 #define VFD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -216,6 +217,18 @@ int vfd_abi_version(void);
 const char* vfd_last_error(void);
 /* Fills the reference defaults of DFSPHSimulationDescription.h:11-49. */
 int vfd_desc_default(vfd_desc* desc);
+
+/* Scene-description I/O, as VFD's scene files store a simulation component: serialize(Archive&,
+ * DFSPHSimulationDescription&) and DFSPHSimulationComponent::save/load (Scene/Components/
+ * DFSPHSimulationComponent.h:7-39, 57-70) write the 26 fields under cereal names ("timeStepSize", ...,
+ * "gravity": {"x", "y", "z"}) inside a "description" object. vfd_read_descriptions parses a JSON text (e.g. a whole
+ * scene file such as Resources/Scenes/New/DFSPH/default.json) and fills out[k] for the k-th "description" object
+ * found, k < capacity, starting from the reference defaults for fields the object does not name; *count receives the
+ * number of objects found (out may be NULL to count). vfd_write_description writes {"description": {...}} in the
+ * same form (floats with 17 significant digits, so a read gives back the same values); buffer may be NULL to query
+ * *length (the terminating NUL needs one more byte). */
+int vfd_read_descriptions(const char* json, size_t length, vfd_desc* out, uint32_t capacity, uint32_t* count);
+int vfd_write_description(const vfd_desc* desc, char* buffer, size_t capacity, size_t* length);
 
 /* DFSPHSimulation(desc) (DFSPHSimulator.cpp:10-18) / DFSPHImplementation ctor (DFSPHImplementation.cu:14-21). */
 int vfd_create(const vfd_desc* desc, vfd_sim** out);

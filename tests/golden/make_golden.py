@@ -8,9 +8,13 @@ data that pin parts of the path are two numeric tables, read here as text (nothi
   - VFD/Source/Core/Math/GaussQuadrature.cpp: Gauss-Legendre abscissae/weights; row p = 30 (16 points) is the one
     the volume-map build uses (RigidBody.cu:71, GaussQuadrature.cpp:5621-5655).
 
+  - VFD/Resources/Scenes/New/DFSPH/default.json: the DFSPH example scene (cereal JSON of an entt snapshot), whose
+    simulation component stores a DFSPHSimulationDescription (DFSPHSimulationComponent.h:7-39).
+
 Outputs (small JSON fixtures, committed):
   halton_samples.json    every 97th point of the table plus the first and last 64 (index -> xyz)
   gauss_legendre_16.json the 16 abscissae and weights of row p = 30
+  scene_default.json     the example scene file, byte for byte (input of the scene-description reader)
 
 Run in a container where the reference is mounted:  python tests/golden/make_golden.py [/root/reference]
 """
@@ -60,13 +64,20 @@ def gauss_legendre(ref):
     return {"source": "VFD/Source/Core/Math/GaussQuadrature.cpp (p = 30)", "abscissae": x, "weights": w}
 
 
+def scene(ref):
+    with open(os.path.join(ref, "VFD/Resources/Scenes/New/DFSPH/default.json"), "rb") as f:
+        return f.read()
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    with open(os.path.join(HERE, "scene_default.json"), "wb") as f:
+        f.write(scene(ref))
     with open(os.path.join(HERE, "halton_samples.json"), "w") as f:
         json.dump(halton(ref), f, separators=(",", ":"))
     with open(os.path.join(HERE, "gauss_legendre_16.json"), "w") as f:
         json.dump(gauss_legendre(ref), f, indent=1)
-    print("wrote halton_samples.json, gauss_legendre_16.json")
+    print("wrote halton_samples.json, gauss_legendre_16.json, scene_default.json")
 
 
 if __name__ == "__main__":

@@ -8,10 +8,10 @@ does not load it; the first DFSPHSimulation does, and raises if the library is m
 from ._abi import (BoundaryDesc, DebugInfo, Desc, Info, VfdError, load_library, make_boundary, make_desc,
                    EXPORTED, FIELDS, PHASES)
 from .simulation import (DFSPHBatch, DFSPHParticleFrame, DFSPHSimulation, DFSPHSimulationDescription, FluidObject,
-                         RigidBodyDescription)
+                         RigidBodyDescription, read_descriptions, write_description)
 
 __all__ = [
     "BoundaryDesc", "DebugInfo", "Desc", "Info", "VfdError", "load_library", "make_boundary", "make_desc",
     "EXPORTED", "FIELDS", "PHASES", "DFSPHBatch", "DFSPHParticleFrame", "DFSPHSimulation",
-    "DFSPHSimulationDescription", "FluidObject", "RigidBodyDescription",
+    "DFSPHSimulationDescription", "FluidObject", "RigidBodyDescription", "read_descriptions", "write_description",
 ]

@@ -176,6 +176,10 @@ _PROTOS = {
     "vfd_abi_version": (ctypes.c_int, []),
     "vfd_last_error": (ctypes.c_char_p, []),
     "vfd_desc_default": (ctypes.c_int, [ctypes.POINTER(Desc)]),
+    "vfd_read_descriptions": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(Desc), _U32,
+                                             ctypes.POINTER(_U32)]),
+    "vfd_write_description": (ctypes.c_int, [ctypes.POINTER(Desc), ctypes.c_char_p, ctypes.c_size_t,
+                                             ctypes.POINTER(ctypes.c_size_t)]),
     "vfd_create": (ctypes.c_int, [ctypes.POINTER(Desc), ctypes.POINTER(_P)]),
     "vfd_batch_create": (ctypes.c_int, [ctypes.POINTER(Desc), _U32, ctypes.POINTER(_P)]),
     "vfd_destroy": (ctypes.c_int, [_P]),

@@ -45,6 +45,7 @@ static int fail(int code, const std::string& msg) {
     g_lastError = msg;
     return code;
 }
+int set_error(int code, const std::string& msg) { return fail(code, msg); }  // for the other host sources
 
 #define VFD_HIP(expr)                                                                                  \
     do {                                                                                               \

@@ -61,6 +61,35 @@ def DFSPHSimulationDescription(**fields):
     return _abi.make_desc(**fields)
 
 
+def read_descriptions(source):
+    """Every DFSPH description in a VFD scene file (cereal JSON, DFSPHSimulationComponent.h:7-39, 57-70): `source`
+    is a path or the JSON text. Fields a description does not name keep the reference defaults."""
+    import os
+    text = source
+    if isinstance(source, (str, os.PathLike)) and os.path.exists(str(source)):
+        with open(source, "rb") as f:
+            text = f.read()
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    lib = _abi.load_library()
+    n = _abi.ctypes.c_uint32()
+    check(lib.vfd_read_descriptions(data, len(data), None, 0, _abi.ctypes.byref(n)), "vfd_read_descriptions")
+    arr = (_abi.Desc * max(n.value, 1))()
+    check(lib.vfd_read_descriptions(data, len(data), arr, n.value, _abi.ctypes.byref(n)), "vfd_read_descriptions")
+    return [arr[k] for k in range(n.value)]
+
+
+def write_description(desc):
+    """The description as a scene file stores it: {"description": {...}} (cereal names)."""
+    lib = _abi.load_library()
+    size = _abi.ctypes.c_size_t()
+    check(lib.vfd_write_description(_abi.ctypes.byref(desc), None, 0, _abi.ctypes.byref(size)),
+          "vfd_write_description")
+    buf = _abi.ctypes.create_string_buffer(size.value + 1)
+    check(lib.vfd_write_description(_abi.ctypes.byref(desc), buf, size.value + 1, _abi.ctypes.byref(size)),
+          "vfd_write_description")
+    return buf.value.decode()
+
+
 class DFSPHParticleFrame:
     """DFSPHParticleFrame (ParticleBuffer/DFSPHParticleBuffer.h:13-19)."""
 
