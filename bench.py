@@ -244,6 +244,36 @@ def adjoint_ratio(reps=3):
             "dloss_dmu": float(mu.grad)}
 
 
+def frame_capture_cost(sim, snap, steps=10):
+    """Frame output on the headline workload (a frame is due every step there): the same `steps` steps from `snap`
+    with capture off and on. With capture on, each frame is packed in original order on the GPU and copied to pinned
+    host memory while the next steps run; `readback_ms_per_frame` is the host copy-out of a finished frame."""
+    d = sim.GetDescription()
+    d.FrameCount = sim.GetDebugInfo().FrameIndex + 4 * steps  # room for a frame in every measured step
+    sim.SetDescription(d)
+
+    def run(on):
+        sim.restore(snap)
+        sim.set_frame_capture(on)
+        d0 = sim.GetDebugInfo().FrameIndex
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.OnUpdate()
+        dt = time.perf_counter() - t0
+        return dt, sim.GetDebugInfo().FrameIndex - d0
+    off, _ = run(False)
+    on, due = run(True)
+    frames = sim.GetParticleFrameBuffer()  # waits for the copies, then copies every frame out
+    t0 = time.perf_counter()
+    frames = sim.GetParticleFrameBuffer()
+    back = time.perf_counter() - t0
+    sim.set_frame_capture(False)
+    n = sim.GetParticleCount()
+    return {"steps": steps, "frames": len(frames), "frames_due": int(due), "frame_bytes": 36 * n,
+            "ms_per_step_capture_off": round(1000 * off / steps, 3), "ms_per_step_capture_on": round(1000 * on / steps, 3),
+            "readback_ms_per_frame": round(1000 * back / max(len(frames), 1), 3)}
+
+
 def copy_probe():
     import torch
     n = COPY_PROBE_BYTES // 4
@@ -267,6 +297,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-gain measurement (config C4)")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement (config C5)")
+    ap.add_argument("--no-frames", action="store_true", help="skip the frame-output measurement")
+    ap.add_argument("--frame-steps", type=int, default=10)
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"))
     ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_DIR")
     args = ap.parse_args()
@@ -333,6 +365,11 @@ def main():
     check_its = (d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount)
     x1_gpu = sim.field("position")
 
+    fc = None
+    if not args.no_frames:
+        print("[bench] frame output: %d steps with capture off and on" % args.frame_steps, file=sys.stderr, flush=True)
+        fc = frame_capture_cost(sim, snap, args.frame_steps)
+
     nbar = float(np.mean(nb))
     nbody = len(bodies[0])
     kernels = {}
@@ -373,6 +410,7 @@ def main():
         "profiled_replay_identical": replay_identical,
         "gpu_kernel_ms_per_step": round(total_gpu_ms / args.steps, 3),
         "iterations_per_step": its,
+        "frame_output": fc,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline: oracle, up to %.0f s" % args.cpu_seconds, file=sys.stderr, flush=True)

@@ -277,7 +277,10 @@ int vfd_advance(vfd_sim* sim, uint32_t steps);
 int vfd_simulate(vfd_sim* sim);
 /* Testing aid: runs the current step up to and including `phase` (VFD_PHASE_*). */
 int vfd_step_until(vfd_sim* sim, int32_t phase);
-/* Enables/disables frame capture in vfd_step (default on, as in OnUpdate :148-167). */
+/* Enables/disables frame capture in vfd_step (default on, as in OnUpdate :148-167). A captured frame is packed in
+ * original order on the GPU and copied into pinned host memory on a copy stream of its own, overlapping the
+ * following steps (36 bytes per particle per frame stay pinned until vfd_simulate, vfd_set_fluid or vfd_destroy
+ * releases the frames to a pool the next frames reuse). */
 int vfd_set_frame_capture(vfd_sim* sim, int32_t enabled);
 
 /* Reads and writes of particle state, original order. vfd_get_particles = ConvertParticlesToBuffer + copy
@@ -299,7 +302,8 @@ int vfd_get_neighbors(vfd_sim* sim, uint32_t scene, uint32_t* counts, uint32_t* 
 int vfd_get_debug_info(vfd_sim* sim, uint32_t scene, vfd_debug_info* info);
 
 /* DFSPHParticleBuffer (ParticleBuffer/DFSPHParticleBuffer.h:13-45): number of captured frames and one frame's
- * {x, v, a} (36 bytes per particle, DFSPHParticleSimple) with its MaxVelocityMagnitude and CurrentTimeStep. */
+ * {x, v, a} (36 bytes per particle, DFSPHParticleSimple) with its MaxVelocityMagnitude and CurrentTimeStep.
+ * vfd_get_frame waits for that frame's copy to complete. */
 int vfd_get_frame_count(vfd_sim* sim, uint32_t scene, uint32_t* count);
 int vfd_get_frame(vfd_sim* sim, uint32_t scene, uint32_t frame, float* xva, float* max_velocity, float* time_step);
 

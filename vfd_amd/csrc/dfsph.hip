@@ -430,6 +430,28 @@ __global__ void __launch_bounds__(kTile) k_time_step_finish(const SceneParams* _
     }
 }
 
+// ConvertParticlesToBuffer (ParticleBuffer/DFSPHParticleBuffer.cu:6-22): {x, v, a} of the particles of one scene,
+// 9 floats each (DFSPHParticleSimple), written at their original index so that the frame is in input order
+__global__ void __launch_bounds__(kTile) k_frame_pack(uint32_t base, uint32_t count, const uint32_t* __restrict__ id,
+                                                      const float4* __restrict__ x, const float4* __restrict__ v,
+                                                      const float4* __restrict__ a, float* out) {
+    const uint32_t k = blockIdx.x * kTile + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t i = base + k;
+    const uint32_t j = id[i];
+    if (j >= count) return;  // ids of a scene are 0..count-1; anything else is not written
+    const float4 xi = x[i], vi = v[i], ai = a[i];
+    float* o = out + (size_t)9 * j;
+    o[0] = xi.x, o[1] = xi.y, o[2] = xi.z;
+    o[3] = vi.x, o[4] = vi.y, o[5] = vi.z;
+    o[6] = ai.x, o[7] = ai.y, o[8] = ai.z;
+}
+void launch_frame_pack(hipStream_t s, uint32_t base, uint32_t count, const uint32_t* id, const float4* x,
+                       const float4* v, const float4* a, float* out) {
+    if (count) hipLaunchKernelGGL(k_frame_pack, dim3((count + kTile - 1) / kTile), dim3(kTile), 0, s, base, count, id,
+                                  x, v, a, out);
+}
+
 // ComputeVelocityKernel (DFSPHKernels.cu:39-52)
 __global__ void __launch_bounds__(kTile) k_velocity(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, Fields F, TileGrid tg) {

@@ -174,6 +174,8 @@ void launch_visc_apply(const Launch& L, const Fields& F);
 void launch_time_step(const Launch& L, const Fields& F);
 void launch_time_step_finish(const Launch& L, const Fields& F);
 void launch_velocity(const Launch& L, const Fields& F);
+void launch_frame_pack(hipStream_t s, uint32_t base, uint32_t count, const uint32_t* id, const float4* x,
+                       const float4* v, const float4* a, float* out);
 void launch_scene_init_pres(const Launch& L, const Fields& F);
 void launch_pres_init(const Launch& L, const KTabHost& kt, const Fields& F);
 void launch_pres_accel(const Launch& L, const KTabHost& kt, const Fields& F);

@@ -220,9 +220,17 @@ static void make_gauss_legendre(std::vector<float>& xs, std::vector<float>& ws) 
 }
 
 // ---- scenes -------------------------------------------------------------------------------------------------------
+// One captured frame: {x, v, a} per particle in original order (DFSPHParticleSimple), 9 floats each. The data lands
+// in pinned host memory by an asynchronous copy that overlaps the following steps; `done` marks its completion.
+// When pinned memory cannot be had, the frame is copied synchronously into `pageable`.
 struct Frame {
-    std::vector<float> xva;  // original order, {x, v, a} per particle (DFSPHParticleSimple)
+    float* host = nullptr;  // pinned, owned by the simulation's pool
+    size_t cap = 0;         // floats in `host`
+    size_t n = 0;           // floats of this frame
+    hipEvent_t done = nullptr;
+    std::vector<float> pageable;
     float maxVel = 0, dt = 0;
+    const float* data() const { return host ? host : pageable.data(); }
 };
 
 struct Body {
@@ -408,6 +416,72 @@ public:
     bool useGraphs = VFD_DEFAULT_USE_GRAPHS != 0;
     hipGraphExec_t loopGraph[3][2] = {};
     hipStream_t ownStream = nullptr;  // default stream (capture needs a stream other than the legacy null stream)
+    // frame output: frames are packed in original order on the simulation's stream into one of two device staging
+    // buffers and copied to pinned host memory on copyStream, so that the copy overlaps the next steps
+    hipStream_t copyStream = nullptr;
+    DevBuf<float> frameStage[2];
+    hipEvent_t stageFree[2] = {}, stagePacked[2] = {};
+    bool stageUsed[2] = {false, false};
+    uint32_t stageNext = 0;
+    std::vector<std::pair<float*, size_t>> pinnedPool;  // pinned buffers of released frames, for reuse
+    std::vector<hipEvent_t> frameEvents;                // events of released frames, for reuse
+    int capture_frame(uint32_t s, Frame& fr) {
+        const Scene& S = *sc[s];
+        fr.n = (size_t)9 * S.count;
+        // pinned destination: the smallest pooled buffer that fits, else a new one
+        size_t best = pinnedPool.size();
+        for (size_t k = 0; k < pinnedPool.size(); ++k)
+            if (pinnedPool[k].second >= fr.n && (best == pinnedPool.size() || pinnedPool[k].second < pinnedPool[best].second))
+                best = k;
+        if (best < pinnedPool.size()) {
+            fr.host = pinnedPool[best].first;
+            fr.cap = pinnedPool[best].second;
+            pinnedPool.erase(pinnedPool.begin() + (long)best);
+        } else if (hipHostMalloc((void**)&fr.host, fr.n * sizeof(float), hipHostMallocDefault) == hipSuccess) {
+            fr.cap = fr.n;
+        } else {
+            (void)hipGetLastError();
+            fr.host = nullptr;
+        }
+        const uint32_t b = stageNext;
+        stageNext ^= 1u;
+        VFD_TRY(frameStage[b].alloc(fr.n));
+        if (stageUsed[b]) VFD_HIP(hipStreamWaitEvent(stream, stageFree[b], 0));  // its previous copy has finished
+        launch_frame_pack(stream, S.base, S.count, id[cur].p, posNext.p, vel[cur].p, acc.p, frameStage[b].p);
+        VFD_HIP(hipGetLastError());
+        if (!fr.host) {
+            fr.pageable.resize(fr.n);
+            VFD_HIP(hipMemcpyAsync(fr.pageable.data(), frameStage[b].p, fr.n * sizeof(float), hipMemcpyDeviceToHost,
+                                   stream));
+            VFD_HIP(hipStreamSynchronize(stream));
+            return VFD_OK;
+        }
+        if (frameEvents.empty()) {
+            hipEvent_t e;
+            VFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            frameEvents.push_back(e);
+        }
+        fr.done = frameEvents.back();
+        frameEvents.pop_back();
+        VFD_HIP(hipEventRecord(stagePacked[b], stream));
+        VFD_HIP(hipStreamWaitEvent(copyStream, stagePacked[b], 0));
+        VFD_HIP(hipMemcpyAsync(fr.host, frameStage[b].p, fr.n * sizeof(float), hipMemcpyDeviceToHost, copyStream));
+        VFD_HIP(hipEventRecord(fr.done, copyStream));
+        VFD_HIP(hipEventRecord(stageFree[b], copyStream));
+        stageUsed[b] = true;
+        return VFD_OK;
+    }
+    // frames of a scene go back to the pools once their copies have completed
+    void release_frames(Scene& S) {
+        for (Frame& f : S.frames) {
+            if (f.done) {
+                (void)hipEventSynchronize(f.done);
+                frameEvents.push_back(f.done);
+            }
+            if (f.host) pinnedPool.emplace_back(f.host, f.cap);
+        }
+        S.frames.clear();
+    }
     void drop_graphs() {
         for (auto& row : loopGraph)
             for (auto& g : row)
@@ -441,6 +515,15 @@ public:
         }
         if (hostLive) (void)hipHostFree(hostLive);
         drop_graphs();
+        if (copyStream) (void)hipStreamSynchronize(copyStream);
+        for (Scene* s : sc) release_frames(*s);
+        for (auto& pb : pinnedPool) (void)hipHostFree(pb.first);
+        for (auto& e : frameEvents) (void)hipEventDestroy(e);
+        for (int b = 0; b < 2; ++b) {
+            if (stageFree[b]) (void)hipEventDestroy(stageFree[b]);
+            if (stagePacked[b]) (void)hipEventDestroy(stagePacked[b]);
+        }
+        if (copyStream) (void)hipStreamDestroy(copyStream);
         if (ownStream) (void)hipStreamDestroy(ownStream);
         for (auto& e : evPool) (void)hipEventDestroy(e);
         for (Scene* s : sc) delete s;
@@ -523,6 +606,11 @@ public:
         VFD_HIP(hipHostMalloc((void**)&hostLive, 2 * sizeof(uint32_t), hipHostMallocDefault));
         VFD_HIP(hipStreamCreateWithFlags(&ownStream, hipStreamNonBlocking));
         stream = ownStream;
+        VFD_HIP(hipStreamCreateWithFlags(&copyStream, hipStreamNonBlocking));
+        for (int b = 0; b < 2; ++b) {
+            VFD_HIP(hipEventCreateWithFlags(&stageFree[b], hipEventDisableTiming));
+            VFD_HIP(hipEventCreateWithFlags(&stagePacked[b], hipEventDisableTiming));
+        }
         return VFD_OK;
     }
 
@@ -585,7 +673,7 @@ public:
         else
             S.v0.assign((size_t)3 * n, 0.0f);
         S.count = n;
-        S.frames.clear();
+        release_frames(S);
         S.dbg = vfd_debug_info{};
         committed = false;
         tapeCount = 0;  // recorded steps belong to the particles they were recorded with
@@ -762,7 +850,7 @@ public:
                 ids[s->base + i] = i;
             }
             s->dbg = vfd_debug_info{};
-            s->frames.clear();
+            release_frames(*s);
         }
         cur = 0;
         VFD_HIP(hipMemcpy(posNext.p, x.data(), c * sizeof(float4), hipMemcpyHostToDevice));
@@ -1120,8 +1208,9 @@ public:
                     Frame fr;
                     fr.maxVel = s->st.maxVel;
                     fr.dt = s->st.dt;
-                    VFD_TRY(frame_data((uint32_t)k, fr.xva));
-                    s->frames.push_back(std::move(fr));
+                    const int rc = capture_frame((uint32_t)k, fr);
+                    s->frames.push_back(std::move(fr));  // (owned by the scene even on failure, so it is released)
+                    VFD_TRY(rc);
                 }
                 d.FrameTime = 0.0f;
                 d.FrameIndex++;
@@ -1433,24 +1522,6 @@ public:
             }
         }
         return fail(VFD_ERROR_INVALID_ARGUMENT, "field is not writable");
-    }
-
-    int frame_data(uint32_t s, std::vector<float>& xva) {
-        const Scene& S = *sc[s];
-        std::vector<uint32_t> idv;
-        VFD_TRY(ids(s, idv));
-        std::vector<float4> x, v, a;
-        VFD_TRY(fetch(posNext.p, s, x));
-        VFD_TRY(fetch(vel[cur].p, s, v));
-        VFD_TRY(fetch(acc.p, s, a));
-        xva.assign((size_t)9 * S.count, 0.0f);
-        for (uint32_t k = 0; k < S.count; ++k) {
-            float* o = &xva[(size_t)9 * idv[k]];
-            o[0] = x[k].x, o[1] = x[k].y, o[2] = x[k].z;
-            o[3] = v[k].x, o[4] = v[k].y, o[5] = v[k].z;
-            o[6] = a[k].x, o[7] = a[k].y, o[8] = a[k].z;
-        }
-        return VFD_OK;
     }
 
     int get_neighbors(uint32_t s, uint32_t* counts, uint32_t* offsets, uint32_t* out, uint64_t* total) {
@@ -1765,7 +1836,8 @@ int vfd_get_frame(vfd_sim* sim, uint32_t scene, uint32_t frame, float* xva, floa
     CHECK_SCENE(sim, scene);
     const auto& fr = sim->impl.sc[scene]->frames;
     if (frame >= fr.size()) return fail(VFD_ERROR_INVALID_ARGUMENT, "frame index out of range");
-    if (xva) std::memcpy(xva, fr[frame].xva.data(), fr[frame].xva.size() * sizeof(float));
+    if (fr[frame].done) VFD_HIP(hipEventSynchronize(fr[frame].done));  // its asynchronous copy has landed
+    if (xva) std::memcpy(xva, fr[frame].data(), fr[frame].n * sizeof(float));
     if (max_velocity) *max_velocity = fr[frame].maxVel;
     if (time_step) *time_step = fr[frame].dt;
     return VFD_OK;
