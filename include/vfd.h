@@ -141,7 +141,8 @@ typedef struct vfd_debug_info {
 
 /* Boundary shapes. The reference builds its volume map from a triangle mesh (RigidBodyDescription,
  * VFD/Source/Simulation/DFSPH/RigidBody/RigidBody.cuh:12-21; mesh distance in Utility/SDF/MeshDistance.cpp); this
- * build takes the exact signed distance of an analytic shape instead. d(x) < 0 inside the shape. */
+ * build takes either a mesh (vfd_add_boundary_mesh) or the exact signed distance of an analytic shape
+ * (vfd_add_boundary). d(x) < 0 inside the shape. */
 enum {
     VFD_SHAPE_BOX = 0,      /* Center, HalfExtents */
     VFD_SHAPE_SPHERE = 1,   /* Center, Radius */
@@ -165,6 +166,32 @@ typedef struct vfd_boundary_desc {
     float DomainMin[3];
     float DomainMax[3];
 } vfd_boundary_desc;
+
+/* A triangle mesh as TriangleMesh::GetVertices / GetTriangles hold it (Renderer/Mesh/TriangleMesh.cpp:39-128):
+ * Vertices is float[3*VertexCount], Triangles is uint32[3*TriangleCount] of zero-based vertex indices, wound
+ * counter-clockwise seen from outside (the face normal cross(x1 - x0, x2 - x0) points out of the solid). */
+typedef struct vfd_mesh {
+    const float* Vertices;
+    uint32_t VertexCount;
+    const uint32_t* Triangles;
+    uint32_t TriangleCount;
+} vfd_mesh;
+
+/* RigidBodyDescription (RigidBody.cuh:15-24) without its mesh pointer: Transform is the column-major 4x4 matrix
+ * (glm::mat4, Transform[4*c + r] = row r of column c) applied to every vertex as Transform * (v, 1). */
+typedef struct vfd_mesh_boundary_desc {
+    float Transform[16];
+    int32_t Inverted;
+    float Padding;
+    uint32_t CollisionMapResolution[3];
+} vfd_mesh_boundary_desc;
+
+/* SampleMode (Utility/Sampler/ParticleSampler.h:7-12). */
+enum {
+    VFD_SAMPLE_MIN_DENSITY = 0,
+    VFD_SAMPLE_MEDIUM_DENSITY = 1,
+    VFD_SAMPLE_MAX_DENSITY = 2
+};
 
 /* Per-particle fields readable with vfd_get_field (original order). Components per particle in brackets. The names
  * are the DFSPHParticle members (VFD/Source/Simulation/DFSPH/Structures/DFSPHParticle.h:8-32) plus solver vectors. */
@@ -230,6 +257,16 @@ int vfd_desc_default(vfd_desc* desc);
 int vfd_read_descriptions(const char* json, size_t length, vfd_desc* out, uint32_t capacity, uint32_t* count);
 int vfd_write_description(const vfd_desc* desc, char* buffer, size_t capacity, size_t* length);
 
+/* FluidObject ctor (FluidObject.cpp:6-24) with ParticleSampler::SampleMeshVolume (ParticleSampler.cpp:7-91): the
+ * mesh's vertices are transformed by `transform` (column-major 4x4 as in vfd_mesh_boundary_desc; NULL = identity),
+ * a one-field signed-distance map of the mesh (SDF.cu:16-35: bounding box grown by 0.001 of its diagonal, negated
+ * when `inverted`) is built with `resolution` cells, and the lattice of `mode` (VFD_SAMPLE_*) over the bounding box
+ * is kept where the interpolated distance is negative, in the reference's z, y, x loop order. Runs on the current
+ * HIP device. out (float[3*capacity], may be NULL) receives the first min(capacity, *count) samples; *count receives
+ * the number of samples. */
+int vfd_sample_mesh_volume(const vfd_mesh* mesh, const float* transform, float radius, const uint32_t resolution[3],
+                           int32_t inverted, int32_t mode, float* out, uint32_t capacity, uint32_t* count);
+
 /* DFSPHSimulation(desc) (DFSPHSimulator.cpp:10-18) / DFSPHImplementation ctor (DFSPHImplementation.cu:14-21). */
 int vfd_create(const vfd_desc* desc, vfd_sim** out);
 /* Batch of scene_count independent scenes advanced by one set of launches. Every scene must use the same
@@ -260,6 +297,14 @@ int vfd_batch_set_fluid(vfd_sim* sim, uint32_t scene, const float* positions, co
  * map (signed distance, boundary volume) on the GPU. Call after vfd_set_fluid, as in the reference. */
 int vfd_add_boundary(vfd_sim* sim, const vfd_boundary_desc* body);
 int vfd_batch_add_boundary(vfd_sim* sim, uint32_t scene, const vfd_boundary_desc* body);
+/* RigidBody ctor for a mesh body (RigidBody.cu:10-73): the vertices are transformed, the map domain is their bounding
+ * box (which, as BoundingBox(vertices) starts from a zero box, always contains the origin) grown by 8*SupportRadius +
+ * (Padding - ParticleRadius), and field 0 is sign * (signed mesh distance - (Padding - ParticleRadius)) with the
+ * signed distance of MeshDistance::SignedDistance (MeshDistance.cpp:186-225: closest triangle, sign from the
+ * angle-weighted vertex, edge or face normal of the closest feature). Both fields are built on the GPU. */
+int vfd_add_boundary_mesh(vfd_sim* sim, const vfd_mesh_boundary_desc* body, const vfd_mesh* mesh);
+int vfd_batch_add_boundary_mesh(vfd_sim* sim, uint32_t scene, const vfd_mesh_boundary_desc* body,
+                                const vfd_mesh* mesh);
 int vfd_clear_boundaries(vfd_sim* sim, uint32_t scene);
 /* Number of map nodes and the node coefficients of one field (0 = distance, 1 = volume), node layout of
  * SDF::AddFunction (Utility/SDF/SDF.cu:45-139). nodes may be NULL to query the count. */

@@ -34,6 +34,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <string>
 #include <vector>
 #include <type_traits>
@@ -556,14 +557,189 @@ template <class T> struct VolumeMap {
     }
 };
 
+// ---------------------------------------------------------------------------------------------------------------
+// Triangle-mesh signed distance: MeshDistance (Utility/SDF/MeshDistance.cpp:13-608) with precomputed normals, over
+// the half-edge mesh of EdgeMesh(vertices, faces) (Renderer/Mesh/EdgeMesh.cpp:132-166), vertices transformed as
+// RigidBody.cu:18-22 / FluidObject.cpp:9-14 do. The reference finds the closest triangle through a bounding-sphere
+// hierarchy; the result is the least squared distance over all triangles, which this restatement takes by a scan in
+// triangle order (first minimum). Pinned by tests/test_oracle_mesh.py (cube and sphere meshes against the analytic
+// box / polyhedron distance, brute-force closest points, sign = inside/outside by ray parity).
+// ---------------------------------------------------------------------------------------------------------------
+template <class T> inline T gdot(V3<T> a, V3<T> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }  // glm::dot
+template <class T> inline V3<T> gnormalize(V3<T> a) { return a * (T(1) / std::sqrt(gdot(a, a))); }
+
+template <class T> struct TriMesh {
+    std::vector<V3<T>> x;                 // transformed vertices
+    std::vector<unsigned> f;              // 3 per triangle
+    std::vector<V3<T>> faceN, vertexN;    // MeshDistance.cpp:28-56
+    std::vector<long> opp;                // opposite half-edge (3 face + edge) or -1 at a border
+    V3<T> lo{0, 0, 0}, hi{0, 0, 0};       // BoundingBox(vertices) from the zero box (BoundingBox.h:16-31)
+
+    // glm::mat4 * vec4(v, 1) = (c0 x + c1 y) + (c2 z + c3 w), columns c of the column-major matrix
+    static V3<T> transform(const float* m, V3<T> v) {
+        T o[3];
+        for (int r = 0; r < 3; ++r)
+            o[r] = ((T)m[r] * v.x + (T)m[4 + r] * v.y) + ((T)m[8 + r] * v.z + (T)m[12 + r] * T(1));
+        return v3<T>(o[0], o[1], o[2]);
+    }
+    void build(const float* verts, unsigned nv, const unsigned* tris, unsigned nt, const float* m) {
+        x.resize(nv);
+        for (unsigned i = 0; i < nv; ++i) {
+            const V3<T> v = v3<T>((T)verts[3 * i], (T)verts[3 * i + 1], (T)verts[3 * i + 2]);
+            x[i] = m ? transform(m, v) : v;
+        }
+        f.assign(tris, tris + 3 * (size_t)nt);
+        lo = hi = v3<T>(0, 0, 0);
+        for (const V3<T>& v : x) {  // glm::min(a, b) = b < a ? b : a; glm::max(a, b) = a < b ? b : a
+            lo = v3<T>(v.x < lo.x ? v.x : lo.x, v.y < lo.y ? v.y : lo.y, v.z < lo.z ? v.z : lo.z);
+            hi = v3<T>(hi.x < v.x ? v.x : hi.x, hi.y < v.y ? v.y : hi.y, hi.z < v.z ? v.z : hi.z);
+        }
+        faceN.resize(nt);
+        vertexN.assign(nv, v3<T>(0, 0, 0));
+        for (unsigned t = 0; t < nt; ++t) {
+            const V3<T>& x0 = x[f[3 * t]];
+            const V3<T>& x1 = x[f[3 * t + 1]];
+            const V3<T>& x2 = x[f[3 * t + 2]];
+            const V3<T> n = gnormalize(cross(x1 - x0, x2 - x0));
+            const V3<T> e1 = gnormalize(x1 - x0), e2 = gnormalize(x2 - x1), e3 = gnormalize(x0 - x2);
+            const T ax = std::acos(gdot(e1, -e3)), ay = std::acos(gdot(e2, -e1)), az = std::acos(gdot(e3, -e2));
+            vertexN[f[3 * t]] = vertexN[f[3 * t]] + ax * n;
+            vertexN[f[3 * t + 1]] = vertexN[f[3 * t + 1]] + ay * n;
+            vertexN[f[3 * t + 2]] = vertexN[f[3 * t + 2]] + az * n;
+            faceN[t] = n;
+        }
+        // pairing of opposite half-edges: a half-edge (a -> b) meets a waiting (b -> a); earliest waiting first
+        opp.assign(3 * (size_t)nt, -1);
+        std::map<std::pair<unsigned, unsigned>, std::vector<long>> waiting;
+        for (unsigned t = 0; t < nt; ++t)
+            for (unsigned e = 0; e < 3; ++e) {
+                const unsigned a = f[3 * t + e], b = f[3 * t + (e + 1) % 3];
+                const long he = 3 * (long)t + e;
+                auto it = waiting.find({b, a});
+                if (it != waiting.end() && !it->second.empty()) {
+                    opp[he] = it->second.front();
+                    opp[it->second.front()] = he;
+                    it->second.erase(it->second.begin());
+                } else {
+                    waiting[{a, b}].push_back(he);
+                }
+            }
+    }
+    // PointTriangleDistanceSquared (MeshDistance.cpp:303-607): Eberly's region classification of the minimiser
+    // (s, t) of |x0 + s e0 + t e1 - p|^2, entity 0..2 = vertex A, B, C; 3..5 = edge D (x0x1), E (x1x2), F (x2x0);
+    // 6 = face G.
+    T point_triangle_d2(V3<T> point, unsigned t3, V3<T>* closest, int* entity) const {
+        const V3<T>& x0 = x[f[t3]];
+        const V3<T> diff = x0 - point;
+        const V3<T> edge0 = x[f[t3 + 1]] - x0;
+        const V3<T> edge1 = x[f[t3 + 2]] - x0;
+        const T a00 = gdot(edge0, edge0), a01 = gdot(edge0, edge1), a11 = gdot(edge1, edge1);
+        const T b0 = gdot(diff, edge0), b1 = gdot(diff, edge1), c = gdot(diff, diff);
+        const T det = std::fabs(a00 * a11 - a01 * a01);
+        T s = a01 * b1 - a11 * b0;
+        T t = a01 * b0 - a00 * b1;
+        T d2 = T(-1);
+        int ent = -1;
+        const T two = T(2);
+        auto quad = [&] { return s * (a00 * s + a01 * t + two * b0) + t * (a01 * s + a11 * t + two * b1) + c; };
+        auto vA = [&] { ent = 0; s = 0; t = 0; d2 = c; };
+        auto vB = [&] { ent = 1; s = 1; t = 0; d2 = a00 + two * b0 + c; };
+        auto vC = [&] { ent = 2; s = 0; t = 1; d2 = a11 + two * b1 + c; };
+        auto eD = [&] { ent = 3; t = 0; s = -b0 / a00; d2 = b0 * s + c; };
+        auto eF = [&] { ent = 5; s = 0; t = -b1 / a11; d2 = b1 * t + c; };
+        if (s + t <= det) {
+            if (s < 0) {
+                if (t < 0 && b0 < 0) {
+                    if (-b0 >= a00) vB(); else eD();
+                } else {  // region 4 with b0 >= 0, or region 3
+                    if (b1 >= 0) vA(); else if (-b1 >= a11) vC(); else eF();
+                }
+            } else if (t < 0) {  // region 5
+                if (b0 >= 0) vA(); else if (-b0 >= a00) vB(); else eD();
+            } else {  // region 0
+                ent = 6;
+                const T invDet = T(1) / det;
+                s *= invDet;
+                t *= invDet;
+                d2 = quad();
+            }
+        } else {
+            const T denom = a00 - two * a01 + a11;
+            if (s < 0) {  // region 2
+                const T tmp0 = a01 + b0, tmp1 = a11 + b1;
+                if (tmp1 > tmp0) {
+                    const T numer = tmp1 - tmp0;
+                    if (numer >= denom) vB();
+                    else { ent = 4; s = numer / denom; t = 1 - s; d2 = quad(); }
+                } else {
+                    if (tmp1 <= 0) vC(); else if (b1 >= 0) vA(); else eF();
+                }
+            } else if (t < 0) {  // region 6
+                const T tmp0 = a01 + b1, tmp1 = a00 + b0;
+                if (tmp1 > tmp0) {
+                    const T numer = tmp1 - tmp0;
+                    if (numer >= denom) vC();
+                    else { ent = 4; t = numer / denom; s = 1 - t; d2 = quad(); }
+                } else {
+                    if (tmp1 <= 0) vB(); else if (b0 >= 0) vA(); else eD();
+                }
+            } else {  // region 1
+                const T numer = a11 + b1 - a01 - b0;
+                if (numer <= 0) vC();
+                else if (numer >= denom) vB();
+                else { ent = 4; s = numer / denom; t = 1 - s; d2 = quad(); }
+            }
+        }
+        if (d2 < 0) d2 = 0;
+        if (closest) *closest = x0 + s * edge0 + t * edge1;
+        if (entity) *entity = ent;
+        return d2;
+    }
+    // SignedDistance (MeshDistance.cpp:186-225): distance to the closest triangle, negative when (p - closest point)
+    // points against the closest entity's normal (vertex: angle-weighted; edge: sum of the two face normals, or the
+    // face's alone at a border; face: its normal).
+    T signed_distance(V3<T> p) const {
+        const unsigned nt = (unsigned)(f.size() / 3);
+        T best = std::numeric_limits<T>::infinity();
+        unsigned face = 0;
+        for (unsigned t = 0; t < nt; ++t) {
+            const T d2 = point_triangle_d2(p, 3 * t, nullptr, nullptr);
+            if (d2 < best) {
+                best = d2;
+                face = t;
+            }
+        }
+        V3<T> cp;
+        int ent;
+        T dist = std::sqrt(point_triangle_d2(p, 3 * face, &cp, &ent));
+        V3<T> n;
+        if (ent <= 2)
+            n = vertexN[f[3 * face + ent]];
+        else if (ent <= 5) {
+            const long o = opp[3 * (size_t)face + (ent - 3)];
+            n = o < 0 ? faceN[face] : faceN[face] + faceN[(size_t)o / 3];
+        } else
+            n = faceN[face];
+        if (gdot(p - cp, n) < T(0)) dist *= T(-1);
+        return dist;
+    }
+};
+
 // Builds both fields of a body's volume map (RigidBody.cu:10-73).
+// A mesh body (mesh != nullptr) takes its domain from the transformed vertices' bounding box and its distance from
+// TriMesh::signed_distance; b then supplies Inverted, Padding and CollisionMapResolution only.
 template <class T>
-void build_volume_map(VolumeMap<T>& map, const vfd_boundary_desc& b, const CubicKernel<T>& kernel, T particleRadius) {
+void build_volume_map(VolumeMap<T>& map, const vfd_boundary_desc& b, const CubicKernel<T>& kernel, T particleRadius,
+                      const TriMesh<T>* mesh = nullptr) {
     const T R = kernel.R;
     const T tol = (T)b.Padding - particleRadius;
     const T sign = b.Inverted ? T(-1) : T(1);
     Box<T> dom;
-    if (b.HasDomain) {
+    if (mesh) {
+        const T grow = T(8) * R + tol;  // RigidBody.cu:34-36
+        dom.min = mesh->lo - v3<T>(grow, grow, grow);
+        dom.max = mesh->hi + v3<T>(grow, grow, grow);
+    } else if (b.HasDomain) {
         dom.min = v3<T>((T)b.DomainMin[0], (T)b.DomainMin[1], (T)b.DomainMin[2]);
         dom.max = v3<T>((T)b.DomainMax[0], (T)b.DomainMax[1], (T)b.DomainMax[2]);
     } else {
@@ -588,7 +764,7 @@ void build_volume_map(VolumeMap<T>& map, const vfd_boundary_desc& b, const Cubic
 #pragma omp parallel for schedule(static)
     for (long l = 0; l < (long)map.nodeCount; ++l) {
         const V3<T> x = map.node_position((unsigned)l);
-        map.nodes[0][l] = sign * (shape_distance<T>(b, x) - tol);
+        map.nodes[0][l] = sign * ((mesh ? mesh->signed_distance(x) : shape_distance<T>(b, x)) - tol);
     }
     // Gauss-Legendre 16 points on [-R, R]^3; nodes/weights stored in T like the reference's float table.
     std::vector<double> gx, gw;
@@ -766,6 +942,71 @@ template <class T> struct Info {  // Structures/DFSPHSimulationInfo.h:5-44
     V3<T> g;
 };
 
+// ParticleSampler::SampleMeshVolume (Utility/Sampler/ParticleSampler.cpp:7-91): a one-field map of the mesh's signed
+// distance over its bounding box grown by 0.001 of the diagonal (SDF.cu:16-35; the upper corner first, then the lower
+// one with the grown box's diagonal), and the lattice of the sample mode kept where the interpolated distance is
+// negative, in the reference's z, y, x loop order with its float loop variables. Pinned by tests/test_oracle_mesh.py
+// (sample counts of a box mesh against the closed-form lattice count).
+template <class T>
+void sample_mesh_volume(const TriMesh<T>& mesh, T radius, const unsigned res[3], bool inverted, int mode,
+                        std::vector<T>& out) {
+    const Box<T> bounds{mesh.lo, mesh.hi};
+    Box<T> dom = bounds;
+    const T g = T(0.001f);
+    V3<T> diag = dom.max - dom.min;
+    const T up = g * std::sqrt(gdot(diag, diag));
+    dom.max = dom.max + v3<T>(up, up, up);
+    diag = dom.max - dom.min;
+    const T down = g * std::sqrt(gdot(diag, diag));
+    dom.min = dom.min - v3<T>(down, down, down);
+    VolumeMap<T> sdf;
+    sdf.init(dom, res);
+    sdf.nodes[0].assign(sdf.nodeCount, T(0));
+    const T factor = inverted ? T(-1) : T(1);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long l = 0; l < (long)sdf.nodeCount; ++l) sdf.nodes[0][l] = factor * mesh.signed_distance(sdf.node_position((unsigned)l));
+    const T diameter = T(2) * radius;
+    T shiftX = diameter, shiftY = diameter;
+    if (mode == VFD_SAMPLE_MEDIUM_DENSITY) {
+        shiftY = std::sqrt(T(3)) * radius;
+    } else if (mode == VFD_SAMPLE_MAX_DENSITY) {
+        shiftX = std::sqrt(T(3)) * radius;
+        shiftY = std::sqrt(T(6)) * diameter / T(3);
+    }
+    out.clear();
+    unsigned counterX = 0, counterY = 0;
+    for (T z = bounds.min.z; z <= bounds.max.z; z += diameter) {
+        for (T y = bounds.min.y; y <= bounds.max.y; y += shiftY) {
+            for (T x = bounds.min.x; x <= bounds.max.x; x += shiftX) {
+                V3<T> p;
+                if (mode == VFD_SAMPLE_MIN_DENSITY) {
+                    p = v3<T>(x + radius, y + radius, z + radius);
+                } else if (mode == VFD_SAMPLE_MEDIUM_DENSITY) {
+                    p = counterY % 2 == 0 ? v3<T>(x, y + radius, z + radius) : v3<T>(x + radius, y + radius, z);
+                } else {
+                    p = v3<T>(x, y + radius, z + radius);
+                    V3<T> shift = v3<T>(0, 0, 0);
+                    if (counterX % 2) shift.z += diameter / (T(2) * T(counterY % 2 ? -1 : 1));
+                    if (counterY % 2) {
+                        shift.x += shiftX / T(2);
+                        shift.z += diameter / T(2);
+                    }
+                    p = p + shift;
+                }
+                if (sdf.interp_at(0, p) - T(0) < T(0)) {  // SDF::GetDistance(p, 0) (SDF.cu:217-225)
+                    out.push_back(p.x);
+                    out.push_back(p.y);
+                    out.push_back(p.z);
+                }
+                ++counterX;
+            }
+            counterX = 0;
+            ++counterY;
+        }
+        counterY = 0;
+    }
+}
+
 template <class T> struct Body {
     vfd_boundary_desc desc;
     VolumeMap<T> map;
@@ -775,6 +1016,7 @@ template <class T> struct Body {
 
 template <class T> class Sim {
 public:
+    using Real = T;
     vfd_desc desc{};
     Info<T> info{};
     CubicKernel<T> kernel;
@@ -875,11 +1117,11 @@ public:
         vb_b.assign(m, v3<T>(0, 0, 0));
         vb_g = vb_r = vb_p = vb_z = vb_tmp = vb_ap = vb_b;
     }
-    void add_boundary(const vfd_boundary_desc& d) {
+    void add_boundary(const vfd_boundary_desc& d, const TriMesh<T>* mesh = nullptr) {
         bodies.emplace_back();
         Body<T>& b = bodies.back();
         b.desc = d;
-        build_volume_map(b.map, d, kernel, info.r);
+        build_volume_map(b.map, d, kernel, info.r, mesh);
         b.xb.assign(info.particleCount, v3<T>(0, 0, 0));
         b.vb.assign(info.particleCount, T(0));
         info.bodyCount = (unsigned)bodies.size();
@@ -1917,6 +2159,53 @@ int orc_set_fluid(void* h, const float* x, const float* v, uint32_t n) {
     return VFD_OK;
 }
 int orc_add_boundary(void* h, const vfd_boundary_desc* b) { DISPATCH(h, s->add_boundary(*b)); return VFD_OK; }
+// RigidBody ctor for a mesh body (RigidBody.cu:10-73); the mesh is the caller's (indices checked by the caller)
+int orc_add_boundary_mesh(void* h, const vfd_mesh_boundary_desc* m, const float* verts, uint32_t nv,
+                          const uint32_t* tris, uint32_t nt) {
+    vfd_boundary_desc d{};
+    d.Shape = -1;
+    d.Inverted = m->Inverted;
+    d.Padding = m->Padding;
+    for (int a = 0; a < 3; ++a) d.CollisionMapResolution[a] = m->CollisionMapResolution[a];
+    DISPATCH(h, {
+        using T = typename std::remove_reference<decltype(*s)>::type::Real;
+        TriMesh<T> mesh;
+        mesh.build(verts, nv, tris, nt, m->Transform);
+        s->add_boundary(d, &mesh);
+    });
+    return VFD_OK;
+}
+// FluidObject ctor + SampleMeshVolume (FluidObject.cpp:6-24): transform (column-major, NULL = identity), sample.
+int orc_sample_mesh_volume(int fp64, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                           const float* transform, float radius, const uint32_t* res, int inverted, int mode,
+                           double* out, uint32_t capacity, uint32_t* count) {
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        TriMesh<T> mesh;
+        mesh.build(verts, nv, tris, nt, transform);
+        std::vector<T> pts;
+        const unsigned r[3] = {res[0], res[1], res[2]};
+        sample_mesh_volume<T>(mesh, (T)radius, r, inverted != 0, mode, pts);
+        *count = (uint32_t)(pts.size() / 3);
+        for (size_t k = 0; out && k < std::min<size_t>(pts.size(), 3 * (size_t)capacity); ++k) out[k] = (double)pts[k];
+    };
+    if (fp64) run(double()); else run(float());
+    return VFD_OK;
+}
+// MeshDistance::SignedDistance of the transformed mesh at n points (float points for the FP32 mirror)
+int orc_mesh_signed_distance(int fp64, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                             const float* transform, const double* pts, uint32_t n, double* out) {
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        TriMesh<T> mesh;
+        mesh.build(verts, nv, tris, nt, transform);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long i = 0; i < (long)n; ++i)
+            out[i] = (double)mesh.signed_distance(v3<T>((T)pts[3 * i], (T)pts[3 * i + 1], (T)pts[3 * i + 2]));
+    };
+    if (fp64) run(double()); else run(float());
+    return VFD_OK;
+}
 int orc_step(void* h) { DISPATCH(h, s->step()); return VFD_OK; }
 int orc_step_until(void* h, int phase) { DISPATCH(h, s->step(phase)); return VFD_OK; }
 int orc_set_time_step(void* h, double dt) {

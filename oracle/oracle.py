@@ -55,6 +55,11 @@ def lib():
             "orc_viscosity_system": [P, P, P],
             "orc_set_smooth_kernel": [P, ctypes.c_int], "orc_tape": [P, ctypes.c_int],
             "orc_adjoint": [P, P, P, P, P, P, P],
+            "orc_add_boundary_mesh": [P, P, P, ctypes.c_uint32, P, ctypes.c_uint32],
+            "orc_sample_mesh_volume": [ctypes.c_int, P, ctypes.c_uint32, P, ctypes.c_uint32, P, ctypes.c_float, P,
+                                       ctypes.c_int, ctypes.c_int, P, ctypes.c_uint32, P],
+            "orc_mesh_signed_distance": [ctypes.c_int, P, ctypes.c_uint32, P, ctypes.c_uint32, P, P,
+                                         ctypes.c_uint32, P],
         }.items():
             fn = getattr(L, name)
             fn.restype = ctypes.c_int
@@ -122,6 +127,12 @@ class OracleSim:
 
     def add_boundary(self, b):
         self._L.orc_add_boundary(self._h, ctypes.byref(b))
+
+    def add_boundary_mesh(self, desc, vertices, triangles):
+        """A mesh body: desc is a vfd_mesh_boundary_desc (ctypes), vertices (n, 3) float32, triangles (m, 3)."""
+        v = np.ascontiguousarray(vertices, np.float32)
+        t = np.ascontiguousarray(triangles, np.uint32)
+        self._L.orc_add_boundary_mesh(self._h, ctypes.byref(desc), _p(v), len(v), _p(t), len(t))
 
     def step(self):
         self._L.orc_step(self._h)
@@ -289,3 +300,34 @@ def search_grid(x, R):
     u = np.zeros(7, np.uint32)
     lib().orc_search_grid(_p(x), len(x), float(R), _p(f), _p(u))
     return dict(gridMin=f[:3], gridDelta=f[3:], dims=u[:3], meta=u[3:6], cellCount=int(u[6]))
+
+
+def _mesh_args(vertices, triangles, transform):
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    if len(t) == 0 or int(t.max()) >= len(v):
+        raise ValueError("bad mesh")
+    tr = None if transform is None else np.ascontiguousarray(np.asarray(transform, np.float32).reshape(4, 4).T).ravel()
+    return v, t, tr
+
+
+def sample_mesh_volume(vertices, triangles, radius, resolution, inverted=False, mode=0, transform=None, fp64=False):
+    """ParticleSampler::SampleMeshVolume of the transformed mesh (transform: 4x4 acting on (x, y, z, 1))."""
+    v, t, tr = _mesh_args(vertices, triangles, transform)
+    res = np.ascontiguousarray(resolution, np.uint32)
+    n = ctypes.c_uint32()
+    lib().orc_sample_mesh_volume(1 if fp64 else 0, _p(v), len(v), _p(t), len(t), _p(tr), float(radius), _p(res),
+                                 int(bool(inverted)), int(mode), None, 0, ctypes.byref(n))
+    out = np.zeros((n.value, 3))
+    lib().orc_sample_mesh_volume(1 if fp64 else 0, _p(v), len(v), _p(t), len(t), _p(tr), float(radius), _p(res),
+                                 int(bool(inverted)), int(mode), _p(out), n.value, ctypes.byref(n))
+    return out
+
+
+def mesh_signed_distance(vertices, triangles, points, transform=None, fp64=True):
+    """MeshDistance::SignedDistance at each point (negative inside)."""
+    v, t, tr = _mesh_args(vertices, triangles, transform)
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    out = np.zeros(len(p))
+    lib().orc_mesh_signed_distance(1 if fp64 else 0, _p(v), len(v), _p(t), len(t), _p(tr), _p(p), len(p), _p(out))
+    return out

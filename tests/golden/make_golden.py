@@ -11,10 +11,14 @@ data that pin parts of the path are two numeric tables, read here as text (nothi
   - VFD/Resources/Scenes/New/DFSPH/default.json: the DFSPH example scene (cereal JSON of an entt snapshot), whose
     simulation component stores a DFSPHSimulationDescription (DFSPHSimulationComponent.h:7-39).
 
-Outputs (small JSON fixtures, committed):
+  - VFD/Resources/Models/{Cube,Sphere,Cone,Torus}.obj: triangle meshes (Wavefront OBJ) the example scenes use as
+    rigid bodies and fluid objects; Cube.obj and Cone.obj are the two meshes of the DFSPH example scene.
+
+Outputs (small fixtures, committed):
   halton_samples.json    every 97th point of the table plus the first and last 64 (index -> xyz)
   gauss_legendre_16.json the 16 abscissae and weights of row p = 30
   scene_default.json     the example scene file, byte for byte (input of the scene-description reader)
+  models/*.obj           the four meshes, byte for byte (inputs of the mesh distance and sampler tests)
 
 Run in a container where the reference is mounted:  python tests/golden/make_golden.py [/root/reference]
 """
@@ -69,15 +73,24 @@ def scene(ref):
         return f.read()
 
 
+MODELS = ("Cube.obj", "Sphere.obj", "Cone.obj", "Torus.obj")
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     with open(os.path.join(HERE, "scene_default.json"), "wb") as f:
         f.write(scene(ref))
+    os.makedirs(os.path.join(HERE, "models"), exist_ok=True)
+    for name in MODELS:
+        with open(os.path.join(ref, "VFD/Resources/Models", name), "rb") as f:
+            data = f.read()
+        with open(os.path.join(HERE, "models", name), "wb") as f:
+            f.write(data)
     with open(os.path.join(HERE, "halton_samples.json"), "w") as f:
         json.dump(halton(ref), f, separators=(",", ":"))
     with open(os.path.join(HERE, "gauss_legendre_16.json"), "w") as f:
         json.dump(gauss_legendre(ref), f, indent=1)
-    print("wrote halton_samples.json, gauss_legendre_16.json, scene_default.json")
+    print("wrote halton_samples.json, gauss_legendre_16.json, scene_default.json, models/")
 
 
 if __name__ == "__main__":

@@ -54,5 +54,8 @@ def make_oracle(descs, fluids, bodies, fp64=False, scene=0):
     o = OracleSim(descs[scene], fp64=fp64)
     o.set_fluid(*fluids[scene])
     for b in bodies[scene]:
-        o.add_boundary(b.to_abi())
+        if getattr(b, "is_mesh", False):
+            o.add_boundary_mesh(b.to_abi(), b.mesh.vertices, b.mesh.triangles)
+        else:
+            o.add_boundary(b.to_abi())
     return o

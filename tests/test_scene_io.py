@@ -105,3 +105,39 @@ def test_missing_fields_keep_the_defaults_and_unknown_ones_are_skipped():
 def test_malformed_text_is_an_invalid_argument(text):
     with pytest.raises(VfdError, match="INVALID_ARGUMENT"):
         read_descriptions(text)
+
+
+def test_scene_file_entities_become_mesh_inputs():
+    """The example scene's entities (Scene.cpp:10-40 snapshot order; editor bake, ComponentPanel.cpp:608-656): a
+    Cone.obj fluid object (medium density, 20^3 SDF, flipped and lifted by 3) and a Cube.obj rigid body scaled
+    (2, 0.2, 2)."""
+    from vfd_amd import scenes
+    d, fluids, bodies = scenes.load_scene_file(SCENE, os.path.join(HERE, "golden", "models"))
+    assert np.float32(d.ParticleRadius) == np.float32(0.025)
+    assert len(fluids) == 1 and len(bodies) == 1
+    fo, rb = fluids[0], bodies[0]
+    assert fo.mesh.triangles.shape == (62, 3) and fo.sample_mode == 1 and fo.resolution == (20, 20, 20)
+    assert fo.transform[1, 3] == np.float32(3.0) and fo.transform[1, 1] == np.float32(-1.0)
+    assert rb.is_mesh and rb.mesh.triangles.shape == (12, 3) and rb.resolution == (20, 20, 20)
+    assert np.array_equal(np.diag(rb.transform), np.array([2.0, 0.2, 2.0, 1.0], np.float32))
+    assert not rb.inverted and rb.padding == 0.0
+
+
+@pytest.mark.parametrize("case", ["index", "empty", "mode", "radius"])
+def test_mesh_sampler_rejects_bad_input(case):
+    """Argument checks of vfd_sample_mesh_volume run before any GPU work."""
+    from vfd_amd.mesh import TriangleMesh
+    from vfd_amd.simulation import sample_mesh_volume
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    t = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]], np.uint32)
+    mesh, radius, mode = TriangleMesh(v, t), 0.1, 0
+    if case == "index":
+        mesh.triangles = np.array([[0, 2, 7]], np.uint32)
+    elif case == "empty":
+        mesh.triangles = np.zeros((0, 3), np.uint32)
+    elif case == "mode":
+        mode = 3
+    else:
+        radius = 0.0
+    with pytest.raises(VfdError, match="INVALID_ARGUMENT"):
+        sample_mesh_volume(mesh, radius, (8, 8, 8), False, mode)

@@ -7,11 +7,14 @@ does not load it; the first DFSPHSimulation does, and raises if the library is m
 """
 from ._abi import (BoundaryDesc, DebugInfo, Desc, Info, VfdError, load_library, make_boundary, make_desc,
                    EXPORTED, FIELDS, PHASES)
+from .mesh import TriangleMesh
 from .simulation import (DFSPHBatch, DFSPHParticleFrame, DFSPHSimulation, DFSPHSimulationDescription, FluidObject,
-                         RigidBodyDescription, read_descriptions, write_description)
+                         FluidObjectDescription, RigidBodyDescription, read_descriptions, sample_mesh_volume,
+                         write_description)
 
 __all__ = [
     "BoundaryDesc", "DebugInfo", "Desc", "Info", "VfdError", "load_library", "make_boundary", "make_desc",
     "EXPORTED", "FIELDS", "PHASES", "DFSPHBatch", "DFSPHParticleFrame", "DFSPHSimulation",
-    "DFSPHSimulationDescription", "FluidObject", "RigidBodyDescription", "read_descriptions", "write_description",
+    "DFSPHSimulationDescription", "FluidObject", "FluidObjectDescription", "RigidBodyDescription", "TriangleMesh",
+    "read_descriptions", "sample_mesh_volume", "write_description",
 ]

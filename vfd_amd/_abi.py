@@ -138,6 +138,21 @@ class BoundaryDesc(ctypes.Structure):
     ]
 
 
+class Mesh(ctypes.Structure):
+    """vfd_mesh: TriangleMesh::GetVertices / GetTriangles (Renderer/Mesh/TriangleMesh.cpp:39-128)."""
+    _fields_ = [("Vertices", ctypes.c_void_p), ("VertexCount", ctypes.c_uint32), ("Triangles", ctypes.c_void_p),
+                ("TriangleCount", ctypes.c_uint32)]
+
+
+class MeshBoundaryDesc(ctypes.Structure):
+    """vfd_mesh_boundary_desc == RigidBodyDescription (RigidBody/RigidBody.cuh:15-24) without its mesh pointer."""
+    _fields_ = [("Transform", ctypes.c_float * 16), ("Inverted", ctypes.c_int32), ("Padding", ctypes.c_float),
+                ("CollisionMapResolution", c_uint3)]
+
+
+SAMPLE_MIN_DENSITY, SAMPLE_MEDIUM_DENSITY, SAMPLE_MAX_DENSITY = 0, 1, 2
+
+
 def make_boundary(shape, center=(0, 0, 0), half_extents=(0, 0, 0), radius=0.0, normal=(0, 1, 0), inverted=False,
                   padding=0.0, resolution=(20, 20, 20), domain=None):
     b = BoundaryDesc()
@@ -194,6 +209,11 @@ _PROTOS = {
     "vfd_batch_set_fluid": (ctypes.c_int, [_P, _U32, _P, _P, _U32]),
     "vfd_add_boundary": (ctypes.c_int, [_P, ctypes.POINTER(BoundaryDesc)]),
     "vfd_batch_add_boundary": (ctypes.c_int, [_P, _U32, ctypes.POINTER(BoundaryDesc)]),
+    "vfd_add_boundary_mesh": (ctypes.c_int, [_P, ctypes.POINTER(MeshBoundaryDesc), ctypes.POINTER(Mesh)]),
+    "vfd_batch_add_boundary_mesh": (ctypes.c_int, [_P, _U32, ctypes.POINTER(MeshBoundaryDesc),
+                                                   ctypes.POINTER(Mesh)]),
+    "vfd_sample_mesh_volume": (ctypes.c_int, [ctypes.POINTER(Mesh), _P, ctypes.c_float, _P, ctypes.c_int32,
+                                              ctypes.c_int32, _P, _U32, ctypes.POINTER(_U32)]),
     "vfd_clear_boundaries": (ctypes.c_int, [_P, _U32]),
     "vfd_get_boundary_map": (ctypes.c_int, [_P, _U32, _U32, _U32, _P, ctypes.POINTER(_U32)]),
     "vfd_step": (ctypes.c_int, [_P]),

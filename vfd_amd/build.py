@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvfd_hip.so")
-SOURCES = ["search.hip", "boundary.hip", "dfsph.hip", "adjoint.hip", "sim.cpp", "scene_io.cpp"]
+SOURCES = ["search.hip", "boundary.hip", "dfsph.hip", "adjoint.hip", "sim.cpp", "scene_io.cpp", "mesh.cpp"]
 ARCH = os.environ.get("VFD_OFFLOAD_ARCH", "gfx950")
 
 

@@ -258,11 +258,199 @@ __global__ void __launch_bounds__(64) k_map_field1(MapDev map, const float* __re
     nodes1[l] = 0.8f * res;
 }
 
-void launch_map_build(hipStream_t st, const ShapeDev& shape, const MapDev& map, float* nodes0, float* nodes1,
-                      const KTabHost& kt, const float* glx, const float* glw) {
+void launch_map_field0(hipStream_t st, const ShapeDev& shape, const MapDev& map, float* nodes0) {
     hipLaunchKernelGGL(k_map_field0, dim3((map.nodeCount + 255) / 256), dim3(256), 0, st, shape, map, nodes0);
-    hipLaunchKernelGGL(k_map_field1, dim3((map.nodeCount + 63) / 64), dim3(64), 0, st, map, (const float*)nodes0,
-                       nodes1, kt, glx, glw);
+}
+
+void launch_map_field1(hipStream_t st, const MapDev& map, const float* nodes0, float* nodes1, const KTabHost& kt,
+                       const float* glx, const float* glw) {
+    hipLaunchKernelGGL(k_map_field1, dim3((map.nodeCount + 63) / 64), dim3(64), 0, st, map, nodes0, nodes1, kt, glx,
+                       glw);
+}
+
+// ---- triangle-mesh signed distance (MeshDistance::SignedDistance, Utility/SDF/MeshDistance.cpp:186-225) ----------
+// The reference's host code evaluates glm::dot as (x x' + y y') + z z' without fused terms; so does this code.
+__device__ __forceinline__ float gdot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+// Squared distance from p to one triangle (PointTriangleDistanceSquared, MeshDistance.cpp:303-607): the minimiser
+// (s, t) of |x0 + s e0 + t e1 - p|^2 over the triangle, found by classifying the unconstrained minimum into the seven
+// regions around the triangle (Eberly, "Distance Between Point and Triangle in 3D", Geometric Tools), with the
+// reference's branch order and float expressions. kWhere also returns the closest point and its feature:
+// 0..2 = vertex x0, x1, x2; 3..5 = edge x0x1, x1x2, x2x0; 6 = interior.
+template <bool kWhere>
+__device__ __forceinline__ float tri_d2(float4 r0, float4 r1, float4 r2, float4 r3, f3 p, f3* cp, int* feature) {
+    const f3 x0 = mk3(r0.x, r0.y, r0.z), e0 = mk3(r1.x, r1.y, r1.z), e1 = mk3(r2.x, r2.y, r2.z);
+    const float a00 = r0.w, a01 = r1.w, a11 = r2.w, det = r3.x, denom = r3.y;
+    const f3 diff = x0 - p;
+    const float b0 = gdot(diff, e0), b1 = gdot(diff, e1), c = gdot(diff, diff);
+    float s = a01 * b1 - a11 * b0;
+    float t = a01 * b0 - a00 * b1;
+    float d2;
+    int f;
+    // the five boundary outcomes shared by several regions
+    auto at_x0 = [&] { s = 0.0f; t = 0.0f; d2 = c; f = 0; };
+    auto at_x1 = [&] { s = 1.0f; t = 0.0f; d2 = (a00 + 2.0f * b0) + c; f = 1; };
+    auto at_x2 = [&] { s = 0.0f; t = 1.0f; d2 = (a11 + 2.0f * b1) + c; f = 2; };
+    auto on_e0 = [&] { t = 0.0f; s = -b0 / a00; d2 = b0 * s + c; f = 3; };
+    auto on_e2 = [&] { s = 0.0f; t = -b1 / a11; d2 = b1 * t + c; f = 5; };
+    auto quadratic = [&] { return (s * ((a00 * s + a01 * t) + 2.0f * b0) + t * ((a01 * s + a11 * t) + 2.0f * b1)) + c; };
+    // edge x1x2 (s + t = 1) at s = u
+    auto on_e1 = [&](float u, bool uIsS) {
+        if (uIsS) { s = u; t = 1.0f - s; } else { t = u; s = 1.0f - t; }
+        d2 = quadratic();
+        f = 4;
+    };
+    // s = 0 edge, vertices x0/x2 (regions 3 and 4 with b0 >= 0)
+    auto side_s0 = [&] {
+        if (b1 >= 0.0f) at_x0();
+        else if (-b1 >= a11) at_x2();
+        else on_e2();
+    };
+    if (s + t <= det) {
+        if (s < 0.0f) {
+            if (t < 0.0f) {  // region 4
+                if (b0 < 0.0f) {
+                    if (-b0 >= a00) at_x1();
+                    else on_e0();
+                } else {
+                    side_s0();
+                }
+            } else {  // region 3
+                side_s0();
+            }
+        } else if (t < 0.0f) {  // region 5
+            if (b0 >= 0.0f) at_x0();
+            else if (-b0 >= a00) at_x1();
+            else on_e0();
+        } else {  // region 0: interior
+            const float invDet = 1.0f / det;
+            s = s * invDet;
+            t = t * invDet;
+            d2 = quadratic();
+            f = 6;
+        }
+    } else if (s < 0.0f) {  // region 2
+        const float tmp0 = a01 + b0, tmp1 = a11 + b1;
+        if (tmp1 > tmp0) {
+            const float numer = tmp1 - tmp0;
+            if (numer >= denom) at_x1();
+            else on_e1(numer / denom, true);
+        } else if (tmp1 <= 0.0f) at_x2();
+        else if (b1 >= 0.0f) at_x0();
+        else on_e2();
+    } else if (t < 0.0f) {  // region 6
+        const float tmp0 = a01 + b1, tmp1 = a00 + b0;
+        if (tmp1 > tmp0) {
+            const float numer = tmp1 - tmp0;
+            if (numer >= denom) at_x2();
+            else on_e1(numer / denom, false);
+        } else if (tmp1 <= 0.0f) at_x1();
+        else if (b0 >= 0.0f) at_x0();
+        else on_e0();
+    } else {  // region 1
+        const float numer = ((a11 + b1) - a01) - b0;
+        if (numer <= 0.0f) at_x2();
+        else if (numer >= denom) at_x1();
+        else on_e1(numer / denom, true);
+    }
+    if (d2 < 0.0f) d2 = 0.0f;
+    if (kWhere) {
+        *cp = (x0 + e0 * s) + e1 * t;
+        *feature = f;
+    }
+    return d2;
+}
+
+constexpr uint32_t kMeshTile = 256;
+
+// Field 0 of a mesh body: one thread per map node scans every triangle (tiles of kMeshTile triangles staged in LDS
+// and read as broadcasts), keeps the first triangle of least squared distance, then recomputes that triangle's
+// closest point and feature for the sign: negative where (p - closest) points against the feature's normal.
+__global__ void __launch_bounds__(kMeshTile) k_mesh_field0(MeshDev mesh, MapDev map, float sign, float tol,
+                                                          float* nodes0) {
+    __shared__ float4 sTri[4 * kMeshTile];
+    const uint32_t l = blockIdx.x * kMeshTile + threadIdx.x;
+    const bool live = l < map.nodeCount;
+    const f3 p = live ? node_position(map, l) : mk3(0.0f, 0.0f, 0.0f);
+    float best = INFINITY;
+    uint32_t bestTri = 0;
+    for (uint32_t base = 0; base < mesh.triCount; base += kMeshTile) {
+        __syncthreads();
+        const uint32_t tl = base + threadIdx.x;
+        if (tl < mesh.triCount)
+            for (int q = 0; q < 4; ++q) sTri[4 * threadIdx.x + q] = mesh.tri[4 * (size_t)tl + q];
+        __syncthreads();
+        const uint32_t n = min(kMeshTile, mesh.triCount - base);
+        for (uint32_t k = 0; k < n; ++k) {
+            const float d2 = tri_d2<false>(sTri[4 * k], sTri[4 * k + 1], sTri[4 * k + 2], sTri[4 * k + 3], p, nullptr,
+                                           nullptr);
+            if (d2 < best) {
+                best = d2;
+                bestTri = base + k;
+            }
+        }
+    }
+    if (!live) return;
+    const float4* r = mesh.tri + 4 * (size_t)bestTri;
+    f3 cp;
+    int feature;
+    float d = sqrtf(tri_d2<true>(r[0], r[1], r[2], r[3], p, &cp, &feature));
+    const float* nrm = mesh.normals + 21 * (size_t)bestTri + 3 * feature;
+    if (gdot(p - cp, mk3(nrm[0], nrm[1], nrm[2])) < 0.0f) d = -d;
+    nodes0[l] = sign * (d - tol);
+}
+
+void launch_mesh_field0(hipStream_t st, const MeshDev& mesh, const MapDev& map, float sign, float tol, float* nodes0) {
+    hipLaunchKernelGGL(k_mesh_field0, dim3((map.nodeCount + kMeshTile - 1) / kMeshTile), dim3(kMeshTile), 0, st, mesh,
+                       map, sign, tol, nodes0);
+}
+
+// ---- ParticleSampler::SampleMeshVolume (Utility/Sampler/ParticleSampler.cpp:7-91) ---------------------------------
+// Candidate (i, j, k) of the x, y, z loops; i and j are the reference's counterX and counterY.
+__device__ __forceinline__ f3 sample_point(const SampleGrid& g, uint32_t i, uint32_t j, uint32_t k) {
+    const float x = g.xs[i], y = g.ys[j], z = g.zs[k], r = g.radius;
+    if (g.mode == 0) return mk3(x + r, y + r, z + r);
+    if (g.mode == 1) return (j % 2 == 0) ? mk3(x, y + r, z + r) : mk3(x + r, y + r, z);
+    f3 shift = mk3(0.0f, 0.0f, 0.0f);
+    if (i % 2) shift.z = shift.z + g.diameter / (2.0f * ((j % 2) ? -1.0f : 1.0f));
+    if (j % 2) {
+        shift.x = shift.x + g.shiftX / 2.0f;
+        shift.z = shift.z + g.diameter / 2.0f;
+    }
+    return mk3(x, y + r, z + r) + shift;
+}
+
+__global__ void k_sample_flags(SampleGrid g, MapDev map, const float* __restrict__ nodes0, uint32_t* flags) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nxy = g.nx * g.ny;
+    if (c >= nxy * g.nz) return;
+    const uint32_t k = c / nxy, rem = c % nxy;
+    const f3 p = sample_point(g, rem % g.nx, rem / g.nx, k);
+    flags[c] = map_interp_at0(map, nodes0, p) < 0.0f ? 1u : 0u;  // SDF::GetDistance(p, 0) < 0 (SDF.cu:217-225)
+}
+
+__global__ void k_sample_scatter(SampleGrid g, const uint32_t* __restrict__ flags,
+                                 const uint32_t* __restrict__ offsets, float* out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nxy = g.nx * g.ny;
+    if (c >= nxy * g.nz || !flags[c]) return;
+    const uint32_t k = c / nxy, rem = c % nxy;
+    const f3 p = sample_point(g, rem % g.nx, rem / g.nx, k);
+    float* o = out + 3 * (size_t)offsets[c];
+    o[0] = p.x;
+    o[1] = p.y;
+    o[2] = p.z;
+}
+
+void launch_sample_flags(hipStream_t st, const SampleGrid& g, const MapDev& map, const float* nodes0, uint32_t* flags) {
+    const uint32_t n = g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(k_sample_flags, dim3((n + 255) / 256), dim3(256), 0, st, g, map, nodes0, flags);
+}
+
+void launch_sample_scatter(hipStream_t st, const SampleGrid& g, const uint32_t* flags, const uint32_t* offsets,
+                           float* out) {
+    const uint32_t n = g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(k_sample_scatter, dim3((n + 255) / 256), dim3(256), 0, st, g, flags, offsets, out);
 }
 
 // ComputeVolumeAndBoundaryKernel (DFSPHKernels.cu:69-150).

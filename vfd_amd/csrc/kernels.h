@@ -83,6 +83,28 @@ struct MapDev {
     const float* nodes1;  // boundary volume
 };
 
+// A boundary mesh prepared for the distance kernel. Per triangle, four float4 records hold the point-independent
+// terms of the closest-point computation: {x0, a00}, {e0 = x1 - x0, a01}, {e1 = x2 - x0, a11}, {det, denom, 0, 0}
+// with a00 = e0.e0, a01 = e0.e1, a11 = e1.e1, det = |a00 a11 - a01^2|, denom = a00 - 2 a01 + a11; and 21 floats of
+// normals: the angle-weighted vertex normals of x0, x1, x2, the edge normals of x0x1, x1x2, x2x0 and the face normal
+// (MeshDistance.cpp:13-57, 255-284).
+struct MeshDev {
+    const float4* tri;
+    const float* normals;
+    uint32_t triCount;
+};
+
+// The candidate lattice of ParticleSampler::SampleMeshVolume: xs/ys/zs hold the loop variables as the reference's
+// float loops accumulate them.
+struct SampleGrid {
+    const float* xs;
+    const float* ys;
+    const float* zs;
+    uint32_t nx, ny, nz;
+    int32_t mode;
+    float radius, diameter, shiftX;
+};
+
 // Kernel table view (device pointer + scalars).
 struct KTabHost {
     const float2* tab;
@@ -104,8 +126,14 @@ void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, 
                         unsigned long long* nbStats);
 
 // ---- boundary.hip ---------------------------------------------------------------------------------------------
-void launch_map_build(hipStream_t st, const ShapeDev& shape, const MapDev& map, float* nodes0, float* nodes1,
-                      const KTabHost& kt, const float* glx, const float* glw);
+void launch_map_field0(hipStream_t st, const ShapeDev& shape, const MapDev& map, float* nodes0);
+void launch_mesh_field0(hipStream_t st, const MeshDev& mesh, const MapDev& map, float sign, float tol, float* nodes0);
+void launch_map_field1(hipStream_t st, const MapDev& map, const float* nodes0, float* nodes1, const KTabHost& kt,
+                       const float* glx, const float* glw);
+// flags[c] = 1 where candidate c of the lattice lies inside (field 0 < 0), c = (k * ny + j) * nx + i
+void launch_sample_flags(hipStream_t st, const SampleGrid& g, const MapDev& map, const float* nodes0, uint32_t* flags);
+void launch_sample_scatter(hipStream_t st, const SampleGrid& g, const uint32_t* flags, const uint32_t* offsets,
+                           float* out);
 void launch_boundary(const Launch& L, const MapDev* maps, float4* pos, float4* vel, float4* xb, float* vb);
 
 // ---- dfsph.hip ------------------------------------------------------------------------------------------------

@@ -25,6 +25,7 @@
 #include "../../include/vfd.h"
 #include "common.h"
 #include "kernels.h"
+#include "mesh.h"
 
 // run-time knob defaults (vfd_set_tuning); vfd_amd/build.py passes the values chosen by vfd_amd/autotune.py
 #ifndef VFD_DEFAULT_USE_GRAPHS
@@ -682,20 +683,11 @@ public:
 
     // RigidBody ctor (RigidBody.cu:10-73): volume map domain, resolution and both fields, built on the GPU.
     int add_boundary(uint32_t s, const vfd_boundary_desc& b) {
-        Scene& S = *sc[s];
-        if (S.bodies.size() >= kMaxBodies)
-            return fail(VFD_ERROR_NOT_SUPPORTED, "at most " + std::to_string(kMaxBodies) + " boundary bodies per scene");
         if (b.Shape < VFD_SHAPE_BOX || b.Shape > VFD_SHAPE_CYLINDER)
             return fail(VFD_ERROR_INVALID_ARGUMENT, "unknown boundary shape");
         if (b.Shape == VFD_SHAPE_PLANE && !b.HasDomain)
             return fail(VFD_ERROR_INVALID_ARGUMENT, "a plane boundary needs an explicit map domain");
-        for (int a = 0; a < 3; ++a)
-            if (b.CollisionMapResolution[a] == 0)
-                return fail(VFD_ERROR_INVALID_ARGUMENT, "CollisionMapResolution must be positive");
-        const uint64_t cells = (uint64_t)b.CollisionMapResolution[0] * b.CollisionMapResolution[1] *
-                               b.CollisionMapResolution[2];
-        if (cells > (1u << 26)) return fail(VFD_ERROR_INVALID_ARGUMENT, "CollisionMapResolution too large");
-        const float r = S.desc.ParticleRadius, R = kt.R;
+        const float r = sc[s]->desc.ParticleRadius, R = kt.R;
         const float tol = b.Padding - r;
         ShapeDev sh{};
         sh.shape = b.Shape;
@@ -707,13 +699,11 @@ public:
         sh.radius = b.Radius;
         sh.sign = b.Inverted ? -1.0f : 1.0f;
         sh.tol = tol;
-        Body* body = new Body();
-        body->desc = b;
-        MapDev& m = body->map;
+        float lo[3], hi[3];
         if (b.HasDomain) {
             for (int a = 0; a < 3; ++a) {
-                m.domMin[a] = b.DomainMin[a];
-                m.domMax[a] = b.DomainMax[a];
+                lo[a] = b.DomainMin[a];
+                hi[a] = b.DomainMax[a];
             }
         } else {
             float e[3];
@@ -726,12 +716,65 @@ public:
                 for (int a = 0; a < 3; ++a) e[a] = b.HalfExtents[a];
             const float grow = 8.0f * R + tol;  // RigidBody.cu:35-37
             for (int a = 0; a < 3; ++a) {
-                m.domMin[a] = (b.Center[a] - e[a]) - grow;
-                m.domMax[a] = (b.Center[a] + e[a]) + grow;
+                lo[a] = (b.Center[a] - e[a]) - grow;
+                hi[a] = (b.Center[a] + e[a]) + grow;
             }
         }
+        return add_body(s, b.CollisionMapResolution, lo, hi, b,
+                        [&](const MapDev& m, float* n0) { launch_map_field0(stream, sh, m, n0); });
+    }
+
+    // RigidBody ctor for a mesh (RigidBody.cu:10-73): transformed vertices, domain = their bounding box grown by
+    // 8R + tol, field 0 = sign * (MeshDistance::SignedDistance - tol) from the GPU distance kernel.
+    int add_boundary_mesh(uint32_t s, const vfd_mesh_boundary_desc& b, const vfd_mesh& mesh) {
+        MeshPrep prep;
+        VFD_TRY(prepare_mesh(mesh, b.Transform, prep));
+        const float r = sc[s]->desc.ParticleRadius, R = kt.R;
+        const float tol = b.Padding - r;
+        const float grow = 8.0f * R + tol;
+        float lo[3], hi[3];
         for (int a = 0; a < 3; ++a) {
-            m.res[a] = b.CollisionMapResolution[a];
+            lo[a] = prep.lo[a] - grow;
+            hi[a] = prep.hi[a] + grow;
+        }
+        DevBuf<float> tri, nrm;
+        VFD_TRY(tri.alloc(prep.tri.size()));
+        VFD_TRY(nrm.alloc(prep.normals.size()));
+        VFD_HIP(hipMemcpyAsync(tri.p, prep.tri.data(), prep.tri.size() * sizeof(float), hipMemcpyHostToDevice,
+                               stream));
+        VFD_HIP(hipMemcpyAsync(nrm.p, prep.normals.data(), prep.normals.size() * sizeof(float),
+                               hipMemcpyHostToDevice, stream));
+        const MeshDev md{(const float4*)tri.p, nrm.p, mesh.TriangleCount};
+        vfd_boundary_desc d{};
+        d.Shape = -1;  // mesh
+        d.Inverted = b.Inverted;
+        d.Padding = b.Padding;
+        for (int a = 0; a < 3; ++a) d.CollisionMapResolution[a] = b.CollisionMapResolution[a];
+        const float sign = b.Inverted ? -1.0f : 1.0f;
+        // add_body synchronises the stream before returning, so tri/nrm outlive the kernel
+        return add_body(s, b.CollisionMapResolution, lo, hi, d,
+                        [&](const MapDev& m, float* n0) { launch_mesh_field0(stream, md, m, sign, tol, n0); });
+    }
+
+    // Common tail of both: map geometry (SDF.cu:8-14), node storage, field 0 by `field0`, field 1 (the boundary
+    // volume) and the body's registration.
+    template <class Field0>
+    int add_body(uint32_t s, const uint32_t res[3], const float lo[3], const float hi[3], const vfd_boundary_desc& b,
+                 Field0 field0) {
+        Scene& S = *sc[s];
+        if (S.bodies.size() >= kMaxBodies)
+            return fail(VFD_ERROR_NOT_SUPPORTED, "at most " + std::to_string(kMaxBodies) + " boundary bodies per scene");
+        for (int a = 0; a < 3; ++a)
+            if (res[a] == 0) return fail(VFD_ERROR_INVALID_ARGUMENT, "CollisionMapResolution must be positive");
+        const uint64_t cells = (uint64_t)res[0] * res[1] * res[2];
+        if (cells > (1u << 26)) return fail(VFD_ERROR_INVALID_ARGUMENT, "CollisionMapResolution too large");
+        Body* body = new Body();
+        body->desc = b;
+        MapDev& m = body->map;
+        for (int a = 0; a < 3; ++a) {
+            m.domMin[a] = lo[a];
+            m.domMax[a] = hi[a];
+            m.res[a] = res[a];
             m.cellSize[a] = (m.domMax[a] - m.domMin[a]) / (float)m.res[a];
             m.cellSizeInv[a] = 1.0f / m.cellSize[a];
         }
@@ -746,7 +789,8 @@ public:
         }
         m.nodes0 = body->n0.p;
         m.nodes1 = body->n1.p;
-        launch_map_build(stream, sh, m, body->n0.p, body->n1.p, ktab(), dGlx.p, dGlw.p);
+        field0(m, body->n0.p);
+        launch_map_field1(stream, m, body->n0.p, body->n1.p, ktab(), dGlx.p, dGlw.p);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) {
@@ -1719,6 +1763,15 @@ int vfd_batch_add_boundary(vfd_sim* sim, uint32_t scene, const vfd_boundary_desc
     return guard(sim->impl.add_boundary(scene, *body));
 }
 int vfd_add_boundary(vfd_sim* sim, const vfd_boundary_desc* body) { return vfd_batch_add_boundary(sim, 0, body); }
+int vfd_batch_add_boundary_mesh(vfd_sim* sim, uint32_t scene, const vfd_mesh_boundary_desc* body,
+                                const vfd_mesh* mesh) {
+    CHECK_SCENE(sim, scene);
+    if (!body || !mesh) return fail(VFD_ERROR_INVALID_ARGUMENT, "body or mesh is NULL");
+    return guard(sim->impl.add_boundary_mesh(scene, *body, *mesh));
+}
+int vfd_add_boundary_mesh(vfd_sim* sim, const vfd_mesh_boundary_desc* body, const vfd_mesh* mesh) {
+    return vfd_batch_add_boundary_mesh(sim, 0, body, mesh);
+}
 int vfd_clear_boundaries(vfd_sim* sim, uint32_t scene) {
     CHECK_SCENE(sim, scene);
     return sim->impl.clear_boundaries(scene);

@@ -184,3 +184,72 @@ CONFIGS = {
     "batched_uneven_64": batched_uneven,
     "falling_viscous_block_100k": falling_viscous_block,
 }
+
+
+def load_scene_file(path, resource_root):
+    """The DFSPH inputs of a VFD scene file (an entt snapshot in cereal JSON written by Scene::Save,
+    Scene/Scene.cpp:10-40), as the editor's "Bake Objects" builds them (Editor/Panels/ComponentPanel.cpp:608-656):
+    the first simulation description, one FluidObjectDescription per entity holding a fluid-object and a mesh
+    component, one mesh RigidBodyDescription per entity holding a rigid-body and a mesh component.
+
+    Components follow their entity id in the snapshot; the component objects are recognised by their keys
+    ("transform", "meshSource", {"inverted", "padding", "collisionMapResolution"}, {"inverted", "resolution",
+    "sampleMode", "velocity"}, "description"). Mesh paths ("Resources/Models/Cube.obj") are resolved under
+    `resource_root` (its last component may stand for "Resources/Models", as for tests/golden/models). Transforms are
+    the stored column-major matrices of entities without parents; the reference decomposes a loaded matrix into
+    translation, Euler angles and scale and recomposes it (TransformComponent.h:33-38), which may move entries by
+    a few ulps. Returns (description, [FluidObjectDescription], [RigidBodyDescription])."""
+    import json
+    import os
+
+    from .mesh import TriangleMesh
+    from .simulation import FluidObjectDescription, read_descriptions
+
+    with open(path, "r") as f:
+        data = json.load(f)
+    values = list(data.values())
+    comps = {}
+    for k in range(1, len(values)):
+        v, ent = values[k], values[k - 1]
+        if not isinstance(v, dict) or isinstance(ent, (dict, list, bool)) or not isinstance(ent, int):
+            continue
+        keys = set(v)
+        if "transform" in keys:
+            cols = [v["transform"]["value%d" % c] for c in range(4)]
+            m = np.array([[cols[c][a] for c in range(4)] for a in "xyzw"], dtype=np.float32)
+            comps.setdefault(ent, {})["transform"] = m
+        elif "meshSource" in keys:
+            comps.setdefault(ent, {})["mesh"] = v["meshSource"]
+        elif {"inverted", "padding", "collisionMapResolution"} <= keys:
+            comps.setdefault(ent, {})["rigid"] = v
+        elif {"inverted", "resolution", "sampleMode", "velocity"} <= keys:
+            comps.setdefault(ent, {})["fluid"] = v
+    descs = read_descriptions(path)
+    if not descs:
+        raise ValueError("the scene holds no DFSPH simulation description")
+
+    def mesh_of(src):
+        rel = src.replace("\\", "/")
+        for cand in (os.path.join(resource_root, rel), os.path.join(resource_root, os.path.basename(rel))):
+            if os.path.exists(cand):
+                return TriangleMesh.load_obj(cand)
+        raise FileNotFoundError(src)
+
+    fluids, bodies = [], []
+    for ent in sorted(comps):
+        c = comps[ent]
+        if "mesh" not in c:
+            continue
+        tr = c.get("transform", np.eye(4, dtype=np.float32))
+        if "fluid" in c:
+            fo = c["fluid"]
+            fluids.append(FluidObjectDescription(
+                mesh_of(c["mesh"]), transform=tr, inverted=bool(fo["inverted"]),
+                resolution=tuple(int(fo["resolution"][a]) for a in "xyz"), sample_mode=int(fo["sampleMode"]),
+                velocity=tuple(float(fo["velocity"][a]) for a in "xyz")))
+        if "rigid" in c:
+            rb = c["rigid"]
+            bodies.append(RigidBodyDescription(
+                "mesh", mesh=mesh_of(c["mesh"]), transform=tr, inverted=bool(rb["inverted"]),
+                padding=float(rb["padding"]), resolution=tuple(int(rb["collisionMapResolution"][a]) for a in "xyz")))
+    return descs[0], fluids, bodies

@@ -25,6 +25,13 @@ class FluidObject:
         if self.velocities.shape != self.positions.shape:
             raise ValueError("velocity must be a vec3 or one vec3 per position")
 
+    @classmethod
+    def sample(cls, desc, particle_radius):
+        """FluidObject(desc, info) (FluidObject.cpp:6-24): the description's mesh sampled on the GPU."""
+        x = sample_mesh_volume(desc.mesh, particle_radius, desc.resolution, desc.inverted, desc.sample_mode,
+                               desc.transform)
+        return cls(x, desc.velocity)
+
     def GetPositions(self):
         return self.positions
 
@@ -36,11 +43,12 @@ class FluidObject:
 
 
 class RigidBodyDescription:
-    """RigidBodyDescription (RigidBody/RigidBody.cuh:12-21): Inverted, Padding, CollisionMapResolution; the mesh and
-    transform are replaced by an analytic shape ("box", "sphere", "plane", "cylinder")."""
+    """RigidBodyDescription (RigidBody/RigidBody.cuh:15-24): Inverted, Padding, CollisionMapResolution and either a
+    triangle mesh with its Transform (shape "mesh", `mesh` a TriangleMesh, `transform` a 4x4 matrix acting on
+    (x, y, z, 1)) or an analytic shape ("box", "sphere", "plane", "cylinder") standing in for the mesh."""
 
     def __init__(self, shape, center=(0, 0, 0), half_extents=(0, 0, 0), radius=0.0, normal=(0, 1, 0),
-                 inverted=False, padding=0.0, resolution=(20, 20, 20), domain=None):
+                 inverted=False, padding=0.0, resolution=(20, 20, 20), domain=None, mesh=None, transform=None):
         self.shape = shape
         self.center = center
         self.half_extents = half_extents
@@ -50,10 +58,59 @@ class RigidBodyDescription:
         self.padding = padding
         self.resolution = resolution
         self.domain = domain
+        self.mesh = mesh
+        self.transform = np.eye(4, dtype=np.float32) if transform is None else np.asarray(transform, np.float32)
+        if shape == "mesh" and mesh is None:
+            raise ValueError("a mesh body needs a mesh")
+
+    @property
+    def is_mesh(self):
+        return self.shape == "mesh"
 
     def to_abi(self):
+        if self.is_mesh:
+            from .mesh import glm_transform
+            b = _abi.MeshBoundaryDesc()
+            b.Transform[:] = [float(v) for v in glm_transform(self.transform)]
+            b.Inverted = 1 if self.inverted else 0
+            b.Padding = self.padding
+            b.CollisionMapResolution[:] = [int(r) for r in self.resolution]
+            return b
         return _abi.make_boundary(self.shape, self.center, self.half_extents, self.radius, self.normal,
                                   self.inverted, self.padding, self.resolution, self.domain)
+
+
+class FluidObjectDescription:
+    """FluidObjectDescription (FluidObject/FluidObject.h:10-19): a triangle mesh with its Transform, Inverted,
+    Resolution of the sampling SDF, SampleMode (0 min, 1 medium, 2 max density) and one initial Velocity."""
+
+    def __init__(self, mesh, transform=None, inverted=False, resolution=(10, 10, 10), sample_mode=0,
+                 velocity=(0.0, 0.0, 0.0)):
+        self.mesh = mesh
+        self.transform = np.eye(4, dtype=np.float32) if transform is None else np.asarray(transform, np.float32)
+        self.inverted = inverted
+        self.resolution = resolution
+        self.sample_mode = sample_mode
+        self.velocity = velocity
+
+
+def sample_mesh_volume(mesh, radius, resolution, inverted=False, sample_mode=0, transform=None):
+    """ParticleSampler::SampleMeshVolume (Utility/Sampler/ParticleSampler.cpp:7-91) of the mesh transformed by
+    `transform` (FluidObject.cpp:9-14), on the GPU: (n, 3) float32 positions in the reference's loop order."""
+    from .mesh import glm_transform
+    lib = _abi.load_library()
+    m = mesh.to_abi()
+    tr = None if transform is None else glm_transform(transform)
+    res = np.ascontiguousarray(resolution, dtype=np.uint32)
+    n = _abi.ctypes.c_uint32()
+    check(lib.vfd_sample_mesh_volume(_abi.ctypes.byref(m), ptr(tr), float(radius), ptr(res), int(bool(inverted)),
+                                     int(sample_mode), None, 0, _abi.ctypes.byref(n)), "vfd_sample_mesh_volume")
+    out = np.zeros((n.value, 3), dtype=np.float32)
+    if n.value:
+        check(lib.vfd_sample_mesh_volume(_abi.ctypes.byref(m), ptr(tr), float(radius), ptr(res),
+                                         int(bool(inverted)), int(sample_mode), ptr(out), n.value,
+                                         _abi.ctypes.byref(n)), "vfd_sample_mesh_volume")
+    return out
 
 
 def DFSPHSimulationDescription(**fields):
@@ -157,6 +214,11 @@ class DFSPHSimulation:
     def SetRigidBodies(self, rigid_bodies):
         check(self._lib.vfd_clear_boundaries(self._h, self._scene), "vfd_clear_boundaries")
         for rb in rigid_bodies:
+            if isinstance(rb, RigidBodyDescription) and rb.is_mesh:
+                m = rb.mesh.to_abi()
+                check(self._lib.vfd_batch_add_boundary_mesh(self._h, self._scene, _abi.ctypes.byref(rb.to_abi()),
+                                                            _abi.ctypes.byref(m)), "vfd_add_boundary_mesh")
+                continue
             b = rb.to_abi() if isinstance(rb, RigidBodyDescription) else rb
             check(self._lib.vfd_batch_add_boundary(self._h, self._scene, _abi.ctypes.byref(b)),
                   "vfd_add_boundary")
