@@ -134,12 +134,25 @@ def _pmc_table(path):
     return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in acc.items()}
 
 
+def _trace_rows(path):
+    """(kernel, grid_x, microseconds) of every dispatch in a rocprofv3 --kernel-trace output: kernel_trace.csv
+    (--output-format csv) or the default SQLite *_results.db (its `kernels` view)."""
+    try:
+        for r in csv.DictReader(open(_find(path, "kernel_trace.csv"))):
+            yield (_kname(r["Kernel_Name"]), int(r.get("Grid_Size_X", r.get("Grid_Size", 0)) or 0),
+                   (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0)
+    except (IndexError, FileNotFoundError):
+        import sqlite3
+        db = sqlite3.connect(_find(path, "_results.db"))
+        for name, grid, start, end in db.execute("select name, grid_x, start, end from kernels"):
+            yield _kname(name), int(grid), (end - start) / 1000.0
+
+
 def trace_summary(path):
-    """Markdown table from a rocprofv3 --kernel-trace kernel_trace.csv: every dispatch, and the live ones."""
+    """Markdown table from a rocprofv3 --kernel-trace output: every dispatch, and the live ones."""
     acc = {}
-    for r in csv.DictReader(open(_find(path, "kernel_trace.csv"))):
-        us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0
-        acc.setdefault(_kname(r["Kernel_Name"]), []).append(us)
+    for k, _, us in _trace_rows(path):
+        acc.setdefault(k, []).append(us)
     lines = ["| kernel | dispatches | mean us (all) | live dispatches (>= %g us) | mean us (live) | total ms |" % LIVE_US,
              "|---|---:|---:|---:|---:|---:|"]
     for k, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
@@ -274,6 +287,40 @@ def frame_capture_cost(sim, snap, steps=10):
             "readback_ms_per_frame": round(1000 * back / max(len(frames), 1), 3)}
 
 
+def mesh_inputs(map_res=48, sample_radius=0.005):
+    """The mesh inputs at setup: the volume map of a mesh body (the reference's Torus.obj, 1,152 triangles; signed
+    distance of every map node on the GPU, then the boundary-volume field) and the mesh sampler filling the example
+    scene's Cone.obj fluid object at a fine particle radius. Wall-clock of the synchronous calls, after one untimed
+    call of each."""
+    import numpy as np
+    from vfd_amd import DFSPHSimulation, DFSPHSimulationDescription, RigidBodyDescription, TriangleMesh
+    from vfd_amd.simulation import sample_mesh_volume
+    models = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "models")
+    torus = TriangleMesh.load_obj(os.path.join(models, "Torus.obj"))
+    cone = TriangleMesh.load_obj(os.path.join(models, "Cone.obj"))
+    sim = DFSPHSimulation(DFSPHSimulationDescription(ParticleRadius=0.01))
+    sim.set_fluid(np.zeros((1, 3), np.float32))
+    body = RigidBodyDescription("mesh", mesh=torus, transform=np.diag([1.5, 1.5, 1.5, 1.0]),
+                                resolution=(map_res,) * 3)
+    ms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        sim.SetRigidBodies([body])
+        ms.append(1000 * (time.perf_counter() - t0))
+    nodes = len(sim.boundary_map(0, 0))
+    flip = np.array([[1, 0, 0, 0], [0, -1, 0, 3], [0, 0, -1, 0], [0, 0, 0, 1]], np.float32)
+    ss = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        x = sample_mesh_volume(cone, sample_radius, (40, 40, 40), False, 1, flip)
+        ss.append(1000 * (time.perf_counter() - t0))
+    return {"map": {"mesh": "Torus.obj", "triangles": len(torus.triangles), "resolution": map_res, "nodes": nodes,
+                    "build_ms": round(float(np.median(ms[1:])), 2),
+                    "node_triangle_pairs": nodes * len(torus.triangles)},
+            "sampler": {"mesh": "Cone.obj", "radius": sample_radius, "samples": len(x),
+                        "sample_ms": round(float(np.median(ss[1:])), 2)}}
+
+
 def copy_probe():
     import torch
     n = COPY_PROBE_BYTES // 4
@@ -299,6 +346,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement (config C5)")
     ap.add_argument("--no-frames", action="store_true", help="skip the frame-output measurement")
     ap.add_argument("--frame-steps", type=int, default=10)
+    ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-input (map build, sampler) measurement")
+    ap.add_argument("--mesh-only", action="store_true", help="only measure the mesh inputs")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"))
     ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_DIR")
     args = ap.parse_args()
@@ -314,6 +363,10 @@ def main():
         return
     if args.copy_probe:
         copy_probe()
+        return
+
+    if args.mesh_only:
+        print(json.dumps({"mesh_inputs": mesh_inputs()}), flush=True)
         return
 
     import numpy as np
@@ -431,6 +484,9 @@ def main():
     if not args.no_adjoint:
         print("[bench] adjoint/forward: C5", file=sys.stderr, flush=True)
         out["adjoint_over_forward"] = adjoint_ratio()
+    if not args.no_mesh:
+        print("[bench] mesh inputs: map build and sampler", file=sys.stderr, flush=True)
+        out["mesh_inputs"] = mesh_inputs()
     print(json.dumps(out), flush=True)
 
 

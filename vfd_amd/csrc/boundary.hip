@@ -280,7 +280,7 @@ __device__ __forceinline__ float gdot(f3 a, f3 b) { return (a.x * b.x + a.y * b.
 template <bool kWhere>
 __device__ __forceinline__ float tri_d2(float4 r0, float4 r1, float4 r2, float4 r3, f3 p, f3* cp, int* feature) {
     const f3 x0 = mk3(r0.x, r0.y, r0.z), e0 = mk3(r1.x, r1.y, r1.z), e1 = mk3(r2.x, r2.y, r2.z);
-    const float a00 = r0.w, a01 = r1.w, a11 = r2.w, det = r3.x, denom = r3.y;
+    const float a00 = r0.w, a01 = r1.w, a11 = r2.w, det = r3.x, denom = r3.y, invDet = r3.z;
     const f3 diff = x0 - p;
     const float b0 = gdot(diff, e0), b1 = gdot(diff, e1), c = gdot(diff, diff);
     float s = a01 * b1 - a11 * b0;
@@ -322,8 +322,7 @@ __device__ __forceinline__ float tri_d2(float4 r0, float4 r1, float4 r2, float4 
             if (b0 >= 0.0f) at_x0();
             else if (-b0 >= a00) at_x1();
             else on_e0();
-        } else {  // region 0: interior
-            const float invDet = 1.0f / det;
+        } else {  // region 0: interior (invDet = 1 / det, the same float the reference divides out per query)
             s = s * invDet;
             t = t * invDet;
             d2 = quadratic();

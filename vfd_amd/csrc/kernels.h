@@ -84,8 +84,8 @@ struct MapDev {
 };
 
 // A boundary mesh prepared for the distance kernel. Per triangle, four float4 records hold the point-independent
-// terms of the closest-point computation: {x0, a00}, {e0 = x1 - x0, a01}, {e1 = x2 - x0, a11}, {det, denom, 0, 0}
-// with a00 = e0.e0, a01 = e0.e1, a11 = e1.e1, det = |a00 a11 - a01^2|, denom = a00 - 2 a01 + a11; and 21 floats of
+// terms of the closest-point computation: {x0, a00}, {e0 = x1 - x0, a01}, {e1 = x2 - x0, a11}, {det, denom, 1/det,
+// 0} with a00 = e0.e0, a01 = e0.e1, a11 = e1.e1, det = |a00 a11 - a01^2|, denom = a00 - 2 a01 + a11; and 21 floats of
 // normals: the angle-weighted vertex normals of x0, x1, x2, the edge normals of x0x1, x1x2, x2x0 and the face normal
 // (MeshDistance.cpp:13-57, 255-284).
 struct MeshDev {

@@ -118,7 +118,9 @@ int prepare_mesh(const vfd_mesh& mesh, const float* transform, MeshPrep& out) {
         const float det = std::fabs(a00 * a11 - a01 * a01);
         const float denom = (a00 - 2.0f * a01) + a11;
         float* r = &out.tri[16 * (size_t)f];
-        const float rec[16] = {x0.x, x0.y, x0.z, a00, e0.x, e0.y, e0.z, a01, e1.x, e1.y, e1.z, a11, det, denom, 0, 0};
+        const float invDet = 1.0f / det;
+        const float rec[16] = {x0.x, x0.y, x0.z, a00, e0.x, e0.y, e0.z, a01,
+                               e1.x, e1.y, e1.z, a11, det, denom, invDet, 0};
         std::memcpy(r, rec, sizeof(rec));
         V nrm[7];
         for (int k = 0; k < 3; ++k) nrm[k] = vn[F[3 * f + k]];
