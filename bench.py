@@ -308,6 +308,15 @@ def mesh_inputs(map_res=48, sample_radius=0.005):
         sim.SetRigidBodies([body])
         ms.append(1000 * (time.perf_counter() - t0))
     nodes = len(sim.boundary_map(0, 0))
+    # a large mesh: an icosphere of 81,920 triangles with a bumpy surface, same map resolution
+    big = TriangleMesh.icosphere(6, 1.5, 0.2, 1)
+    big_body = RigidBodyDescription("mesh", mesh=big, resolution=(map_res,) * 3)
+    bs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        sim.SetRigidBodies([big_body])
+        bs.append(1000 * (time.perf_counter() - t0))
+    big_nodes = len(sim.boundary_map(0, 0))
     flip = np.array([[1, 0, 0, 0], [0, -1, 0, 3], [0, 0, -1, 0], [0, 0, 0, 1]], np.float32)
     ss = []
     for _ in range(3):
@@ -317,6 +326,10 @@ def mesh_inputs(map_res=48, sample_radius=0.005):
     return {"map": {"mesh": "Torus.obj", "triangles": len(torus.triangles), "resolution": map_res, "nodes": nodes,
                     "build_ms": round(float(np.median(ms[1:])), 2),
                     "node_triangle_pairs": nodes * len(torus.triangles)},
+            "map_large": {"mesh": "icosphere level 6, bumps 0.2", "triangles": len(big.triangles),
+                          "resolution": map_res, "nodes": big_nodes,
+                          "build_ms": round(float(np.median(bs[1:])), 2),
+                          "node_triangle_pairs": big_nodes * len(big.triangles)},
             "sampler": {"mesh": "Cone.obj", "radius": sample_radius, "samples": len(x),
                         "sample_ms": round(float(np.median(ss[1:])), 2)}}
 

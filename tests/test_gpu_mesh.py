@@ -42,7 +42,19 @@ BODIES = {
                                                   padding=0.04, resolution=(12, 12, 12)),
     "torus_inverted": lambda: RigidBodyDescription("mesh", mesh=load("Torus.obj"), transform=_rot_x(90, s=(0.8,) * 3),
                                                    inverted=True, resolution=(16, 10, 16)),
+    # the reference's AABB mesh (TriangleMesh.cpp:18-37): coplanar triangles make exact distance ties everywhere
+    "aabb_box": lambda: RigidBodyDescription("mesh", mesh=TriangleMesh.box((-0.5, -0.3, -0.4), 1.0, 0.6, 0.8),
+                                             resolution=(14, 10, 12)),
+    # 5,120 triangles in shuffled order: the culled, Morton-ordered scan must still pick the first closest triangle
+    "icosphere_shuffled": lambda: RigidBodyDescription("mesh", mesh=_shuffled(TriangleMesh.icosphere(4, 0.5, 0.2, 3)),
+                                                       transform=_rot_x(10, (0.05, 0.0, -0.02)),
+                                                       resolution=(16, 16, 16)),
 }
+
+
+def _shuffled(m, seed=9):
+    order = np.random.default_rng(seed).permutation(len(m.triangles))
+    return TriangleMesh(m.vertices, m.triangles[order])
 
 
 @pytest.mark.parametrize("name", sorted(BODIES))

@@ -361,46 +361,115 @@ __device__ __forceinline__ float tri_d2(float4 r0, float4 r1, float4 r2, float4 
 }
 
 constexpr uint32_t kMeshTile = 256;
+static_assert(kMeshTile % kMeshCluster == 0 && kMeshCluster == 64, "a wave decides one cluster at a time");
 
-// Field 0 of a mesh body: one thread per map node scans every triangle (tiles of kMeshTile triangles staged in LDS
-// and read as broadcasts), keeps the first triangle of least squared distance, then recomputes that triangle's
-// closest point and feature for the sign: negative where (p - closest) points against the feature's normal.
+// The map's nodes as four grids (SDF.cu:45-139 numbering): vertices (nx+1, ny+1, nz+1), x-edge nodes (2nx, ny+1,
+// nz+1), y-edge nodes (2ny, nz+1, nx+1) and z-edge nodes (2nz, nx+1, ny+1), each linear with its first dimension
+// fastest. The distance kernel gives every wave one 4 x 4 x 4 brick of one grid, so that a wave's nodes are close
+// together and its cluster test can skip most of a large mesh.
+struct NodeBricks {
+    uint32_t dim[4][3], nb[4][3], first[5], base[4];
+    __host__ __device__ explicit NodeBricks(const MapDev& m) {
+        const uint32_t nx = m.res[0], ny = m.res[1], nz = m.res[2];
+        const uint32_t d[4][3] = {{nx + 1, ny + 1, nz + 1}, {2 * nx, ny + 1, nz + 1}, {2 * ny, nz + 1, nx + 1},
+                                  {2 * nz, nx + 1, ny + 1}};
+        first[0] = 0;
+        base[0] = 0;
+        for (int c = 0; c < 4; ++c) {
+            for (int a = 0; a < 3; ++a) {
+                dim[c][a] = d[c][a];
+                nb[c][a] = (d[c][a] + 3) / 4;
+            }
+            first[c + 1] = first[c] + nb[c][0] * nb[c][1] * nb[c][2];
+            if (c < 3) base[c + 1] = base[c] + d[c][0] * d[c][1] * d[c][2];
+        }
+    }
+    __host__ __device__ uint32_t waves() const { return first[4]; }
+    // node of lane `lane` of brick-wave w, or ~0u past the grid's edge
+    __device__ uint32_t node(uint32_t w, uint32_t lane) const {
+        if (w >= first[4]) return ~0u;
+        int c = 0;
+        while (w >= first[c + 1]) ++c;
+        const uint32_t b = w - first[c];
+        const uint32_t i0 = (b % nb[c][0]) * 4 + (lane & 3u);
+        const uint32_t i1 = ((b / nb[c][0]) % nb[c][1]) * 4 + ((lane >> 2) & 3u);
+        const uint32_t i2 = (b / (nb[c][0] * nb[c][1])) * 4 + (lane >> 4);
+        if (i0 >= dim[c][0] || i1 >= dim[c][1] || i2 >= dim[c][2]) return ~0u;
+        return base[c] + (i2 * dim[c][1] + i1) * dim[c][0] + i0;
+    }
+};
+
+// Field 0 of a mesh body: one thread per map node, a wave per brick of nodes (NodeBricks). A block stages
+// kMeshTile triangles at a time in LDS (read as broadcasts); each wave skips a cluster of kMeshCluster triangles when
+// every lane's node is provably farther from the cluster's bounding sphere than from a triangle already seen (or than
+// the farthest point of some cluster, the bound before any triangle is seen). The winner is the least squared distance, ties to the smaller mesh index - the
+// first minimum of the reference's order - so the skipped clusters cannot change it: the test keeps a margin of
+// 4e-5 (|p - c| + r)^2 over the rounding of the squared distances, which stays below 1e-6 of that scale. The
+// winner's closest point and feature give the sign: negative where (p - closest) points against the feature's normal.
 __global__ void __launch_bounds__(kMeshTile) k_mesh_field0(MeshDev mesh, MapDev map, float sign, float tol,
                                                           float* nodes0) {
     __shared__ float4 sTri[4 * kMeshTile];
-    const uint32_t l = blockIdx.x * kMeshTile + threadIdx.x;
+    const NodeBricks bricks(map);
+    const uint32_t t = blockIdx.x * kMeshTile + threadIdx.x;
+    const uint32_t l = bricks.node(t / 64u, t % 64u);
     const bool live = l < map.nodeCount;
     const f3 p = live ? node_position(map, l) : mk3(0.0f, 0.0f, 0.0f);
+    const uint32_t clusters = (mesh.triCount + kMeshCluster - 1) / kMeshCluster;
+    // |p - c| and the margin of one cluster
+    auto reach = [&](uint32_t c, float& dc, float& r) {
+        const float4 sp = mesh.spheres[c];
+        const f3 d = p - mk3(sp.x, sp.y, sp.z);
+        dc = sqrtf(gdot(d, d));
+        r = sp.w;
+    };
+    float bound = INFINITY;  // no triangle is farther than (|p - c| + r)^2 from p, for every cluster c
+    for (uint32_t c = 0; c < clusters; ++c) {
+        float dc, r;
+        reach(c, dc, r);
+        const float u = dc + r;
+        bound = fminf(bound, u * u);
+    }
     float best = INFINITY;
-    uint32_t bestTri = 0;
+    uint32_t bestIdx = 0xffffffffu, bestPos = mesh.firstPos;
     for (uint32_t base = 0; base < mesh.triCount; base += kMeshTile) {
         __syncthreads();
         const uint32_t tl = base + threadIdx.x;
         if (tl < mesh.triCount)
             for (int q = 0; q < 4; ++q) sTri[4 * threadIdx.x + q] = mesh.tri[4 * (size_t)tl + q];
         __syncthreads();
-        const uint32_t n = min(kMeshTile, mesh.triCount - base);
-        for (uint32_t k = 0; k < n; ++k) {
-            const float d2 = tri_d2<false>(sTri[4 * k], sTri[4 * k + 1], sTri[4 * k + 2], sTri[4 * k + 3], p, nullptr,
-                                           nullptr);
-            if (d2 < best) {
-                best = d2;
-                bestTri = base + k;
+        for (uint32_t cb = 0; cb < kMeshTile && base + cb < mesh.triCount; cb += kMeshCluster) {
+            float dc, r;
+            reach((base + cb) / kMeshCluster, dc, r);
+            const float lb = dc - r, u = dc + r;
+            const float lim = fminf(best, bound) + 4e-5f * (u * u);
+            const bool skip = !live || (lb > 0.0f && lb * lb > lim);
+            if (__all(skip)) continue;  // wave-uniform
+            const uint32_t n = min(kMeshCluster, mesh.triCount - (base + cb));
+            for (uint32_t k = cb; k < cb + n; ++k) {
+                const float4 r3 = sTri[4 * k + 3];
+                const float d2 = tri_d2<false>(sTri[4 * k], sTri[4 * k + 1], sTri[4 * k + 2], r3, p, nullptr, nullptr);
+                const uint32_t idx = __float_as_uint(r3.w);
+                if (d2 < best || (d2 == best && idx < bestIdx)) {
+                    best = d2;
+                    bestIdx = idx;
+                    bestPos = base + k;
+                }
             }
         }
     }
     if (!live) return;
-    const float4* r = mesh.tri + 4 * (size_t)bestTri;
+    const float4* rp = mesh.tri + 4 * (size_t)bestPos;
     f3 cp;
     int feature;
-    float d = sqrtf(tri_d2<true>(r[0], r[1], r[2], r[3], p, &cp, &feature));
-    const float* nrm = mesh.normals + 21 * (size_t)bestTri + 3 * feature;
+    float d = sqrtf(tri_d2<true>(rp[0], rp[1], rp[2], rp[3], p, &cp, &feature));
+    const float* nrm = mesh.normals + 21 * (size_t)bestPos + 3 * feature;
     if (gdot(p - cp, mk3(nrm[0], nrm[1], nrm[2])) < 0.0f) d = -d;
     nodes0[l] = sign * (d - tol);
 }
 
 void launch_mesh_field0(hipStream_t st, const MeshDev& mesh, const MapDev& map, float sign, float tol, float* nodes0) {
-    hipLaunchKernelGGL(k_mesh_field0, dim3((map.nodeCount + kMeshTile - 1) / kMeshTile), dim3(kMeshTile), 0, st, mesh,
+    const uint32_t waves = NodeBricks(map).waves();
+    hipLaunchKernelGGL(k_mesh_field0, dim3((waves * 64u + kMeshTile - 1) / kMeshTile), dim3(kMeshTile), 0, st, mesh,
                        map, sign, tol, nodes0);
 }
 

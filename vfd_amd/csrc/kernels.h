@@ -83,15 +83,20 @@ struct MapDev {
     const float* nodes1;  // boundary volume
 };
 
-// A boundary mesh prepared for the distance kernel. Per triangle, four float4 records hold the point-independent
-// terms of the closest-point computation: {x0, a00}, {e0 = x1 - x0, a01}, {e1 = x2 - x0, a11}, {det, denom, 1/det,
-// 0} with a00 = e0.e0, a01 = e0.e1, a11 = e1.e1, det = |a00 a11 - a01^2|, denom = a00 - 2 a01 + a11; and 21 floats of
-// normals: the angle-weighted vertex normals of x0, x1, x2, the edge normals of x0x1, x1x2, x2x0 and the face normal
-// (MeshDistance.cpp:13-57, 255-284).
+// A boundary mesh prepared for the distance kernel. The triangles are stored in the Morton order of their centroids
+// (the mesh's own index travels with each record and decides ties). Per triangle, four float4 records hold the
+// point-independent terms of the closest-point computation: {x0, a00}, {e0 = x1 - x0, a01}, {e1 = x2 - x0, a11},
+// {det, denom, 1/det, index as bits} with a00 = e0.e0, a01 = e0.e1, a11 = e1.e1, det = |a00 a11 - a01^2|,
+// denom = a00 - 2 a01 + a11; and 21 floats of normals: the angle-weighted vertex normals of x0, x1, x2, the edge
+// normals of x0x1, x1x2, x2x0 and the face normal (MeshDistance.cpp:13-57, 255-284). spheres[c] = {centre, radius}
+// bounds the triangles [kMeshCluster c, kMeshCluster (c + 1)) of that order.
+constexpr uint32_t kMeshCluster = 64;
 struct MeshDev {
     const float4* tri;
     const float* normals;
+    const float4* spheres;
     uint32_t triCount;
+    uint32_t firstPos;  // stored position of the mesh's triangle 0
 };
 
 // The candidate lattice of ParticleSampler::SampleMeshVolume: xs/ys/zs hold the loop variables as the reference's

@@ -108,19 +108,69 @@ int prepare_mesh(const vfd_mesh& mesh, const float* transform, MeshPrep& out) {
             }
         }
 
+    // storage order: Morton code of the centroid over the bounding box (10 bits per axis), then the mesh index, so
+    // that each cluster of kMeshCluster stored triangles is compact in space
+    std::vector<uint64_t> key(nt);
+    for (uint32_t f = 0; f < nt; ++f) {
+        uint64_t code = 0;
+        for (int a = 0; a < 3; ++a) {
+            const double c = ((double)(&x[F[3 * f]].x)[a] + (&x[F[3 * f + 1]].x)[a] + (&x[F[3 * f + 2]].x)[a]) / 3.0;
+            const double ext = (double)hi[a] - lo[a];
+            double u = ext > 0.0 ? (c - lo[a]) / ext : 0.0;
+            u = std::isfinite(u) ? std::min(std::max(u, 0.0), 1.0) : 0.0;
+            const uint64_t q = (uint64_t)(u * 1023.0);
+            for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (3 * b + a);
+        }
+        key[f] = (code << 32) | f;
+    }
+    std::sort(key.begin(), key.end());
     out.tri.assign(16 * (size_t)nt, 0.0f);
     out.normals.assign(21 * (size_t)nt, 0.0f);
-    for (uint32_t f = 0; f < nt; ++f) {
+    const uint32_t clusters = (nt + kMeshCluster - 1) / kMeshCluster;
+    out.spheres.assign(4 * (size_t)clusters, 0.0f);
+    for (uint32_t c = 0; c < clusters; ++c) {
+        // bounding sphere of the cluster's vertices: centre of their box, radius rounded up so that every vertex
+        // (hence every triangle) lies inside it in exact arithmetic
+        double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        const uint32_t end = std::min(nt, (c + 1) * kMeshCluster);
+        for (uint32_t k = c * kMeshCluster; k < end; ++k)
+            for (int v = 0; v < 3; ++v)
+                for (int a = 0; a < 3; ++a) {
+                    const double w = (&x[F[3 * (key[k] & 0xffffffffu) + v]].x)[a];
+                    blo[a] = std::min(blo[a], w);
+                    bhi[a] = std::max(bhi[a], w);
+                }
+        const double cc[3] = {0.5 * (blo[0] + bhi[0]), 0.5 * (blo[1] + bhi[1]), 0.5 * (blo[2] + bhi[2])};
+        const float cf[3] = {(float)cc[0], (float)cc[1], (float)cc[2]};
+        double r2 = 0.0;
+        for (uint32_t k = c * kMeshCluster; k < end; ++k)
+            for (int v = 0; v < 3; ++v) {
+                double d2 = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    const double d = (double)(&x[F[3 * (key[k] & 0xffffffffu) + v]].x)[a] - (double)cf[a];
+                    d2 += d * d;
+                }
+                r2 = std::max(r2, d2);
+            }
+        const float rf = (float)(std::sqrt(r2) * (1.0 + 1e-6)) * (1.0f + 1e-6f);
+        const float sph[4] = {cf[0], cf[1], cf[2], std::isfinite(rf) ? rf : INFINITY};
+        std::memcpy(&out.spheres[4 * (size_t)c], sph, sizeof(sph));
+    }
+    for (uint32_t pos = 0; pos < nt; ++pos) {
+        const uint32_t f = (uint32_t)(key[pos] & 0xffffffffu);
+        if (f == 0) out.firstPos = pos;
         const V x0 = x[F[3 * f]], x1 = x[F[3 * f + 1]], x2 = x[F[3 * f + 2]];
         // the point-independent terms of PointTriangleDistanceSquared (MeshDistance.cpp:303-315)
         const V e0 = x1 - x0, e1 = x2 - x0;
         const float a00 = gdot(e0, e0), a01 = gdot(e0, e1), a11 = gdot(e1, e1);
         const float det = std::fabs(a00 * a11 - a01 * a01);
         const float denom = (a00 - 2.0f * a01) + a11;
-        float* r = &out.tri[16 * (size_t)f];
+        float* r = &out.tri[16 * (size_t)pos];
         const float invDet = 1.0f / det;
+        float index;
+        std::memcpy(&index, &f, sizeof(index));
         const float rec[16] = {x0.x, x0.y, x0.z, a00, e0.x, e0.y, e0.z, a01,
-                               e1.x, e1.y, e1.z, a11, det, denom, invDet, 0};
+                               e1.x, e1.y, e1.z, a11, det, denom, invDet, index};
         std::memcpy(r, rec, sizeof(rec));
         V nrm[7];
         for (int k = 0; k < 3; ++k) nrm[k] = vn[F[3 * f + k]];
@@ -130,7 +180,7 @@ int prepare_mesh(const vfd_mesh& mesh, const float* transform, MeshPrep& out) {
             nrm[3 + k] = o < 0 ? fn[f] : fn[f] + fn[(size_t)o / 3];
         }
         nrm[6] = fn[f];
-        std::memcpy(&out.normals[21 * (size_t)f], nrm, sizeof(nrm));
+        std::memcpy(&out.normals[21 * (size_t)pos], nrm, sizeof(nrm));
     }
     return VFD_OK;
 }
@@ -230,9 +280,10 @@ extern "C" int vfd_sample_mesh_volume(const vfd_mesh* mesh, const float* transfo
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } guard{st};
-    Dev<float> dTri, dNrm, dNodes, dX, dY, dZ, dOut;
+    Dev<float> dTri, dNrm, dSph, dNodes, dX, dY, dZ, dOut;
     Dev<uint32_t> dFlags, dOffs, dScratch;
     if ((e = dTri.alloc(prep.tri.size())) != hipSuccess || (e = dNrm.alloc(prep.normals.size())) != hipSuccess ||
+        (e = dSph.alloc(prep.spheres.size())) != hipSuccess ||
         (e = dNodes.alloc(map.nodeCount)) != hipSuccess || (e = dX.alloc(xs.size())) != hipSuccess ||
         (e = dY.alloc(ys.size())) != hipSuccess || (e = dZ.alloc(zs.size())) != hipSuccess ||
         (e = dFlags.alloc(n)) != hipSuccess || (e = dOffs.alloc(n)) != hipSuccess ||
@@ -243,11 +294,12 @@ extern "C" int vfd_sample_mesh_volume(const vfd_mesh* mesh, const float* transfo
         const void* h;
         size_t bytes;
     } up[] = {{dTri.p, prep.tri.data(), prep.tri.size() * 4}, {dNrm.p, prep.normals.data(), prep.normals.size() * 4},
+              {dSph.p, prep.spheres.data(), prep.spheres.size() * 4},
               {dX.p, xs.data(), xs.size() * 4}, {dY.p, ys.data(), ys.size() * 4}, {dZ.p, zs.data(), zs.size() * 4}};
     for (const auto& u : up)
         if ((e = hipMemcpyAsync(u.d, u.h, u.bytes, hipMemcpyHostToDevice, st)) != hipSuccess)
             return set_error(VFD_ERROR_HIP, std::string("sampler upload: ") + hipGetErrorString(e));
-    const MeshDev md{(const float4*)dTri.p, dNrm.p, nt};
+    const MeshDev md{(const float4*)dTri.p, dNrm.p, (const float4*)dSph.p, nt, prep.firstPos};
     launch_mesh_field0(st, md, map, inverted ? -1.0f : 1.0f, 0.0f, dNodes.p);
     map.nodes0 = dNodes.p;
     const SampleGrid g{dX.p, dY.p, dZ.p, (uint32_t)xs.size(), (uint32_t)ys.size(), (uint32_t)zs.size(), mode, radius,

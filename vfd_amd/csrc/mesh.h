@@ -8,8 +8,10 @@
 namespace vfd {
 
 struct MeshPrep {
-    std::vector<float> tri;      // 16 floats per triangle, the MeshDev::tri records
-    std::vector<float> normals;  // 21 floats per triangle, the MeshDev::normals records
+    std::vector<float> tri;      // 16 floats per triangle, the MeshDev::tri records (Morton order of the centroids)
+    std::vector<float> normals;  // 21 floats per triangle, the MeshDev::normals records (same order)
+    std::vector<float> spheres;  // 4 floats per cluster of kMeshCluster stored triangles: centre, radius
+    uint32_t firstPos = 0;       // stored position of the mesh's triangle 0
     float lo[3], hi[3];          // BoundingBox(vertices) of the transformed vertices (always holds the origin)
 };
 

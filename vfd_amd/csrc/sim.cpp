@@ -737,14 +737,17 @@ public:
             lo[a] = prep.lo[a] - grow;
             hi[a] = prep.hi[a] + grow;
         }
-        DevBuf<float> tri, nrm;
+        DevBuf<float> tri, nrm, sph;
         VFD_TRY(tri.alloc(prep.tri.size()));
         VFD_TRY(nrm.alloc(prep.normals.size()));
+        VFD_TRY(sph.alloc(prep.spheres.size()));
+        VFD_HIP(hipMemcpyAsync(sph.p, prep.spheres.data(), prep.spheres.size() * sizeof(float), hipMemcpyHostToDevice,
+                               stream));
         VFD_HIP(hipMemcpyAsync(tri.p, prep.tri.data(), prep.tri.size() * sizeof(float), hipMemcpyHostToDevice,
                                stream));
         VFD_HIP(hipMemcpyAsync(nrm.p, prep.normals.data(), prep.normals.size() * sizeof(float),
                                hipMemcpyHostToDevice, stream));
-        const MeshDev md{(const float4*)tri.p, nrm.p, mesh.TriangleCount};
+        const MeshDev md{(const float4*)tri.p, nrm.p, (const float4*)sph.p, mesh.TriangleCount, prep.firstPos};
         vfd_boundary_desc d{};
         d.Shape = -1;  // mesh
         d.Inverted = b.Inverted;

@@ -1,5 +1,5 @@
 """Triangle meshes: TriangleMesh (VFD/Source/Renderer/Mesh/TriangleMesh.cpp) as the DFSPH inputs use it - vertices
-and triangles, loaded from Wavefront OBJ files or built as a box.
+and triangles, loaded from Wavefront OBJ files or built as a box (or, for tests and benchmarks, an icosphere).
 
     mesh = TriangleMesh.load_obj("Cube.obj", scale=(1, 1, 1))
     body = RigidBodyDescription("mesh", mesh=mesh, transform=M, resolution=(20, 20, 20))
@@ -86,6 +86,39 @@ class TriangleMesh:
         t = [(0, 1, 2), (0, 2, 3), (4, 7, 6), (4, 6, 5), (0, 3, 7), (0, 7, 4), (1, 5, 6), (1, 6, 2),
              (0, 4, 5), (0, 5, 1), (3, 2, 6), (3, 6, 7)]
         return cls(np.asarray(v, dtype=np.float32), np.asarray(t, dtype=np.uint32))
+
+    @classmethod
+    def icosphere(cls, level, radius=1.0, bumps=0.0, seed=0):
+        """A closed, outward-wound sphere mesh: the icosahedron with every triangle split in four `level` times
+        (20 * 4^level triangles), vertices on the sphere of `radius`, then moved radially by up to `bumps` * radius
+        with a smooth random field (a test and benchmark mesh; no reference counterpart)."""
+        t = (1.0 + 5.0 ** 0.5) / 2.0
+        v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+        v = [np.asarray(p, np.float64) / np.linalg.norm(p) for p in v]
+        f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+        for _ in range(level):
+            mid = {}
+
+            def m(a, b):
+                k = (min(a, b), max(a, b))
+                if k not in mid:
+                    c = v[a] + v[b]
+                    v.append(c / np.linalg.norm(c))
+                    mid[k] = len(v) - 1
+                return mid[k]
+            f = [g for a, b, c in f for g in ((a, m(a, b), m(a, c)), (b, m(b, c), m(a, b)), (c, m(a, c), m(b, c)),
+                                              (m(a, b), m(b, c), m(a, c)))]
+        x = np.asarray(v)
+        if bumps:
+            rng = np.random.default_rng(seed)
+            k = rng.normal(size=(6, 3))
+            ph = rng.uniform(0, 2 * np.pi, 6)
+            h = sum(np.sin(3.0 * x @ k[i] + ph[i]) for i in range(6)) / 6.0
+            x = x * (1.0 + bumps * h)[:, None]
+        return cls((x * radius).astype(np.float32), np.asarray(f, np.uint32))
 
     def to_abi(self):
         """The vfd_mesh view; keep `self` alive while the struct is in use."""

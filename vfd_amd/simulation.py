@@ -103,14 +103,15 @@ def sample_mesh_volume(mesh, radius, resolution, inverted=False, sample_mode=0, 
     tr = None if transform is None else glm_transform(transform)
     res = np.ascontiguousarray(resolution, dtype=np.uint32)
     n = _abi.ctypes.c_uint32()
-    check(lib.vfd_sample_mesh_volume(_abi.ctypes.byref(m), ptr(tr), float(radius), ptr(res), int(bool(inverted)),
-                                     int(sample_mode), None, 0, _abi.ctypes.byref(n)), "vfd_sample_mesh_volume")
-    out = np.zeros((n.value, 3), dtype=np.float32)
-    if n.value:
-        check(lib.vfd_sample_mesh_volume(_abi.ctypes.byref(m), ptr(tr), float(radius), ptr(res),
-                                         int(bool(inverted)), int(sample_mode), ptr(out), n.value,
-                                         _abi.ctypes.byref(n)), "vfd_sample_mesh_volume")
-    return out
+    cap = 1 << 22  # one call for up to 4M samples; a larger object is sampled again with the exact room
+    while True:
+        out = np.zeros((cap, 3), dtype=np.float32)
+        check(lib.vfd_sample_mesh_volume(_abi.ctypes.byref(m), ptr(tr), float(radius), ptr(res), int(bool(inverted)),
+                                         int(sample_mode), ptr(out), cap, _abi.ctypes.byref(n)),
+              "vfd_sample_mesh_volume")
+        if n.value <= cap:
+            return out[:n.value].copy()
+        cap = n.value
 
 
 def DFSPHSimulationDescription(**fields):
