@@ -167,7 +167,9 @@ __device__ __forceinline__ float visc_k(const AdjK& a, float h, float rhoi, floa
 }
 // out = A^T w: out_i = w_i - sum_j r r^T (k_ij w_i - k_ji w_j)
 __global__ void __launch_bounds__(kTile) k_adj_visc_T(AdjK a, float h, const float* __restrict__ rho,
-                                                      const float4* __restrict__ w, float4* out) {
+                                                      const float4* __restrict__ w, float4* out,
+                                                      const BicgState* __restrict__ st) {
+    if (st && st->done) return;  // a device-side solve that has converged (block-uniform)
     const uint32_t i = pid();
     if (i >= a.n) return;
     const f3 xi = ld3(a.x[i]), wi = ld3(w[i]);
@@ -272,6 +274,86 @@ __global__ void k_adj_bicg_p(uint32_t n, const float4* __restrict__ r, float bet
     if (i < n) p[i] = st4(ld3(r[i]) + beta * (ld3(p[i]) - omega * ld3(v[i])));
 }
 
+// ---- BiCGSTAB with its scalars on the device --------------------------------------------------------------------
+// The solve of the transposed viscosity system runs a fixed sequence of launches per iteration; the scalars live in
+// one BicgState, every kernel returns at once when `done` is set, and the host reads `done` only every few
+// iterations. Dots are FP64 per-block partials summed in a fixed order, scalar updates are in double and the vector
+// updates round the scalars to float.
+__global__ void __launch_bounds__(kTile) k_bicg_dot(uint32_t n, const float4* __restrict__ p,
+                                                    const float4* __restrict__ q, const float4* __restrict__ p2,
+                                                    const float4* __restrict__ q2, double* part,
+                                                    const BicgState* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t i = pid();
+    double v[2] = {i < n ? (double)dot3(ld3(p[i]), ld3(q[i])) : 0.0,
+                   (p2 && i < n) ? (double)dot3(ld3(p2[i]), ld3(q2[i])) : 0.0};
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[gridDim.x + blockIdx.x] = v[1];
+    }
+}
+// the dots' block partials summed by the block in a fixed order (strided per thread, then block_sum's tree), then
+// the scalar update of stage `which`
+__global__ void __launch_bounds__(kTile) k_bicg_scalars(const double* __restrict__ part, uint32_t blocks,
+                                                        BicgState* st, int which) {
+    if (st->done) return;  // block-uniform
+    double v[2] = {0.0, 0.0};
+    for (uint32_t k = threadIdx.x; k < blocks; k += blockDim.x) {
+        v[0] += part[k];
+        if (which == kBicgTsTt) v[1] += part[blocks + k];
+    }
+    block_sum<2>(v);
+    if (threadIdx.x != 0) return;
+    const double a = v[0], b = v[1];
+    switch (which) {
+        case kBicgRho:
+            st->rhoNew = a;
+            st->beta = (a / st->rhoOld) * (st->alpha / st->omega);
+            break;
+        case kBicgRv:
+            st->alpha = st->rhoNew / a;
+            break;
+        case kBicgTsTt:
+            st->omega = b > 0.0 ? a / b : 0.0;
+            break;
+        default: {  // kBicgRr: end of an iteration
+            st->rr = a;
+            st->rhoOld = st->rhoNew;
+            st->it += 1;
+            if (st->omega == 0.0 || !(st->it < st->maxIt && a > st->tol2)) st->done = 1u;
+        }
+    }
+}
+// p = r + beta (p - omega v)
+__global__ void k_bicg_p(uint32_t n, const float4* __restrict__ r, const float4* __restrict__ v, float4* p,
+                         const BicgState* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t i = pid();
+    const float beta = (float)st->beta, omega = (float)st->omega;
+    if (i < n) p[i] = st4(ld3(r[i]) + beta * (ld3(p[i]) - omega * ld3(v[i])));
+}
+// s = r - alpha v
+__global__ void k_bicg_s(uint32_t n, const float4* __restrict__ r, const float4* __restrict__ v, float4* s,
+                         const BicgState* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t i = pid();
+    const float na = (float)-st->alpha;
+    if (i < n) s[i] = st4(ld3(r[i]) + na * ld3(v[i]));
+}
+// x += alpha p, then x += omega s; r = s - omega t
+__global__ void k_bicg_xr(uint32_t n, const float4* __restrict__ p, const float4* __restrict__ s,
+                          const float4* __restrict__ t, float4* x, float4* r, const BicgState* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t i = pid();
+    const float alpha = (float)st->alpha, omega = (float)st->omega, no = (float)-st->omega;
+    if (i < n) {
+        const f3 si = ld3(s[i]);
+        x[i] = st4((ld3(x[i]) + alpha * ld3(p[i])) + omega * si);
+        r[i] = st4(si + no * ld3(t[i]));
+    }
+}
+
 // cotangents between the step's slot order and the original particle order
 __global__ void k_adj_to_slots(uint32_t n, const uint32_t* __restrict__ id, const float* __restrict__ orig,
                                float4* slot) {
@@ -335,8 +417,9 @@ void adj_launch_axpy4(hipStream_t s, uint32_t n, float f, const float4* in, floa
 void adj_launch_fill(hipStream_t s, float* p, float v, uint32_t n) {
     hipLaunchKernelGGL(k_adj_fill, grid_of(n), dim3(kTile), 0, s, p, v, n);
 }
-void adj_launch_visc_T(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* w, float4* out) {
-    hipLaunchKernelGGL(k_adj_visc_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), dt, rho, w, out);
+void adj_launch_visc_T(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* w, float4* out,
+                       const BicgState* st) {
+    hipLaunchKernelGGL(k_adj_visc_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), dt, rho, w, out, st);
 }
 void adj_launch_visc_vjp(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* y,
                          const float4* lb, float4* lx, float* lrho, double* muPart) {
@@ -368,6 +451,25 @@ AdjK AdjKHost::dev() const {
 
 void adj_launch_contact(hipStream_t s, uint32_t n, uint32_t cap, uint32_t bodies, const float* vb, uint32_t* flag) {
     if (n) hipLaunchKernelGGL(k_adj_contact, grid_of(n), dim3(kTile), 0, s, n, cap, bodies, vb, flag);
+}
+
+void adj_launch_bicg_iteration(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const BicgVectors& v,
+                               double* part, BicgState* st) {
+    const uint32_t n = h.n;
+    const uint32_t blocks = grid_of(n).x;
+    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.rh, v.r, nullptr, nullptr, part, st);
+    hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(kTile), 0, s, part, blocks, st, (int)kBicgRho);
+    hipLaunchKernelGGL(k_bicg_p, grid_of(n), dim3(kTile), 0, s, n, v.r, v.v, v.p, st);
+    adj_launch_visc_T(s, h, dt, rho, v.p, v.v, st);
+    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.rh, v.v, nullptr, nullptr, part, st);
+    hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(kTile), 0, s, part, blocks, st, (int)kBicgRv);
+    hipLaunchKernelGGL(k_bicg_s, grid_of(n), dim3(kTile), 0, s, n, v.r, v.v, v.s, st);
+    adj_launch_visc_T(s, h, dt, rho, v.s, v.t, st);
+    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.t, v.s, v.t, v.t, part, st);
+    hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(kTile), 0, s, part, blocks, st, (int)kBicgTsTt);
+    hipLaunchKernelGGL(k_bicg_xr, grid_of(n), dim3(kTile), 0, s, n, v.p, v.s, v.t, v.x, v.r, st);
+    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.r, v.r, nullptr, nullptr, part, st);
+    hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(kTile), 0, s, part, blocks, st, (int)kBicgRr);
 }
 
 }  // namespace vfd

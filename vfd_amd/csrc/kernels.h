@@ -51,7 +51,20 @@ void adj_launch_div_init(hipStream_t s, uint32_t n, float h, const float* alpha,
 void adj_launch_scale4(hipStream_t s, uint32_t n, const float4* in, float f, float4* out);
 void adj_launch_axpy4(hipStream_t s, uint32_t n, float f, const float4* in, float4* out);
 void adj_launch_fill(hipStream_t s, float* p, float v, uint32_t n);
-void adj_launch_visc_T(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* w, float4* out);
+// scalars of a device-driven BiCGSTAB (adjoint viscosity solve); kernels given a state return at once once `done`
+struct BicgState {
+    double rhoOld, rhoNew, alpha, omega, beta, rr, tol2;  // tol2: continue while rr > tol2
+    uint32_t it, maxIt, done, pad;
+};
+enum { kBicgRho = 0, kBicgRv = 1, kBicgTsTt = 2, kBicgRr = 3 };
+struct BicgVectors {
+    float4 *x, *r, *rh, *p, *v, *s, *t;
+};
+void adj_launch_visc_T(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* w, float4* out,
+                       const BicgState* st = nullptr);
+// one BiCGSTAB iteration for A^T x = b on the device; part holds 2 * ceil(n / kTile) doubles
+void adj_launch_bicg_iteration(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const BicgVectors& v,
+                               double* part, BicgState* st);
 void adj_launch_visc_vjp(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* y,
                          const float4* lb, float4* lx, float* lrho, double* muPart);
 void adj_launch_factor_density(hipStream_t s, const AdjKHost& h, const float* alpha, const float* lalpha, float* lS,

@@ -1286,10 +1286,16 @@ public:
         for (double v : h) out += v;
         return VFD_OK;
     }
-    // BiCGSTAB for A^T x = b (the transposed viscosity system of one recorded step), relative residual 1e-6
+    // BiCGSTAB for A^T x = b (the transposed viscosity system of one recorded step), relative residual 1e-6. The
+    // iterations run on the device with their scalars in a BicgState; the host reads the state every
+    // kBicgCheck iterations.
+    DevBuf<BicgState> dBicg;
+    BicgState hBicg{};
+    static constexpr int kBicgCheck = 4;
     int adj_visc_solve(const AdjKHost& A, float h, const float* rho, const float4* b, float4* x) {
         const uint32_t n = A.n;
         for (auto* q : {&bR, &bRh, &bP, &bV, &bS, &bT}) VFD_TRY(q->alloc(n));
+        VFD_TRY(dBicg.alloc(1));
         VFD_HIP(hipMemcpyAsync(x, b, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
         adj_launch_visc_T(stream, A, h, rho, x, bT.p);
         VFD_HIP(hipMemcpyAsync(bR.p, b, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
@@ -1297,31 +1303,22 @@ public:
         VFD_HIP(hipMemcpyAsync(bRh.p, bR.p, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
         VFD_HIP(hipMemsetAsync(bP.p, 0, n * sizeof(float4), stream));
         VFD_HIP(hipMemsetAsync(bV.p, 0, n * sizeof(float4), stream));
-        double bn2 = 0, rr = 0, rhoOld = 1, alpha = 1, omega = 1;
+        double bn2 = 0, rr = 0;
         VFD_TRY(adj_dot(b, b, n, bn2));
         if (bn2 == 0) return VFD_OK;
         VFD_TRY(adj_dot(bR.p, bR.p, n, rr));
-        for (int it = 0; it < 300 && rr > 1e-12 * bn2; ++it) {
-            double rhoNew = 0, rv = 0, ts = 0, tt = 0;
-            VFD_TRY(adj_dot(bRh.p, bR.p, n, rhoNew));
-            const double beta = (rhoNew / rhoOld) * (alpha / omega);
-            adj_launch_bicg_p(stream, n, bR.p, (float)beta, (float)omega, bV.p, bP.p);
-            adj_launch_visc_T(stream, A, h, rho, bP.p, bV.p);
-            VFD_TRY(adj_dot(bRh.p, bV.p, n, rv));
-            alpha = rhoNew / rv;
-            VFD_HIP(hipMemcpyAsync(bS.p, bR.p, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-            adj_launch_axpy4(stream, n, (float)-alpha, bV.p, bS.p);
-            adj_launch_visc_T(stream, A, h, rho, bS.p, bT.p);
-            VFD_TRY(adj_dot(bT.p, bS.p, n, ts));
-            VFD_TRY(adj_dot(bT.p, bT.p, n, tt));
-            omega = tt > 0 ? ts / tt : 0.0;
-            adj_launch_axpy4(stream, n, (float)alpha, bP.p, x);
-            adj_launch_axpy4(stream, n, (float)omega, bS.p, x);
-            VFD_HIP(hipMemcpyAsync(bR.p, bS.p, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-            adj_launch_axpy4(stream, n, (float)-omega, bT.p, bR.p);
-            VFD_TRY(adj_dot(bR.p, bR.p, n, rr));
-            rhoOld = rhoNew;
-            if (omega == 0.0) break;
+        const double tol2 = 1e-12 * bn2;
+        if (!(rr > tol2)) return VFD_OK;
+        hBicg = BicgState{1.0, 0.0, 1.0, 1.0, 0.0, rr, tol2, 0u, 300u, 0u, 0u};
+        VFD_HIP(hipMemcpyAsync(dBicg.p, &hBicg, sizeof(BicgState), hipMemcpyHostToDevice, stream));
+        VFD_TRY(aPart.alloc(2 * (size_t)((n + kTile - 1) / kTile)));
+        const BicgVectors vec{x, bR.p, bRh.p, bP.p, bV.p, bS.p, bT.p};
+        for (uint32_t done = 0; !done;) {
+            for (int k = 0; k < kBicgCheck; ++k) adj_launch_bicg_iteration(stream, A, h, rho, vec, aPart.p, dBicg.p);
+            VFD_HIP(hipGetLastError());
+            VFD_HIP(hipMemcpyAsync(&hBicg, dBicg.p, sizeof(BicgState), hipMemcpyDeviceToHost, stream));
+            VFD_HIP(hipStreamSynchronize(stream));
+            done = hBicg.done || hBicg.it >= hBicg.maxIt;
         }
         return VFD_OK;
     }
