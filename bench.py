@@ -334,6 +334,46 @@ def mesh_inputs(map_res=48, sample_radius=0.005):
                         "sample_ms": round(float(np.median(ss[1:])), 2)}}
 
 
+def other_configs(steps=20, warmup=3):
+    """Throughput of the two smaller benchmark configurations: the 8k dam break (C1) and the 250k viscous column with
+    surface tension, a ground plane and walls (C2), each `steps` plain steps after `warmup`, then the same steps again
+    with per-kernel HIP-event timing for the kernels that only C2 runs (surface tension) and its viscosity solve."""
+    import numpy as np
+    from vfd_amd import DFSPHSimulation, scenes
+    out = {}
+    for name in ("dam_break_8k", "viscous_coil_250k"):
+        descs, fluids, bodies = scenes.CONFIGS[name]()
+        sim = DFSPHSimulation(descs[0])
+        sim.set_fluid(*fluids[0])
+        sim.SetRigidBodies(bodies[0])
+        sim.set_frame_capture(False)
+        sim.advance(warmup)
+        snap = sim.checkpoint()
+        n = sim.GetParticleCount()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.OnUpdate()
+        el = time.perf_counter() - t0
+        sim.restore(snap)
+        sim.set_profiling(True)
+        its = []
+        for _ in range(steps):
+            sim.OnUpdate()
+            d = sim.GetDebugInfo()
+            its.append((d.DivergenceSolverIterationCount, d.PressureSolverIterationCount,
+                        d.ViscositySolverIterationCount))
+        prof = sim.profile()
+        sim.set_profiling(False)
+        keep = ("surface_tension", "visc_precond", "cg_matvec", "density_factor", "div_iter", "build_lists")
+        out[name] = {"particles": n, "steps": steps, "warmup": warmup, "ms_per_step": round(1000 * el / steps, 3),
+                     "particle_updates_per_s": n * steps / el,
+                     "mean_iterations": [round(float(x), 2) for x in np.asarray(its, float).mean(0)],
+                     "kernels_mean_us": {k: round(1000 * ms / max(l, 1), 2) for k, (l, ms) in prof.items()
+                                         if k in keep}}
+        del sim
+    return out
+
+
 def copy_probe():
     import torch
     n = COPY_PROBE_BYTES // 4
@@ -360,6 +400,7 @@ def main():
     ap.add_argument("--no-frames", action="store_true", help="skip the frame-output measurement")
     ap.add_argument("--frame-steps", type=int, default=10)
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-input (map build, sampler) measurement")
+    ap.add_argument("--no-other-configs", action="store_true", help="skip the 8k and 250k configuration throughputs")
     ap.add_argument("--mesh-only", action="store_true", help="only measure the mesh inputs")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"))
     ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_DIR")
@@ -500,6 +541,9 @@ def main():
     if not args.no_mesh:
         print("[bench] mesh inputs: map build and sampler", file=sys.stderr, flush=True)
         out["mesh_inputs"] = mesh_inputs()
+    if not args.no_other_configs:
+        print("[bench] other configurations: 8k dam break, 250k viscous column", file=sys.stderr, flush=True)
+        out["other_configs"] = other_configs()
     print(json.dumps(out), flush=True)
 
 

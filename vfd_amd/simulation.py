@@ -103,7 +103,17 @@ def sample_mesh_volume(mesh, radius, resolution, inverted=False, sample_mode=0, 
     tr = None if transform is None else glm_transform(transform)
     res = np.ascontiguousarray(resolution, dtype=np.uint32)
     n = _abi.ctypes.c_uint32()
-    cap = 1 << 22  # one call for up to 4M samples; a larger object is sampled again with the exact room
+    # room for every lattice candidate of the bounding box (which holds the origin), so that one call suffices; a
+    # second call with the exact room covers an estimate that falls short
+    from .mesh import transform_vertices
+    v = mesh.vertices if transform is None else transform_vertices(transform, mesh.vertices)
+    lo = np.minimum(v.min(axis=0), 0.0) if len(v) else np.zeros(3)
+    hi = np.maximum(v.max(axis=0), 0.0) if len(v) else np.zeros(3)
+    step = 2.0 * float(radius) * np.array([np.sqrt(3.0) / 2.0 if sample_mode == 2 else 1.0,
+                                           np.sqrt(3.0) / 2.0 if sample_mode == 1 else
+                                           (np.sqrt(6.0) / 3.0 if sample_mode == 2 else 1.0), 1.0])
+    est = np.prod(np.floor((hi - lo) / step) + 2.0) if radius > 0 else 1.0
+    cap = int(min(max(est, 1.0), float(1 << 24)))
     while True:
         out = np.zeros((cap, 3), dtype=np.float32)
         check(lib.vfd_sample_mesh_volume(_abi.ctypes.byref(m), ptr(tr), float(radius), ptr(res), int(bool(inverted)),
