@@ -45,11 +45,22 @@ BODIES = {
     # the reference's AABB mesh (TriangleMesh.cpp:18-37): coplanar triangles make exact distance ties everywhere
     "aabb_box": lambda: RigidBodyDescription("mesh", mesh=TriangleMesh.box((-0.5, -0.3, -0.4), 1.0, 0.6, 0.8),
                                              resolution=(14, 10, 12)),
+    "degenerate": lambda: RigidBodyDescription("mesh", mesh=_degenerate(), transform=np.diag([0.6, 0.5, 0.7, 1.0]),
+                                               resolution=(10, 10, 10)),
     # 5,120 triangles in shuffled order: the culled, Morton-ordered scan must still pick the first closest triangle
     "icosphere_shuffled": lambda: RigidBodyDescription("mesh", mesh=_shuffled(TriangleMesh.icosphere(4, 0.5, 0.2, 3)),
                                                        transform=_rot_x(10, (0.05, 0.0, -0.02)),
                                                        resolution=(16, 16, 16)),
 }
+
+
+def _degenerate():
+    """Cube.obj with a zero-area triangle (three collinear points inside the box) and one face listed twice (its
+    three edges then join three faces): NaN normals and ambiguous half-edge pairs must match the oracle too."""
+    m = load("Cube.obj")
+    v = np.concatenate([m.vertices, np.array([[0.1, 0.2, 0.3], [0.2, 0.3, 0.4], [0.3, 0.4, 0.5]], np.float32)])
+    t = np.concatenate([m.triangles, np.array([[8, 9, 10]], np.uint32), m.triangles[4:5]])
+    return TriangleMesh(v, t)
 
 
 def _shuffled(m, seed=9):
