@@ -138,15 +138,29 @@ __device__ __forceinline__ void tile_partials(double (&v)[NV], double* partials,
     __syncthreads();  // the block-sum scratch is reused by the next tile of a persistent block
 }
 
-// Inside a finalize block (kTile threads) for scene s: sum of the scene's tile partials, valid in thread 0.
+// Inside a finalize block (kTile threads) for scene s: sum of the scene's tile partials, valid in thread 0. Each
+// thread adds the tiles threadIdx.x, threadIdx.x + kTile, ... in that order; the loads of eight of them are issued
+// together before they are added, so the latency is paid once per eight.
 template <int NV>
 __device__ __forceinline__ void scene_sum(const double* __restrict__ partials, uint32_t tileBegin, uint32_t tileEnd,
                                           double (&out)[NV]) {
+    constexpr uint32_t kBatch = 8;
 #pragma unroll
     for (int i = 0; i < NV; ++i) out[i] = 0.0;
-    for (uint32_t t = tileBegin + threadIdx.x; t < tileEnd; t += kTile)
+    for (uint32_t t0 = tileBegin + threadIdx.x; t0 < tileEnd; t0 += kBatch * kTile) {
+        double v[kBatch][NV];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) out[i] += partials[(size_t)t * kPartialSlots + i];
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            const uint32_t t = t0 + u * kTile;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) v[u][i] = t < tileEnd ? partials[(size_t)t * kPartialSlots + i] : 0.0;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u)
+            if (t0 + u * kTile < tileEnd)
+#pragma unroll
+                for (int i = 0; i < NV; ++i) out[i] += v[u][i];
+    }
     block_sum<NV>(out);
 }
 
