@@ -382,6 +382,8 @@ typedef struct vfd_tuning {
     uint32_t SolverCheckInterval; /* Jacobi / CG iterations enqueued between two convergence checks, >= 1 */
     uint32_t HaloLimit;           /* largest group halo staged in LDS (<= 3808); 0: every neighbour from HBM */
     uint32_t UseGraphs;           /* 1: each chunk of iterations is one launch of a captured HIP graph */
+    uint32_t PredictIterations;   /* 1: a solver's first chunk is one iteration short of what it ran in the
+                                     previous step, so that few empty iterations follow its convergence (default 0) */
 } vfd_tuning;
 int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out);
 int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* tuning);

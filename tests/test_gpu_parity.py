@@ -215,8 +215,8 @@ def test_determinism():
 
 @pytest.mark.parametrize("kind", ["dam", "all", "push"])
 def test_tuning_never_changes_results(kind):
-    """Halo staging (all groups, a limit that splits them, none), the convergence-check interval and the graph
-    capture of the solver loops change how the step runs, not what it computes: every combination ends bit-identical
+    """Halo staging (all groups, a limit that splits them, none), the convergence-check interval, the predicted first
+    chunk and the graph capture of the solver loops change how the step runs, not what it computes: every combination ends bit-identical
     after three steps (each step replays the captured iteration graphs several times, with eager launches between)."""
     scene = small_scene(kind)
     ref = make_gpu(*scene)
@@ -227,7 +227,8 @@ def test_tuning_never_changes_results(kind):
     split = max(1, d.MaxHaloSize - 1)
     assert ref.tuning()["UseGraphs"] == 1
     for knobs in [dict(HaloLimit=0), dict(HaloLimit=split), dict(SolverCheckInterval=1),
-                  dict(SolverCheckInterval=7, HaloLimit=0), dict(UseGraphs=0), dict(UseGraphs=0, SolverCheckInterval=2)]:
+                  dict(SolverCheckInterval=7, HaloLimit=0), dict(UseGraphs=0), dict(UseGraphs=0, SolverCheckInterval=2),
+                  dict(PredictIterations=0), dict(PredictIterations=0, SolverCheckInterval=1)]:
         sim = make_gpu(*scene)
         sim.set_tuning(**knobs)
         for _ in range(3):

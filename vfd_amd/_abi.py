@@ -181,7 +181,7 @@ class KernelStat(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """vfd_tuning: launch and solver-control knobs (no reference counterpart)."""
     _fields_ = [("SolverCheckInterval", ctypes.c_uint32), ("HaloLimit", ctypes.c_uint32),
-                ("UseGraphs", ctypes.c_uint32)]
+                ("UseGraphs", ctypes.c_uint32), ("PredictIterations", ctypes.c_uint32)]
 
 
 _P = ctypes.c_void_p

@@ -34,6 +34,9 @@
 #ifndef VFD_DEFAULT_CHECK_INTERVAL
 #define VFD_DEFAULT_CHECK_INTERVAL 4
 #endif
+#ifndef VFD_DEFAULT_PREDICT_ITERATIONS
+#define VFD_DEFAULT_PREDICT_ITERATIONS 0
+#endif
 #ifndef VFD_DEFAULT_HALO_LIMIT
 #define VFD_DEFAULT_HALO_LIMIT vfd::kHaloMax
 #endif
@@ -416,6 +419,13 @@ public:
     // (solver, cur); they are dropped whenever buffers are reallocated or the chunk length changes.
     bool useGraphs = VFD_DEFAULT_USE_GRAPHS != 0;
     hipGraphExec_t loopGraph[3][2] = {};
+    // Iterations each solver loop ran in the previous step (most over the scenes; Jacobi divergence, Jacobi
+    // pressure, CG). With predictIterations the loop's first chunk is one iteration shorter than that, so a solve
+    // that needs as many iterations as last time ends with one empty iteration in flight instead of a whole chunk.
+    // Off by default: on the 2M dam break it saved 0.1 ms of empty launches per step but only 0.2% of the step
+    // (with SolverCheckInterval 1 or 2), within run-to-run noise.
+    bool predictIterations = VFD_DEFAULT_PREDICT_ITERATIONS != 0;
+    uint32_t lastIterations[3] = {0, 0, 0};
     hipStream_t ownStream = nullptr;  // default stream (capture needs a stream other than the legacy null stream)
     // frame output: frames are packed in original order on the simulation's stream into one of two device staging
     // buffers and copied to pinned host memory on copyStream, so that the copy overlaps the next steps
@@ -1081,7 +1091,9 @@ public:
             return VFD_OK;
         };
         int cur = 0;
-        VFD_TRY(issue(std::max<uint32_t>(first, 1), cur));
+        uint32_t n0 = std::max<uint32_t>(first, 1);
+        if (predictIterations && lastIterations[which] > 1) n0 = std::max(n0, lastIterations[which] - 1);
+        VFD_TRY(issue(n0, cur));
         for (;;) {
             VFD_TRY(issue(pollChunk, cur ^ 1));
             VFD_HIP(hipEventSynchronize(pollEv[cur]));
@@ -1453,6 +1465,14 @@ public:
 
     // solver iteration counts and errors of the last (possibly partial) step, DFSPHDebugInfo semantics
     void solver_debug() {
+        uint32_t ran[3] = {0, 0, 0};
+        for (Scene* s : sc) {
+            if (!s->count) continue;
+            ran[0] = std::max(ran[0], s->st.div.it);
+            ran[1] = std::max(ran[1], s->st.pres.it);
+            ran[2] = std::max(ran[2], s->st.cg.loops);
+        }
+        std::memcpy(lastIterations, ran, sizeof(ran));
         for (Scene* s : sc) {
             vfd_debug_info& d = s->dbg;
             d.DivergenceSolverIterationCount = s->desc.EnableDivergenceSolverError ? s->st.div.it : 0;
@@ -1943,6 +1963,7 @@ int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out) {
     out->SolverCheckInterval = sim->impl.pollChunk;
     out->HaloLimit = sim->impl.haloLimit;
     out->UseGraphs = sim->impl.useGraphs ? 1u : 0u;
+    out->PredictIterations = sim->impl.predictIterations ? 1u : 0u;
     return VFD_OK;
 }
 int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* t) {
@@ -1956,6 +1977,7 @@ int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* t) {
     sim->impl.pollChunk = t->SolverCheckInterval;
     sim->impl.haloLimit = t->HaloLimit;
     sim->impl.useGraphs = t->UseGraphs != 0;
+    sim->impl.predictIterations = t->PredictIterations != 0;
     return VFD_OK;
 }
 int vfd_get_kernel_table(vfd_sim* sim, float* w, float* gradient_w, float* scalars) {
