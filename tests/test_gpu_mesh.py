@@ -13,7 +13,7 @@ from oracle import oracle as orc
 from scene_util import make_gpu, make_oracle
 from vfd_amd import scenes
 from vfd_amd.mesh import TriangleMesh
-from vfd_amd.simulation import (DFSPHSimulationDescription, FluidObject, RigidBodyDescription,
+from vfd_amd.simulation import (DFSPHSimulation, DFSPHSimulationDescription, FluidObject, RigidBodyDescription,
                                 sample_mesh_volume)
 
 pytestmark = pytest.mark.gpu
@@ -110,7 +110,9 @@ def test_example_scene_against_the_oracle():
     x = np.concatenate([f.positions for f in fluids])
     v = np.concatenate([f.velocities for f in fluids])
     scene = ([desc], [(x, v)], [bodies])
-    gpu, o = make_gpu(*scene), make_oracle(*scene)
+    o = make_oracle(*scene)
+    gpu = DFSPHSimulation.from_scene_file(os.path.join(HERE, "golden", "scene_default.json"), MODELS)
+    assert np.array_equal(gpu.field("position"), x)
     r = desc.ParticleRadius
     for f in (0, 1):
         assert np.array_equal(gpu.boundary_map(0, f), o.boundary_map(0, f)[0].astype(np.float32))

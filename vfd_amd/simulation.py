@@ -170,6 +170,19 @@ class DFSPHParticleFrame:
 class DFSPHSimulation:
     """DFSPHSimulation (DFSPHSimulator.h:10-56) backed by the HIP library. `scene` selects one scene of a batch."""
 
+    @classmethod
+    def from_scene_file(cls, path, resource_root):
+        """A VFD scene file baked as the editor's "Bake Objects" does (Editor/Panels/ComponentPanel.cpp:608-656): the
+        simulation from the file's description, every fluid object sampled on the GPU (SetFluidObjects), then every
+        mesh rigid body (SetRigidBodies). Mesh paths are resolved under `resource_root` (see
+        vfd_amd.scenes.load_scene_file)."""
+        from .scenes import load_scene_file
+        desc, fluid_descs, bodies = load_scene_file(path, resource_root)
+        sim = cls(desc)
+        sim.SetFluidObjects([FluidObject.sample(fd, desc.ParticleRadius) for fd in fluid_descs])
+        sim.SetRigidBodies(bodies)
+        return sim
+
     def __init__(self, desc=None, _handle=None, _scene=0, _owner=None):
         self._lib = _abi.load_library()
         self._scene = _scene
