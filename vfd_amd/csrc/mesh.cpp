@@ -27,6 +27,7 @@ namespace {
 
 struct V {
     float x, y, z;
+    float operator[](int a) const { return a == 0 ? x : (a == 1 ? y : z); }
 };
 V operator+(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 V operator-(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -114,7 +115,7 @@ int prepare_mesh(const vfd_mesh& mesh, const float* transform, MeshPrep& out) {
     for (uint32_t f = 0; f < nt; ++f) {
         uint64_t code = 0;
         for (int a = 0; a < 3; ++a) {
-            const double c = ((double)(&x[F[3 * f]].x)[a] + (&x[F[3 * f + 1]].x)[a] + (&x[F[3 * f + 2]].x)[a]) / 3.0;
+            const double c = ((double)x[F[3 * f]][a] + x[F[3 * f + 1]][a] + x[F[3 * f + 2]][a]) / 3.0;
             const double ext = (double)hi[a] - lo[a];
             double u = ext > 0.0 ? (c - lo[a]) / ext : 0.0;
             u = std::isfinite(u) ? std::min(std::max(u, 0.0), 1.0) : 0.0;
@@ -136,7 +137,7 @@ int prepare_mesh(const vfd_mesh& mesh, const float* transform, MeshPrep& out) {
         for (uint32_t k = c * kMeshCluster; k < end; ++k)
             for (int v = 0; v < 3; ++v)
                 for (int a = 0; a < 3; ++a) {
-                    const double w = (&x[F[3 * (key[k] & 0xffffffffu) + v]].x)[a];
+                    const double w = x[F[3 * (key[k] & 0xffffffffu) + v]][a];
                     blo[a] = std::min(blo[a], w);
                     bhi[a] = std::max(bhi[a], w);
                 }
@@ -147,7 +148,7 @@ int prepare_mesh(const vfd_mesh& mesh, const float* transform, MeshPrep& out) {
             for (int v = 0; v < 3; ++v) {
                 double d2 = 0.0;
                 for (int a = 0; a < 3; ++a) {
-                    const double d = (double)(&x[F[3 * (key[k] & 0xffffffffu) + v]].x)[a] - (double)cf[a];
+                    const double d = (double)x[F[3 * (key[k] & 0xffffffffu) + v]][a] - (double)cf[a];
                     d2 += d * d;
                 }
                 r2 = std::max(r2, d2);
