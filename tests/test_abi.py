@@ -48,7 +48,8 @@ def test_null_handles_are_rejected(vfd_lib):
 
 def test_struct_layout_matches_header(tmp_path):
     structs = {"vfd_desc": _abi.Desc, "vfd_info": _abi.Info, "vfd_debug_info": _abi.DebugInfo,
-               "vfd_boundary_desc": _abi.BoundaryDesc, "vfd_kernel_stat": _abi.KernelStat}
+               "vfd_boundary_desc": _abi.BoundaryDesc, "vfd_kernel_stat": _abi.KernelStat,
+               "vfd_mesh": _abi.Mesh, "vfd_mesh_boundary_desc": _abi.MeshBoundaryDesc, "vfd_tuning": _abi.Tuning}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vfd.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
