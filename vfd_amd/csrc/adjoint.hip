@@ -354,6 +354,17 @@ __global__ void k_bicg_xr(uint32_t n, const float4* __restrict__ p, const float4
     }
 }
 
+// three copies in one launch (a recorded Jacobi iteration)
+__global__ void k_adj_copy3(uint32_t n, const float* __restrict__ a, float* da, const float4* __restrict__ b,
+                            float4* db, const float* __restrict__ c, float* dc) {
+    const uint32_t i = pid();
+    if (i < n) {
+        da[i] = a[i];
+        db[i] = b[i];
+        dc[i] = c[i];
+    }
+}
+
 // cotangents between the step's slot order and the original particle order
 __global__ void k_adj_to_slots(uint32_t n, const uint32_t* __restrict__ id, const float* __restrict__ orig,
                                float4* slot) {
@@ -437,6 +448,10 @@ void adj_launch_dot(hipStream_t s, uint32_t n, const float4* p, const float4* q,
 void adj_launch_bicg_p(hipStream_t s, uint32_t n, const float4* r, float beta, float omega, const float4* v,
                        float4* p) {
     hipLaunchKernelGGL(k_adj_bicg_p, grid_of(n), dim3(kTile), 0, s, n, r, beta, omega, v, p);
+}
+void adj_launch_copy3(hipStream_t s, uint32_t n, const float* a, float* da, const float4* b, float4* db,
+                      const float* c, float* dc) {
+    if (n) hipLaunchKernelGGL(k_adj_copy3, grid_of(n), dim3(kTile), 0, s, n, a, da, b, db, c, dc);
 }
 void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float4* slot) {
     hipLaunchKernelGGL(k_adj_to_slots, grid_of(n), dim3(kTile), 0, s, n, id, orig, slot);

@@ -72,6 +72,8 @@ void adj_launch_factor_density(hipStream_t s, const AdjKHost& h, const float* al
 void adj_launch_dot(hipStream_t s, uint32_t n, const float4* p, const float4* q, double* part);
 void adj_launch_bicg_p(hipStream_t s, uint32_t n, const float4* r, float beta, float omega, const float4* v,
                        float4* p);
+void adj_launch_copy3(hipStream_t s, uint32_t n, const float* a, float* da, const float4* b, float4* db,
+                      const float* c, float* dc);
 void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float4* slot);
 void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const float4* slot, float* orig);
 void adj_launch_contact(hipStream_t s, uint32_t n, uint32_t cap, uint32_t bodies, const float* vb, uint32_t* flag);

@@ -406,6 +406,19 @@ public:
         while (q.size() <= k) q.emplace_back();
         return tape_copy(q[k], src, count);
     }
+    // the end of Jacobi iteration k in one launch: a^k (from the acceleration pass), the residual of iteration k and
+    // kappa^(k+1) (the updated kappa, the next iteration's start)
+    int tape_iteration(std::deque<DevBuf<float>>& kq, std::deque<DevBuf<float4>>& aq, std::deque<DevBuf<float>>& rq,
+                       uint32_t k, uint32_t n, const float* kap, const float4* a, const float* r) {
+        while (kq.size() <= k + 1) kq.emplace_back();
+        while (aq.size() <= k) aq.emplace_back();
+        while (rq.size() <= k) rq.emplace_back();
+        VFD_TRY(kq[k + 1].alloc(n));
+        VFD_TRY(aq[k].alloc(n));
+        VFD_TRY(rq[k].alloc(n));
+        adj_launch_copy3(stream, n, kap, kq[k + 1].p, a, aq[k].p, r, rq[k].p);
+        return VFD_OK;
+    }
     uint32_t haloLimit = VFD_DEFAULT_HALO_LIMIT;  // largest halo staged in LDS (tuning)
     uint32_t cellCap = 0;
     hipEvent_t ev[8] = {};
@@ -1161,12 +1174,11 @@ public:
         if (tp) VFD_TRY(tape_copy(tp->D, densAdv.p, tp->n));
         VFD_TRY(poll_loop(
             [&](uint32_t k) {
-                if (tp) (void)tape_slot(tp->kv, k, kappaV.p, tp->n);
+                if (tp && k == 0) (void)tape_slot(tp->kv, 0, kappaV.p, tp->n);
                 (void)timed("div_accel", [&] { launch_div_accel(L, K, F); }, 0, k);
-                if (tp) (void)tape_slot(tp->av, k, ap.p, tp->n);
                 (void)timed("div_iter", [&] { launch_div_iter(L, K, F); }, 0, k);
                 (void)timed("jacobi_finish", [&] { launch_jacobi_finish(L, F, 0); }, 0, k);
-                if (tp) (void)tape_slot(tp->rv, k, residuum.p, tp->n);
+                if (tp) (void)tape_iteration(tp->kv, tp->av, tp->rv, k, tp->n, kappaV.p, ap.p, residuum.p);
             },
             minDiv, 0));
         if (tp) VFD_TRY(tape_copy(tp->kvEnd, kappaV.p, tp->n));
@@ -1212,12 +1224,11 @@ public:
         if (tp) VFD_TRY(tape_copy(tp->rhoadv, densAdv.p, tp->n));
         VFD_TRY(poll_loop(
             [&](uint32_t k) {
-                if (tp) (void)tape_slot(tp->kp, k, kappa.p, tp->n);
+                if (tp && k == 0) (void)tape_slot(tp->kp, 0, kappa.p, tp->n);
                 (void)timed("pres_accel", [&] { launch_pres_accel(L, K, F); }, 1, k);
-                if (tp) (void)tape_slot(tp->ap, k, ap.p, tp->n);
                 (void)timed("pres_iter", [&] { launch_pres_iter(L, K, F); }, 1, k);
                 (void)timed("jacobi_finish", [&] { launch_jacobi_finish(L, F, 1); }, 1, k);
-                if (tp) (void)tape_slot(tp->rp, k, residuum.p, tp->n);
+                if (tp) (void)tape_iteration(tp->kp, tp->ap, tp->rp, k, tp->n, kappa.p, ap.p, residuum.p);
             },
             minPres, 1));
         if (tp) VFD_TRY(tape_copy(tp->kpEnd, kappa.p, tp->n));
