@@ -114,6 +114,24 @@ def dam_break_2m(seed=3):
     return [desc], [(x, v)], [[tank, sphere]]
 
 
+def dam_break_16m(seed=6):
+    """The 2M dam break at eight times the particles (a scale point, not a benchmark configuration of the reference):
+    16,000,000 particles, a 400 x 200 x 200 lattice at r = 0.002 in the same 2 x 1 x 1 m tank with the same sphere."""
+    rng = np.random.default_rng(seed)
+    r = 0.002
+    desc = DFSPHSimulationDescription(
+        ParticleRadius=r, Viscosity=0.01, BoundaryViscosity=0.01, MaxViscositySolverError=0.1,
+        EnableSurfaceTensionSolver=0, MaxDivergenceSolverError=10.0, MaxPressureSolverError=10.0,
+        MinDivergenceSolverIterations=2, MinPressureSolverIterations=2, FrameCount=100)
+    x = lattice_block((4 * r, 4 * r, 4 * r), (400, 200, 200), 2 * r, 0.01 * r, rng)
+    v = np.zeros_like(x)
+    tank = RigidBodyDescription("box", center=(1.0, 0.5, 0.5), half_extents=(1.0, 0.5, 0.5), inverted=True,
+                                resolution=(80, 48, 48))
+    sphere = RigidBodyDescription("sphere", center=(1.8, 0.15, 0.5), radius=0.15, padding=2 * r,
+                                  resolution=(32, 32, 32))
+    return [desc], [(x, v)], [[tank, sphere]]
+
+
 def batched_uneven(seed=4, scenes=64, r=0.01):
     """Config 4: 64 dam breaks with N_s = round(exp U(ln 5e3, ln 6e4)) and mu_s = exp U(ln 0.1, ln 100), one seed;
     every scene has its own box."""
@@ -181,6 +199,7 @@ CONFIGS = {
     "dam_break_8k": dam_break_8k,
     "viscous_coil_250k": viscous_coil_250k,
     "dam_break_2m": dam_break_2m,
+    "dam_break_16m": dam_break_16m,
     "batched_uneven_64": batched_uneven,
     "falling_viscous_block_100k": falling_viscous_block,
 }
