@@ -433,25 +433,33 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
     volatile uint32_t* vkey = bkey;
     volatile uint32_t* vmask = bmask;
     volatile uint32_t* vused = &used;
-    for (uint32_t j = 0; j < nr; ++j) {
-        if (j >= n || *vused > kBucketLimit) continue;
-        const uint32_t t = nbrList[(size_t)j * cap + i];
-        const uint32_t b = t >> 5, bit = 1u << (t & 31u);
-        uint32_t h = bucket_hash(b);
-        for (;;) {
-            uint32_t cur = vkey[h];
-            if (cur == kHashEmpty) {
-                cur = atomicCAS(&bkey[h], kHashEmpty, b);
+    // the list entries are loaded kHaloBatch at a time (their latencies overlap), then inserted in list order
+    constexpr uint32_t kHaloBatch = 8;
+    for (uint32_t j0 = 0; j0 < nr; j0 += kHaloBatch) {
+        uint32_t tb[kHaloBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kHaloBatch; ++u) tb[u] = j0 + u < n ? nbrList[(size_t)(j0 + u) * cap + i] : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < kHaloBatch; ++u) {
+            if (j0 + u >= n || *vused > kBucketLimit) continue;
+            const uint32_t t = tb[u];
+            const uint32_t b = t >> 5, bit = 1u << (t & 31u);
+            uint32_t h = bucket_hash(b);
+            for (;;) {
+                uint32_t cur = vkey[h];
                 if (cur == kHashEmpty) {
-                    atomicAdd(&used, 1u);
-                    cur = b;
+                    cur = atomicCAS(&bkey[h], kHashEmpty, b);
+                    if (cur == kHashEmpty) {
+                        atomicAdd(&used, 1u);
+                        cur = b;
+                    }
                 }
+                if (cur == b) {
+                    if (!(vmask[h] & bit)) atomicOr(&bmask[h], bit);
+                    break;
+                }
+                h = (h + 1) & (kBucketSlots - 1);
             }
-            if (cur == b) {
-                if (!(vmask[h] & bit)) atomicOr(&bmask[h], bit);
-                break;
-            }
-            h = (h + 1) & (kBucketSlots - 1);
         }
     }
     __syncthreads();
@@ -500,14 +508,19 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
     }
     __syncthreads();
     // staged entries: the halo number of each neighbour, in list order
-    for (uint32_t j = 0; j < nr; ++j) {
-        if (j >= n) continue;
-        const size_t e = (size_t)j * cap + i;
-        const uint32_t t = nbrList[e];
-        const uint32_t b = t >> 5, below = (1u << (t & 31u)) - 1u;
-        uint32_t h = bucket_hash(b);
-        while (bkey[h] != b) h = (h + 1) & (kBucketSlots - 1);
-        nbrLocal[e] = (uint16_t)(bbase[h] + __popc(bmask[h] & below));
+    for (uint32_t j0 = 0; j0 < nr; j0 += kHaloBatch) {
+        uint32_t tb[kHaloBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kHaloBatch; ++u) tb[u] = j0 + u < n ? nbrList[(size_t)(j0 + u) * cap + i] : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < kHaloBatch; ++u) {
+            if (j0 + u >= n) continue;
+            const uint32_t t = tb[u];
+            const uint32_t b = t >> 5, below = (1u << (t & 31u)) - 1u;
+            uint32_t h = bucket_hash(b);
+            while (bkey[h] != b) h = (h + 1) & (kBucketSlots - 1);
+            nbrLocal[(size_t)(j0 + u) * cap + i] = (uint16_t)(bbase[h] + __popc(bmask[h] & below));
+        }
     }
 }
 
