@@ -837,8 +837,17 @@ __global__ void __launch_bounds__(kTile) k_st_classify(const SceneParams* __rest
         F.kMC[c.i] = 1.0f / R;
         return;
     }
+    // neighbour positions are loaded kStBatch at a time so that their latencies overlap; sums stay in list order
+    constexpr uint32_t kStBatch = 8;
     f3 com = mk3(0.0f, 0.0f, 0.0f);
-    for (uint32_t j = 0; j < n; ++j) com = com + (ld3(F.pos[nbr(F, cap, c.i, j)]) - xi);
+    for (uint32_t j0 = 0; j0 < n; j0 += kStBatch) {
+        f3 xb[kStBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kStBatch; ++u) xb[u] = ld3(F.pos[nbr(F, cap, c.i, min(j0 + u, n - 1u))]);
+#pragma unroll
+        for (uint32_t u = 0; u < kStBatch; ++u)
+            if (j0 + u < n) com = com + (xb[u] - xi);
+    }
     com = com / R;
     const float ci = length(com) / (float)n;
     if (!((float)n <= p.slope * ci + p.constant + 0.0f)) return;
@@ -853,12 +862,13 @@ __global__ void __launch_bounds__(kTile) k_st_classify(const SceneParams* __rest
         const f3 pt = R * mk3(F.halton[i3 % kHaltonFloats], F.halton[(i3 + 1u) % kHaltonFloats],
                               F.halton[(i3 + 2u) % kHaltonFloats]);
         bool covered = false;
-        for (uint32_t j = 0; j < n; ++j) {
-            const f3 pd = ld3(F.pos[nbr(F, cap, c.i, j)]) - xi;
-            if (length2(pt - pd) <= lim) {
-                covered = true;
-                break;
-            }
+        for (uint32_t j0 = 0; j0 < n && !covered; j0 += kStBatch) {
+            f3 xb[kStBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kStBatch; ++u) xb[u] = ld3(F.pos[nbr(F, cap, c.i, min(j0 + u, n - 1u))]);
+#pragma unroll
+            for (uint32_t u = 0; u < kStBatch; ++u)
+                if (j0 + u < n && length2(pt - (xb[u] - xi)) <= lim) covered = true;
         }
         if (covered) continue;
         nsum = nsum + pt;
