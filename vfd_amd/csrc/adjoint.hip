@@ -56,6 +56,9 @@ __device__ __forceinline__ uint32_t adj_nbr(const AdjK& a, uint32_t i, uint32_t 
     return a.list[(size_t)j * a.cap + i];
 }
 __device__ __forceinline__ uint32_t pid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+// The reverse neighbour loops fetch kAdjBatch list entries, then those neighbours' data, before computing any of
+// them, so that the dependent loads overlap; the terms are still added in list order.
+constexpr uint32_t kAdjBatch = 4;
 
 // lk_i -= V sum_j (la_i - la_j) . gradW_ij            (A^T la = -D(la))
 // lx_i += sum_j vjp(r_ij, -V (k_i + k_j)(la_i - la_j))  (positions through A(k))
@@ -67,13 +70,26 @@ __global__ void __launch_bounds__(kTile) k_adj_accel_T(AdjK a, const float* __re
     const float ki = kap[i];
     float d = 0.0f;
     f3 gx = mk3(0.0f, 0.0f, 0.0f);
-    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
-        const uint32_t j = adj_nbr(a, i, jj);
-        const f3 r = xi - ld3(a.x[j]);
-        const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
-        const f3 dl = lai - ld3(la[j]);
-        d += dot3(dl, g * r);
-        gx = gx + vjp_grad(a, r, rl, g, (-a.V * (ki + kap[j])) * dl);
+    const uint32_t cnt = a.count[i];
+    for (uint32_t j0 = 0; j0 < cnt; j0 += kAdjBatch) {
+        f3 xj[kAdjBatch], laj[kAdjBatch];
+        float kj[kAdjBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kAdjBatch; ++u) {
+            const uint32_t j = adj_nbr(a, i, min(j0 + u, cnt - 1u));
+            xj[u] = ld3(a.x[j]);
+            laj[u] = ld3(la[j]);
+            kj[u] = kap[j];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kAdjBatch; ++u) {
+            if (j0 + u >= cnt) break;
+            const f3 r = xi - xj[u];
+            const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
+            const f3 dl = lai - laj[u];
+            d += dot3(dl, g * r);
+            gx = gx + vjp_grad(a, r, rl, g, (-a.V * (ki + kj[u])) * dl);
+        }
     }
     lk[i] -= a.V * d;
     lx[i] = st4(ld3(lx[i]) + gx);
@@ -88,13 +104,26 @@ __global__ void __launch_bounds__(kTile) k_adj_div_T(AdjK a, const float* __rest
     const f3 xi = ld3(a.x[i]), ui = ld3(u[i]);
     const float lfi = lf[i];
     f3 s = mk3(0.0f, 0.0f, 0.0f), gx = mk3(0.0f, 0.0f, 0.0f);
-    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
-        const uint32_t j = adj_nbr(a, i, jj);
-        const f3 r = xi - ld3(a.x[j]);
-        const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
-        const float l2 = lfi + lf[j];
-        s = s + (l2 * g) * r;
-        gx = gx + vjp_grad(a, r, rl, g, (a.V * l2) * (ui - ld3(u[j])));
+    const uint32_t cnt = a.count[i];
+    for (uint32_t j0 = 0; j0 < cnt; j0 += kAdjBatch) {
+        f3 xj[kAdjBatch], uj[kAdjBatch];
+        float lfj[kAdjBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) {
+            const uint32_t j = adj_nbr(a, i, min(j0 + q, cnt - 1u));
+            xj[q] = ld3(a.x[j]);
+            uj[q] = ld3(u[j]);
+            lfj[q] = lf[j];
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) {
+            if (j0 + q >= cnt) break;
+            const f3 r = xi - xj[q];
+            const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
+            const float l2 = lfi + lfj[q];
+            s = s + (l2 * g) * r;
+            gx = gx + vjp_grad(a, r, rl, g, (a.V * l2) * (ui - uj[q]));
+        }
     }
     lu[i] = st4(ld3(lu[i]) + a.V * s);
     lx[i] = st4(ld3(lx[i]) + gx);
@@ -173,13 +202,27 @@ __global__ void __launch_bounds__(kTile) k_adj_visc_T(AdjK a, float h, const flo
     const uint32_t i = pid();
     if (i >= a.n) return;
     const f3 xi = ld3(a.x[i]), wi = ld3(w[i]);
+    const float rhoi = rho[i];
     f3 o = wi;
-    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
-        const uint32_t j = adj_nbr(a, i, jj);
-        const f3 r = xi - ld3(a.x[j]);
-        const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
-        const f3 u = visc_k(a, h, rho[i], rho[j], rl, g) * wi - visc_k(a, h, rho[j], rho[i], rl, g) * ld3(w[j]);
-        o = o - dot3(r, u) * r;
+    const uint32_t cnt = a.count[i];
+    for (uint32_t j0 = 0; j0 < cnt; j0 += kAdjBatch) {
+        f3 xj[kAdjBatch], wj[kAdjBatch];
+        float rj[kAdjBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) {
+            const uint32_t j = adj_nbr(a, i, min(j0 + q, cnt - 1u));
+            xj[q] = ld3(a.x[j]);
+            wj[q] = ld3(w[j]);
+            rj[q] = rho[j];
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kAdjBatch; ++q) {
+            if (j0 + q >= cnt) break;
+            const f3 r = xi - xj[q];
+            const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
+            const f3 u = visc_k(a, h, rhoi, rj[q], rl, g) * wi - visc_k(a, h, rj[q], rhoi, rl, g) * wj[q];
+            o = o - dot3(r, u) * r;
+        }
     }
     out[i] = st4(o);
 }
