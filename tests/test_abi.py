@@ -90,3 +90,9 @@ def test_tuning_struct_and_defaults_without_gpu(vfd_lib):
     t = _abi.Tuning()
     assert vfd_lib.vfd_get_tuning(None, ctypes.byref(t)) == 1  # NULL handle rejected before any GPU call
     assert b"NULL" in vfd_lib.vfd_last_error()
+
+
+def test_abi_version_agrees_between_header_library_and_binding(vfd_lib):
+    m = re.search(r"#define VFD_ABI_VERSION (\d+)", open(HEADER).read())
+    assert m, "VFD_ABI_VERSION missing from include/vfd.h"
+    assert int(m.group(1)) == vfd_lib.vfd_abi_version() == _abi.ABI_VERSION

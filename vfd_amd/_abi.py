@@ -244,6 +244,7 @@ _PROTOS = {
 EXPORTED = sorted(_PROTOS)
 
 _lib = None
+ABI_VERSION = 1  # VFD_ABI_VERSION of include/vfd.h, whose structs the ctypes classes above mirror
 
 
 class VfdError(RuntimeError):
@@ -265,6 +266,9 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.vfd_abi_version() != ABI_VERSION:
+        raise RuntimeError("%s implements ABI version %d, this binding describes version %d (rebuild with python -m "
+                           "vfd_amd.build)" % (p, lib.vfd_abi_version(), ABI_VERSION))
     if path is None:
         _lib = lib
     return lib

@@ -283,6 +283,12 @@ extern "C" int vfd_sample_mesh_volume(const vfd_mesh* mesh, const float* transfo
     } guard{st};
     Dev<float> dTri, dNrm, dSph, dNodes, dX, dY, dZ, dOut;
     Dev<uint32_t> dFlags, dOffs, dScratch;
+    // declared after the buffers, so destroyed before them: every return drains the stream before the buffers and
+    // the host vectors that queued copies read are released
+    struct DrainGuard {
+        hipStream_t s;
+        ~DrainGuard() { (void)hipStreamSynchronize(s); }
+    } drain{st};
     if ((e = dTri.alloc(prep.tri.size())) != hipSuccess || (e = dNrm.alloc(prep.normals.size())) != hipSuccess ||
         (e = dSph.alloc(prep.spheres.size())) != hipSuccess ||
         (e = dNodes.alloc(map.nodeCount)) != hipSuccess || (e = dX.alloc(xs.size())) != hipSuccess ||

@@ -764,6 +764,12 @@ public:
         VFD_TRY(tri.alloc(prep.tri.size()));
         VFD_TRY(nrm.alloc(prep.normals.size()));
         VFD_TRY(sph.alloc(prep.spheres.size()));
+        // destroyed before the buffers and `prep`: every return, early error ones included, drains the uploads and
+        // the distance kernel that read them
+        struct DrainGuard {
+            hipStream_t s;
+            ~DrainGuard() { (void)hipStreamSynchronize(s); }
+        } drain{stream};
         VFD_HIP(hipMemcpyAsync(sph.p, prep.spheres.data(), prep.spheres.size() * sizeof(float), hipMemcpyHostToDevice,
                                stream));
         VFD_HIP(hipMemcpyAsync(tri.p, prep.tri.data(), prep.tri.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -777,7 +783,6 @@ public:
         d.Padding = b.Padding;
         for (int a = 0; a < 3; ++a) d.CollisionMapResolution[a] = b.CollisionMapResolution[a];
         const float sign = b.Inverted ? -1.0f : 1.0f;
-        // add_body synchronises the stream before returning, so tri/nrm outlive the kernel
         return add_body(s, b.CollisionMapResolution, lo, hi, d,
                         [&](const MapDev& m, float* n0) { launch_mesh_field0(stream, md, m, sign, tol, n0); });
     }
