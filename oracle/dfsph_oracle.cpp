@@ -2,7 +2,7 @@
 //
 // This file restates, on the CPU, the algorithm of VFD's GPU DFSPH step (Goubermouche/VFD; the reference has no CPU
 // path of its own). It exists to check the HIP product: only tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg may load it, and only as the checker. The product (vfd_amd/) never links or calls it.
+// cpu_baseline leg may load it, and only as the result they compare against. The product (vfd_amd/) never links or calls it.
 //
 // One template, two instantiations:
 //   T = float   "FP32 mirror": the reference's operation order in IEEE single precision, compiled with
@@ -10,7 +10,7 @@
 //   T = double  "FP64 twin": the same code in double precision, used to state each field's FP32 error.
 //
 // Arithmetic contract shared with the HIP kernels (the reference's nvcc --use_fast_math build leaves these
-// unpinned; this build fixes them):
+// unpinned; this implementation fixes them):
 //   dot3(a,b)  = fma(a.z,b.z, fma(a.y,b.y, a.x*b.x))      (every dot, length2 and neighbour-search distance)
 //   length(a)  = sqrt(dot3(a,a)), IEEE sqrt and division (no fast math)
 //   reductions: Jacobi residual sums and CG dot products accumulate per-particle float terms in double,

@@ -1,5 +1,6 @@
-"""The FP64 reference adjoint of the oracle (the checker for the HIP adjoint), pinned by central finite differences
-of a forward whose kernel is the spline itself (oracle smooth mode), where the adjoint is the exact derivative.
+"""The FP64 reference adjoint of the oracle (the result the HIP adjoint is compared with), pinned by central finite
+differences of a forward whose kernel is the spline itself (oracle smooth mode), where the adjoint is the exact
+derivative.
 
 Setup: a free 512-particle block with viscosity 20, fixed time step and fixed Jacobi iteration counts (the adjoint
 holds time steps, iteration counts, clamp masks and neighbour lists fixed, as the forward's derivative does between

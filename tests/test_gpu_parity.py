@@ -1,6 +1,6 @@
 """Parity of the HIP step against the CPU oracle (FP32 mirror), called through the C ABI.
 
-Bars (BASELINE north star): neighbour sets bit-exact; densities, pressure factors and viscosity solutions within
+Bars: neighbour sets bit-exact; densities, pressure factors and viscosity solutions within
 1e-5 relative (inf-norm: max|gpu - oracle| / max|oracle|); positions within 1e-4 r after one step. The kernels keep
 the oracle's operation order, so most fields agree to the last bit; every comparison prints its measured error.
 """

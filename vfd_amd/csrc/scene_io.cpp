@@ -235,7 +235,7 @@ bool read_description(Reader& r, vfd_desc& d) {
                 r.error = why;
                 return false;
             }
-        } else if (!r.value(nullptr)) {  // a field this build does not know: skipped
+        } else if (!r.value(nullptr)) {  // a field this parser does not know: skipped
             return false;
         }
         r.ws();

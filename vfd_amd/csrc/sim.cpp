@@ -1299,7 +1299,7 @@ public:
         return VFD_OK;
     }
 
-    // ---- adjoint: reverse of the recorded steps (kernels in adjoint.hip; the FP64 reference is the oracle's) -------
+    // ---- adjoint: reverse of the recorded steps (kernels in adjoint.hip) ----------------------------------------
     DevBuf<float4> aLx, aLv, aLa, aTmp4, aGs, aLy, aLb, bR, bRh, bP, bV, bS, bT;
     DevBuf<float> aLk, aLsrc, aLs, aLf, aLrho, aLalpha, aLd, aLS, aOrigX, aOrigV;
     DevBuf<double> aPart;
