@@ -115,6 +115,28 @@ def test_checkpointed_rollout_matches_oracle_fp64():
     assert e["v0"] < 1e-3 and e["mu"] < 1e-3 and e["v0_inf"] < 1e-2
 
 
+def test_second_forward_keeps_the_first_gradient():
+    """Two forwards before one backward (as gradcheck, a line search or a loss over two rollouts do): the first call's
+    gradient must come from its own checkpoints and viscosity, bit for bit the gradient of a lone call."""
+    import torch
+    from vfd_amd import adjoint
+
+    ro, v0, loss = adjoint.viscosity_inversion(n_side=(16, 8, 16), steps=10, every=5)
+
+    def grads(second):
+        v = v0.clone().requires_grad_(True)
+        mu = torch.tensor(5.0, dtype=torch.float64, requires_grad=True)
+        x, _ = ro(v, mu)
+        if second:
+            ro(v0.clone() * 0.5, torch.tensor(12.0, dtype=torch.float64))  # other checkpoints, other viscosity
+        loss(x).backward()
+        return v.grad.clone(), float(mu.grad)
+
+    g1, m1 = grads(False)
+    g2, m2 = grads(True)
+    assert torch.equal(g1, g2) and m1 == m2
+
+
 def test_c5_gradient_against_differences_and_fp64_reference():
     """The 100k-particle inversion workload (60 steps, checkpoints every 10, mu = 5 against a target at mu = 20).
 

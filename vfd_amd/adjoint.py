@@ -64,6 +64,7 @@ class Rollout:
         self.sim = DFSPHSimulation(desc)
         self.sim.SetRigidBodies(list(bodies))
         self.checkpoints = []
+        self._last = None
         self.mu = None
         self.timings = {}
 
@@ -82,29 +83,45 @@ class Rollout:
         self.mu = float(np.float32(d.Viscosity)) * RHO0  # the viscosity the steps actually use
 
     def forward_np(self, x0, v0, mu):
-        """Run the rollout (host arrays in, host arrays out) and keep the checkpoints for backward_np."""
+        """Run the rollout (host arrays in, host arrays out); backward_np() without a record then differentiates
+        this run."""
+        x, v, _ = self.run_np(x0, v0, mu)
+        return x, v
+
+    def run_np(self, x0, v0, mu):
+        """Run the rollout and return (x, v, record). `record` holds what the backward pass of this run needs: its
+        checkpoints and its viscosity. The autograd function keeps the record of each call, so that a second forward
+        before the first backward (gradcheck, a line search, a loss over two rollouts) leaves the first call's
+        gradient unchanged."""
         t0 = time.perf_counter()
         self._set_mu(mu)
         self.sim.set_fluid(x0, v0)
-        self.checkpoints = []
+        checkpoints = []
         for k in range(self.steps):
             if k % self.every == 0:
-                self.checkpoints.append(self.sim.checkpoint())
+                checkpoints.append(self.sim.checkpoint())
             self.sim.OnUpdate()
         x, v = self.sim.field("position"), self.sim.field("velocity")
         self.timings["forward_s"] = time.perf_counter() - t0
-        return x, v
+        record = {"checkpoints": checkpoints, "mu": float(mu)}
+        self.checkpoints = checkpoints
+        self._last = record
+        return x, v, record
 
-    def backward_np(self, lx, lv):
-        """Cotangents of the final (x, v) -> those of (x0, v0) and d/dmu, segment by segment from the last."""
+    def backward_np(self, lx, lv, record=None):
+        """Cotangents of the final (x, v) of the run that produced `record` (default: the last run) -> those of
+        (x0, v0) and d/dmu, segment by segment from the last. The segments are re-run at that run's viscosity."""
         t0 = time.perf_counter()
-        if not self.checkpoints:
+        record = record if record is not None else getattr(self, "_last", None)
+        if not record or not record["checkpoints"]:
             raise RuntimeError("backward before forward")
+        checkpoints = record["checkpoints"]
+        self._set_mu(record["mu"])
         lx = np.ascontiguousarray(lx, np.float32)
         lv = np.ascontiguousarray(lv, np.float32)
         lmu, t_rev = 0.0, 0.0
-        for s in reversed(range(len(self.checkpoints))):
-            self.sim.restore(self.checkpoints[s])
+        for s in reversed(range(len(checkpoints))):
+            self.sim.restore(checkpoints[s])
             self.sim.tape(True)
             for _ in range(min(self.every, self.steps - s * self.every)):
                 self.sim.OnUpdate()
@@ -121,15 +138,16 @@ class Rollout:
 class _RolloutFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x0, v0, mu, rollout):
-        x, v = rollout.forward_np(x0.detach().cpu().numpy(), v0.detach().cpu().numpy(), float(mu))
+        x, v, record = rollout.run_np(x0.detach().cpu().numpy(), v0.detach().cpu().numpy(), float(mu))
         ctx.rollout = rollout
+        ctx.record = record  # this call's checkpoints and viscosity
         ctx.meta = (x0.device, x0.dtype, v0.device, v0.dtype, mu.device, mu.dtype)
         return torch.as_tensor(x, device=v0.device), torch.as_tensor(v, device=v0.device)
 
     @staticmethod
     def backward(ctx, gx, gv):
         xd, xt, vd, vt, md, mt = ctx.meta
-        lx, lv, lmu = ctx.rollout.backward_np(gx.detach().cpu().numpy(), gv.detach().cpu().numpy())
+        lx, lv, lmu = ctx.rollout.backward_np(gx.detach().cpu().numpy(), gv.detach().cpu().numpy(), ctx.record)
         return (torch.as_tensor(lx, device=xd, dtype=xt), torch.as_tensor(lv, device=vd, dtype=vt),
                 torch.as_tensor(lmu, device=md, dtype=mt), None)
 

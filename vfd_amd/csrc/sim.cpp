@@ -24,6 +24,7 @@
 
 #include "../../include/vfd.h"
 #include "common.h"
+#include "grid.h"
 #include "kernels.h"
 #include "mesh.h"
 
@@ -1028,38 +1029,24 @@ public:
         std::vector<SceneGrid> grids(S);
         for (uint32_t k = 0; k < S; ++k) {
             Scene& s = *sc[k];
-            SceneGrid& g = grids[k];
-            std::memset(&g, 0, sizeof(g));
-            g.cellBase = cellTotal;
-            g.startCorr = (int32_t)s.base - (int32_t)prefix;
+            SceneGrid g{};
+            const int32_t startCorr = (int32_t)s.base - (int32_t)prefix;
             prefix += s.count;
-            if (s.count == 0) continue;
-            const float R = s.prm.R;
-            const int* lo = &mm[6 * k];
-            const int* hi = &mm[6 * k + 3];
-            float bmin[3], bmax[3];
-            for (int a = 0; a < 3; ++a) {
-                bmin[a] = (float)lo[a] * R;
-                bmax[a] = (float)(hi[a] + 1) * R;
+            if (s.count != 0) {
+                const uint64_t cells = size_scene_grid(&mm[6 * k], &mm[6 * k + 3], s.prm.R, cellTotal, g);
+                if (cells == 0)
+                    return fail(VFD_ERROR_NOT_SUPPORTED,
+                                "scene " + std::to_string(k) + ": particle cloud too sparse for the neighbour grid "
+                                "(more than " + std::to_string(kMaxGridCells) + " cells)");
             }
-            uint64_t cells = kMetaBlock;
-            for (int a = 0; a < 3; ++a) {
-                const float size = bmax[a] - bmin[a];
-                g.dims[a] = (uint32_t)std::ceil(size / R) + 4;
-                g.gridMin[a] = bmin[a] - R * 2.0f;
-                g.meta[a] = (uint32_t)std::ceil((float)g.dims[a] / (float)kMetaGroup);
-                const float gs = (float)g.dims[a] * R;
-                g.gridDelta[a] = (float)g.dims[a] / gs;
-                cells *= g.meta[a];
-            }
-            if (cells + cellTotal > 0xF0000000ull)
-                return fail(VFD_ERROR_NOT_SUPPORTED, "particle cloud too sparse for the neighbour grid");
-            g.cellCount = (uint32_t)cells;
-            cellTotal += g.cellCount;
+            g.cellBase = cellTotal;
+            g.startCorr = startCorr;
+            cellTotal += g.cellCount;  // <= kMaxGridCells, checked by size_scene_grid
+            grids[k] = g;
             s.grid = g;
         }
         if (cellTotal > cellCap) {
-            const uint32_t want = cellTotal + cellTotal / 4;
+            const uint32_t want = (uint32_t)grid_capacity(cellTotal);
             VFD_TRY(cellCount.alloc(want));
             VFD_TRY(cellStart.alloc(want));
             VFD_TRY(scanScratch.alloc(scan_scratch_words(want)));
@@ -1087,9 +1074,11 @@ public:
     template <class Fn>
     int poll_loop(Fn enqueue, uint32_t first, int which) {
         uint32_t iter = 0;
+        // graphs are keyed by the double-buffer index of the particle arrays their launches captured
+        const int buf = this->cur;
         auto issue = [&](uint32_t n, int slot) -> int {
             if (useGraphs && !profiling && !taping && n == pollChunk && stream != nullptr) {
-                hipGraphExec_t& g = loopGraph[which][cur];
+                hipGraphExec_t& g = loopGraph[which][buf];
                 if (!g) {
                     hipGraph_t graph = nullptr;
                     VFD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -1108,15 +1097,15 @@ public:
             VFD_HIP(hipEventRecord(pollEv[slot], stream));
             return VFD_OK;
         };
-        int cur = 0;
+        int slot = 0;  // pinned poll slot of the chunk in flight
         uint32_t n0 = std::max<uint32_t>(first, 1);
         if (predictIterations && lastIterations[which] > 1) n0 = std::max(n0, lastIterations[which] - 1);
-        VFD_TRY(issue(n0, cur));
+        VFD_TRY(issue(n0, slot));
         for (;;) {
-            VFD_TRY(issue(pollChunk, cur ^ 1));
-            VFD_HIP(hipEventSynchronize(pollEv[cur]));
-            if (hostLive[cur] == 0) return VFD_OK;
-            cur ^= 1;
+            VFD_TRY(issue(pollChunk, slot ^ 1));
+            VFD_HIP(hipEventSynchronize(pollEv[slot]));
+            if (hostLive[slot] == 0) return VFD_OK;
+            slot ^= 1;
         }
     }
 
