@@ -11,8 +11,13 @@
  *   - vec3 arrays are tightly packed float[3*n] (x0,y0,z0,x1,...), in the caller's ORIGINAL particle order. The
  *     library keeps particles internally in Morton (cell) order and maps back on every read.
  *   - A vfd_sim holds one or more scenes ("batch"). The plain functions act on scene 0; vfd_batch_* take a scene index.
- *   - One vfd_sim must not be used from two threads at once. Work is enqueued on the stream set with vfd_set_stream
- *     (default: a non-blocking stream the simulation creates on the current device).
+ *   - Threads: one thread at a time drives a vfd_sim (every call not listed below). While it runs a call - for
+ *     instance vfd_simulate, or while vfd_simulate_async's worker runs - any other thread may poll with
+ *     vfd_get_simulation_state, vfd_get_debug_info, vfd_get_description, vfd_get_info, vfd_get_particle_count,
+ *     vfd_scene_count, vfd_get_frame_count and vfd_get_frame (the editor's pattern, ComponentPanel.cpp:446-705,
+ *     Scene.cpp:323-399); a second driving call meanwhile returns VFD_ERROR_STATE instead of racing. Work is
+ *     enqueued on the stream set with vfd_set_stream (default: a non-blocking stream the simulation creates on the
+ *     current device).
  */
 #ifndef VFD_H
 #define VFD_H
@@ -24,7 +29,7 @@
 extern "C" {
 #endif
 
-#define VFD_ABI_VERSION 1
+#define VFD_ABI_VERSION 2
 
 /* Status codes. */
 enum {
@@ -318,8 +323,22 @@ int vfd_step(vfd_sim* sim);
 /* Runs `steps` steps. */
 int vfd_advance(vfd_sim* sim, uint32_t steps);
 /* DFSPHImplementation::Simulate (DFSPHImplementation.cu:33-61): restores the baked particles, resets dt and runs
- * until FrameCount frames are captured. Synchronous (the reference runs it on a worker thread). */
+ * until FrameCount frames are captured, on the calling thread (vfd_simulate_async runs it on a worker thread, as the
+ * reference's DFSPHSimulation::Simulate does). */
 int vfd_simulate(vfd_sim* sim);
+/* DFSPHImplementation::SimulationState (DFSPHImplementation.h:27-32): None after creation, vfd_set_fluid and the
+ * boundary calls (SetFluidObjects :177, SetRigidBodies :259), Simulating while vfd_simulate runs, Ready once it has
+ * captured FrameCount frames (Simulate :48, :58). A failed vfd_simulate leaves None. */
+enum { VFD_STATE_NONE = 0, VFD_STATE_SIMULATING = 1, VFD_STATE_READY = 2 };
+/* GetSimulationState (DFSPHImplementation.h:45, .cu:273-276). Any thread, any time. */
+int vfd_get_simulation_state(const vfd_sim* sim, int32_t* state);
+/* DFSPHSimulation::Simulate (DFSPHSimulator.cpp:156-166), which the editor calls from its UI thread: restores the
+ * baked particles and releases the previous frames on the calling thread, then runs the frame loop of vfd_simulate on
+ * a worker thread of the library (on the calling thread's HIP device) and returns at once. Poll it with the calls the
+ * header lists as safe from other threads; vfd_wait joins the worker and returns vfd_simulate's status (with its
+ * message in vfd_last_error). Until vfd_wait, driving calls return VFD_ERROR_STATE; vfd_destroy waits for the worker. */
+int vfd_simulate_async(vfd_sim* sim);
+int vfd_wait(vfd_sim* sim);
 /* Testing aid: runs the current step up to and including `phase` (VFD_PHASE_*). */
 int vfd_step_until(vfd_sim* sim, int32_t phase);
 /* Enables/disables frame capture in vfd_step (default on, as in OnUpdate :148-167). A captured frame is packed in

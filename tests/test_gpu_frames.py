@@ -7,7 +7,7 @@ particles and captures the same frames again."""
 import numpy as np
 import pytest
 
-from scene_util import make_gpu, small_scene
+from scene_util import make_gpu, make_oracle, small_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -60,3 +60,35 @@ def test_capture_off_keeps_frame_timing():
     sim.set_frame_capture(True)
     sim.OnUpdate()
     assert len(_frames(sim)) == 1
+
+
+def test_frames_match_the_oracle():
+    """Frames against the CPU oracle stepped alongside (DFSPHImplementation.cu:148-167): a frame is due at the same
+    steps (FrameTime accumulates dt until it reaches FrameLength), and each frame's {x, v, a} in original order, its
+    MaxVelocityMagnitude and CurrentTimeStep equal the oracle's state after that step."""
+    descs, fluids, bodies = small_scene("st")
+    descs[0].FrameLength = 0.007
+    descs[0].FrameCount = 6
+    sim = make_gpu(descs, fluids, bodies)
+    orc = make_oracle(descs, fluids, bodies)
+    r = descs[0].ParticleRadius
+    expected, steps = [], []
+    for k in range(16):
+        before = orc.debug_info().FrameIndex
+        sim.OnUpdate()
+        orc.step()
+        d = orc.debug_info()
+        assert sim.GetDebugInfo().FrameIndex == d.FrameIndex, k
+        if d.FrameIndex > before:
+            xva = np.concatenate([orc.field("position"), orc.field("velocity"), orc.field("acceleration")], axis=1)
+            expected.append((xva, d.MaxVelocityMagnitude, d.TimeStepSize))
+            steps.append(k)
+    frames = sim.GetParticleFrameBuffer()
+    print("frames due after steps", steps, flush=True)
+    assert len(frames) == len(expected) == 6 and steps[-1] < 15  # the FrameCount cap stops capture
+    for fr, (xva, maxv, dt) in zip(frames, expected):
+        d = np.abs(fr.ParticleData.astype(np.float64) - xva)
+        assert d[:, :3].max() <= 1e-4 * r
+        assert d[:, 3:].max() <= 1e-5 * max(np.abs(xva[:, 3:]).max(), 1.0)
+        assert fr.MaxVelocityMagnitude == pytest.approx(maxv, rel=1e-6)
+        assert fr.CurrentTimeStep == pytest.approx(dt, rel=1e-6)

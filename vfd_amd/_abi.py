@@ -19,6 +19,10 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "STATE", 4: "OUT_OF
 
 SHAPE_BOX, SHAPE_SPHERE, SHAPE_PLANE, SHAPE_CYLINDER = 0, 1, 2, 3
 
+# DFSPHImplementation::SimulationState (DFSPHImplementation.h:27-32)
+STATE_NONE, STATE_SIMULATING, STATE_READY = 0, 1, 2
+STATE_NAMES = {STATE_NONE: "None", STATE_SIMULATING: "Simulating", STATE_READY: "Ready"}
+
 FIELDS = {
     "position": (0, 3), "velocity": (1, 3), "acceleration": (2, 3), "pressure_acceleration": (3, 3),
     "pressure_residuum": (4, 1), "density": (5, 1), "density_advection": (6, 1), "pressure_rho2": (7, 1),
@@ -219,6 +223,9 @@ _PROTOS = {
     "vfd_step": (ctypes.c_int, [_P]),
     "vfd_advance": (ctypes.c_int, [_P, _U32]),
     "vfd_simulate": (ctypes.c_int, [_P]),
+    "vfd_simulate_async": (ctypes.c_int, [_P]),
+    "vfd_wait": (ctypes.c_int, [_P]),
+    "vfd_get_simulation_state": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
     "vfd_step_until": (ctypes.c_int, [_P, _I32]),
     "vfd_set_frame_capture": (ctypes.c_int, [_P, _I32]),
     "vfd_get_particles": (ctypes.c_int, [_P, _P, _P, _P]),
@@ -244,7 +251,7 @@ _PROTOS = {
 EXPORTED = sorted(_PROTOS)
 
 _lib = None
-ABI_VERSION = 1  # VFD_ABI_VERSION of include/vfd.h, whose structs the ctypes classes above mirror
+ABI_VERSION = 2  # VFD_ABI_VERSION of include/vfd.h, whose structs the ctypes classes above mirror
 
 
 class VfdError(RuntimeError):

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cfloat>
 #include <climits>
@@ -19,7 +20,9 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vfd.h"
@@ -331,6 +334,46 @@ public:
     hipStream_t stream = nullptr;
     bool capture = true;
     bool committed = false;
+    int device = 0;
+
+    // ---- threads (DFSPHSimulator.cpp:156-166 runs Simulate on a ThreadPool worker while the editor's UI thread
+    //      polls the state, the debug info and the frames) ----------------------------------------------------------
+    // hostMu guards the host data those polls read: each scene's debug info, frame list, description, time step and
+    // particle count. A step takes it only for its host-side bookkeeping after the GPU work has drained.
+    mutable std::mutex hostMu;
+    std::atomic<int> simState{VFD_STATE_NONE};
+    // set for the duration of every driving call (and of the asynchronous worker): a second driver fails instead of
+    // racing the first
+    std::atomic<bool> busy{false};
+    std::thread worker;
+    std::mutex workerMu;  // serialises joins of `worker`
+    int workerStatus = VFD_OK;
+    std::string workerError;
+    // DFSPHImplementation::Simulate's frame loop (:47-58); the caller has restored the baked particles
+    int simulate_frames() {
+        simState = VFD_STATE_SIMULATING;
+        for (;;) {
+            bool done = true;
+            {
+                std::lock_guard<std::mutex> lk(hostMu);
+                for (Scene* s : sc)
+                    if (s->count && s->dbg.FrameIndex < s->desc.FrameCount) done = false;
+            }
+            if (done) break;
+            const int rc = step(VFD_PHASE_POSITION);
+            if (rc != VFD_OK) {
+                simState = VFD_STATE_NONE;
+                return rc;
+            }
+        }
+        simState = VFD_STATE_READY;
+        return VFD_OK;
+    }
+    int join_worker() {
+        std::lock_guard<std::mutex> lk(workerMu);
+        if (worker.joinable()) worker.join();
+        return workerStatus;
+    }
 
     KernelTable kt;
     DevBuf<float2> dTab;
@@ -534,6 +577,7 @@ public:
     size_t evUsed = 0;
 
     ~Simulation() {
+        join_worker();
         if (eventsMade) {
             for (auto& e : ev) (void)hipEventDestroy(e);
             for (auto& e : pollEv) (void)hipEventDestroy(e);
@@ -614,6 +658,7 @@ public:
         }
         int dev = 0, cus = 0;
         VFD_HIP(hipGetDevice(&dev));
+        device = dev;
         VFD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         numCU = cus > 0 ? (uint32_t)cus : 256u;
         if (configure_kernels() != 0) return fail(VFD_ERROR_HIP, "hipFuncSetAttribute (dynamic LDS) failed");
@@ -670,7 +715,10 @@ public:
                 return fail(VFD_ERROR_STATE, "ParticleRadius cannot change after boundaries are built");
             VFD_TRY(build_tables(d.ParticleRadius));
         }
-        sc[s]->desc = d;
+        {
+            std::lock_guard<std::mutex> lk(hostMu);
+            sc[s]->desc = d;
+        }
         const SceneParams old = sc[s]->prm;
         derive_params(d, sc[s]->prm);
         sc[s]->prm.base = old.base;
@@ -678,7 +726,10 @@ public:
         sc[s]->prm.tileBegin = old.tileBegin;
         sc[s]->prm.tileEnd = old.tileEnd;
         sc[s]->prm.bodyCount = old.bodyCount;
-        set_time_step(sc[s]->st, d, d.TimeStepSize);  // SetDescription resets the time step (:352-356)
+        {
+            std::lock_guard<std::mutex> lk(hostMu);
+            set_time_step(sc[s]->st, d, d.TimeStepSize);  // SetDescription resets the time step (:352-356)
+        }
         if (committed) {
             VFD_HIP(hipMemcpyAsync(dP.p + s, &sc[s]->prm, sizeof(SceneParams), hipMemcpyHostToDevice, stream));
             VFD_TRY(push_state());
@@ -697,9 +748,13 @@ public:
             S.v0.assign(v, v + (size_t)3 * n);
         else
             S.v0.assign((size_t)3 * n, 0.0f);
-        S.count = n;
-        release_frames(S);
-        S.dbg = vfd_debug_info{};
+        {
+            std::lock_guard<std::mutex> lk(hostMu);
+            S.count = n;
+            release_frames(S);
+            S.dbg = vfd_debug_info{};
+        }
+        simState = VFD_STATE_NONE;  // SetFluidObjects (:177)
         committed = false;
         tapeCount = 0;  // recorded steps belong to the particles they were recorded with
         return VFD_OK;
@@ -831,6 +886,7 @@ public:
         }
         S.bodies.push_back(body);
         S.prm.bodyCount = (uint32_t)S.bodies.size();
+        simState = VFD_STATE_NONE;  // SetRigidBodies (:259)
         committed = false;
         return VFD_OK;
     }
@@ -839,6 +895,7 @@ public:
         for (Body* b : sc[s]->bodies) delete b;
         sc[s]->bodies.clear();
         sc[s]->prm.bodyCount = 0;
+        simState = VFD_STATE_NONE;
         committed = false;
         return VFD_OK;
     }
@@ -925,6 +982,7 @@ public:
                 v[s->base + i] = make_float4(s->v0[3 * i], s->v0[3 * i + 1], s->v0[3 * i + 2], 0.0f);
                 ids[s->base + i] = i;
             }
+            std::lock_guard<std::mutex> lk(hostMu);
             s->dbg = vfd_debug_info{};
             release_frames(*s);
         }
@@ -1230,6 +1288,8 @@ public:
         VFD_HIP(hipEventRecord(ev[6], stream));
         if (last == VFD_PHASE_PRESSURE) return finish_partial();
         VFD_HIP(hipGetLastError());
+        VFD_HIP(hipStreamSynchronize(stream));
+        std::unique_lock<std::mutex> lk(hostMu);  // the host bookkeeping that polling threads read
         VFD_TRY(pull_state());
         solver_debug();
         VFD_TRY(neighbor_stats());
@@ -1444,6 +1504,8 @@ public:
     int finish_partial() {
         VFD_HIP(hipMemcpyAsync(posNext.p, pos.p, (size_t)cap * sizeof(float4), hipMemcpyDeviceToDevice, stream));
         VFD_HIP(hipGetLastError());
+        VFD_HIP(hipStreamSynchronize(stream));
+        std::lock_guard<std::mutex> lk(hostMu);
         VFD_TRY(pull_state());
         solver_debug();
         VFD_TRY(neighbor_stats());
@@ -1649,6 +1711,29 @@ struct vfd_sim {
 
 static int guard(int rc) { return rc; }
 
+// Held by every driving call for its duration: a call from a second thread while one runs (or while the worker of
+// vfd_simulate_async runs) fails with VFD_ERROR_STATE.
+struct DriveGuard {
+    std::atomic<bool>& flag;
+    bool ok;
+    explicit DriveGuard(std::atomic<bool>& f) : flag(f) {
+        bool idle = false;
+        ok = flag.compare_exchange_strong(idle, true);
+    }
+    ~DriveGuard() {
+        if (ok) flag.store(false);
+    }
+};
+#define DRIVE(sim)                                                                                             \
+    CHECK_SIM(sim);                                                                                            \
+    DriveGuard drive_((sim)->impl.busy);                                                                       \
+    if (!drive_.ok)                                                                                            \
+        return fail(VFD_ERROR_STATE, "the simulation is running on another thread (vfd_simulate, vfd_step or "  \
+                                     "vfd_simulate_async without vfd_wait)")
+#define DRIVE_SCENE(sim, s)                                                                                   \
+    DRIVE(sim);                                                                                               \
+    if ((s) >= (sim)->impl.sc.size()) return fail(VFD_ERROR_INVALID_ARGUMENT, "scene index out of range")
+
 extern "C" {
 
 int vfd_abi_version(void) { return VFD_ABI_VERSION; }
@@ -1704,13 +1789,14 @@ int vfd_batch_create(const vfd_desc* descs, uint32_t n, vfd_sim** out) {
 int vfd_create(const vfd_desc* desc, vfd_sim** out) { return vfd_batch_create(desc, 1, out); }
 int vfd_destroy(vfd_sim* sim) {
     if (sim) {
+        sim->impl.join_worker();
         if (sim->impl.stream) (void)hipStreamSynchronize(sim->impl.stream);
         delete sim;
     }
     return VFD_OK;
 }
 int vfd_set_stream(vfd_sim* sim, void* stream) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     sim->impl.drop_graphs();
     sim->impl.stream = stream ? (hipStream_t)stream : sim->impl.ownStream;
     return VFD_OK;
@@ -1722,7 +1808,7 @@ int vfd_scene_count(const vfd_sim* sim, uint32_t* count) {
     return VFD_OK;
 }
 int vfd_batch_set_description(vfd_sim* sim, uint32_t scene, const vfd_desc* desc) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     if (!desc) return fail(VFD_ERROR_INVALID_ARGUMENT, "desc is NULL");
     return guard(sim->impl.set_description(scene, *desc));
 }
@@ -1730,12 +1816,14 @@ int vfd_set_description(vfd_sim* sim, const vfd_desc* desc) { return vfd_batch_s
 int vfd_get_description(const vfd_sim* sim, vfd_desc* desc) {
     CHECK_SIM(sim);
     if (!desc) return fail(VFD_ERROR_INVALID_ARGUMENT, "desc is NULL");
+    std::lock_guard<std::mutex> lk(sim->impl.hostMu);
     *desc = sim->impl.sc[0]->desc;
     return VFD_OK;
 }
 int vfd_batch_get_info(const vfd_sim* sim, uint32_t scene, vfd_info* info) {
     CHECK_SCENE(sim, scene);
     if (!info) return fail(VFD_ERROR_INVALID_ARGUMENT, "info is NULL");
+    std::lock_guard<std::mutex> lk(sim->impl.hostMu);
     const vfd::Scene& S = *sim->impl.sc[scene];
     const vfd::SceneParams& p = S.prm;
     vfd_info& o = *info;
@@ -1776,21 +1864,21 @@ int vfd_batch_get_info(const vfd_sim* sim, uint32_t scene, vfd_info* info) {
 int vfd_get_info(const vfd_sim* sim, vfd_info* info) { return vfd_batch_get_info(sim, 0, info); }
 
 int vfd_batch_set_fluid(vfd_sim* sim, uint32_t scene, const float* x, const float* v, uint32_t n) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     return guard(sim->impl.set_fluid(scene, x, v, n));
 }
 int vfd_set_fluid(vfd_sim* sim, const float* x, const float* v, uint32_t n) {
     return vfd_batch_set_fluid(sim, 0, x, v, n);
 }
 int vfd_batch_add_boundary(vfd_sim* sim, uint32_t scene, const vfd_boundary_desc* body) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     if (!body) return fail(VFD_ERROR_INVALID_ARGUMENT, "body is NULL");
     return guard(sim->impl.add_boundary(scene, *body));
 }
 int vfd_add_boundary(vfd_sim* sim, const vfd_boundary_desc* body) { return vfd_batch_add_boundary(sim, 0, body); }
 int vfd_batch_add_boundary_mesh(vfd_sim* sim, uint32_t scene, const vfd_mesh_boundary_desc* body,
                                 const vfd_mesh* mesh) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     if (!body || !mesh) return fail(VFD_ERROR_INVALID_ARGUMENT, "body or mesh is NULL");
     return guard(sim->impl.add_boundary_mesh(scene, *body, *mesh));
 }
@@ -1798,12 +1886,12 @@ int vfd_add_boundary_mesh(vfd_sim* sim, const vfd_mesh_boundary_desc* body, cons
     return vfd_batch_add_boundary_mesh(sim, 0, body, mesh);
 }
 int vfd_clear_boundaries(vfd_sim* sim, uint32_t scene) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     return sim->impl.clear_boundaries(scene);
 }
 int vfd_get_boundary_map(vfd_sim* sim, uint32_t scene, uint32_t body, uint32_t field, float* nodes,
                          uint32_t* node_count) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     const vfd::Scene& S = *sim->impl.sc[scene];
     if (body >= S.bodies.size() || field > 1) return fail(VFD_ERROR_INVALID_ARGUMENT, "body or field out of range");
     const vfd::Body& b = *S.bodies[body];
@@ -1817,11 +1905,11 @@ int vfd_get_boundary_map(vfd_sim* sim, uint32_t scene, uint32_t body, uint32_t f
 }
 
 int vfd_step(vfd_sim* sim) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     return guard(sim->impl.step(VFD_PHASE_POSITION));
 }
 int vfd_advance(vfd_sim* sim, uint32_t steps) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     for (uint32_t i = 0; i < steps; ++i) {
         const int rc = sim->impl.step(VFD_PHASE_POSITION);
         if (rc != VFD_OK) return rc;
@@ -1829,37 +1917,71 @@ int vfd_advance(vfd_sim* sim, uint32_t steps) {
     return VFD_OK;
 }
 int vfd_step_until(vfd_sim* sim, int32_t phase) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     if (phase < 0 || phase >= VFD_PHASE_COUNT) return fail(VFD_ERROR_INVALID_ARGUMENT, "phase out of range");
     return guard(sim->impl.step(phase));
 }
 int vfd_simulate(vfd_sim* sim) {
+    DRIVE(sim);
+    Simulation& S = sim->impl;
+    const int rc = S.committed ? S.upload_baked() : S.commit();
+    if (rc != VFD_OK) return rc;
+    return S.simulate_frames();
+}
+int vfd_simulate_async(vfd_sim* sim) {
     CHECK_SIM(sim);
     Simulation& S = sim->impl;
-    if (!S.committed) {
-        const int rc = S.commit();
-        if (rc != VFD_OK) return rc;
-    } else {
-        const int rc = S.upload_baked();
-        if (rc != VFD_OK) return rc;
+    bool idle = false;
+    if (!S.busy.compare_exchange_strong(idle, true))
+        return fail(VFD_ERROR_STATE, "the simulation is running on another thread");
+    S.join_worker();  // a finished worker nobody waited for
+    // baked particles restored and earlier frames released here, before any poll can read them
+    const int rc = S.committed ? S.upload_baked() : S.commit();
+    if (rc != VFD_OK) {
+        S.busy = false;
+        return rc;
     }
-    for (;;) {
-        bool done = true;
-        for (vfd::Scene* s : S.sc)
-            if (s->count && s->dbg.FrameIndex < s->desc.FrameCount) done = false;
-        if (done) return VFD_OK;
-        const int rc = S.step(VFD_PHASE_POSITION);
-        if (rc != VFD_OK) return rc;
+    S.workerStatus = VFD_OK;
+    S.workerError.clear();
+    S.simState = VFD_STATE_SIMULATING;
+    std::lock_guard<std::mutex> lk(S.workerMu);
+    try {
+        S.worker = std::thread([&S] {
+            int r = hipSetDevice(S.device) == hipSuccess ? VFD_OK : fail(VFD_ERROR_HIP, "hipSetDevice in the worker");
+            if (r == VFD_OK) r = S.simulate_frames();
+            if (r != VFD_OK) {
+                S.workerError = vfd::g_lastError;  // this thread's message, handed to vfd_wait
+                S.simState = VFD_STATE_NONE;
+            }
+            S.workerStatus = r;
+            S.busy = false;
+        });
+    } catch (...) {
+        S.simState = VFD_STATE_NONE;
+        S.busy = false;
+        return fail(VFD_ERROR_STATE, "could not start the simulation thread");
     }
+    return VFD_OK;
+}
+int vfd_wait(vfd_sim* sim) {
+    CHECK_SIM(sim);
+    const int rc = sim->impl.join_worker();
+    return rc == VFD_OK ? VFD_OK : fail(rc, sim->impl.workerError);
+}
+int vfd_get_simulation_state(const vfd_sim* sim, int32_t* state) {
+    CHECK_SIM(sim);
+    if (!state) return fail(VFD_ERROR_INVALID_ARGUMENT, "state is NULL");
+    *state = sim->impl.simState.load();
+    return VFD_OK;
 }
 int vfd_set_frame_capture(vfd_sim* sim, int32_t enabled) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     sim->impl.capture = enabled != 0;
     return VFD_OK;
 }
 
 int vfd_batch_get_particles(vfd_sim* sim, uint32_t scene, float* x, float* v, float* a) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     if (x) {
         const int rc = sim->impl.get_field(scene, VFD_FIELD_POSITION, 0, x);
         if (rc != VFD_OK) return rc;
@@ -1876,29 +1998,31 @@ int vfd_batch_get_particles(vfd_sim* sim, uint32_t scene, float* x, float* v, fl
 }
 int vfd_get_particles(vfd_sim* sim, float* x, float* v, float* a) { return vfd_batch_get_particles(sim, 0, x, v, a); }
 int vfd_get_field(vfd_sim* sim, uint32_t scene, int32_t field, uint32_t body, float* out) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     if (!out) return fail(VFD_ERROR_INVALID_ARGUMENT, "out is NULL");
     return guard(sim->impl.get_field(scene, field, body, out));
 }
 int vfd_set_field(vfd_sim* sim, uint32_t scene, int32_t field, const float* values) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     if (!values) return fail(VFD_ERROR_INVALID_ARGUMENT, "values is NULL");
     return guard(sim->impl.set_field(scene, field, values));
 }
 int vfd_get_particle_count(const vfd_sim* sim, uint32_t scene, uint32_t* count) {
     CHECK_SCENE(sim, scene);
     if (!count) return fail(VFD_ERROR_INVALID_ARGUMENT, "count is NULL");
+    std::lock_guard<std::mutex> lk(sim->impl.hostMu);
     *count = sim->impl.sc[scene]->count;
     return VFD_OK;
 }
 int vfd_get_neighbors(vfd_sim* sim, uint32_t scene, uint32_t* counts, uint32_t* offsets, uint32_t* ids,
                       uint64_t* total) {
-    CHECK_SCENE(sim, scene);
+    DRIVE_SCENE(sim, scene);
     return guard(sim->impl.get_neighbors(scene, counts, offsets, ids, total));
 }
 int vfd_get_debug_info(vfd_sim* sim, uint32_t scene, vfd_debug_info* info) {
     CHECK_SCENE(sim, scene);
     if (!info) return fail(VFD_ERROR_INVALID_ARGUMENT, "info is NULL");
+    std::lock_guard<std::mutex> lk(sim->impl.hostMu);
     *info = sim->impl.sc[scene]->dbg;
     info->TimeStepSize = sim->impl.sc[scene]->st.dt;
     info->MaxVelocityMagnitude = sim->impl.sc[scene]->st.maxVel;
@@ -1907,21 +2031,36 @@ int vfd_get_debug_info(vfd_sim* sim, uint32_t scene, vfd_debug_info* info) {
 int vfd_get_frame_count(vfd_sim* sim, uint32_t scene, uint32_t* count) {
     CHECK_SCENE(sim, scene);
     if (!count) return fail(VFD_ERROR_INVALID_ARGUMENT, "count is NULL");
+    std::lock_guard<std::mutex> lk(sim->impl.hostMu);
     *count = (uint32_t)sim->impl.sc[scene]->frames.size();
     return VFD_OK;
 }
 int vfd_get_frame(vfd_sim* sim, uint32_t scene, uint32_t frame, float* xva, float* max_velocity, float* time_step) {
     CHECK_SCENE(sim, scene);
-    const auto& fr = sim->impl.sc[scene]->frames;
-    if (frame >= fr.size()) return fail(VFD_ERROR_INVALID_ARGUMENT, "frame index out of range");
-    if (fr[frame].done) VFD_HIP(hipEventSynchronize(fr[frame].done));  // its asynchronous copy has landed
-    if (xva) std::memcpy(xva, fr[frame].data(), fr[frame].n * sizeof(float));
-    if (max_velocity) *max_velocity = fr[frame].maxVel;
-    if (time_step) *time_step = fr[frame].dt;
-    return VFD_OK;
+    Simulation& S = sim->impl;
+    // the frame's asynchronous copy must have landed; wait for it without holding the lock that a running step's
+    // bookkeeping takes, then copy out under the lock (frames are released only by driving calls)
+    for (;;) {
+        hipEvent_t done = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(S.hostMu);
+            const auto& fr = S.sc[scene]->frames;
+            if (frame >= fr.size()) return fail(VFD_ERROR_INVALID_ARGUMENT, "frame index out of range");
+            const vfd::Frame& f = fr[frame];
+            if (!f.done || hipEventQuery(f.done) == hipSuccess) {
+                if (xva) std::memcpy(xva, f.data(), f.n * sizeof(float));
+                if (max_velocity) *max_velocity = f.maxVel;
+                if (time_step) *time_step = f.dt;
+                return VFD_OK;
+            }
+            done = f.done;
+        }
+        (void)hipGetLastError();  // the query's "not ready" is no error of this thread
+        VFD_HIP(hipEventSynchronize(done));
+    }
 }
 int vfd_set_profiling(vfd_sim* sim, int32_t enabled) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     sim->impl.profiling = enabled != 0;
     sim->impl.kstats.clear();
     sim->impl.pending.clear();
@@ -1929,7 +2068,7 @@ int vfd_set_profiling(vfd_sim* sim, int32_t enabled) {
     return VFD_OK;
 }
 int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint32_t* count) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     const auto& ks = sim->impl.kstats;
     if (count) *count = (uint32_t)ks.size();
     if (out)
@@ -1942,7 +2081,7 @@ int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint3
     return VFD_OK;
 }
 int vfd_tape(vfd_sim* sim, int32_t on) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     auto& S = sim->impl;
     S.taping = on != 0;  // (solver chunks run without their graphs while recording)
     if (on) {
@@ -1952,18 +2091,18 @@ int vfd_tape(vfd_sim* sim, int32_t on) {
     return VFD_OK;
 }
 int vfd_tape_steps(vfd_sim* sim, uint32_t* steps) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     if (!steps) return fail(VFD_ERROR_INVALID_ARGUMENT, "steps must not be NULL");
     *steps = (uint32_t)sim->impl.tapeCount;
     return VFD_OK;
 }
 int vfd_adjoint(vfd_sim* sim, const float* lx_out, const float* lv_out, float* lx_in, float* lv_in, double* lmu) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     if (!lx_out || !lx_in || !lv_in) return fail(VFD_ERROR_INVALID_ARGUMENT, "cotangent buffers must not be NULL");
     return sim->impl.adjoint(lx_out, lv_out, lx_in, lv_in, lmu);
 }
 int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     if (!out) return fail(VFD_ERROR_INVALID_ARGUMENT, "out must not be NULL");
     out->SolverCheckInterval = sim->impl.pollChunk;
     out->HaloLimit = sim->impl.haloLimit;
@@ -1972,7 +2111,7 @@ int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out) {
     return VFD_OK;
 }
 int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* t) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     if (!t) return fail(VFD_ERROR_INVALID_ARGUMENT, "tuning must not be NULL");
     if (t->SolverCheckInterval < 1 || t->SolverCheckInterval > 1024)
         return fail(VFD_ERROR_INVALID_ARGUMENT, "SolverCheckInterval must be in [1, 1024]");
@@ -1986,7 +2125,7 @@ int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* t) {
     return VFD_OK;
 }
 int vfd_get_kernel_table(vfd_sim* sim, float* w, float* gradient_w, float* scalars) {
-    CHECK_SIM(sim);
+    DRIVE(sim);
     const vfd::KernelTable& k = sim->impl.kt;
     if (w) std::memcpy(w, k.W.data(), k.W.size() * sizeof(float));
     if (gradient_w) std::memcpy(gradient_w, k.G.data(), k.G.size() * sizeof(float));

@@ -251,8 +251,25 @@ class DFSPHSimulation:
     def OnUpdate(self):
         check(self._lib.vfd_step(self._h), "vfd_step")
 
-    def Simulate(self):
-        check(self._lib.vfd_simulate(self._h), "vfd_simulate")
+    def Simulate(self, asynchronous=False):
+        """DFSPHImplementation::Simulate (DFSPHImplementation.cu:33-61): restore the baked particles and run until
+        FrameCount frames are captured. asynchronous=True runs the frame loop on a worker thread of the library and
+        returns at once, as the reference's DFSPHSimulation::Simulate does (DFSPHSimulator.cpp:156-166); poll with
+        GetSimulationState / GetDebugInfo / GetParticleFrameBuffer and finish with Wait()."""
+        if asynchronous:
+            check(self._lib.vfd_simulate_async(self._h), "vfd_simulate_async")
+        else:
+            check(self._lib.vfd_simulate(self._h), "vfd_simulate")
+
+    def Wait(self):
+        """Join the worker of Simulate(asynchronous=True); raises VfdError with its status if the run failed."""
+        check(self._lib.vfd_wait(self._h), "vfd_wait")
+
+    def GetSimulationState(self):
+        """0 None, 1 Simulating, 2 Ready (DFSPHImplementation::SimulationState, DFSPHImplementation.h:27-32)."""
+        st = _abi.ctypes.c_int32()
+        check(self._lib.vfd_get_simulation_state(self._h, _abi.ctypes.byref(st)), "vfd_get_simulation_state")
+        return st.value
 
     def advance(self, steps):
         check(self._lib.vfd_advance(self._h, int(steps)), "vfd_advance")
