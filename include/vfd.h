@@ -320,6 +320,21 @@ int vfd_get_boundary_map(vfd_sim* sim, uint32_t scene, uint32_t body, uint32_t f
  * VFD_ERROR_NONFINITE (after completing the step; the state keeps its values) when a particle's new position or
  * velocity is not finite; the reference has no such check and lets NaN propagate. */
 int vfd_step(vfd_sim* sim);
+/* No counterpart in the reference. vfd_step for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc): the same step with the same results, which calls no hipMalloc, hipFree,
+ * hipHostMalloc or graph instantiation. The neighbour grid's cell arrays and the surface-tension curvature tables
+ * come from a static scratch array of the library (32 MiB + 272 KiB on each device the library runs on, held until
+ * the process ends); every other buffer is the simulation's own and must already exist: they are allocated by the
+ * first vfd_step (or vfd_get_particles) after vfd_set_fluid, and reused for any smaller particle count later.
+ * Limits: at most 4,194,304 grid cells in all scenes together (VFD_ERROR_NOT_SUPPORTED, checked before the step
+ * changes any state), at most 65,536 curvature-table floats over the surface-tension scenes at the largest time step
+ * the descriptions allow and 4,096 scenes (VFD_ERROR_NOT_SUPPORTED). Returns VFD_ERROR_STATE, without stepping, when
+ * a buffer would have to be allocated or grown, or when frame capture (vfd_set_frame_capture), adjoint recording
+ * (vfd_tape) or profiling is on. Iteration loops are launched directly, not through HIP graphs. The scratch is shared
+ * by every no-alloc step on one device and holds the last call's contents: calls that overlap on one device (two
+ * threads, two simulations) race; calls one after another are safe, since the step waits for its work before it
+ * returns. */
+int vfd_step_no_alloc(vfd_sim* sim);
 /* Runs `steps` steps. */
 int vfd_advance(vfd_sim* sim, uint32_t steps);
 /* DFSPHImplementation::Simulate (DFSPHImplementation.cu:33-61): restores the baked particles, resets dt and runs

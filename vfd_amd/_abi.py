@@ -221,6 +221,7 @@ _PROTOS = {
     "vfd_clear_boundaries": (ctypes.c_int, [_P, _U32]),
     "vfd_get_boundary_map": (ctypes.c_int, [_P, _U32, _U32, _U32, _P, ctypes.POINTER(_U32)]),
     "vfd_step": (ctypes.c_int, [_P]),
+    "vfd_step_no_alloc": (ctypes.c_int, [_P]),
     "vfd_advance": (ctypes.c_int, [_P, _U32]),
     "vfd_simulate": (ctypes.c_int, [_P]),
     "vfd_simulate_async": (ctypes.c_int, [_P]),

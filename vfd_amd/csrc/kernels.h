@@ -145,6 +145,24 @@ void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, 
                         const uint32_t* cellCount, uint32_t* nbrCount, uint32_t* nbrList,
                         unsigned long long* nbStats);
 
+// Static scratch of the allocation-free step, in 32-bit words: the grid's cell counts and cell starts (kStaticCells
+// each), the scan's chunk sums, the surface-tension curvature tables (kStaticMcFloats floats) and their per-scene
+// offsets (kStaticMcScenes words). 32 MiB + 272 KiB per device.
+constexpr uint32_t kStaticCells = 1u << 22;
+constexpr uint32_t kStaticScanWords = kStaticCells / 2048u + 1u;  // scan_scratch_words(kStaticCells)
+constexpr uint32_t kStaticMcFloats = 1u << 16;
+constexpr uint32_t kStaticMcScenes = 4096;
+constexpr size_t kStaticCellCountAt = 0;
+constexpr size_t kStaticCellStartAt = kStaticCells;
+constexpr size_t kStaticScanAt = 2 * (size_t)kStaticCells;
+constexpr size_t kStaticMcTabAt = (kStaticScanAt + kStaticScanWords + 63) / 64 * 64;
+constexpr size_t kStaticMcOffAt = kStaticMcTabAt + kStaticMcFloats;
+constexpr size_t kStaticScratchWords = kStaticMcOffAt + kStaticMcScenes;
+constexpr uint32_t kMaxStaticScratchDevices = 64;
+// Device address of that device's static scratch (kStaticScratchWords words); looked up once per device, which must
+// be the current device on the first call. Returns 0 on success.
+int static_scratch_address(int device, uint32_t** out);
+
 // ---- boundary.hip ---------------------------------------------------------------------------------------------
 void launch_map_field0(hipStream_t st, const ShapeDev& shape, const MapDev& map, float* nodes0);
 void launch_mesh_field0(hipStream_t st, const MeshDev& mesh, const MapDev& map, float sign, float tol, float* nodes0);

@@ -8,6 +8,8 @@
 //     contiguous cells; the reference keeps particles in input order and only sorts index arrays;
 //   - the order inside a cell is ascending original id (deterministic) instead of atomicAdd order;
 //   - lists hold slot indices in a transposed [j][i] layout with a fixed capacity of 70 per particle.
+#include <mutex>
+
 #include "device.h"
 #include "kernels.h"
 
@@ -286,6 +288,8 @@ __global__ void __launch_bounds__(kTile) k_build_lists(const float4* __restrict_
 
 // ---- exclusive scan of uint32 (cell counts) --------------------------------------------------------------------
 constexpr unsigned kScanThreads = 256, kScanItems = 8, kScanChunk = kScanThreads * kScanItems;
+static_assert(kStaticScanWords >= (kStaticCells + kScanChunk - 1) / kScanChunk + 1,
+              "static scratch must hold the chunk sums of kStaticCells cells");
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {
     __shared__ uint32_t wsum[kScanThreads / kWave];
@@ -545,6 +549,37 @@ int configure_search_kernels() {
     if (kLdsListStage == 0) return 0;
     return hipFuncSetAttribute((const void*)k_build_lists, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kLdsListStage) == hipSuccess ? 0 : 1;
+}
+
+// ---- static scratch of the allocation-free step -------------------------------------------------------------------
+// The grid cell arrays (counts, starts, scan partials) and the surface-tension curvature tables of vfd_step_no_alloc
+// live here instead of in buffers the simulation allocates. Each device holds its own copy for the life of the
+// process; every no-alloc step on a device shares it (overlapping calls on one device race). Its address is looked up
+// once per device with that device current, and only that address is handed to kernels and copies: in host code
+// `scratch` names a host-side stand-in, not device memory.
+static __device__ __align__(256) uint32_t scratch[kStaticScratchWords];
+
+namespace {
+std::mutex g_scratchMu;
+uint32_t* g_scratchPtr[kMaxStaticScratchDevices] = {};
+}  // namespace
+
+int static_scratch_address(int device, uint32_t** out) {
+    *out = nullptr;
+    if (device < 0 || device >= (int)kMaxStaticScratchDevices) return 1;
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchPtr[device]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess || cur != device) return 1;
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) {
+            (void)hipGetLastError();
+            return 1;
+        }
+        g_scratchPtr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratchPtr[device];
+    return 0;
 }
 
 }  // namespace vfd

@@ -55,6 +55,15 @@ static int fail(int code, const std::string& msg) {
 }
 int set_error(int code, const std::string& msg) { return fail(code, msg); }  // for the other host sources
 
+// Set by vfd_step_no_alloc for the duration of its call on this thread: a device buffer that would have to be
+// allocated or grown fails the call instead, and the step takes its grid and curvature-table scratch from the
+// library's static scratch.
+static thread_local bool t_noAlloc = false;
+struct NoAllocScope {
+    NoAllocScope() { t_noAlloc = true; }
+    ~NoAllocScope() { t_noAlloc = false; }
+};
+
 #define VFD_HIP(expr)                                                                                  \
     do {                                                                                               \
         hipError_t e_ = (expr);                                                                        \
@@ -72,6 +81,11 @@ template <class T> struct DevBuf {
     size_t n = 0;
     int alloc(size_t count) {
         if (count <= n && p) return VFD_OK;
+        if (t_noAlloc && count != 0)
+            return fail(VFD_ERROR_STATE, "vfd_step_no_alloc: a device buffer of " +
+                                             std::to_string(count * sizeof(T)) +
+                                             " bytes would have to be allocated (take one vfd_step, or call "
+                                             "vfd_get_particles, after vfd_set_fluid to allocate the state first)");
         release();
         if (count == 0) return VFD_OK;
         hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
@@ -385,6 +399,19 @@ public:
     // cos(acos(1 - 2 count / S) + MonteCarloFactor) for count = 0..S of every surface-tension scene, in float with the
     // host's libm, the expression of ComputeSurfaceTensionClassificationKernel (DFSPHKernels.cu:950-953); the
     // classification kernel reads it instead of evaluating cos and acos on the device
+    // Whether the curvature tables of every surface-tension scene fit the static scratch at any time step the
+    // descriptions allow (dt <= max(TimeStepSize, MaxTimeStepSize)), so a no-alloc step cannot fail half-way.
+    bool mc_tables_fit_static() const {
+        if (sc.size() > kStaticMcScenes) return false;
+        double total = 1.0;
+        for (const Scene* s : sc) {
+            if (!s->desc.EnableSurfaceTensionSolver) continue;
+            const double dtMax = std::max((double)s->desc.TimeStepSize, (double)s->desc.MaxTimeStepSize);
+            const double S = s->desc.CSDFix > 0 ? (double)s->desc.CSDFix : std::floor((double)s->desc.CSD * dtMax) + 1.0;
+            total += S + 1.0;
+        }
+        return total <= (double)kStaticMcFloats;
+    }
     int upload_mc_tables(Fields& F) {
         mcOff.assign(sc.size(), 0u);
         mcTab.clear();
@@ -397,13 +424,23 @@ public:
                 mcTab.push_back(std::cos(std::acos(1.0f - 2.0f * ((float)count / (float)S)) + s.prm.mcFactor));
         }
         if (mcTab.empty()) mcTab.push_back(0.0f);
-        VFD_TRY(dMcTab.alloc(mcTab.size()));
-        VFD_TRY(dMcOff.alloc(mcOff.size()));
-        VFD_HIP(hipMemcpyAsync(dMcTab.p, mcTab.data(), mcTab.size() * sizeof(float), hipMemcpyHostToDevice, stream));
-        VFD_HIP(hipMemcpyAsync(dMcOff.p, mcOff.data(), mcOff.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                               stream));
-        F.mcTab = dMcTab.p;
-        F.mcOff = dMcOff.p;
+        float* tab = nullptr;
+        uint32_t* off = nullptr;
+        if (t_noAlloc) {  // sizes checked by vfd_step_no_alloc before the step began (mc_table_fits)
+            if (mcTab.size() > kStaticMcFloats || mcOff.size() > kStaticMcScenes)
+                return fail(VFD_ERROR_NOT_SUPPORTED, "vfd_step_no_alloc: curvature tables exceed the static scratch");
+            tab = reinterpret_cast<float*>(staticScratch + kStaticMcTabAt);
+            off = staticScratch + kStaticMcOffAt;
+        } else {
+            VFD_TRY(dMcTab.alloc(mcTab.size()));
+            VFD_TRY(dMcOff.alloc(mcOff.size()));
+            tab = dMcTab.p;
+            off = dMcOff.p;
+        }
+        VFD_HIP(hipMemcpyAsync(tab, mcTab.data(), mcTab.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        VFD_HIP(hipMemcpyAsync(off, mcOff.data(), mcOff.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        F.mcTab = tab;
+        F.mcOff = off;
         return VFD_OK;
     }
 
@@ -476,6 +513,8 @@ public:
     // (solver, cur); they are dropped whenever buffers are reallocated or the chunk length changes.
     bool useGraphs = VFD_DEFAULT_USE_GRAPHS != 0;
     hipGraphExec_t loopGraph[3][2] = {};
+    bool graphsStale = false;            // dropped during a no-alloc step; destroyed by the next call that may free
+    uint32_t* staticScratch = nullptr;   // this device's static scratch (vfd_step_no_alloc), looked up at init
     // Iterations each solver loop ran in the previous step (most over the scenes; Jacobi divergence, Jacobi
     // pressure, CG). With predictIterations the loop's first chunk is one iteration shorter than that, so a solve
     // that needs as many iterations as last time ends with one empty iteration in flight instead of a whole chunk.
@@ -551,6 +590,11 @@ public:
         S.frames.clear();
     }
     void drop_graphs() {
+        if (t_noAlloc) {  // hipGraphExecDestroy frees: leave it to the next call that may, and use no graph till then
+            graphsStale = true;
+            return;
+        }
+        graphsStale = false;
         for (auto& row : loopGraph)
             for (auto& g : row)
                 if (g) {
@@ -662,6 +706,8 @@ public:
         VFD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         numCU = cus > 0 ? (uint32_t)cus : 256u;
         if (configure_kernels() != 0) return fail(VFD_ERROR_HIP, "hipFuncSetAttribute (dynamic LDS) failed");
+        if (static_scratch_address(dev, &staticScratch) != 0)
+            return fail(VFD_ERROR_HIP, "hipGetSymbolAddress of the static scratch failed");
         for (uint32_t i = 0; i < n; ++i) {
             Scene* s = new Scene();
             s->desc = descs[i];
@@ -1103,23 +1149,37 @@ public:
             grids[k] = g;
             s.grid = g;
         }
-        if (cellTotal > cellCap) {
-            const uint32_t want = (uint32_t)grid_capacity(cellTotal);
-            VFD_TRY(cellCount.alloc(want));
-            VFD_TRY(cellStart.alloc(want));
-            VFD_TRY(scanScratch.alloc(scan_scratch_words(want)));
-            cellCap = want;
+        uint32_t *cCount = nullptr, *cStart = nullptr, *cScan = nullptr;
+        if (t_noAlloc) {  // the static scratch: checked before any kernel writes the particle state
+            if (cellTotal > kStaticCells)
+                return fail(VFD_ERROR_NOT_SUPPORTED,
+                            "vfd_step_no_alloc: the neighbour grid needs " + std::to_string(cellTotal) +
+                                " cells, more than the static scratch holds (" + std::to_string(kStaticCells) + ")");
+            cCount = staticScratch + kStaticCellCountAt;
+            cStart = staticScratch + kStaticCellStartAt;
+            cScan = staticScratch + kStaticScanAt;
+        } else {
+            if (cellTotal > cellCap) {
+                const uint32_t want = (uint32_t)grid_capacity(cellTotal);
+                VFD_TRY(cellCount.alloc(want));
+                VFD_TRY(cellStart.alloc(want));
+                VFD_TRY(scanScratch.alloc(scan_scratch_words(want)));
+                cellCap = want;
+            }
+            cCount = cellCount.p;
+            cStart = cellStart.p;
+            cScan = scanScratch.p;
         }
         VFD_HIP(hipMemcpyAsync(dGrid.p, grids.data(), S * sizeof(SceneGrid), hipMemcpyHostToDevice, stream));
         const int alt = cur ^ 1;
         VFD_TRY(timed("sort", [&] {
             launch_sort(L, dGrid.p, cellTotal, posNext.p, vel[cur].p, dv[cur].p, dk[cur].p, id[cur].p, pos.p,
-                        vel[alt].p, dv[alt].p, dk[alt].p, id[alt].p, cellOf.p, rank.p, tmpId.p, cellCount.p,
-                        cellStart.p, scanScratch.p);
+                        vel[alt].p, dv[alt].p, dk[alt].p, id[alt].p, cellOf.p, rank.p, tmpId.p, cCount, cStart,
+                        cScan);
         }));
         cur = alt;
         VFD_TRY(timed("build_lists", [&] {
-            launch_build_lists(L, dGrid.p, pos.p, cellStart.p, cellCount.p, nbrCount.p, nbrList.p, nbStats.p);
+            launch_build_lists(L, dGrid.p, pos.p, cStart, cCount, nbrCount.p, nbrList.p, nbStats.p);
         }));
         VFD_HIP(hipGetLastError());
         return VFD_OK;
@@ -1135,7 +1195,8 @@ public:
         // graphs are keyed by the double-buffer index of the particle arrays their launches captured
         const int buf = this->cur;
         auto issue = [&](uint32_t n, int slot) -> int {
-            if (useGraphs && !profiling && !taping && n == pollChunk && stream != nullptr) {
+            if (useGraphs && !profiling && !taping && n == pollChunk && stream != nullptr && !t_noAlloc) {
+                if (graphsStale) drop_graphs();
                 hipGraphExec_t& g = loopGraph[which][buf];
                 if (!g) {
                     hipGraph_t graph = nullptr;
@@ -1907,6 +1968,23 @@ int vfd_get_boundary_map(vfd_sim* sim, uint32_t scene, uint32_t body, uint32_t f
 int vfd_step(vfd_sim* sim) {
     DRIVE(sim);
     return guard(sim->impl.step(VFD_PHASE_POSITION));
+}
+int vfd_step_no_alloc(vfd_sim* sim) {
+    DRIVE(sim);
+    Simulation& S = sim->impl;
+    if (S.capture)
+        return fail(VFD_ERROR_STATE, "vfd_step_no_alloc: frame capture allocates frame memory; switch it off with "
+                                     "vfd_set_frame_capture(sim, 0)");
+    if (S.taping) return fail(VFD_ERROR_STATE, "vfd_step_no_alloc: adjoint recording allocates its records");
+    if (S.profiling) return fail(VFD_ERROR_STATE, "vfd_step_no_alloc: profiling creates timing events");
+    if (!S.staticScratch) return fail(VFD_ERROR_STATE, "vfd_step_no_alloc: no static scratch on this device");
+    if (!S.mc_tables_fit_static())
+        return fail(VFD_ERROR_NOT_SUPPORTED, "vfd_step_no_alloc: the surface-tension curvature tables of these "
+                                             "descriptions can exceed the static scratch (" +
+                                                 std::to_string(vfd::kStaticMcFloats) + " floats, " +
+                                                 std::to_string(vfd::kStaticMcScenes) + " scenes)");
+    vfd::NoAllocScope scope;
+    return guard(S.step(VFD_PHASE_POSITION));
 }
 int vfd_advance(vfd_sim* sim, uint32_t steps) {
     DRIVE(sim);

@@ -251,6 +251,11 @@ class DFSPHSimulation:
     def OnUpdate(self):
         check(self._lib.vfd_step(self._h), "vfd_step")
 
+    def step_no_alloc(self):
+        """OnUpdate without device allocation (no reference counterpart): the grid and curvature-table scratch come
+        from the library's static scratch; needs frame capture off and the state already allocated."""
+        check(self._lib.vfd_step_no_alloc(self._h), "vfd_step_no_alloc")
+
     def Simulate(self, asynchronous=False):
         """DFSPHImplementation::Simulate (DFSPHImplementation.cu:33-61): restore the baked particles and run until
         FrameCount frames are captured. asynchronous=True runs the frame loop on a worker thread of the library and
@@ -445,6 +450,9 @@ class DFSPHBatch:
 
     def step(self):
         check(self._lib.vfd_step(self._h), "vfd_step")
+
+    def step_no_alloc(self):
+        check(self._lib.vfd_step_no_alloc(self._h), "vfd_step_no_alloc")
 
     def advance(self, steps):
         check(self._lib.vfd_advance(self._h, int(steps)), "vfd_advance")
