@@ -119,8 +119,9 @@ template <class T> inline void orthogonal_vectors(V3<T> vec, V3<T>& x, V3<T>& y)
 
 // ---------------------------------------------------------------------------------------------------------------
 // Tabulated cubic spline kernel. Kernel/DFSPHKernels.h:7-143 (SetRadius :13-41, GetW :43-65, GetGradientW :67-80,
-// CalculateW :86-104, CalculateGradientW :110-131). Pinned by tests/test_oracle_kernel.py (closed-form spline,
-// normalisation integral, W(R)=0, gradient finite differences) and by GPU bit-equality of the tables.
+// CalculateW :86-104, CalculateGradientW :110-131). Pinned by tests/test_oracle_pins.py
+// (test_kernel_table_closed_form: closed-form spline and gradient at both radii;
+// test_fp32_kernel_table_rounds_the_same_formula) and by GPU bit-equality of the tables.
 // ---------------------------------------------------------------------------------------------------------------
 template <class T> struct CubicKernel {
     std::vector<T> W, G;  // W[10000], G[10001]
@@ -313,8 +314,9 @@ static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w
 // 32-node cubic serendipity shape functions on [-1,1]^3 (Discregrid layout): SDF.cu:403-...,
 // SDFDeviceData.cuh:36-370. Node order: 8 corners (bit0 = x, bit1 = y, bit2 = z), then x-edge, y-edge and z-edge
 // nodes at -1/3 and +1/3. Written from the polynomials (corner: (1/64)(1+-x)(1+-y)(1+-z)(9(x^2+y^2+z^2)-19);
-// edge: (9/64)(1-s^2)(1+-3s)(1+-a)(1+-b)). Pinned by tests/test_oracle_sdf.py (partition of unity, cubic
-// reproduction, nodal interpolation, gradient vs finite differences).
+// edge: (9/64)(1-s^2)(1+-3s)(1+-a)(1+-b)). Pinned by tests/test_oracle_pins.py
+// (test_shape_functions_interpolate_cubics: exact reproduction of cubics and their gradients;
+// test_cell_node_layout_matches_local_coordinates).
 // ---------------------------------------------------------------------------------------------------------------
 template <class T> void shape_function(V3<T> xi, T N[32], V3<T> dN[32]) {
     const T x = xi.x, y = xi.y, z = xi.z;
@@ -416,8 +418,8 @@ template <class T> struct Box {
 // Volume map: SDF(domain,res) SDF.cu:8-14, AddFunction :45-139, host Interpolate :141-215, IndexToNodePosition
 // :313-373, device DetermineShapeFunction / Interpolate SDFDeviceData.cuh:372-474. Field 0 = signed distance
 // sign*(d - (Padding - r)) (RigidBody.cu:39-42), field 1 = boundary volume 0.8 * GL16^3 integral (RigidBody.cu:44-72,
-// GaussQuadrature.cpp:5621-5655). Pinned by tests/test_oracle_sdf.py (closed-form plane volume, deep-solid limit,
-// node values = analytic distance).
+// GaussQuadrature.cpp:5621-5655). Pinned by tests/test_oracle_pins.py (test_volume_map_distance_field_is_exact:
+// node values = analytic distance; test_volume_field_plane_closed_form: closed-form plane volume).
 // ---------------------------------------------------------------------------------------------------------------
 template <class T> struct VolumeMap {
     Box<T> domain;

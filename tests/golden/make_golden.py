@@ -16,6 +16,7 @@ data that pin parts of the path are two numeric tables, read here as text (nothi
 
 Outputs (small fixtures, committed):
   halton_samples.json    every 97th point of the table plus the first and last 64 (index -> xyz)
+  halton_table.npz       the whole table: 49,152 values as written in the source (float64 "values")
   gauss_legendre_16.json the 16 abscissae and weights of row p = 30
   scene_default.json     the example scene file, byte for byte (input of the scene-description reader)
   models/*.obj           the four meshes, byte for byte (inputs of the mesh distance and sampler tests)
@@ -46,6 +47,14 @@ def halton(ref):
         "count": 16384,
         "points": {str(i): vals[3 * i: 3 * i + 3] for i in idx},
     }
+
+
+def halton_table(ref):
+    path = os.path.join(ref, "VFD/Source/Simulation/DFSPH/HaltonVec323.cuh")
+    text = open(path, encoding="utf-8", errors="replace").read()
+    vals = _numbers(text[text.index("{") + 1: text.rindex("}")])
+    assert len(vals) == 49152, len(vals)
+    return vals
 
 
 def gauss_legendre(ref):
@@ -88,9 +97,11 @@ def main():
             f.write(data)
     with open(os.path.join(HERE, "halton_samples.json"), "w") as f:
         json.dump(halton(ref), f, separators=(",", ":"))
+    import numpy as np
+    np.savez_compressed(os.path.join(HERE, "halton_table.npz"), values=np.array(halton_table(ref), np.float64))
     with open(os.path.join(HERE, "gauss_legendre_16.json"), "w") as f:
         json.dump(gauss_legendre(ref), f, indent=1)
-    print("wrote halton_samples.json, gauss_legendre_16.json, scene_default.json, models/")
+    print("wrote halton_samples.json, halton_table.npz, gauss_legendre_16.json, scene_default.json, models/")
 
 
 if __name__ == "__main__":

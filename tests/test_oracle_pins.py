@@ -34,6 +34,11 @@ def test_halton_matches_reference_table():
         assert np.array_equal(h32[3 * i: 3 * i + 3], np.array(v, dtype=np.float32))
     pts = h64.reshape(-1, 3)
     assert np.abs(np.linalg.norm(pts, axis=1) - 1).max() < 1e-14
+    # the whole table, 49,152 values as the reference's source writes them (15 decimals)
+    full = np.load(os.path.join(GOLDEN, "halton_table.npz"))["values"]
+    assert full.shape == (49152,)
+    assert np.abs(h64 - full).max() < 4e-15  # the rounding of the 15th decimal
+    assert np.array_equal(h32, full.astype(np.float32))
 
 
 def test_gauss_legendre_matches_reference_table():
