@@ -4,7 +4,12 @@ Compile-time shapes (each candidate is a separate build of the library, vfd_amd.
   VFD_LIST_CHUNK     halo-slot list entries fetched per batch in the one-record staged neighbour loops
   VFD_LIST_PREFETCH  the next batch is requested before the current one is consumed (1) or not (0)
   VFD_LIST_CHUNK2, VFD_LIST_PREFETCH2  the same for the two-record loops
-  VFD_PACK_RECORDS   halo record halves read as three floats stored in 12 bytes (1) or as 16-byte float4 (0)
+  VFD_PACK_RECORDS   halo record halves read as three floats stored in 12 bytes (1), as 16-byte float4 (0), or a
+                     record of two three-float halves as 16 + 8 bytes (2)
+  VFD_GROUP_TILES    tiles per neighbour group (4: 1024-thread blocks, one per CU; 2: 512-thread blocks, two per
+                     CU, halos of at most 2048 particles)
+  VFD_HALO_MAX       largest halo a 4-tile group stages in LDS
+  VFD_SEARCH_CELLS   neighbour lists built one grid cell per wave (1) or one particle per thread (0)
   VFD_SEARCH_UNROLL  candidate positions loaded per batch in the neighbour search
   VFD_SEARCH_PREFETCH  the ranges of all 27 cells are loaded before the traversal (1) or cell by cell (0)
   VFD_STAGE_LISTS    neighbour lists staged in LDS and written row by row (1) or stored as found (0)
@@ -35,18 +40,33 @@ TUNING_JSON = os.path.join(HERE, "tuning.json")
 
 AXES = {
     "VFD_LIST_CHUNK": [4, 8],
-    "VFD_LIST_PREFETCH": [0, 1],
+    "VFD_LIST_PREFETCH": [0, 1, 2],
     "VFD_LIST_CHUNK2": [2, 4, 8],
-    "VFD_LIST_PREFETCH2": [0, 1],
+    "VFD_LIST_PREFETCH2": [1, 2],
     "VFD_SEARCH_UNROLL": [4, 8],
     "VFD_SEARCH_PREFETCH": [0, 1],
     "VFD_STAGE_LISTS": [0, 1],
-    "VFD_PACK_RECORDS": [0, 1],
+    "VFD_PACK_RECORDS": [0, 1, 2],
+    "VFD_GROUP_TILES": [2, 4],
+    "VFD_HALO_MAX": [3072, 3808],
+    "VFD_SEARCH_CELLS": [0, 1],
 }
-DEFAULTS = {"VFD_LIST_CHUNK": 8, "VFD_LIST_PREFETCH": 1, "VFD_LIST_CHUNK2": 4, "VFD_LIST_PREFETCH2": 1,
-            "VFD_SEARCH_UNROLL": 8, "VFD_SEARCH_PREFETCH": 1, "VFD_STAGE_LISTS": 0, "VFD_PACK_RECORDS": 1}
+DEFAULTS = {"VFD_LIST_CHUNK": 4, "VFD_LIST_PREFETCH": 1, "VFD_LIST_CHUNK2": 2, "VFD_LIST_PREFETCH2": 2,
+            "VFD_SEARCH_UNROLL": 8, "VFD_SEARCH_PREFETCH": 1, "VFD_STAGE_LISTS": 0, "VFD_PACK_RECORDS": 2,
+            "VFD_GROUP_TILES": 4, "VFD_HALO_MAX": 3808, "VFD_SEARCH_CELLS": 0}
+# shapes that only make sense together: 2-tile groups take at most 2048-particle halos, two blocks per CU
+LINKED = {("VFD_GROUP_TILES", 2): {"VFD_HALO_MAX": 2048, "VFD_GROUPS_PER_CU": 2}}
 INTERVALS = [2, 4, 8, 16]
 DEFAULT_INTERVAL = 4
+
+
+def shape_defines(shape):
+    """The defines of a shape, with the values LINKED to its choices (they override the shape's own)."""
+    d = dict(shape)
+    for (k, v), extra in LINKED.items():
+        if d.get(k) == v:
+            d.update(extra)
+    return ["%s=%d" % kv for kv in sorted(d.items())]
 
 
 def chosen_defines():
@@ -56,7 +76,7 @@ def chosen_defines():
         t = json.load(open(TUNING_JSON))
         shape.update({k: int(v) for k, v in t.get("shape", {}).items() if k in AXES})
         interval = int(t.get("SolverCheckInterval", interval))
-    return ["%s=%d" % kv for kv in sorted(shape.items())] + ["VFD_DEFAULT_CHECK_INTERVAL=%d" % interval]
+    return shape_defines(shape) + ["VFD_DEFAULT_CHECK_INTERVAL=%d" % interval]
 
 
 def _name(shape):
@@ -75,7 +95,7 @@ def _build_one(shape):
     from vfd_amd import build as vb
     out = os.path.join(TUNE_DIR, _name(shape), "libvfd_hip.so")
     os.makedirs(os.path.dirname(out), exist_ok=True)
-    vb.build(out=out, defines=["%s=%d" % kv for kv in sorted(shape.items())])
+    vb.build(out=out, defines=shape_defines(shape))
     import shutil
     shutil.rmtree(os.path.join(os.path.dirname(out), "_build"), ignore_errors=True)
     return out

@@ -38,9 +38,23 @@ constexpr double kEps = 1.0e-5;              // Core/Math/Math.h:6 (double liter
 // staged in LDS by the neighbour-loop kernels. A group's halo holds at most kHaloMax particles; a group with more
 // keeps gathering from the global arrays (haloCount = kHaloNone). The lists of a staged group are also stored as
 // 16-bit halo slots.
-constexpr unsigned kGroupTiles = 4;
+// VFD_GROUP_TILES, VFD_HALO_MAX and VFD_GROUPS_PER_CU (the persistent grid of the group kernels, in blocks per CU)
+// are compile-time launch shapes the autotuner may vary.
+#ifndef VFD_GROUP_TILES
+#define VFD_GROUP_TILES 4
+#endif
+#ifndef VFD_HALO_MAX
+#define VFD_HALO_MAX 3808
+#endif
+#ifndef VFD_GROUPS_PER_CU
+#define VFD_GROUPS_PER_CU 1
+#endif
+constexpr unsigned kGroupTiles = VFD_GROUP_TILES;
 constexpr unsigned kGroupSize = kTile * kGroupTiles;
-constexpr unsigned kHaloMax = 3808;
+constexpr unsigned kHaloMax = VFD_HALO_MAX;
+constexpr unsigned kGroupsPerCU = VFD_GROUPS_PER_CU;
+static_assert(kGroupTiles == 1 || kGroupTiles == 2 || kGroupTiles == 4, "a group is 1, 2 or 4 tiles");
+static_assert(kHaloMax <= 4 * kGroupSize && kHaloMax <= 65535, "halo slots are 16-bit, filled four per thread");
 constexpr uint32_t kHaloNone = 0xFFFFFFFFu;
 
 // Constant per-scene parameters (DFSPHSimulationInfo + DFSPHSimulationDescription, DFSPHImplementation.cu:318-368).

@@ -289,7 +289,10 @@ struct KLds {
 // estimate s (within 1 ulp) and its two neighbours s-, s+ (one ulp apart); the root rounds to s- when x - s- s <= 0
 // and to s+ when x - s+ s > 0. Both residuals are exact-sign single-rounding fmas, and x - s- s is a multiple of
 // ulp(s)^2, so it cannot fall strictly between 0 and the midpoint gap ulp(s)^2 / 4 - the tests decide the rounding
-// exactly. Above 2^-64 the residuals stay normal numbers. Smaller, zero, negative and NaN inputs take sqrtf.
+// exactly. Above 2^-64 the residuals stay normal numbers. A smaller input is scaled by 2^126 first (exact, and the
+// scaled value is at least 2^-23) and the root by 2^-63 after (exact: the root is a normal number), which commutes
+// with the rounding; zero gives zero and NaN gives NaN. No branch, so that the loops can issue the LDS reads of a
+// whole chunk before the first root.
 // The neighbour loops use it where it pays: the loops of one-record-and-scalar and viscosity records (LoadXS,
 // LoadXSV: 5% and 1% faster on the 2M dam break); the two-record and density loops were 2-3% slower with it and keep
 // sqrtf (both give the same value).
@@ -298,12 +301,13 @@ struct KLds {
 #endif
 __device__ __forceinline__ float sqrt_rn(float x) {
 #if VFD_FAST_SQRT
-    if (__builtin_expect(!(x >= 0x1.0p-64f), 0)) return sqrtf(x);
-    const float s = __builtin_amdgcn_sqrtf(x);
+    const bool small = x < 0x1.0p-64f;
+    const float xs = small ? x * 0x1.0p126f : x;
+    const float s = __builtin_amdgcn_sqrtf(xs);
     const float sd = __uint_as_float(__float_as_uint(s) - 1u), su = __uint_as_float(__float_as_uint(s) + 1u);
-    float r = fmaf(-sd, s, x) <= 0.0f ? sd : s;
-    r = fmaf(-su, s, x) > 0.0f ? su : r;
-    return r;
+    float r = fmaf(-sd, s, xs) <= 0.0f ? sd : s;
+    r = fmaf(-su, s, xs) > 0.0f ? su : r;
+    return small ? r * 0x1.0p-63f : r;
 #else
     return sqrtf(x);
 #endif
@@ -316,9 +320,9 @@ template <bool kShortSqrt = false>
 __device__ __forceinline__ uint32_t gcode(float R, float invStep, f3 r) {
     const float r2 = dot3(r, r);
     const float rl = kShortSqrt ? sqrt_rn(r2) : sqrtf(r2);
-    if (!(rl <= R)) return kGCodeZero;
     const unsigned p0 = (unsigned)(rl * invStep);
-    return p0 < kTableEntries - 1u ? p0 : kTableEntries - 1u;
+    const unsigned p = p0 < kTableEntries - 1u ? p0 : kTableEntries - 1u;
+    return rl <= R ? p : kGCodeZero;
 }
 __device__ __forceinline__ f3 gradW(const KLds& k, f3 r) {
     const float rl = sqrtf(dot3(r, r));
@@ -337,7 +341,7 @@ __device__ __forceinline__ f3 gradW(const KLds& k, f3 r) {
 // list entry. Both paths hand the same values to the same arithmetic in the same list order.
 struct NbrSrc {
     const float* rec;                      // LDS records of the halo (layout: HaloRec); nullptr: global gathers
-    const uint16_t* __restrict__ local;    // [j * cap + i] halo slots (staged groups)
+    const uint32_t* __restrict__ local;    // [j * cap + i] halo slot | table code << 16 (staged groups)
     const uint32_t* __restrict__ list;     // [j * cap + i] particle slots
     const uint32_t* __restrict__ count;    // [i]
     uint32_t cap;
@@ -417,16 +421,34 @@ template <int R> struct ListShape {
 // Halo records in LDS, as two arrays of kHaloMax records: lo of slot s at rec[L * s], hi at rec[L * kHaloMax + H * s]
 // (H = 0 for one-record loops). With VFD_PACK_RECORDS a half that the neighbour terms read as three floats is stored
 // in 12 bytes (L or H = 3), which spreads a wave's random reads over all 32 banks instead of 8 groups of 4; otherwise
-// every half is a 16-byte float4.
+// every half is a 16-byte float4. VFD_PACK_RECORDS = 2 stores a record of two three-float halves as 16 + 8 bytes
+// instead ({lo.xyz, hi.x} and {hi.y, hi.z}: L = 4, H = 2), read with one 16-byte and one 8-byte LDS load rather than
+// two 12-byte halves of two loads each.
 #ifndef VFD_PACK_RECORDS
 #define VFD_PACK_RECORDS 1
 #endif
+#ifndef VFD_SCHED_PHASES
+#define VFD_SCHED_PHASES 1
+#endif
 template <class Load> struct HaloRec {
-    static constexpr int L = VFD_PACK_RECORDS ? Load::kLo : 4;
-    static constexpr int H = Load::kHi == 0 ? 0 : (VFD_PACK_RECORDS ? Load::kHi : 4);
+    static constexpr bool kSplit = VFD_PACK_RECORDS == 2 && Load::kLo == 3 && Load::kHi == 3;
+    static constexpr int L = kSplit ? 4 : (VFD_PACK_RECORDS ? Load::kLo : 4);
+    static constexpr int H = Load::kHi == 0 ? 0 : (kSplit ? 2 : (VFD_PACK_RECORDS ? Load::kHi : 4));
+    static constexpr size_t kBytes = (size_t)(L + H) * sizeof(float) * kHaloMax;  // LDS of the records
     static __device__ __forceinline__ float4 get(const float* p, int n) {
         if (n == 4) return *reinterpret_cast<const float4*>(p);
         return make_float4(p[0], p[1], p[2], 0.0f);
+    }
+    // both halves of slot s
+    static __device__ __forceinline__ void load(const float* rec, uint32_t s, float4& lo, float4& hi) {
+        if (kSplit) {
+            lo = *reinterpret_cast<const float4*>(rec + 4 * s);
+            const float2 t = *reinterpret_cast<const float2*>(rec + 4 * kHaloMax + 2 * s);
+            hi = make_float4(lo.w, t.x, t.y, 0.0f);
+        } else {
+            lo = get(rec + L * s, L);
+            hi = H ? get(rec + L * kHaloMax + H * s, H) : lo;
+        }
     }
     static __device__ __forceinline__ void put(float* p, int n, const float4& v) {
         if (n == 4) {
@@ -442,6 +464,11 @@ template <class Load> struct HaloRec {
         return get(rec + L * kHaloMax + H * s, H);
     }
     static __device__ __forceinline__ void store(float* rec, uint32_t s, const float4& l, const float4& h) {
+        if (kSplit) {
+            *reinterpret_cast<float4*>(rec + 4 * s) = make_float4(l.x, l.y, l.z, h.x);
+            *reinterpret_cast<float2*>(rec + 4 * kHaloMax + 2 * s) = make_float2(h.y, h.z);
+            return;
+        }
         put(rec + L * s, L, l);
         if (H) put(rec + L * kHaloMax + H * s, H, h);
     }
@@ -457,7 +484,7 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
         if (n == 0) return;
         const float* rec = src.rec;
         using HR = HaloRec<Load>;
-        const uint16_t* __restrict__ li = src.local + i;
+        const uint32_t* __restrict__ li = src.local + i;
         const uint32_t last = n - 1u;
         auto fetch = [&](uint32_t (&e)[C], uint32_t b) {
 #pragma unroll
@@ -466,12 +493,14 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
         auto consume = [&](const uint32_t (&e)[C], uint32_t b) {
             float4 lo[C], hi[C];
             TV tv[C];
+            // every record read of the chunk is issued before the first root, every table read before the first
+            // neighbour term (VFD_SCHED_PHASES; the arithmetic and its order are the same either way)
 #pragma unroll
             for (uint32_t u = 0; u < C; ++u) {
-                lo[u] = HR::lo(rec, e[u]);
-                hi[u] = HR::H ? HR::hi(rec, e[u]) : lo[u];
-                tv[u] = tab[gcode<Load::kShortSqrt>(cutR, invStep, xi - ld3(lo[u]))];
+                HR::load(rec, e[u] & 0xFFFFu, lo[u], hi[u]);
+                tv[u] = tab[e[u] >> 16];  // the pair's table code, computed once per step by k_build_halo
             }
+            if (VFD_SCHED_PHASES) __builtin_amdgcn_sched_barrier(0);
             if (b + C <= n) {
 #pragma unroll
                 for (uint32_t u = 0; u < C; ++u) f(lo[u], hi[u], tv[u]);
@@ -481,17 +510,19 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
                     if (b + u < n) f(lo[u], hi[u], tv[u]);
             }
         };
-        // ring of D chunk buffers: chunk c lives in e[c % D]; chunk c + D - 1 is requested before chunk c is consumed
+        // ring of D chunk buffers: chunk c lives in e[c % D]; chunk c + D - 1 is requested before chunk c is consumed.
+        // The requests are unconditional (rows past a list's end repeat its last row): under a per-lane condition
+        // the compiler cannot count the loads in flight and waits for all of them, the prefetch included, before
+        // the current chunk's first use.
         constexpr uint32_t D = ListShape<R>::depth;
         uint32_t e[D][C];
 #pragma unroll
-        for (uint32_t k = 0; k + 1 < D; ++k)
-            if (k * C < n) fetch(e[k], k * C);
+        for (uint32_t k = 0; k + 1 < D; ++k) fetch(e[k], k * C);
         for (; j < n; j += D * C) {
 #pragma unroll
             for (uint32_t k = 0; k < D; ++k) {
                 const uint32_t b = j + k * C;
-                if (b + (D - 1) * C < n) fetch(e[(k + D - 1) % D], b + (D - 1) * C);
+                fetch(e[(k + D - 1) % D], b + (D - 1) * C);
                 if (b < n) consume(e[k], b);
             }
         }

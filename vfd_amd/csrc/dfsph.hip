@@ -24,9 +24,17 @@ __device__ __forceinline__ NbrSrc global_src(const Fields& F, uint32_t cap) {
     return NbrSrc{nullptr, F.nbrLocal, F.nbrList, F.nbrCount, cap};
 }
 
+// Dynamic LDS of the staged kernels: the kernel table, then the halo records of the current group.
+constexpr size_t kLdsDensity = kLdsWG + HaloRec<LoadX>::kBytes;
+constexpr size_t kLdsXS = kLdsG + HaloRec<LoadXS>::kBytes;
+constexpr size_t kLdsXV = kLdsG + HaloRec<LoadXV>::kBytes;
+constexpr size_t kLdsXSV = kLdsG + HaloRec<LoadXSV>::kBytes;
+static_assert(kLdsDensity <= 160 * 1024 && kLdsXS <= 160 * 1024 && kLdsXV <= 160 * 1024 && kLdsXSV <= 160 * 1024,
+              "staged kernels must fit the 160 KiB LDS of a CU");
+
 // ---- ComputeDensityKernel (DFSPHKernels.cu:152-187) + ComputeDFSPHFactorKernel (:189-231) ----------------------
 // One table fetch per pair gives W (cutoff r^2 <= R^2) and gradW (cutoff |r| <= R); the {W, G} table sits in LDS.
-__global__ void __launch_bounds__(kGroupSize) k_density_factor(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_density_factor(const SceneParams* __restrict__ P,
                                                            const uint32_t* __restrict__ tileScene, KTabHost kh,
                                                            Fields F, uint32_t cap, TileGrid tg) {
     extern __shared__ float4 lds[];
@@ -69,7 +77,7 @@ __global__ void __launch_bounds__(kGroupSize) k_density_factor(const SceneParams
 }
 
 void launch_density_factor(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_density_factor, dim3(L.gridGroups), dim3(kGroupSize), kLdsWG + kHaloMax * sizeof(float4),
+    hipLaunchKernelGGL(k_density_factor, dim3(L.gridGroups), dim3(kGroupSize), kLdsDensity,
                        L.stream, L.P, L.tileScene, kt, F, L.capacity, TileGrid(L.tg));
 }
 
@@ -97,11 +105,12 @@ __device__ __forceinline__ f3 pressure_accel(const SceneParams& p, const KLds& k
                                              uint32_t i, const float* __restrict__ kap, const NbrSrc& src) {
     const f3 xi = ld3(F.pos[i]);
     const float ki = kap[i];
+    const float negV = -p.V;  // read before the loop: a global read inside it would wait for the list prefetch
     f3 a = mk3(0.0f, 0.0f, 0.0f);
     each_neighbor_rec<1>(src, i, k.R, k.invStep, xi, k.sG, LoadXS{F.pos, kap}, [&](float4 lo, float4, float g) {
         const float s = ki + lo.w;
         if (above_eps(fabsf(s))) {
-            const f3 gj = (-p.V) * (g * (xi - ld3(lo)));
+            const f3 gj = negV * (g * (xi - ld3(lo)));
             a = a + s * gj;
         }
     });
@@ -194,7 +203,7 @@ __global__ void __launch_bounds__(kTile) k_jacobi_finish(const SceneParams* __re
 
 // ---- divergence solve (ComputeDivergence, DFSPHImplementation.cu:506-575) ---------------------------------------
 // ComputeDensityChangeKernel (DFSPHKernels.cu:483-529)
-__global__ void __launch_bounds__(kGroupSize) k_div_init(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_div_init(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                     uint32_t cap, TileGrid tg) {
     VFD_HALO_KERNEL_PROLOGUE(kh)
@@ -212,7 +221,7 @@ __global__ void __launch_bounds__(kGroupSize) k_div_init(const SceneParams* __re
 }
 
 // ComputePressureAccelerationAndDivergenceKernel (:388-432)
-__global__ void __launch_bounds__(kGroupSize) k_div_accel(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_div_accel(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
     if (F.live[0] == 0u) return;  // every scene has left this solve (block-uniform)
@@ -225,7 +234,7 @@ __global__ void __launch_bounds__(kGroupSize) k_div_accel(const SceneParams* __r
 }
 
 // DivergenceSolveIterationKernel (:531-553) + per-tile residual partial sums (DFSPHImplementation.cu:554-563)
-__global__ void __launch_bounds__(kGroupSize) k_div_iter(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_div_iter(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                     uint32_t cap, TileGrid tg) {
     if (F.live[0] == 0u) return;  // every scene has left this solve (block-uniform)
@@ -249,7 +258,7 @@ __global__ void __launch_bounds__(kGroupSize) k_div_iter(const SceneParams* __re
 }
 
 // ComputePressureAccelerationAndFactorKernel (:555-602) + ClearAccelerationKernel (:24-37)
-__global__ void __launch_bounds__(kGroupSize) k_div_final(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_div_final(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
     VFD_HALO_KERNEL_PROLOGUE(kh)
@@ -272,15 +281,15 @@ void launch_scene_init_div(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_jacobi_begin, dim3(1), dim3(kTile), 0, L.stream, L.P, F.state, F.live, L.sceneCount, 0);
 }
 void launch_div_init(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_init, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_div_init, dim3(L.gridGroups), dim3(kGroupSize), kLdsXV, L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 void launch_div_accel(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_accel, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_div_accel, dim3(L.gridGroups), dim3(kGroupSize), kLdsXS, L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 void launch_div_iter(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_iter, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_div_iter, dim3(L.gridGroups), dim3(kGroupSize), kLdsXV, L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 // the residual reduction and loop condition after a Jacobi iteration; which = 0 divergence, 1 pressure
@@ -288,13 +297,13 @@ void launch_jacobi_finish(const Launch& L, const Fields& F, int which) {
     hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F, which);
 }
 void launch_div_final(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_div_final, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_div_final, dim3(L.gridGroups), dim3(kGroupSize), kLdsXS, L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 
 // ---- pressure solve (ComputePressure, DFSPHImplementation.cu:443-504) -------------------------------------------
 // ComputeDensityAdvectionKernel (DFSPHKernels.cu:233-278)
-__global__ void __launch_bounds__(kGroupSize) k_pres_init(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_pres_init(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
     VFD_HALO_KERNEL_PROLOGUE(kh)
@@ -314,7 +323,7 @@ __global__ void __launch_bounds__(kGroupSize) k_pres_init(const SceneParams* __r
 }
 
 // ComputePressureAccelerationKernel (:342-386)
-__global__ void __launch_bounds__(kGroupSize) k_pres_accel(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_pres_accel(const SceneParams* __restrict__ P,
                                                       const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                       uint32_t cap, TileGrid tg) {
     if (F.live[1] == 0u) return;  // every scene has left this solve (block-uniform)
@@ -327,7 +336,7 @@ __global__ void __launch_bounds__(kGroupSize) k_pres_accel(const SceneParams* __
 }
 
 // PressureSolveIterationKernel (:317-340) + per-tile residual partial sums (DFSPHImplementation.cu:483-491)
-__global__ void __launch_bounds__(kGroupSize) k_pres_iter(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_pres_iter(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
     if (F.live[1] == 0u) return;  // every scene has left this solve (block-uniform)
@@ -351,7 +360,7 @@ __global__ void __launch_bounds__(kGroupSize) k_pres_iter(const SceneParams* __r
 }
 
 // ComputePressureAccelerationAndVelocityKernel (:434-481) + ComputePositionKernel (:54-67) into posNext
-__global__ void __launch_bounds__(kGroupSize) k_pres_final(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_pres_final(const SceneParams* __restrict__ P,
                                                       const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                       uint32_t cap, TileGrid tg, int writeNext) {
     VFD_HALO_KERNEL_PROLOGUE(kh)
@@ -376,19 +385,19 @@ void launch_scene_init_pres(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_jacobi_begin, dim3(1), dim3(kTile), 0, L.stream, L.P, F.state, F.live, L.sceneCount, 1);
 }
 void launch_pres_init(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_pres_init, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_pres_init, dim3(L.gridGroups), dim3(kGroupSize), kLdsXV, L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 void launch_pres_accel(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_pres_accel, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
+    hipLaunchKernelGGL(k_pres_accel, dim3(L.gridGroups), dim3(kGroupSize), kLdsXS, L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg));
 }
 void launch_pres_iter(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_pres_iter, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_pres_iter, dim3(L.gridGroups), dim3(kGroupSize), kLdsXV, L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 void launch_pres_final(const Launch& L, const KTabHost& kt, const Fields& F, int writeNext) {
-    hipLaunchKernelGGL(k_pres_final, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
+    hipLaunchKernelGGL(k_pres_final, dim3(L.gridGroups), dim3(kGroupSize), kLdsXS, L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg), writeNext);
 }
 
@@ -538,7 +547,7 @@ __device__ __forceinline__ bool tangential(const SceneParams& p, const KLds& k, 
 
 // ComputeViscosityPreconditionerKernel (:604-684) + ComputeViscosityGradientKernel (:686-756): inverse diagonal
 // block, right-hand side b = v (the boundary rhs term is dot(0, .) = 0) and initial guess x0 = v + dv.
-__global__ void __launch_bounds__(kGroupSize) k_visc_precond(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_visc_precond(const SceneParams* __restrict__ P,
                                                         const uint32_t* __restrict__ tileScene, KTabHost kh,
                                                         Fields F, uint32_t cap, TileGrid tg) {
     VFD_HALO_KERNEL_PROLOGUE(kh)
@@ -548,6 +557,7 @@ __global__ void __launch_bounds__(kGroupSize) k_visc_precond(const SceneParams* 
         if (!c.valid || !p.enableVisc) return;
         const f3 xi = ld3(F.pos[c.i]);
         const float d001 = 0.01f * p.R2;
+        const float mu10 = 10.0f * p.mu, m = p.m;  // read before the loop (see pressure_accel)
         float M[9];
 #pragma unroll
         for (int e = 0; e < 9; ++e) M[e] = 0.0f;
@@ -555,7 +565,7 @@ __global__ void __launch_bounds__(kGroupSize) k_visc_precond(const SceneParams* 
                              [&](float4 lo, float4, float gt) {
             const f3 xij = xi - ld3(lo);
             const f3 gw = gt * xij;
-            const float s = 10.0f * p.mu * (p.m / lo.w) / (length2(xij) + d001);
+            const float s = mu10 * (m / lo.w) / (length2(xij) + d001);
             const float a[3] = {xij.x, xij.y, xij.z}, g[3] = {gw.x, gw.y, gw.z};
 #pragma unroll
             for (int r = 0; r < 3; ++r)
@@ -597,12 +607,13 @@ __device__ __forceinline__ f3 visc_matvec(const SceneParams& p, const KLds& k, c
                                           uint32_t i, const float4* __restrict__ in, float dt, const NbrSrc& src) {
     const f3 xi = ld3(F.pos[i]), vi = ld3(in[i]);
     const float d001 = 0.01f * p.R2;
+    const float mu10 = 10.0f * p.mu, m = p.m;  // read before the loop (see pressure_accel)
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
     each_neighbor_rec<2>(src, i, k.R, k.invStep, xi, k.sG, LoadXSV{F.pos, F.density, in},
                          [&](float4 lo, float4 hi, float g) {
         const f3 xij = xi - ld3(lo);
         const f3 gw = g * xij;
-        const float s = 10.0f * p.mu * (p.m / lo.w) * dot3(vi - ld3(hi), xij) / (length2(xij) + d001);
+        const float s = mu10 * (m / lo.w) * dot3(vi - ld3(hi), xij) / (length2(xij) + d001);
         acc = acc + s * gw;
     });
     if (p.muB != 0.0f) {
@@ -624,7 +635,7 @@ __device__ __forceinline__ f3 visc_matvec(const SceneParams& p, const KLds& k, c
 }
 
 // SolveViscosity set-up (DFSPHImplementation.cu:627-691): r = b - A x0, ||b||^2, ||r||^2, p = M^-1 r, |r.p|
-__global__ void __launch_bounds__(kGroupSize) k_cg_begin(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_cg_begin(const SceneParams* __restrict__ P,
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                     uint32_t cap, TileGrid tg) {
     VFD_HALO_KERNEL_PROLOGUE(kh)
@@ -679,7 +690,7 @@ __global__ void __launch_bounds__(kTile) k_cg_begin_finish(const SceneParams* __
 }
 
 // CG matvec Ap and p.Ap -> alpha (:695-720)
-__global__ void __launch_bounds__(kGroupSize) k_cg_matvec(const SceneParams* __restrict__ P,
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_cg_matvec(const SceneParams* __restrict__ P,
                                                      const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                      uint32_t cap, TileGrid tg) {
     if (F.live[2] == 0u) return;  // every scene has left this solve (block-uniform)
@@ -791,18 +802,18 @@ __global__ void __launch_bounds__(kTile) k_visc_apply(const SceneParams* __restr
 }
 
 void launch_visc_precond(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_visc_precond, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 1 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
+    hipLaunchKernelGGL(k_visc_precond, dim3(L.gridGroups), dim3(kGroupSize), kLdsXS, L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg));
 }
 void launch_cg_begin(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_begin, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F, L.capacity,
+    hipLaunchKernelGGL(k_cg_begin, dim3(L.gridGroups), dim3(kGroupSize), kLdsXSV, L.stream, L.P, L.tileScene, kt, F, L.capacity,
                        TileGrid(L.tg));
 }
 void launch_cg_begin_finish(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_cg_begin_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
 }
 void launch_cg_matvec(const Launch& L, const KTabHost& kt, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_matvec, dim3(L.gridGroups), dim3(kGroupSize), kLdsG + 2 * kHaloMax * sizeof(float4), L.stream, L.P, L.tileScene, kt, F,
+    hipLaunchKernelGGL(k_cg_matvec, dim3(L.gridGroups), dim3(kGroupSize), kLdsXSV, L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg));
 }
 void launch_cg_alpha(const Launch& L, const Fields& F) {
@@ -957,18 +968,18 @@ int configure_kernels() {
         const void* fn;
         size_t bytes;
     } const k[] = {
-        {(const void*)k_density_factor, kLdsWG + kHaloMax * sizeof(float4)},
-        {(const void*)k_div_init, kLdsG + 2 * kHaloMax * sizeof(float4)},
-        {(const void*)k_div_accel, kLdsG + kHaloMax * sizeof(float4)},
-        {(const void*)k_div_iter, kLdsG + 2 * kHaloMax * sizeof(float4)},
-        {(const void*)k_div_final, kLdsG + kHaloMax * sizeof(float4)},
-        {(const void*)k_pres_init, kLdsG + 2 * kHaloMax * sizeof(float4)},
-        {(const void*)k_pres_accel, kLdsG + kHaloMax * sizeof(float4)},
-        {(const void*)k_pres_iter, kLdsG + 2 * kHaloMax * sizeof(float4)},
-        {(const void*)k_pres_final, kLdsG + kHaloMax * sizeof(float4)},
-        {(const void*)k_visc_precond, kLdsG + kHaloMax * sizeof(float4)},
-        {(const void*)k_cg_begin, kLdsG + 2 * kHaloMax * sizeof(float4)},
-        {(const void*)k_cg_matvec, kLdsG + 2 * kHaloMax * sizeof(float4)},
+        {(const void*)k_density_factor, kLdsDensity},
+        {(const void*)k_div_init, kLdsXV},
+        {(const void*)k_div_accel, kLdsXS},
+        {(const void*)k_div_iter, kLdsXV},
+        {(const void*)k_div_final, kLdsXS},
+        {(const void*)k_pres_init, kLdsXV},
+        {(const void*)k_pres_accel, kLdsXS},
+        {(const void*)k_pres_iter, kLdsXV},
+        {(const void*)k_pres_final, kLdsXS},
+        {(const void*)k_visc_precond, kLdsXS},
+        {(const void*)k_cg_begin, kLdsXSV},
+        {(const void*)k_cg_matvec, kLdsXSV},
     };
     for (const auto& e : k)
         if (hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.bytes) != hipSuccess) return 1;

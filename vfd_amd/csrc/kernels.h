@@ -140,7 +140,8 @@ void launch_sort(const Launch& L, const SceneGrid* G, uint32_t totalCells, const
                  float4* dvOut, float* dkOut, uint32_t* idOut, uint32_t* cellOf, uint32_t* rank, uint32_t* tmpId,
                  uint32_t* cellCount, uint32_t* cellStart, uint32_t* scanScratch);
 void launch_build_halo(const Launch& L, const uint32_t* nbrCount, const uint32_t* nbrList, uint32_t haloLimit,
-                       uint32_t* haloCount, uint32_t* haloIds, uint16_t* nbrLocal, uint32_t* haloStats);
+                       uint32_t* haloCount, uint32_t* haloIds, uint32_t* nbrLocal, uint32_t* haloStats,
+                       const float4* pos, const KTabHost& kt);
 void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, const uint32_t* cellStart,
                         const uint32_t* cellCount, uint32_t* nbrCount, uint32_t* nbrList,
                         unsigned long long* nbStats);
@@ -198,7 +199,7 @@ struct Fields {
     float* vb;        // [kMaxBodies][capacity]
     const uint32_t* nbrCount;
     const uint32_t* nbrList;
-    const uint16_t* nbrLocal;    // [j][capacity] halo slots (staged groups)
+    const uint32_t* nbrLocal;    // [j][capacity] halo slot | table code << 16 (staged groups)
     const uint32_t* haloCount;   // [groups] kHaloNone: global gathers
     const uint32_t* haloIds;     // [groups][kHaloMax]
     // viscosity solver

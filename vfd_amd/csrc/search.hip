@@ -285,6 +285,173 @@ __global__ void __launch_bounds__(kTile) k_build_lists(const float4* __restrict_
     }
 }
 
+// ---- cell-parallel list build --------------------------------------------------------------------------------------
+// The same lists as k_build_lists, built one grid cell at a time by one wave: every particle of a cell has the same 27
+// candidate ranges, so the wave loads the candidates once, 64 per chunk with a lane per candidate (consecutive slots
+// of a range are consecutive lanes), and tests them against each particle of the cell in turn; a ballot gives every
+// accepted candidate its rank in the particle's list (candidates are visited in list order: ranges in z, y, x order,
+// slots ascending), so the first kMaxNeighbors are kept exactly as the per-particle traversal keeps them. A cell's
+// lists are staged in LDS ([row][particle], kCellBatch particles at a time) and written row by row: each store
+// instruction writes 64-byte runs of four rows. A wave handles the cells whose first slot lies in its 64-slot window
+// (a cell may extend past the window); blocks take tiles in XCD-aware order, so one XCD writes one contiguous stretch
+// of every list row.
+#ifndef VFD_SEARCH_CELLS
+#define VFD_SEARCH_CELLS 1
+#endif
+#ifndef VFD_CELL_BATCH
+#define VFD_CELL_BATCH 16
+#endif
+constexpr uint32_t kCellBatch = VFD_CELL_BATCH;  // particles of a cell tested together (a divisor of 64)
+static_assert(kWave % kCellBatch == 0, "the row writes cover 64 / kCellBatch rows per store");
+constexpr uint32_t kCellWaves = kTile / kWave;
+
+__device__ __forceinline__ uint32_t slot_cell(const SceneGrid& g, const float4& x) {  // k_insert's key
+    const float fx = (x.x - g.gridMin[0]) * g.gridDelta[0];
+    const float fy = (x.y - g.gridMin[1]) * g.gridDelta[1];
+    const float fz = (x.z - g.gridMin[2]) * g.gridDelta[2];
+    return cell_key(g, clampu((int)fx, g.dims[0]), clampu((int)fy, g.dims[1]), clampu((int)fz, g.dims[2]));
+}
+
+__global__ void __launch_bounds__(kTile) k_build_lists_cells(const float4* __restrict__ pos,
+                                                             const SceneParams* __restrict__ P,
+                                                             const SceneGrid* __restrict__ G,
+                                                             const uint32_t* __restrict__ tileScene, uint32_t tiles,
+                                                             uint32_t per, const uint32_t* __restrict__ cellStart,
+                                                             const uint32_t* __restrict__ cellCount,
+                                                             uint32_t* nbrCount, uint32_t* nbrList, uint32_t capacity,
+                                                             unsigned long long* nbStats) {
+    __shared__ uint32_t stage[kCellWaves][kMaxNeighbors][kCellBatch];
+    __shared__ uint32_t pre[kCellWaves][32];  // exclusive prefix of the 27 range lengths
+    __shared__ uint32_t cst[kCellWaves][32];  // first slot of each range
+    const uint32_t tile = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (tile >= tiles) return;  // block-uniform
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
+    const uint32_t s = tileScene[tile];
+    const SceneParams& p = P[s];
+    const SceneGrid& g = G[s];
+    const uint32_t end = p.base + p.count;
+    const uint32_t i = tile * kTile + threadIdx.x;
+    const bool valid = i < end;
+    const uint32_t key = valid ? slot_cell(g, pos[i]) : 0u;
+    uint32_t prevKey = __shfl_up(key, 1, kWave);
+    if (lane == 0) prevKey = valid && i > p.base ? slot_cell(g, pos[i - 1]) : 0xFFFFFFFFu;
+    unsigned long long leaders = __ballot(valid && (i == p.base || key != prevKey));
+    const float R2 = p.R * p.R;
+    unsigned long long tot = 0;
+    uint32_t mx = 0;
+    while (leaders) {
+        const uint32_t b = (uint32_t)__ffsll(leaders) - 1u;
+        leaders &= leaders - 1ull;
+        const uint32_t first = tile * kTile + w * kWave + b;
+        const float4 xl = pos[first];
+        // the cell of the key (for particles inside the grid, the cell neighbor_list takes with floorf; a particle
+        // with a non-finite position sits in the clamped cell of k_insert and finds no neighbour)
+        const int gx = (int)clampu((int)((xl.x - g.gridMin[0]) * g.gridDelta[0]), g.dims[0]);
+        const int gy = (int)clampu((int)((xl.y - g.gridMin[1]) * g.gridDelta[1]), g.dims[1]);
+        const int gz = (int)clampu((int)((xl.z - g.gridMin[2]) * g.gridDelta[2]), g.dims[2]);
+        uint32_t cn = 0, cs = 0;
+        if (lane < 27) {
+            const int cx = gx + (int)(lane % 3) - 1, cy = gy + (int)((lane / 3) % 3) - 1, cz = gz + (int)(lane / 9) - 1;
+            if (cx >= 0 && cx < (int)g.dims[0] && cy >= 0 && cy < (int)g.dims[1] && cz >= 0 && cz < (int)g.dims[2]) {
+                const uint32_t k = cell_key(g, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+                cn = cellCount[k];
+                cs = (uint32_t)((int32_t)cellStart[k] + g.startCorr);
+            }
+        }
+        uint32_t incl = cn;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, kWave);
+            if ((int)lane >= o) incl += t;
+        }
+        const uint32_t T = __shfl(incl, 26, kWave);  // candidates in the 27 ranges
+        const uint32_t m = __shfl(cn, 13, kWave);    // particles of this cell (the centre range)
+        if (lane < 27) {
+            pre[w][lane] = incl - cn;
+            cst[w][lane] = cs;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t p0 = 0; p0 < m; p0 += kCellBatch) {
+            const uint32_t mb = min(kCellBatch, m - p0);
+            float xix[kCellBatch], xiy[kCellBatch], xiz[kCellBatch];  // wave-uniform
+            uint32_t cnt[kCellBatch];
+#pragma unroll
+            for (uint32_t k = 0; k < kCellBatch; ++k) {
+                const float4 x = k < mb ? pos[first + p0 + k] : xl;
+                xix[k] = x.x;
+                xiy[k] = x.y;
+                xiz[k] = x.z;
+                cnt[k] = 0;
+            }
+            for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
+                const uint32_t t = c0 + lane;
+                const bool has = t < T;
+                uint32_t slot = 0;
+                float4 xj = xl;
+                if (has) {  // the range holding candidate t: the last range whose prefix is <= t
+                    uint32_t r = 0;
+#pragma unroll
+                    for (uint32_t step = 16; step > 0; step >>= 1)
+                        if (r + step < 27 && pre[w][r + step] <= t) r += step;
+                    slot = cst[w][r] + (t - pre[w][r]);
+                    xj = pos[slot];
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kCellBatch; ++k) {
+                    if (k < mb) {  // wave-uniform
+                        const float ddx = xj.x - xix[k], ddy = xj.y - xiy[k], ddz = xj.z - xiz[k];
+                        const float d2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+                        const bool acc = has && d2 < R2 && d2 > 0.0f;
+                        const unsigned long long bal = __ballot(acc);
+                        if (acc) {
+                            const uint32_t rank = cnt[k] + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                            if (rank < kMaxNeighbors) stage[w][rank][k] = slot;
+                        }
+                        cnt[k] += (uint32_t)__popcll(bal);
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // this lane's particle (lane % kCellBatch) and its kept count
+            const uint32_t kk = lane % kCellBatch;
+            uint32_t mine = 0, rows = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kCellBatch; ++k) {
+                const uint32_t c = k < mb ? min(cnt[k], kMaxNeighbors) : 0u;
+                if (kk == k) mine = c;
+                rows = max(rows, c);
+            }
+            for (uint32_t r0 = 0; r0 < rows; r0 += kWave / kCellBatch) {
+                const uint32_t r = r0 + lane / kCellBatch;
+                if (kk < mb && r < mine) nbrList[(size_t)r * capacity + first + p0 + kk] = stage[w][r][kk];
+            }
+            if (lane < mb) nbrCount[first + p0 + lane] = mine;
+            if (lane < kCellBatch && lane < mb) {
+                tot += mine;
+                mx = max(mx, mine);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    // per-scene total and maximum neighbour count (debug statistics)
+    unsigned long long t = tot;
+    uint32_t x = mx;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        t += __shfl_xor(t, o, kWave);
+        x = max(x, (uint32_t)__shfl_xor(x, o, kWave));
+    }
+    if (lane == 0 && t > 0) {
+        atomicAdd(&nbStats[2 * s], t);
+        atomicMax(&nbStats[2 * s + 1], (unsigned long long)x);
+    }
+}
 
 // ---- exclusive scan of uint32 (cell counts) --------------------------------------------------------------------
 constexpr unsigned kScanThreads = 256, kScanItems = 8, kScanChunk = kScanThreads * kScanItems;
@@ -393,8 +560,11 @@ __global__ void k_fill_u64(unsigned long long* p, uint32_t n) {
 // One block per group. The distinct particle slots among the lists of its kGroupSize particles are collected in an
 // LDS hash set of 32-slot buckets (bucket = slot >> 5, Fibonacci-hashed, linear probing) with a presence mask each;
 // halo numbers are handed out bucket by bucket, ascending inside a bucket, so runs of consecutive particle slots stay
-// consecutive in the halo. Every list entry is then rewritten as its halo number. A group with more than `limit`
-// distinct neighbours (or more than kBucketLimit buckets) gets kHaloNone and is served by the global lists.
+// consecutive in the halo. Every list entry is then rewritten as its halo number, packed with the pair's
+// kernel-table code (gcode: the table index GetGradientW reads, or kGCodeZero beyond the cutoff) in the upper 16
+// bits: positions do not change between here and the step's last neighbour loop, so the loops read the code instead
+// of recomputing the distance, its square root and the index in every pass. A group with more than `limit` distinct
+// neighbours (or more than kBucketLimit buckets) gets kHaloNone and is served by the global lists.
 // haloStats = {groups staged, groups gathered, largest halo}.
 constexpr uint32_t kBucketSlots = 2048;
 constexpr uint32_t kBucketLimit = 896;  // + one bucket per thread in flight < kBucketSlots: probing always ends
@@ -402,18 +572,21 @@ constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
 static_assert(kBucketLimit + kGroupSize < kBucketSlots, "bucket table must never fill");
 
 __device__ __forceinline__ uint32_t bucket_hash(uint32_t b) { return (b * 0x9E3779B1u) >> 21; }  // 11 bits
+constexpr size_t kLdsHaloPos = (size_t)kHaloMax * 3 * sizeof(float);  // halo positions of k_build_halo
 
 __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __restrict__ P,
                                                            const uint32_t* __restrict__ tileScene, uint32_t tiles,
                                                            const uint32_t* __restrict__ nbrCount,
                                                            const uint32_t* __restrict__ nbrList, uint32_t cap,
                                                            uint32_t limit, uint32_t* haloCount, uint32_t* haloIds,
-                                                           uint16_t* nbrLocal, uint32_t* haloStats) {
+                                                           uint32_t* nbrLocal, uint32_t* haloStats,
+                                                           const float4* __restrict__ pos, float R, float invStep) {
     __shared__ uint32_t bkey[kBucketSlots];
     __shared__ uint32_t bmask[kBucketSlots];
     __shared__ uint16_t bbase[kBucketSlots];
     __shared__ uint32_t waveTotal[kGroupSize / kWave];
     __shared__ uint32_t used, rows;
+    extern __shared__ float hpos[];  // [kHaloMax * 3]: positions of the halo, by halo number
     const uint32_t g = blockIdx.x;
     const uint32_t tile = g * kGroupTiles + threadIdx.x / kTile;
     const uint32_t i = g * kGroupSize + threadIdx.x;
@@ -511,7 +684,28 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
         atomicMax(&haloStats[2], m);
     }
     __syncthreads();
-    // staged entries: the halo number of each neighbour, in list order
+    // the halo's positions, by halo number (ids written above by the block, visible after the barrier)
+    if (m) {  // up to four entries per thread (kHaloMax <= 4 kGroupSize): all id loads, then all gathers
+        uint32_t t[4];
+        float4 x[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) t[u] = ids[min(threadIdx.x + u * kGroupSize, m - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) x[u] = pos[t[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t k = threadIdx.x + u * kGroupSize;
+            if (k < m) {
+                hpos[3 * k] = x[u].x;
+                hpos[3 * k + 1] = x[u].y;
+                hpos[3 * k + 2] = x[u].z;
+            }
+        }
+    }
+    __syncthreads();
+    // staged entries: the halo number of each neighbour | its table code << 16, in list order (the neighbour's
+    // position from the halo's copy in LDS)
+    const f3 xi = n ? ld3(pos[i]) : mk3(0.0f, 0.0f, 0.0f);
     for (uint32_t j0 = 0; j0 < nr; j0 += kHaloBatch) {
         uint32_t tb[kHaloBatch];
 #pragma unroll
@@ -523,17 +717,20 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
             const uint32_t b = t >> 5, below = (1u << (t & 31u)) - 1u;
             uint32_t h = bucket_hash(b);
             while (bkey[h] != b) h = (h + 1) & (kBucketSlots - 1);
-            nbrLocal[(size_t)(j0 + u) * cap + i] = (uint16_t)(bbase[h] + __popc(bmask[h] & below));
+            const uint32_t s = bbase[h] + __popc(bmask[h] & below);
+            const uint32_t code = gcode<false>(R, invStep, xi - mk3(hpos[3 * s], hpos[3 * s + 1], hpos[3 * s + 2]));
+            nbrLocal[(size_t)(j0 + u) * cap + i] = s | (code << 16);
         }
     }
 }
 
 void launch_build_halo(const Launch& L, const uint32_t* nbrCount, const uint32_t* nbrList, uint32_t haloLimit,
-                       uint32_t* haloCount, uint32_t* haloIds, uint16_t* nbrLocal, uint32_t* haloStats) {
+                       uint32_t* haloCount, uint32_t* haloIds, uint32_t* nbrLocal, uint32_t* haloStats,
+                       const float4* pos, const KTabHost& kt) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, L.stream, haloStats, 0u, 3u);
-    hipLaunchKernelGGL(k_build_halo, dim3(L.groups), dim3(kGroupSize), 0, L.stream, L.P, L.tileScene, L.tiles,
+    hipLaunchKernelGGL(k_build_halo, dim3(L.groups), dim3(kGroupSize), kLdsHaloPos, L.stream, L.P, L.tileScene, L.tiles,
                        nbrCount, nbrList, L.capacity, haloLimit < kHaloMax ? haloLimit : kHaloMax, haloCount, haloIds,
-                       nbrLocal, haloStats);
+                       nbrLocal, haloStats, pos, kt.R, kt.invStep);
 }
 
 void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, const uint32_t* cellStart,
@@ -541,11 +738,18 @@ void launch_build_lists(const Launch& L, const SceneGrid* G, const float4* pos, 
                         unsigned long long* nbStats) {
     hipLaunchKernelGGL(k_fill_u64, dim3((2 * L.sceneCount + 63) / 64), dim3(64), 0, L.stream, nbStats,
                        2 * L.sceneCount);
-    hipLaunchKernelGGL(k_build_lists, dim3(L.tiles), dim3(kTile), kLdsListStage, L.stream, pos, L.P, G, L.tileScene,
-                       cellStart, cellCount, nbrCount, nbrList, L.capacity, nbStats);
+    if (VFD_SEARCH_CELLS)
+        hipLaunchKernelGGL(k_build_lists_cells, dim3(L.gridAll), dim3(kTile), 0, L.stream, pos, L.P, G, L.tileScene,
+                           L.tiles, L.tg.per, cellStart, cellCount, nbrCount, nbrList, L.capacity, nbStats);
+    else
+        hipLaunchKernelGGL(k_build_lists, dim3(L.tiles), dim3(kTile), kLdsListStage, L.stream, pos, L.P, G,
+                           L.tileScene, cellStart, cellCount, nbrCount, nbrList, L.capacity, nbStats);
 }
 
 int configure_search_kernels() {
+    if (hipFuncSetAttribute((const void*)k_build_halo, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kLdsHaloPos) != hipSuccess)
+        return 1;
     if (kLdsListStage == 0) return 0;
     return hipFuncSetAttribute((const void*)k_build_lists, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kLdsListStage) == hipSuccess ? 0 : 1;

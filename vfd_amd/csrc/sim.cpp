@@ -459,7 +459,7 @@ public:
     DevBuf<float> partialsF;
     DevBuf<unsigned long long> nbStats;
     DevBuf<uint32_t> haloCount, haloIds, haloStats;
-    DevBuf<uint16_t> nbrLocal;
+    DevBuf<uint32_t> nbrLocal;
 
     // ---- adjoint recording (one scene): per step, everything the reverse needs, in that step's slot order -----
     struct DevTape {
@@ -1075,7 +1075,7 @@ public:
         L.grid4 = std::min(L.gridAll, p4);
         L.grid2 = std::min(L.gridAll, p2);
         L.groups = 8 * (per / kGroupTiles);
-        L.gridGroups = std::min(L.groups, std::max<uint32_t>(8, numCU / 8 * 8));
+        L.gridGroups = std::min(L.groups, std::max<uint32_t>(8, numCU * kGroupsPerCU / 8 * 8));
         L.tg.tiles = tiles;
         L.tg.per = per;
         return L;
@@ -1251,7 +1251,8 @@ public:
         // halos are built after the boundary has pushed penetrating particles out (the lists themselves come from the
         // positions before)
         VFD_TRY(timed("build_halo", [&] {
-            launch_build_halo(L, nbrCount.p, nbrList.p, haloLimit, haloCount.p, haloIds.p, nbrLocal.p, haloStats.p);
+            launch_build_halo(L, nbrCount.p, nbrList.p, haloLimit, haloCount.p, haloIds.p, nbrLocal.p, haloStats.p,
+                              pos.p, K);
         }));
         if (last == VFD_PHASE_BOUNDARY) return finish_partial();
         DevTape* tp = nullptr;

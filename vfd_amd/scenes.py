@@ -41,6 +41,39 @@ def lattice_block(origin, counts, spacing, jitter, rng):
     return p.astype(np.float32)
 
 
+def sampler_lattice(mode, origin, counts, r, jitter, rng):
+    """The lattice patterns of ParticleSampler::SampleMeshVolume (Utility/Sampler/ParticleSampler.cpp:19-73) over
+    counts[0] x counts[1] x counts[2] points (x fastest, then y, then z, the reference's loop order), starting at
+    `origin`, with uniform jitter in [-jitter, jitter]:
+      "min"    MinDensity: cubic, spacing 2r, points at the cell centres (+r);
+      "medium" MediumDensity: x step 2r, y step sqrt(3) r, z step 2r; even rows at (x, y + r, z + r), odd rows at
+               (x + r, y + r, z) - 6.93 r^3 per particle, 34 neighbours within 4r in the interior;
+      "max"    MaxDensity: x step sqrt(3) r, y step 2 sqrt(6) r / 3, z step 2r; at (x, y + r, z + r) shifted by
+               +-r in z on odd columns (sign by row parity) and by (sqrt(3) r / 2, 0, r) on odd rows - 5.66 r^3 per
+               particle, 50 neighbours within 4r in the interior.
+    The point of column i, row j, layer k uses the loop variables i * shiftX, j * shiftY, k * 2r (the reference
+    accumulates them in float; here they are products, a difference of a few ulps in a synthetic input)."""
+    nx, ny, nz = counts
+    d = 2.0 * r
+    sx, sy = {"min": (d, d), "medium": (d, math.sqrt(3.0) * r),
+              "max": (math.sqrt(3.0) * r, math.sqrt(6.0) * d / 3.0)}[mode]
+    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    x, y, z = I * sx, J * sy, K * d
+    odd_row, odd_col = (J % 2).astype(np.float64), (I % 2).astype(np.float64)
+    if mode == "min":
+        p = np.stack([x + r, y + r, z + r], 1)
+    elif mode == "medium":
+        p = np.stack([x + odd_row * r, y + r, z + (1.0 - odd_row) * r], 1)
+    else:
+        dz = odd_col * np.where(J % 2 == 1, -r, r) + odd_row * r
+        p = np.stack([x + odd_row * (sx / 2.0), y + r, z + r + dz], 1)
+    p = p + np.asarray(origin, np.float64)
+    if jitter:
+        p = p + rng.uniform(-jitter, jitter, size=p.shape)
+    return p.astype(np.float32)
+
+
 def dam_break_8k(seed=1):
     """Config 1: 20^3 = 8,000 particles, r = 0.025, spacing 2r, jitter +-0.01r, in the corner of a 2 m box
     ([-1, 1]^3, the span of the reference's Cube.obj) given as an inverted-box volume map, the block one support radius (4r) off the walls; no viscosity, no surface
@@ -93,17 +126,25 @@ def viscous_coil_250k(seed=2, n=250_000):
     return [desc], [(x, v.astype(np.float32))], [[plane, walls]]
 
 
-def dam_break_2m(seed=3):
-    """Config 3: 2,000,000 particles (200 x 100 x 100 cubic lattice, r = 0.004, spacing 2r, jitter +-0.01r) in a
-    2 x 1 x 1 m tank (inverted box) with a sphere obstacle of radius 0.15; viscosity 0.01. The block is 1.6 x 0.8 x
-    0.8 m (51 % of the tank; 40 % cannot hold 2M particles on a 2r cubic lattice)."""
+def dam_break_2m(seed=3, lattice="medium"):
+    """Config 3: 2,000,000 particles (r = 0.004, jitter +-0.01r) in a 2 x 1 x 1 m tank (inverted box) with a sphere
+    obstacle of radius 0.15; viscosity 0.01. The block is 200 x 100 x 100 points of a sampler lattice:
+      "medium" (default): the reference's MediumDensity pattern, the one its DFSPH example scene uses, with about 35
+               neighbours per particle as the configuration states; the block is 1.6 x 0.69 x 0.8 m, 44 % of the
+               tank (40 % holds 1.8M particles at this density);
+      "cubic": a cubic lattice of spacing 2r (28.5 neighbours), 1.6 x 0.8 x 0.8 m, 51 % of the tank;
+      "max":   the MaxDensity pattern (about 48 neighbours, 1.13 rest densities), 1.39 x 0.65 x 0.8 m, 36 %."""
     rng = np.random.default_rng(seed)
     r = 0.004
     desc = DFSPHSimulationDescription(
         ParticleRadius=r, Viscosity=0.01, BoundaryViscosity=0.01, MaxViscositySolverError=0.1,
         EnableSurfaceTensionSolver=0, MaxDivergenceSolverError=10.0, MaxPressureSolverError=10.0,
         MinDivergenceSolverIterations=2, MinPressureSolverIterations=2, FrameCount=100)
-    x = lattice_block((4 * r, 4 * r, 4 * r), (200, 100, 100), 2 * r, 0.01 * r, rng)
+    if lattice == "cubic":
+        x = lattice_block((4 * r, 4 * r, 4 * r), (200, 100, 100), 2 * r, 0.01 * r, rng)
+    else:
+        origin = (4 * r, 3 * r, 4 * r) if lattice == "medium" else (4 * r, 3 * r, 3 * r)  # lowest points 4r in
+        x = sampler_lattice(lattice, origin, (200, 100, 100), r, 0.01 * r, rng)
     v = np.zeros_like(x)
     tank = RigidBodyDescription("box", center=(1.0, 0.5, 0.5), half_extents=(1.0, 0.5, 0.5), inverted=True,
                                 resolution=(40, 24, 24))
@@ -199,6 +240,8 @@ CONFIGS = {
     "dam_break_8k": dam_break_8k,
     "viscous_coil_250k": viscous_coil_250k,
     "dam_break_2m": dam_break_2m,
+    "dam_break_2m_cubic": lambda seed=3: dam_break_2m(seed, lattice="cubic"),
+    "dam_break_2m_max": lambda seed=3: dam_break_2m(seed, lattice="max"),
     "dam_break_16m": dam_break_16m,
     "batched_uneven_64": batched_uneven,
     "falling_viscous_block_100k": falling_viscous_block,
