@@ -5,6 +5,7 @@
     python bench.py --copy-probe                      # plain 512 MiB device copies, for counter calibration
     python bench.py --pmc-summary FETCH WRITE COPY_FETCH COPY_WRITE   # rocprofv3 --pmc output dirs -> traffic table
     python bench.py --trace-summary DIR               # rocprofv3 --kernel-trace output -> per-kernel means
+    python bench.py --counter-summary LDS_DIR VALU_DIR  # rocprofv3 --pmc LDS / instruction counters -> table
 
 Prints one JSON line. A step is one full DFSPH step (neighbour search, boundary, density/factor, divergence solve,
 viscosity solve, time step, pressure solve, integration) of every particle; particles stay resident in HBM and frame
@@ -105,6 +106,143 @@ def cpu_baseline(cfg, snap, max_steps, budget_s):
             "seconds": dt, "x1": x1, "iterations": its, "steps": len(its)}
 
 
+def cpu_model():
+    """The host CPU's model name as /proc/cpuinfo reports it."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_scene(descs, fluids, bodies, s=0, fp64=False):
+    from oracle.oracle import OracleSim
+    o = OracleSim(descs[s], fp64=fp64)
+    o.set_fluid(*fluids[s])
+    for b in bodies[s]:
+        o.add_boundary(b.to_abi())
+    return o
+
+
+def _iters(d):
+    return (d.DivergenceSolverIterationCount, d.PressureSolverIterationCount, d.ViscositySolverIterationCount)
+
+
+def cpu_baselines_other(c5_steps=60):
+    """The CPU oracle (FP32 mirror, OpenMP over the host cores) on the other benchmark configurations, each beside the
+    GPU over exactly the same steps from the same initial state (setup excluded on both):
+      C1 dam_break_8k: all 200 steps;  C2 viscous_coil_250k: the first 5 steps;
+      C4 batched_uneven_64: the first 3 steps of every scene, stepped one scene after another on the CPU and as one
+         batch on the GPU;
+      C5 falling_viscous_block_100k: the first `c5_steps` steps of its 60-step forward, plus the FP64 oracle
+         adjoint of a 2,028-particle version (3 recorded steps, the oracle's reference gradient), timed forward and
+         backward.
+    Solver iteration counts are reported per config so that equal work can be checked."""
+    import numpy as np
+    from oracle.oracle import build_oracle
+    from vfd_amd import DFSPHBatch, DFSPHSimulation, scenes
+    if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
+        build_oracle()
+    out = {"cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "cpu_model": cpu_model(),
+           "kind": "port"}
+
+    def gpu_single(descs, fluids, bodies, steps):
+        sim = DFSPHSimulation(descs[0])
+        sim.set_fluid(*fluids[0])
+        sim.SetRigidBodies(bodies[0])
+        sim.set_frame_capture(False)
+        sim.field("position")  # state allocated before the clock starts, as the oracle's setup is excluded
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.OnUpdate()
+            its.append(_iters(sim.GetDebugInfo()))
+        return time.perf_counter() - t0, its
+
+    for name, steps in (("dam_break_8k", 200), ("viscous_coil_250k", 5)):
+        descs, fluids, bodies = scenes.CONFIGS[name]()
+        n = len(fluids[0][0])
+        o = _oracle_scene(descs, fluids, bodies)
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.step()
+            its.append(_iters(o.debug_info()))
+        cpu_s = time.perf_counter() - t0
+        del o
+        gpu_s, gits = gpu_single(descs, fluids, bodies, steps)
+        out[name] = {"particles": n, "steps": steps, "cpu_value": n * steps / cpu_s, "gpu_value": n * steps / gpu_s,
+                     "unit": "particle-updates/s", "same_iterations": its == gits,
+                     "mean_iterations_cpu": [round(float(v), 2) for v in np.mean(its, 0)]}
+        print("[bench] cpu baseline %s: %.3g (GPU %.3g) updates/s" % (name, out[name]["cpu_value"],
+                                                                       out[name]["gpu_value"]),
+              file=sys.stderr, flush=True)
+
+    descs, fluids, bodies = scenes.batched_uneven()
+    steps = 3
+    total = sum(len(f[0]) for f in fluids)
+    cpu_s, its = 0.0, []
+    for s in range(len(descs)):
+        o = _oracle_scene(descs, fluids, bodies, s)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.step()
+        cpu_s += time.perf_counter() - t0
+        its.append(_iters(o.debug_info()))
+        del o
+    b = DFSPHBatch(descs)
+    for s in range(len(descs)):
+        b.scenes[s].set_fluid(*fluids[s])
+        b.scenes[s].SetRigidBodies(bodies[s])
+    b.set_frame_capture(False)
+    b.scenes[0].field("position")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step()
+    gpu_s = time.perf_counter() - t0
+    gits = [_iters(b.scenes[s].GetDebugInfo()) for s in range(len(descs))]
+    out["batched_uneven_64"] = {"particles": total, "scenes": len(descs), "steps": steps,
+                                "cpu_value": total * steps / cpu_s, "gpu_value": total * steps / gpu_s,
+                                "unit": "particle-updates/s", "same_iterations": its == gits,
+                                "note": "CPU: scenes one after another; GPU: one batch"}
+    del b
+    print("[bench] cpu baseline batched_uneven_64: %.3g (GPU %.3g) updates/s" % (
+        out["batched_uneven_64"]["cpu_value"], out["batched_uneven_64"]["gpu_value"]), file=sys.stderr, flush=True)
+
+    descs, fluids, bodies = scenes.falling_viscous_block()
+    n = len(fluids[0][0])
+    o = _oracle_scene(descs, fluids, bodies)
+    t0 = time.perf_counter()
+    for _ in range(c5_steps):
+        o.step()
+    cpu_s = time.perf_counter() - t0
+    del o
+    gpu_s, _ = gpu_single(descs, fluids, bodies, c5_steps)
+    d2, f2, b2 = scenes.falling_viscous_block(n_side=(13, 12, 13), differentiable=True)
+    o = _oracle_scene(d2, f2, b2, fp64=True)
+    o.tape(True)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        o.step()
+    fwd_s = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    o.adjoint(rng.normal(size=(len(f2[0][0]), 3)), rng.normal(size=(len(f2[0][0]), 3)))
+    adj_s = time.perf_counter() - t0
+    del o
+    out["falling_viscous_block_100k"] = {
+        "particles": n, "steps": c5_steps, "cpu_value": n * c5_steps / cpu_s, "gpu_value": n * c5_steps / gpu_s,
+        "unit": "particle-updates/s",
+        "fp64_adjoint_2k": {"particles": len(f2[0][0]), "steps": 3, "forward_s": round(fwd_s, 4),
+                            "adjoint_s": round(adj_s, 4), "adjoint_over_forward": round(adj_s / fwd_s, 3)}}
+    print("[bench] cpu baseline falling_viscous_block_100k: %.3g (GPU %.3g) updates/s" % (
+        out["falling_viscous_block_100k"]["cpu_value"], out["falling_viscous_block_100k"]["gpu_value"]),
+        file=sys.stderr, flush=True)
+    return out
+
+
 LIVE_US = 20.0  # loop-kernel dispatches shorter than this exited at once (issued after convergence)
 LOOP_KERNELS = {"k_div_accel", "k_div_iter", "k_pres_accel", "k_pres_iter", "k_cg_matvec", "k_cg_update",
                 "k_cg_direction"}
@@ -179,6 +317,31 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
         if k in wr and k.startswith("k_"):
             t = fe[k]["FETCH_SIZE"] * 1024.0 * f_scale + wr[k]["WRITE_SIZE"] * 1024.0 * w_scale
             lines.append("| %s | %.1f | %.1f | %d |" % (k[2:], fe[k]["FETCH_SIZE"], wr[k]["WRITE_SIZE"], t))
+    return "\n".join(lines)
+
+
+SHADER_ENGINES = 32  # MI355X: 8 XCDs x 4 shader engines; SQ_BUSY_CYCLES sums over them
+SIMDS = 1024         # 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles
+
+
+def counter_summary(lds_dir, valu_dir, kernels=("div_iter", "div_accel", "pres_iter", "pres_accel", "cg_matvec",
+                                                   "density_factor", "visc_precond", "build_lists", "build_halo")):
+    """Markdown table of the LDS and instruction counters of the neighbour-loop kernels (means per dispatch, summed
+    over the chip as rocprofv3 reports them): LDS busy = SQ_LDS_IDX_ACTIVE per CU over the kernel's duration
+    (SQ_BUSY_CYCLES / shader engines); VALU busy = 4 cycles per SQ_INSTS_VALU per SIMD over the same duration."""
+    lds, valu = _pmc_table(lds_dir), _pmc_table(valu_dir)
+    lines = ["| kernel | duration cycles | LDS active | bank-conflict share | LDS busy | LDS insts | LDS issue wait | "
+             "VALU insts | VALU busy | VMEM reads | wait any (issue) |", "|---|" + "---:|" * 10]
+    for k in kernels:
+        a, b = lds.get("k_" + k), valu.get("k_" + k)
+        if not a or not b:
+            continue
+        dur = a["SQ_BUSY_CYCLES"] / SHADER_ENGINES
+        lines.append("| %s | %.0f | %.3g | %.0f%% | %.0f%% | %.3g | %.3g | %.3g | %.0f%% | %.3g | %.3g |" % (
+            k, dur, a["SQ_LDS_IDX_ACTIVE"], 100.0 * a["SQ_LDS_BANK_CONFLICT"] / max(a["SQ_LDS_IDX_ACTIVE"], 1.0),
+            100.0 * a["SQ_LDS_IDX_ACTIVE"] / (SIMDS / 4) / max(dur, 1.0), a["SQ_INSTS_LDS"], a["SQ_WAIT_INST_LDS"],
+            b["SQ_INSTS_VALU"], 100.0 * 4.0 * b["SQ_INSTS_VALU"] / SIMDS / max(dur, 1.0), b["SQ_INSTS_VMEM_RD"],
+            b["SQ_WAIT_INST_ANY"]))
     return "\n".join(lines)
 
 
@@ -374,6 +537,39 @@ def other_configs(steps=20, warmup=3):
     return out
 
 
+def lattice_variants(steps, warmup, names=("dam_break_2m_cubic", "dam_break_2m_max")):
+    """The 2M dam break seeded with the other lattices (cubic spacing 2r; the reference's MaxDensity pattern), timed
+    like the headline: `warmup` steps, then `steps` plain steps; with the mean neighbour count and iterations."""
+    import numpy as np
+    from vfd_amd import DFSPHSimulation, scenes
+    out = {}
+    for name in names:
+        descs, fluids, bodies = scenes.CONFIGS[name]()
+        sim = DFSPHSimulation(descs[0])
+        sim.set_fluid(*fluids[0])
+        sim.SetRigidBodies(bodies[0])
+        sim.set_frame_capture(False)
+        n = sim.GetParticleCount()
+        sim.advance(warmup)
+        its, nb, el = [], [], 0.0
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            sim.OnUpdate()
+            el += time.perf_counter() - t0
+            d = sim.GetDebugInfo()
+            its.append(_iters(d))
+            nb.append(d.TotalNeighborCount / max(n, 1))
+        out[name] = {"particles": n, "steps": steps, "warmup": warmup, "value": n * steps / el,
+                     "unit": "particle-updates/s", "ms_per_step": round(1000 * el / steps, 3),
+                     "mean_neighbors": round(float(np.mean(nb)), 2),
+                     "mean_iterations": [round(float(v), 2) for v in np.mean(its, 0)]}
+        print("[bench] %s: %.3g updates/s, mean neighbours %.2f" % (name, out[name]["value"],
+                                                                      out[name]["mean_neighbors"]),
+              file=sys.stderr, flush=True)
+        del sim
+    return out
+
+
 def copy_probe():
     import torch
     n = COPY_PROBE_BYTES // 4
@@ -392,6 +588,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="dam_break_2m")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-other", action="store_true",
+                    help="skip the CPU oracle on configs C1, C2, C4 and C5 (cpu_baselines)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-gain measurement (config C4)")
@@ -401,9 +599,12 @@ def main():
     ap.add_argument("--frame-steps", type=int, default=10)
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-input (map build, sampler) measurement")
     ap.add_argument("--no-other-configs", action="store_true", help="skip the 8k and 250k configuration throughputs")
+    ap.add_argument("--no-lattices", action="store_true",
+                    help="skip the 2M dam break on the cubic and MaxDensity lattices (lattice_variants)")
     ap.add_argument("--mesh-only", action="store_true", help="only measure the mesh inputs")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"))
     ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_DIR")
+    ap.add_argument("--counter-summary", nargs=2, metavar=("LDS_COUNTERS_DIR", "VALU_COUNTERS_DIR"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the DFSPH step runs on a single MI355X (multi-GPU sharding is a design, "
@@ -414,6 +615,9 @@ def main():
         return
     if args.trace_summary:
         print(trace_summary(args.trace_summary))
+        return
+    if args.counter_summary:
+        print(counter_summary(*args.counter_summary))
         return
     if args.copy_probe:
         copy_probe()
@@ -505,7 +709,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (FP64 solver reductions)",
-        "data": "synthetic: seeded cubic lattice, analytic SDF volume maps",
+        "data": "synthetic: seeded lattice (%s), analytic SDF volume maps" % (
+            "the reference sampler's MediumDensity pattern" if args.config == "dam_break_2m" else args.config),
         "config": {"workload": args.config, "particles": n, "particle_radius": descs[0].ParticleRadius,
                    "boundary_bodies": nbody, "mean_neighbors": round(nbar, 2),
                    "mean_iterations": {"divergence": round(it[0], 2), "pressure": round(it[1], 2),
@@ -523,6 +728,7 @@ def main():
         print("[bench] cpu baseline: oracle, up to %.0f s" % args.cpu_seconds, file=sys.stderr, flush=True)
         cb = cpu_baseline(args.config, snap, args.steps, args.cpu_seconds)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
         out["cpu_baseline"]["gpu_value_same_steps"] = n * cb["steps"] / sum(step_s[:cb["steps"]])
         r = descs[0].ParticleRadius
         out["parity_full_size"] = {
@@ -531,6 +737,12 @@ def main():
             "iterations_gpu": check_its, "iterations_cpu": cb["iterations"][0]}
     else:
         out["cpu_baseline"] = None
+    if not args.no_lattices and args.config == "dam_break_2m":
+        print("[bench] the 2M dam break on the cubic and MaxDensity lattices", file=sys.stderr, flush=True)
+        out["lattice_variants"] = lattice_variants(args.steps, args.warmup)
+    if not args.no_cpu_baseline and not args.no_cpu_other:
+        print("[bench] cpu baselines of the other configurations", file=sys.stderr, flush=True)
+        out["cpu_baselines"] = cpu_baselines_other()
     if not args.no_batched:
         del sim
         print("[bench] batched gain: C4, %d steps" % args.batched_steps, file=sys.stderr, flush=True)
