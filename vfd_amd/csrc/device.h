@@ -377,6 +377,34 @@ struct LoadXV {  // lo = {x_t, -}, hi = {v_t, -}
         hi = v[t];
     }
 };
+// Viscosity records: lo = {x_t, m / rho_t} (hi = {v_t, -}). The division is the one the neighbour terms take per pair
+// (m / rho_j, DFSPHKernels.cu:638, :790), formed once per halo record instead; m is the same for every scene of a
+// batch (all scenes share the particle radius).
+struct LoadXQ {
+    static constexpr int kLo = 4, kHi = 0;
+    static constexpr bool kShortSqrt = true;
+    const float4* __restrict__ pos;
+    const float* __restrict__ rho;
+    float m;
+    __device__ __forceinline__ void operator()(uint32_t t, float4& lo, float4& hi) const {
+        const float4 x = pos[t];
+        lo = make_float4(x.x, x.y, x.z, m / rho[t]);
+        hi = lo;
+    }
+};
+struct LoadXQV {
+    static constexpr int kLo = 4, kHi = 3;
+    static constexpr bool kShortSqrt = true;
+    const float4* __restrict__ pos;
+    const float* __restrict__ rho;
+    const float4* __restrict__ v;
+    float m;
+    __device__ __forceinline__ void operator()(uint32_t t, float4& lo, float4& hi) const {
+        const float4 x = pos[t];
+        lo = make_float4(x.x, x.y, x.z, m / rho[t]);
+        hi = v[t];
+    }
+};
 struct LoadXSV {  // lo = {x_t, s_t}, hi = {v_t, -}
     static constexpr int kLo = 4, kHi = 3;
     static constexpr bool kShortSqrt = true;

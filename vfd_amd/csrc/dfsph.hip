@@ -28,7 +28,9 @@ __device__ __forceinline__ NbrSrc global_src(const Fields& F, uint32_t cap) {
 constexpr size_t kLdsDensity = kLdsWG + HaloRec<LoadX>::kBytes;
 constexpr size_t kLdsXS = kLdsG + HaloRec<LoadXS>::kBytes;
 constexpr size_t kLdsXV = kLdsG + HaloRec<LoadXV>::kBytes;
-constexpr size_t kLdsXSV = kLdsG + HaloRec<LoadXSV>::kBytes;
+constexpr size_t kLdsXSV = kLdsG + HaloRec<LoadXQV>::kBytes;
+static_assert(HaloRec<LoadXQV>::kBytes == HaloRec<LoadXSV>::kBytes && HaloRec<LoadXQ>::kBytes == HaloRec<LoadXS>::kBytes,
+              "viscosity records are the size of the records they replace");
 static_assert(kLdsDensity <= 160 * 1024 && kLdsXS <= 160 * 1024 && kLdsXV <= 160 * 1024 && kLdsXSV <= 160 * 1024,
               "staged kernels must fit the 160 KiB LDS of a CU");
 
@@ -551,21 +553,21 @@ __global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_visc_precond(const
                                                         const uint32_t* __restrict__ tileScene, KTabHost kh,
                                                         Fields F, uint32_t cap, TileGrid tg) {
     VFD_HALO_KERNEL_PROLOGUE(kh)
-    group_loop<1>(tg, halo(F), rec, global_src(F, cap), LoadXS{F.pos, F.density}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+    const LoadXQ load{F.pos, F.density, P[0].m};  // lo.w = m / rho_j
+    group_loop<1>(tg, halo(F), rec, global_src(F, cap), load, [&](uint32_t tile, bool ok, const NbrSrc& src) {
         const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         const SceneParams& p = P[c.s];
         if (!c.valid || !p.enableVisc) return;
         const f3 xi = ld3(F.pos[c.i]);
         const float d001 = 0.01f * p.R2;
-        const float mu10 = 10.0f * p.mu, m = p.m;  // read before the loop (see pressure_accel)
+        const float mu10 = 10.0f * p.mu;  // read before the loop (see pressure_accel)
         float M[9];
 #pragma unroll
         for (int e = 0; e < 9; ++e) M[e] = 0.0f;
-        each_neighbor_rec<1>(src, c.i, K.R, K.invStep, xi, K.sG, LoadXS{F.pos, F.density},
-                             [&](float4 lo, float4, float gt) {
+        each_neighbor_rec<1>(src, c.i, K.R, K.invStep, xi, K.sG, load, [&](float4 lo, float4, float gt) {
             const f3 xij = xi - ld3(lo);
             const f3 gw = gt * xij;
-            const float s = mu10 * (m / lo.w) / (length2(xij) + d001);
+            const float s = mu10 * lo.w / (length2(xij) + d001);
             const float a[3] = {xij.x, xij.y, xij.z}, g[3] = {gw.x, gw.y, gw.z};
 #pragma unroll
             for (int r = 0; r < 3; ++r)
@@ -602,18 +604,18 @@ __global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_visc_precond(const
 }
 
 // ComputeMatrixVecProdFunctionKernel (:758-842): y_i = x_i - dt/rho_i * (viscous acceleration of x)
-// records lo = {x_j, rho_j}, hi = {in_j}
+// records lo = {x_j, m / rho_j}, hi = {in_j}
 __device__ __forceinline__ f3 visc_matvec(const SceneParams& p, const KLds& k, const Fields& F, uint32_t cap,
-                                          uint32_t i, const float4* __restrict__ in, float dt, const NbrSrc& src) {
+                                          uint32_t i, const float4* __restrict__ in, float dt, const NbrSrc& src,
+                                          const LoadXQV& load) {
     const f3 xi = ld3(F.pos[i]), vi = ld3(in[i]);
     const float d001 = 0.01f * p.R2;
-    const float mu10 = 10.0f * p.mu, m = p.m;  // read before the loop (see pressure_accel)
+    const float mu10 = 10.0f * p.mu;  // read before the loop (see pressure_accel)
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
-    each_neighbor_rec<2>(src, i, k.R, k.invStep, xi, k.sG, LoadXSV{F.pos, F.density, in},
-                         [&](float4 lo, float4 hi, float g) {
+    each_neighbor_rec<2>(src, i, k.R, k.invStep, xi, k.sG, load, [&](float4 lo, float4 hi, float g) {
         const f3 xij = xi - ld3(lo);
         const f3 gw = g * xij;
-        const float s = mu10 * (m / lo.w) * dot3(vi - ld3(hi), xij) / (length2(xij) + d001);
+        const float s = mu10 * lo.w * dot3(vi - ld3(hi), xij) / (length2(xij) + d001);
         acc = acc + s * gw;
     });
     if (p.muB != 0.0f) {
@@ -639,7 +641,8 @@ __global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_cg_begin(const Sce
                                                     const uint32_t* __restrict__ tileScene, KTabHost kh, Fields F,
                                                     uint32_t cap, TileGrid tg) {
     VFD_HALO_KERNEL_PROLOGUE(kh)
-    group_loop<2>(tg, halo(F), rec, global_src(F, cap), LoadXSV{F.pos, F.density, F.cgX}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+    const LoadXQV load{F.pos, F.density, F.cgX, P[0].m};
+    group_loop<2>(tg, halo(F), rec, global_src(F, cap), load, [&](uint32_t tile, bool ok, const NbrSrc& src) {
         const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         const SceneParams& p = P[c.s];
         const bool act = ok && p.enableVisc;
@@ -647,7 +650,7 @@ __global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_cg_begin(const Sce
         double v[3] = {0.0, 0.0, 0.0};
         if (act && c.valid) {
             const f3 b = ld3(F.vel[c.i]);
-            const f3 ax = visc_matvec(p, K, F, cap, c.i, F.cgX, st.dt, src);
+            const f3 ax = visc_matvec(p, K, F, cap, c.i, F.cgX, st.dt, src, load);
             const f3 r = b - ax;
             const f3 z = minv_mul(F.minv, cap, c.i, r);
             F.cgR[c.i] = st4(r);
@@ -695,13 +698,14 @@ __global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_cg_matvec(const Sc
                                                      uint32_t cap, TileGrid tg) {
     if (F.live[2] == 0u) return;  // every scene has left this solve (block-uniform)
     VFD_HALO_KERNEL_PROLOGUE(kh)
-    group_loop<2>(tg, halo(F), rec, global_src(F, cap), LoadXSV{F.pos, F.density, F.cgP}, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+    const LoadXQV load{F.pos, F.density, F.cgP, P[0].m};
+    group_loop<2>(tg, halo(F), rec, global_src(F, cap), load, [&](uint32_t tile, bool ok, const NbrSrc& src) {
         const TileCtx c = tile_ctx4(tile, ok, tileScene, P);
         const SceneState& st = F.state[c.s];
         const bool act = ok && st.cg.active;
         double v[1] = {0.0};
         if (act && c.valid) {
-            const f3 ap = visc_matvec(P[c.s], K, F, cap, c.i, F.cgP, st.dt, src);
+            const f3 ap = visc_matvec(P[c.s], K, F, cap, c.i, F.cgP, st.dt, src, load);
             F.cgAp[c.i] = st4(ap);
             v[0] = (double)dot3(ld3(F.cgP[c.i]), ap);
         }
