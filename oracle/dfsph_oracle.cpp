@@ -106,6 +106,10 @@ template <class T> inline V3<T> mulmv(const M3<T>& A, V3<T> v) {  // glm mat3 * 
     return {A.m[0] * v.x + A.m[1] * v.y + A.m[2] * v.z, A.m[3] * v.x + A.m[4] * v.y + A.m[5] * v.z,
             A.m[6] * v.x + A.m[7] * v.y + A.m[8] * v.z};
 }
+template <class T> inline V3<T> mulmtv(const M3<T>& A, V3<T> v) {  // A^T v
+    return {A.m[0] * v.x + A.m[3] * v.y + A.m[6] * v.z, A.m[1] * v.x + A.m[4] * v.y + A.m[7] * v.z,
+            A.m[2] * v.x + A.m[5] * v.y + A.m[8] * v.z};
+}
 
 // Math.h:19-33 GetOrthogonalVectors
 template <class T> inline void orthogonal_vectors(V3<T> vec, V3<T>& x, V3<T>& y) {
@@ -375,6 +379,79 @@ template <class T> void shape_function(V3<T> xi, T N[32], V3<T> dN[32]) {
             }
 }
 
+// Second derivatives of the 32 shape functions in the local coordinates, per function (xx, yy, zz, xy, xz, yz):
+// what the adjoint of the boundary pass needs for the derivative of the normal of the interpolated distance (no
+// reference counterpart: the reference never differentiates its maps twice). Pinned against central differences
+// of the first derivatives (test_shape_function_hessian_matches_differences).
+template <class T> void shape_function_hess(V3<T> xi, T H[32][6]) {
+    const T x = xi.x, y = xi.y, z = xi.z;
+    const T lx[2] = {T(1) - x, T(1) + x}, ly[2] = {T(1) - y, T(1) + y}, lz[2] = {T(1) - z, T(1) + z};
+    const T sg[2] = {T(-1), T(1)};
+    const T q = T(9) * (x * x + y * y + z * z) - T(19);
+    const T c = T(1) / T(64);
+    // corner: N = c q a b d with a = 1 + sx x etc.; d2N/dx2 = c (18 a + 36 sx x) b d,
+    // d2N/dxdy = c (18 x a sy + 18 y sx b + q sx sy) d
+    for (int n = 0; n < 8; ++n) {
+        const int ix = n & 1, iy = (n >> 1) & 1, iz = (n >> 2) & 1;
+        const T a = lx[ix], b = ly[iy], d = lz[iz], sx = sg[ix], sy = sg[iy], sz = sg[iz];
+        H[n][0] = c * (T(18) * a + T(36) * sx * x) * b * d;
+        H[n][1] = c * (T(18) * b + T(36) * sy * y) * a * d;
+        H[n][2] = c * (T(18) * d + T(36) * sz * z) * a * b;
+        H[n][3] = c * (T(18) * x * a * sy + T(18) * y * sx * b + q * sx * sy) * d;
+        H[n][4] = c * (T(18) * x * a * sz + T(18) * z * sx * d + q * sx * sz) * b;
+        H[n][5] = c * (T(18) * y * b * sz + T(18) * z * sy * d + q * sy * sz) * a;
+    }
+    // edge: N = k e(s) u v with e = (1 - s^2)(1 + t 3s), t = -1 (side 0) or +1; e' = -2 s (1 + 3 t s) + 3 t (1 - s^2),
+    // e'' = -2 (1 + 3 t s) - 12 t s
+    const T k = T(9) / T(64);
+    auto edge = [](T s, int side, T& f, T& df, T& ddf) {
+        const T t = side == 0 ? T(-1) : T(1);
+        const T u = T(1) + T(3) * t * s;
+        f = (T(1) - s * s) * u;
+        df = T(-2) * s * u + T(3) * t * (T(1) - s * s);
+        ddf = T(-2) * u - T(12) * t * s;
+    };
+    for (int iy = 0; iy < 2; ++iy)  // x edges: 8 + 2 (iz + 2 iy) + side
+        for (int iz = 0; iz < 2; ++iz)
+            for (int side = 0; side < 2; ++side) {
+                T f, df, ddf;
+                edge(x, side, f, df, ddf);
+                T* h = H[8 + 2 * (iz + 2 * iy) + side];
+                h[0] = k * ddf * ly[iy] * lz[iz];
+                h[1] = T(0);
+                h[2] = T(0);
+                h[3] = k * df * sg[iy] * lz[iz];
+                h[4] = k * df * ly[iy] * sg[iz];
+                h[5] = k * f * sg[iy] * sg[iz];
+            }
+    for (int iz = 0; iz < 2; ++iz)  // y edges: 16 + 2 (ix + 2 iz) + side
+        for (int ix = 0; ix < 2; ++ix)
+            for (int side = 0; side < 2; ++side) {
+                T f, df, ddf;
+                edge(y, side, f, df, ddf);
+                T* h = H[16 + 2 * (ix + 2 * iz) + side];
+                h[0] = T(0);
+                h[1] = k * ddf * lx[ix] * lz[iz];
+                h[2] = T(0);
+                h[3] = k * df * sg[ix] * lz[iz];
+                h[4] = k * f * sg[ix] * sg[iz];
+                h[5] = k * df * lx[ix] * sg[iz];
+            }
+    for (int ix = 0; ix < 2; ++ix)  // z edges: 24 + 2 (iy + 2 ix) + side
+        for (int iy = 0; iy < 2; ++iy)
+            for (int side = 0; side < 2; ++side) {
+                T f, df, ddf;
+                edge(z, side, f, df, ddf);
+                T* h = H[24 + 2 * (iy + 2 * ix) + side];
+                h[0] = T(0);
+                h[1] = T(0);
+                h[2] = k * ddf * lx[ix] * ly[iy];
+                h[3] = k * f * sg[ix] * sg[iy];
+                h[4] = k * df * sg[ix] * ly[iy];
+                h[5] = k * df * lx[ix] * sg[iy];
+            }
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Analytic signed distances (stand-in for MeshDistance, Utility/SDF/MeshDistance.cpp). d < 0 inside the shape.
 // ---------------------------------------------------------------------------------------------------------------
@@ -514,7 +591,7 @@ template <class T> struct VolumeMap {
         }
     }
     // Locate the cell, build xi and the shape functions. Returns false outside the domain.
-    bool locate(V3<T> x, unsigned cell[32], V3<T>& c0, T N[32], V3<T> dN[32]) const {
+    bool locate(V3<T> x, unsigned cell[32], V3<T>& c0, T N[32], V3<T> dN[32], V3<T>* xiOut = nullptr) const {
         if (!domain.contains(x)) return false;
         const V3<T> f = cmul(cellSizeInv, x - domain.min);
         unsigned mi[3] = {(unsigned)f.x, (unsigned)f.y, (unsigned)f.z};
@@ -529,6 +606,27 @@ template <class T> struct VolumeMap {
         const V3<T> xi = cmul(c0, x) - c1;
         cell_nodes(mi[0], mi[1], mi[2], cell);
         shape_function(xi, N, dN);
+        if (xiOut) *xiOut = xi;
+        return true;
+    }
+    // Value, gradient and Hessian (xx, yy, zz, xy, xz, yz) of a field's interpolant at x in world coordinates;
+    // false outside the domain. The interpolant is a cubic polynomial per cell, so these are exact within a cell.
+    bool derivatives(unsigned field, V3<T> x, T& phi, V3<T>& grad, T hess[6]) const {
+        unsigned cell[32];
+        V3<T> c0, xi;
+        T N[32];
+        V3<T> dN[32];
+        if (!locate(x, cell, c0, N, dN, &xi)) return false;
+        phi = interp_grad(field, cell, c0, N, dN, grad);
+        T H[32][6];
+        shape_function_hess(xi, H);
+        for (int e = 0; e < 6; ++e) hess[e] = T(0);
+        for (int k = 0; k < 32; ++k) {
+            const T c = nodes[field][cell[k]];
+            for (int e = 0; e < 6; ++e) hess[e] += c * H[k][e];
+        }
+        const T s[6] = {c0.x * c0.x, c0.y * c0.y, c0.z * c0.z, c0.x * c0.y, c0.x * c0.z, c0.y * c0.z};
+        for (int e = 0; e < 6; ++e) hess[e] *= s[e];
         return true;
     }
     T interp(unsigned field, const unsigned cell[32], const T N[32]) const {
@@ -1043,8 +1141,21 @@ public:
         std::vector<std::vector<uint8_t>> mv, mp;      // residual masks (K each)
         T h = 0, H = 0;
         bool visc = false;
+        // Boundary contact sample of a particle against one body: r = x - xb = s n with s = max(phi + r/2, 2r) and
+        // n = grad phi / |grad phi| at the particle, the sample volume vb (> 0; 0 = no sample) and the derivatives
+        // of both in the particle's position (Jr = dr/dx, row-major).
+        struct Contact {
+            T vb = T(0);
+            V3<T> dvb{T(0), T(0), T(0)}, r{T(0), T(0), T(0)};
+            M3<T> Jr = mzero<T>();
+        };
+        std::vector<std::vector<Contact>> contact;  // [body][particle]
+        // Push-out of a particle found inside a body: x <- x + delta n, v <- 0; Jpush = dx_after / dx_before
+        std::vector<uint8_t> pushed;
+        std::vector<M3<T>> Jpush;
     };
     std::vector<StepTape> tapes;
+    std::vector<V3<T>> tapePre;  // positions the boundary pass of a taped step started from
     bool taping = false;
 
     explicit Sim(const vfd_desc& d) {
@@ -1675,6 +1786,7 @@ public:
         for (long i = 0; i < n; ++i) xs[i] = p[i].x;
         find_neighbors(xs, kernel.R, nb);
         if (last == VFD_PHASE_SEARCH) return;
+        if (taping) tapePre = xs;
         boundary();
         if (last == VFD_PHASE_BOUNDARY) return;
         if (taping) tape_begin();
@@ -1722,12 +1834,17 @@ public:
     // Adjoint of the step (reference gradient for the HIP adjoint; TEST INFRASTRUCTURE like the rest of this file).
     //
     // Differentiated map: one step (x, v; mu) -> (x', v') with the neighbour lists, the time steps, the solver
-    // iteration counts and the Jacobi clamp masks of the forward held fixed. Kernel values are the forward's (table
-    // or spline); their derivatives are the spline's (dW/drho, d(gradW/r)/drho), so with `smooth` kernels this is
-    // the exact derivative of the computed step and with the table it is the derivative of the step the table
-    // approximates. The Jacobi solves are reversed iteration by iteration (their transposed iteration), the viscosity
-    // solve by one transposed linear solve. Scope: no boundary contact, no surface tension, symmetric lists (no
-    // particle at the 70-neighbour cap); tape_begin reports anything outside it.
+    // iteration counts, the Jacobi clamp masks and the boundary pass's branches (which particles sample which body,
+    // which are pushed out, the max/min of the sample distance and the push depth) of the forward held fixed. Kernel
+    // values are the forward's (table or spline); their derivatives are the spline's (dW/drho, d(gradW/r)/drho), so
+    // with `smooth` kernels this is the exact derivative of the computed step and with the table it is the
+    // derivative of the step the table approximates. The Jacobi solves are reversed iteration by iteration (their
+    // transposed iteration), the viscosity solve by one transposed linear solve. Boundary contact enters through the
+    // sample terms of density, factor, divergence and pressure acceleration (vb W(x - xb), vb gradW(x - xb)), whose
+    // position derivatives follow the volume map's interpolant to second order (the normal's derivative is
+    // (I - n n^T) Hess(phi) / |grad phi|), and through the push-out's Jacobian. Scope: no surface tension, no
+    // boundary viscosity on a step with contact, no particle pushed out twice or pushed after a contact sample of an
+    // earlier body, symmetric lists (no particle at the 70-neighbour cap); tape_begin reports anything outside it.
     // ==========================================================================================================
     std::string tape_error;
     void tape_begin() {
@@ -1739,11 +1856,113 @@ public:
         for (long i = 0; i < n; ++i) t.x[i] = p[i].x;
         tape_velocity(t.v0);
         if (desc.EnableSurfaceTensionSolver) tape_error = "surface tension is outside the adjoint's scope";
-        for (auto& b : bodies)
-            for (long i = 0; i < n; ++i)
-                if (b.vb[i] > T(0)) tape_error = "boundary contact is outside the adjoint's scope";
+        if (info.muB != T(0))
+            for (auto& b : bodies)
+                for (long i = 0; i < n; ++i)
+                    if (b.vb[i] > T(0)) tape_error = "boundary viscosity at a boundary contact is outside the adjoint's scope";
         for (long i = 0; i < n; ++i)
             if (cnt(i) >= kMaxNeighbors) tape_error = "a particle reached the neighbour cap (lists not symmetric)";
+        tape_boundary(t);
+    }
+    // (I - n n^T) Hs / |g|: the derivative of n = g / |g| for g = grad phi, Hs = Hess(phi) as (xx, yy, zz, xy, xz, yz)
+    static M3<T> normal_jacobian(V3<T> n, T gl, const T hs[6]) {
+        const T Hm[9] = {hs[0], hs[3], hs[4], hs[3], hs[1], hs[5], hs[4], hs[5], hs[2]};
+        const T nv[3] = {n.x, n.y, n.z};
+        M3<T> J;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                T s = Hm[3 * r + c];
+                for (int k = 0; k < 3; ++k) s -= nv[r] * nv[k] * Hm[3 * k + c];
+                J.m[3 * r + c] = s / gl;
+            }
+        return J;
+    }
+    // Replays the boundary pass (boundary()) from the positions it started from, with the same branches, recording
+    // each contact sample and each push-out with their derivatives.
+    void tape_boundary(StepTape& t) {
+        const long n = (long)p.size();
+        t.contact.assign(bodies.size(), std::vector<typename StepTape::Contact>(n));
+        t.pushed.assign(n, 0);
+        t.Jpush.assign(n, mzero<T>());
+        for (long i = 0; i < n; ++i) {
+            V3<T> x = tapePre[i];
+            bool sampled = false;
+            for (size_t bi = 0; bi < bodies.size(); ++bi) {
+                const Body<T>& b = bodies[bi];
+                T phi, hs[6];
+                V3<T> g;
+                if (!b.map.derivatives(0, x, phi, g, hs)) continue;
+                const T gl = length(g);
+                if (b.vb[i] > T(0)) {
+                    const V3<T> nn = g / gl;
+                    const M3<T> Jn = normal_jacobian(nn, gl, hs);
+                    const T s0 = phi + T(0.5) * info.r;
+                    const T s = std::max(s0, info.diam);
+                    const V3<T> ds = s0 < info.diam ? v3<T>(0, 0, 0) : g;
+                    typename StepTape::Contact& c = t.contact[bi][i];
+                    c.vb = b.vb[i];
+                    T vol, hv[6];
+                    b.map.derivatives(1, x, vol, c.dvb, hv);
+                    c.r = x - b.xb[i];
+                    const T nv[3] = {nn.x, nn.y, nn.z}, dv[3] = {ds.x, ds.y, ds.z};
+                    for (int r = 0; r < 3; ++r)
+                        for (int k = 0; k < 3; ++k) c.Jr.m[3 * r + k] = nv[r] * dv[k] + s * Jn.m[3 * r + k];
+                    sampled = true;
+                } else if (phi <= T(0) && gl > T(1.0e-5)) {
+                    if (sampled || t.pushed[i])
+                        tape_error = "a particle pushed out twice or after a contact sample is outside the adjoint's scope";
+                    const V3<T> nn = g / gl;
+                    const M3<T> Jn = normal_jacobian(nn, gl, hs);
+                    const T d0 = info.diam - phi;
+                    const T delta = std::min(d0, T(0.1) * info.r);
+                    const V3<T> dd = T(0.1) * info.r < d0 ? v3<T>(0, 0, 0) : -g;
+                    const T nv[3] = {nn.x, nn.y, nn.z}, dv[3] = {dd.x, dd.y, dd.z};
+                    for (int r = 0; r < 3; ++r)
+                        for (int k = 0; k < 3; ++k)
+                            t.Jpush[i].m[3 * r + k] = (r == k ? T(1) : T(0)) + nv[r] * dv[k] + delta * Jn.m[3 * r + k];
+                    t.pushed[i] = 1;
+                    x = x + delta * nn;
+                }
+            }
+            if (x.x != p[i].x.x || x.y != p[i].x.y || x.z != p[i].x.z)
+                tape_error = "the replay of the boundary pass differs from the forward";
+        }
+    }
+    // boundary part of particle i's kernel-gradient sum: sum over bodies of vb gradW(x_i - xb)
+    V3<T> contact_sum(const StepTape& t, long i) const {
+        V3<T> s = v3<T>(0, 0, 0);
+        for (const auto& cb : t.contact) {
+            const auto& c = cb[i];
+            if (c.vb > T(0)) s = s + (c.vb * kernel.gval(length(c.r))) * c.r;
+        }
+        return s;
+    }
+    // d/dx_i of q . contact_sum(i) for a fixed vector q
+    V3<T> contact_vjp(const StepTape& t, long i, V3<T> q) const {
+        V3<T> o = v3<T>(0, 0, 0);
+        for (const auto& cb : t.contact) {
+            const auto& c = cb[i];
+            if (!(c.vb > T(0))) continue;
+            const T g = kernel.gval(length(c.r));
+            o = o + (g * dot3(c.r, q)) * c.dvb + mulmtv(c.Jr, vjp_grad(c.r, c.vb * q));
+        }
+        return o;
+    }
+    // sum over bodies of vb W(x_i - xb), and (grad != nullptr) its derivative in x_i
+    T contact_density(const StepTape& t, long i, V3<T>* grad) const {
+        T d = T(0);
+        for (const auto& cb : t.contact) {
+            const auto& c = cb[i];
+            if (!(c.vb > T(0))) continue;
+            const T w = kernel.getW(c.r);
+            d += c.vb * w;
+            if (grad) {
+                const T rl = length(c.r);
+                const T dw = rl > T(0) ? kernel.dW(rl) / rl : T(0);
+                *grad = *grad + w * c.dvb + mulmtv(c.Jr, (c.vb * dw) * c.r);
+            }
+        }
+        return d;
     }
     void tape_velocity(std::vector<V3<T>>& out) const {
         out.resize(p.size());
@@ -1769,7 +1988,7 @@ public:
                 const V3<T> r = t.x[i] - t.x[j];
                 a = a + (-info.V * (k[i] + k[j]) * kernel.gval(std::sqrt(dot3(r, r)))) * r;
             }
-            out[i] = a;
+            out[i] = a - k[i] * contact_sum(t, i);
         }
     }
     // D(u)_i = V sum_j (u_i - u_j) . gradW_ij   (divergence of a vector field)
@@ -1783,31 +2002,35 @@ public:
                 const V3<T> r = t.x[i] - t.x[j];
                 d += dot3(u[i] - u[j], kernel.gval(std::sqrt(dot3(r, r))) * r);
             }
-            out[i] = info.V * d;
+            out[i] = info.V * d + dot3(u[i], contact_sum(t, i));
         }
     }
     // position cotangents of A(k) against la, and of D(u) against lf
     void vjp_x_A(const StepTape& t, const std::vector<T>& k, const std::vector<V3<T>>& la,
                  std::vector<V3<T>>& lx) const {
         const long n = (long)t.x.size();
-        for (long i = 0; i < n; ++i)
+        for (long i = 0; i < n; ++i) {
             for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
                 const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
                 const V3<T> g = vjp_grad(t.x[i] - t.x[j], (-info.V * (k[i] + k[j])) * la[i]);
                 lx[i] = lx[i] + g;
                 lx[j] = lx[j] - g;
             }
+            lx[i] = lx[i] + contact_vjp(t, i, (-k[i]) * la[i]);
+        }
     }
     void vjp_x_D(const StepTape& t, const std::vector<V3<T>>& u, const std::vector<T>& lf,
                  std::vector<V3<T>>& lx) const {
         const long n = (long)t.x.size();
-        for (long i = 0; i < n; ++i)
+        for (long i = 0; i < n; ++i) {
             for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
                 const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
                 const V3<T> g = vjp_grad(t.x[i] - t.x[j], (info.V * lf[i]) * (u[i] - u[j]));
                 lx[i] = lx[i] + g;
                 lx[j] = lx[j] - g;
             }
+            lx[i] = lx[i] + contact_vjp(t, i, lf[i] * u[i]);
+        }
     }
     // reverse of one Jacobi solve (divergence: sign = -1, source D; pressure: sign = +1, source 1 - rhoadv):
     // kappa^{k+1} = kappa^k - s o r^k, r^k = M^k o (src - c f^k), f^k = D(A(kappa^k)).
@@ -1942,6 +2165,7 @@ public:
             T d = info.V * kernel.W0;
             for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj)
                 d += info.V * kernel.getW(t.x[i] - t.x[t.nb.ids[t.nb.offsets[i] + jj]]);
+            d += contact_density(t, i, nullptr);
             rho[i] = d * info.rho0;
         }
         if (t.visc) {
@@ -2006,6 +2230,8 @@ public:
                 const V3<T> r = t.x[i] - t.x[t.nb.ids[t.nb.offsets[i] + jj]];
                 gs = gs + (info.V * kernel.gval(std::sqrt(dot3(r, r)))) * r;
             }
+            gs = gs + contact_sum(t, i);
+            lx[i] = lx[i] + contact_vjp(t, i, (T(2) * lS) * gs);
             for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
                 const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
                 const V3<T> r = t.x[i] - t.x[j];
@@ -2015,8 +2241,8 @@ public:
                 lx[j] = lx[j] - g;
             }
         }
-        // density: rho_i = rho0 V (W0 + sum_j W_ij)
-        for (long i = 0; i < n; ++i)
+        // density: rho_i = rho0 (V (W0 + sum_j W_ij) + sum_b vb W(x_i - xb))
+        for (long i = 0; i < n; ++i) {
             for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
                 const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
                 const V3<T> r = t.x[i] - t.x[j];
@@ -2025,6 +2251,16 @@ public:
                 const V3<T> g = (info.rho0 * info.V * lrho[i] * kernel.dW(rl) / rl) * r;
                 lx[i] = lx[i] + g;
                 lx[j] = lx[j] - g;
+            }
+            V3<T> gb = v3<T>(0, 0, 0);
+            contact_density(t, i, &gb);
+            lx[i] = lx[i] + (info.rho0 * lrho[i]) * gb;
+        }
+        // boundary push-out: x = x_before + delta n(x_before), v = 0
+        for (long i = 0; i < n; ++i)
+            if (t.pushed[i]) {
+                lx[i] = mulmtv(t.Jpush[i], lx[i]);
+                lv[i] = v3<T>(0, 0, 0);
             }
     }
 
@@ -2406,6 +2642,12 @@ void orc_shape_function(const double* xi, double* N, double* dN) {
         dN[3 * k + 1] = d[k].y;
         dN[3 * k + 2] = d[k].z;
     }
+}
+void orc_shape_function_hess(const double* xi, double* H) {
+    double h[32][6];
+    shape_function_hess(v3<double>(xi[0], xi[1], xi[2]), h);
+    for (int k = 0; k < 32; ++k)
+        for (int e = 0; e < 6; ++e) H[6 * k + e] = h[k][e];
 }
 uint32_t orc_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3(x, y, z); }
 // cell node indices of a map of resolution res (SDF.cu:81-131)

@@ -67,6 +67,7 @@ def lib():
         L.orc_halton.argtypes = [ctypes.c_int, P]
         L.orc_gauss_legendre.argtypes = [ctypes.c_int, P, P]
         L.orc_shape_function.argtypes = [P, P, P]
+        L.orc_shape_function_hess.argtypes = [P, P]
         L.orc_morton3.restype = ctypes.c_uint32
         L.orc_morton3.argtypes = [ctypes.c_uint32] * 3
         L.orc_cell_nodes.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P]
@@ -250,6 +251,14 @@ def gauss_legendre(n=16):
     w = np.zeros(n)
     lib().orc_gauss_legendre(n, _p(x), _p(w))
     return x, w
+
+
+def shape_function_hess(xi):
+    """(32, 6) second derivatives (xx, yy, zz, xy, xz, yz) of the shape functions at local coordinates xi."""
+    xi = np.ascontiguousarray(xi, np.float64)
+    H = np.zeros(192)
+    lib().orc_shape_function_hess(_p(xi), _p(H))
+    return H.reshape(32, 6)
 
 
 def shape_function(xi):

@@ -199,6 +199,45 @@ def test_shape_functions_interpolate_cubics():
         assert np.allclose(dN.T @ fn, fd, atol=1e-7)
 
 
+def test_shape_function_hessian_matches_differences():
+    """Second derivatives used by the adjoint's boundary terms: a cubic's interpolant is the cubic, so its Hessian is
+    the cubic's own (closed form per monomial); the Hessians sum to zero (partition of unity); and they are the
+    central differences of the first derivatives."""
+    nodes = _corner_xi()
+    rng = np.random.default_rng(6)
+    mon = [(a, b, c) for a in range(4) for b in range(4) for c in range(4) if a + b + c <= 3]
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+
+    def d2(p, cf_mon, i, j):
+        s = 0.0
+        for cf, e in cf_mon:
+            e = list(e)
+            c = cf
+            for ax in (i, j):
+                c *= e[ax]
+                e[ax] -= 1
+                if e[ax] < 0:
+                    c = 0.0
+                    break
+            if c:
+                s += c * p[0] ** e[0] * p[1] ** e[1] * p[2] ** e[2]
+        return s
+
+    for _ in range(20):
+        coef = rng.normal(size=len(mon))
+        f = lambda p: sum(cf * p[0] ** a * p[1] ** b * p[2] ** c for cf, (a, b, c) in zip(coef, mon))
+        fn = np.array([f(p) for p in nodes])
+        xi = rng.uniform(-1, 1, 3)
+        H = O.shape_function_hess(xi)
+        assert np.abs(H.sum(0)).max() < 1e-12
+        for k, (i, j) in enumerate(pairs):
+            assert abs(H[:, k] @ fn - d2(xi, list(zip(coef, mon)), i, j)) < 1e-11, (i, j)
+        for k, (i, j) in enumerate(pairs):
+            e = np.eye(3)[j] * 1e-6
+            fd = (O.shape_function(xi + e)[1][:, i] - O.shape_function(xi - e)[1][:, i]) / 2e-6
+            assert np.allclose(H[:, k], fd, atol=1e-8), (i, j)
+
+
 def test_cell_node_layout_matches_local_coordinates():
     res = (3, 4, 5)
     dmin, dmax = np.array([-1.0, 0.5, 2.0]), np.array([2.0, 2.5, 4.5])
