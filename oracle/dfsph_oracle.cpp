@@ -1856,7 +1856,7 @@ public:
         for (long i = 0; i < n; ++i) t.x[i] = p[i].x;
         tape_velocity(t.v0);
         if (desc.EnableSurfaceTensionSolver) tape_error = "surface tension is outside the adjoint's scope";
-        if (info.muB != T(0))
+        if (info.muB != T(0) && desc.EnableViscositySolver)
             for (auto& b : bodies)
                 for (long i = 0; i < n; ++i)
                     if (b.vb[i] > T(0)) tape_error = "boundary viscosity at a boundary contact is outside the adjoint's scope";

@@ -35,6 +35,36 @@ def small_scene(kind="dam", seed=7):
     return [desc], [(x, v.astype(np.float32))], [[box, ball]]
 
 
+def contact_scene(kind="floor"):
+    """Scenes for the adjoint through boundary contact: fixed 4 ms steps, 3 + 3 Jacobi iterations, viscosity 20 with
+    no boundary viscosity, r = 0.01, velocities -0.5 m/s downwards plus a smooth field. Each has one extra particle
+    starting inside the solid (pushed out by 0.1 r in the first step, velocity zeroed), placed off the map's cell
+    faces (where the interpolant's gradient jumps).
+    floor: a 16 x 8 x 16 block (2,048 particles) whose two lowest layers sample the floor of a 1 m box (flat wall),
+    spaced 2.1 r so that no lattice pair sits at the support radius (a difference quotient would see the lists change).
+    sphere: an 8 x 5 x 8 block on the top of a sphere of radius 0.06 (curved wall: the normal's derivative matters).
+    Returns (desc, x (float64), v (float64), bodies)."""
+    rng = np.random.default_rng(1)
+    r = 0.01
+    if kind == "floor":
+        x = scenes.lattice_block((0.3, 0.025, 0.3), (16, 8, 16), 2.1 * r, 0.01 * r, rng).astype(np.float64)
+        x = np.vstack([x, [[0.311, 0.006, 0.309]]])
+        bodies = [RigidBodyDescription("box", center=(0.5, 0.5, 0.5), half_extents=(0.5, 0.5, 0.5), inverted=True,
+                                       resolution=(20, 20, 20))]
+    else:
+        x = scenes.lattice_block((0.23, 0.362, 0.23), (8, 5, 8), 2 * r, 0.01 * r, rng).astype(np.float64)
+        x = np.vstack([x, [[0.303, 0.348, 0.298]]])
+        bodies = [RigidBodyDescription("sphere", center=(0.3, 0.3, 0.3), radius=0.06, resolution=(16, 16, 16))]
+    v = 0.2 * np.sin(7 * x[:, [1, 2, 0]]) + 0.05 * rng.normal(size=x.shape)
+    v[:, 1] -= 0.5
+    d = DFSPHSimulationDescription(
+        ParticleRadius=r, Viscosity=0.02, BoundaryViscosity=0.0, EnableSurfaceTensionSolver=0, EnableViscositySolver=1,
+        MaxViscositySolverError=1e-5, MaxViscositySolverIterations=200, MinDivergenceSolverIterations=3,
+        MaxDivergenceSolverIterations=3, MinPressureSolverIterations=3, MaxPressureSolverIterations=3,
+        MinTimeStepSize=0.004, MaxTimeStepSize=0.004, TimeStepSize=0.004)
+    return d, x, v, bodies
+
+
 def make_gpu(descs, fluids, bodies):
     from vfd_amd import DFSPHBatch, DFSPHSimulation
     if len(descs) == 1:

@@ -5,8 +5,8 @@ derivative.
 Setup: a free 512-particle block with viscosity 20, fixed time step and fixed Jacobi iteration counts (the adjoint
 holds time steps, iteration counts, clamp masks and neighbour lists fixed, as the forward's derivative does between
 discrete events); the loss is a random linear functional of the positions and velocities after 3 steps. The contact
-case drops a block on a sphere (a curved volume map, so the normal's derivative matters) with one particle starting
-inside the sphere (pushed out in the first step)."""
+cases drop a block on a sphere (a curved volume map, so the normal's derivative matters) and on a flat floor, each
+with one particle starting inside the solid (pushed out in the first step)."""
 import copy
 
 import numpy as np
@@ -14,7 +14,8 @@ import pytest
 
 from oracle.oracle import OracleSim
 from vfd_amd import scenes
-from vfd_amd.simulation import DFSPHSimulationDescription, RigidBodyDescription
+from scene_util import contact_scene
+from vfd_amd.simulation import DFSPHSimulationDescription
 
 STEPS = 3
 
@@ -70,25 +71,6 @@ def test_reference_adjoint_matches_finite_differences():
         assert e < 1e-4, (k, e)
 
 
-def _contact_setup():
-    """8 x 5 x 8 block falling at 0.5 m/s onto the top of a sphere (radius 0.06, map resolution 16^3): 44 particles
-    sample the sphere in the first step. One extra particle starts 2 mm inside the sphere's padded surface (pushed out
-    by 0.1 r with its velocity zeroed) off the map's cell faces, where the map's gradient is continuous."""
-    rng = np.random.default_rng(1)
-    r = 0.01
-    x = scenes.lattice_block((0.23, 0.362, 0.23), (8, 5, 8), 2 * r, 0.01 * r, rng).astype(np.float64)
-    x = np.vstack([x, [[0.303, 0.348, 0.298]]])
-    v = 0.2 * np.sin(7 * x[:, [1, 2, 0]]) + 0.05 * rng.normal(size=x.shape)
-    v[:, 1] -= 0.5
-    d = DFSPHSimulationDescription(
-        ParticleRadius=r, Viscosity=0.02, BoundaryViscosity=0.0, EnableSurfaceTensionSolver=0, EnableViscositySolver=1,
-        MaxViscositySolverError=1e-5, MaxViscositySolverIterations=200, MinDivergenceSolverIterations=3,
-        MaxDivergenceSolverIterations=3, MinPressureSolverIterations=3, MaxPressureSolverIterations=3,
-        MinTimeStepSize=0.004, MaxTimeStepSize=0.004, TimeStepSize=0.004)
-    sphere = RigidBodyDescription("sphere", center=(0.3, 0.3, 0.3), radius=0.06, resolution=(16, 16, 16))
-    return d, x, v, [sphere]
-
-
 def _run_bodies(d, x, v, bodies, tape=False):
     o = OracleSim(d, fp64=True)
     o.set_smooth_kernel(True)
@@ -104,13 +86,14 @@ def _run_bodies(d, x, v, bodies, tape=False):
     return o
 
 
-def test_reference_adjoint_with_boundary_contact_matches_finite_differences():
-    """Directions restricted to the contact particles and to the pushed particle, so that the boundary terms carry
+@pytest.mark.parametrize("kind,contacts", [("sphere", 44), ("floor", 512)])
+def test_reference_adjoint_with_boundary_contact_matches_finite_differences(kind, contacts):
+    """A block landing on a sphere and on a flat floor (tests/scene_util.py contact_scene). Directions restricted to the contact particles and to the pushed particle, so that the boundary terms carry
     the derivative. Measured errors are 7e-6 (contact) and 1e-7 (push), the level of the free block; dropping any
     one boundary term (the normal's derivative in the sample offset or in the push, the distance's gradient, the
     volume's gradient, the density or factor sample terms, the sample term of the divergence) or transposing the
     offset's Jacobian moves one of them to 8e-5 or more."""
-    d, x, v, bodies = _contact_setup()
+    d, x, v, bodies = contact_scene(kind)
     n = len(x)
     probe = OracleSim(d, fp64=True)
     probe.set_smooth_kernel(True)
@@ -122,7 +105,7 @@ def test_reference_adjoint_with_boundary_contact_matches_finite_differences():
     probe.step_until("boundary")
     contact = probe.field("boundary_volume") > 0
     moved = np.abs(probe.field("position") - x).max(1) > 0
-    assert contact.sum() == 44 and moved.tolist() == [False] * (n - 1) + [True]
+    assert contact.sum() == contacts and moved.tolist() == [False] * (n - 1) + [True]
     rng = np.random.default_rng(7)
     wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
 
@@ -147,7 +130,7 @@ def test_reference_adjoint_with_boundary_contact_matches_finite_differences():
 
 
 def test_tape_rejects_boundary_viscosity_at_contact():
-    d, x, v, bodies = _contact_setup()
+    d, x, v, bodies = contact_scene("sphere")
     d.BoundaryViscosity = 0.02
     o = _run_bodies(d, x, v, bodies, tape=True)
     with pytest.raises(RuntimeError, match="boundary viscosity"):

@@ -1,12 +1,15 @@
-"""The HIP adjoint (vfd_tape / vfd_adjoint) against the oracle's reference adjoint on a 2,048-particle viscous block.
+"""The HIP adjoint (vfd_tape / vfd_adjoint) against the oracle's reference adjoint on a 2,048-particle viscous block,
+free and landing on a floor, and on a block landing on a sphere.
 
 The oracle runs the same forward in its FP32 mirror (identical lists, masks and iteration counts, so the two adjoints
 differ only in summation order) and in its FP64 twin (the FP64 reference gradient). The differentiated map and its
-scope are those of vfd_adjoint: fixed time step, fixed Jacobi iteration counts, no boundary contact."""
+scope are those of vfd_adjoint: fixed time step, fixed Jacobi iteration counts, boundary contact without boundary
+viscosity, no surface tension."""
 import numpy as np
 import pytest
 
 from oracle.oracle import OracleSim
+from scene_util import contact_scene
 from vfd_amd import DFSPHSimulation, scenes
 from vfd_amd.simulation import DFSPHSimulationDescription
 
@@ -66,14 +69,77 @@ def test_adjoint_matches_oracle_reference():
             assert e[k] < 1e-3, (tag, k, e[k])
 
 
-def test_adjoint_rejects_boundary_contact():
-    descs, fluids, bodies = scenes.dam_break_8k()
-    gpu = DFSPHSimulation(descs[0])
-    gpu.set_fluid(*fluids[0])
-    gpu.SetRigidBodies(bodies[0])
+@pytest.mark.parametrize("kind,contacts", [("floor", 512), ("sphere", 44)])
+def test_adjoint_through_boundary_contact_matches_oracle(kind, contacts):
+    """Contact samples (density, factor, divergence and pressure terms through the volume map) and the push-out of
+    one particle starting inside the solid. Directions over all particles, over the contact particles of the first
+    step and on the pushed particle; the pushed particle's entry velocity has no effect. Against the FP32 mirror
+    (same forward, bit for bit) the bar is 1e-3; against the FP64 twin it is 1e-3 or twice the mirror's own distance
+    from the twin, whichever is larger: the FP32 and FP64 forwards part slightly (on the sphere the mirror's mu
+    derivative is 8e-4 from the twin's), and the GPU cannot be closer to the twin than the forward it differentiates."""
+    d, x, v, bodies = contact_scene(kind)
+    x32, v32 = x.astype(np.float32), v.astype(np.float32)
+    n = len(x)
+    rng = np.random.default_rng(11)
+    wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    print("contact adjoint", kind, n, "particles", flush=True)
+    gpu = DFSPHSimulation(d)
+    gpu.set_fluid(x32, v32)
+    gpu.SetRigidBodies(bodies)
+    gpu.tape(True)
+    for _ in range(STEPS):
+        gpu.OnUpdate()
+    gx, gv, gmu = gpu.adjoint(wx, wv)
+    probe = OracleSim(d)
+    probe.set_fluid(x32, v32)
+    for b in bodies:
+        probe.add_boundary(b.to_abi())
+    probe.step_until("boundary")
+    contact = probe.field("boundary_volume") > 0
+    assert contact.sum() == contacts
+    push = np.zeros((n, 3))
+    push[-1] = rng.normal(size=3)
+    dirs = {"x0": rng.normal(size=(n, 3)), "x0_contact": rng.normal(size=(n, 3)) * contact[:, None], "x0_push": push,
+            "v0": rng.normal(size=(n, 3)), "v0_contact": rng.normal(size=(n, 3)) * contact[:, None]}
+    assert not gv[-1].any()
+    out, ref = {}, {}
+    for fp64 in (False, True):
+        o = OracleSim(d, fp64=fp64)
+        o.set_fluid(x32, v32)
+        for b in bodies:
+            o.add_boundary(b.to_abi())
+        o.tape(True)
+        for _ in range(STEPS):
+            o.step()
+        if not fp64:
+            assert np.array_equal(o.field("position").astype(np.float32), gpu.field("position"))
+        ref[fp64] = o.adjoint(wx, wv)
+        ox, ov, omu = ref[fp64]
+        e = {k: _rel(float(((gx if k[0] == "x" else gv) * dv).sum()), float(((ox if k[0] == "x" else ov) * dv).sum()))
+             for k, dv in dirs.items()}
+        e["mu"] = _rel(gmu, omu)
+        e["x0_inf"] = float(np.abs(gx - ox).max() / np.abs(ox).max())
+        out["fp64" if fp64 else "fp32"] = e
+        print("gpu contact adjoint vs oracle", kind, "fp64" if fp64 else "fp32", {k: "%.2e" % v for k, v in e.items()},
+              flush=True)
+    (mx, mv, mmu), (tx, tv, tmu) = ref[False], ref[True]
+    mirror = {k: _rel(float(((mx if k[0] == "x" else mv) * dv).sum()), float(((tx if k[0] == "x" else tv) * dv).sum()))
+              for k, dv in dirs.items()}
+    mirror["mu"] = _rel(mmu, tmu)
+    for k in list(dirs) + ["mu"]:
+        assert out["fp32"][k] < 1e-3, ("fp32", k, out["fp32"][k])
+        assert out["fp64"][k] < max(1e-3, 2 * mirror[k]), ("fp64", k, out["fp64"][k], mirror[k])
+
+
+def test_adjoint_rejects_boundary_viscosity_at_contact():
+    d, x, v, bodies = contact_scene("floor")
+    d.BoundaryViscosity = 0.02
+    gpu = DFSPHSimulation(d)
+    gpu.set_fluid(x.astype(np.float32), v.astype(np.float32))
+    gpu.SetRigidBodies(bodies)
     gpu.tape(True)
     gpu.OnUpdate()
-    with pytest.raises(Exception, match="boundary"):
+    with pytest.raises(Exception, match="boundary viscosity"):
         gpu.adjoint(np.zeros((gpu.GetParticleCount(), 3), np.float32))
 
 

@@ -8,6 +8,11 @@
 // when recording), so every position cotangent is gathered per particle: pair (i, j) contributes to both ends from
 // i's list and from j's, without atomics.
 //
+// Boundary contact enters through the sample terms vb W(x - xb) (density) and vb gradW(x - xb) (factor, divergence,
+// pressure acceleration), whose position derivatives the recording kernel (k_adj_boundary_tape) stores per particle
+// and body from the volume map's interpolant (gradient of vb, Jacobian of r = x - xb including the normal's
+// derivative), and through the push-out's Jacobian; a pushed particle's entry velocity has no effect (it is zeroed).
+//
 // Per-step arrays are in that step's sorted slot order; the host maps cotangents through the recorded ids.
 #include "device.h"
 #include "kernels.h"
@@ -21,6 +26,8 @@ struct AdjK {
     const uint32_t* __restrict__ count;
     const uint32_t* __restrict__ list;  // [j * cap + i]
     const float4* __restrict__ x;
+    const float4* __restrict__ con;  // contact records [(5 b + q) n + i] (k_adj_boundary_tape); bodies = 0: none
+    uint32_t bodies;
 };
 
 __device__ __forceinline__ float adj_g(const AdjK& a, float rl) {  // gradW / r as the forward's table gives it
@@ -56,12 +63,63 @@ __device__ __forceinline__ uint32_t adj_nbr(const AdjK& a, uint32_t i, uint32_t 
     return a.list[(size_t)j * a.cap + i];
 }
 __device__ __forceinline__ uint32_t pid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+// Boundary sample records of particle i, body b: {r = x - xb, vb}, {grad vb}, rows of dr/dx.
+struct AdjContact {
+    f3 r, dv, j0, j1, j2;
+    float vb;
+};
+__device__ __forceinline__ bool adj_contact(const AdjK& a, uint32_t b, uint32_t i, AdjContact& c) {
+    const float4* p = a.con + (size_t)5 * b * a.n + i;
+    const float4 r = p[0];
+    if (!(r.w > 0.0f)) return false;
+    c.r = mk3(r.x, r.y, r.z);
+    c.vb = r.w;
+    c.dv = ld3(p[a.n]);
+    c.j0 = ld3(p[2 * (size_t)a.n]);
+    c.j1 = ld3(p[3 * (size_t)a.n]);
+    c.j2 = ld3(p[4 * (size_t)a.n]);
+    return true;
+}
+// sum over bodies of vb gradW(r): the boundary part of particle i's kernel-gradient sum
+__device__ __forceinline__ f3 adj_contact_sum(const AdjK& a, uint32_t i) {
+    f3 s = mk3(0.0f, 0.0f, 0.0f);
+    AdjContact c;
+    for (uint32_t b = 0; b < a.bodies; ++b)
+        if (adj_contact(a, b, i, c)) s = s + (c.vb * adj_g(a, sqrtf(dot3(c.r, c.r)))) * c.r;
+    return s;
+}
+// d/dx_i of q . adj_contact_sum(i) for a fixed q: grad vb (gradW . q) + (dr/dx)^T vjp(r, vb q)
+__device__ __forceinline__ f3 adj_contact_vjp(const AdjK& a, uint32_t i, f3 q) {
+    f3 o = mk3(0.0f, 0.0f, 0.0f);
+    AdjContact c;
+    for (uint32_t b = 0; b < a.bodies; ++b) {
+        if (!adj_contact(a, b, i, c)) continue;
+        const float rl = sqrtf(dot3(c.r, c.r)), g = adj_g(a, rl);
+        const f3 u = vjp_grad(a, c.r, rl, g, c.vb * q);
+        o = o + (g * dot3(c.r, q)) * c.dv + (u.x * c.j0 + u.y * c.j1 + u.z * c.j2);
+    }
+    return o;
+}
+// d/dx_i of sum over bodies of vb W(r)
+__device__ __forceinline__ f3 adj_contact_density_grad(const AdjK& a, uint32_t i) {
+    f3 o = mk3(0.0f, 0.0f, 0.0f);
+    AdjContact c;
+    for (uint32_t b = 0; b < a.bodies; ++b) {
+        if (!adj_contact(a, b, i, c)) continue;
+        const float r2 = dot3(c.r, c.r), rl = sqrtf(r2);
+        const float w = adj_w(a, r2), dw = rl > 0.0f ? c.vb * adj_dW(a, rl) / rl : 0.0f;
+        o = o + w * c.dv + (dw * c.r.x * c.j0 + dw * c.r.y * c.j1 + dw * c.r.z * c.j2);
+    }
+    return o;
+}
+
 // The reverse neighbour loops fetch kAdjBatch list entries, then those neighbours' data, before computing any of
 // them, so that the dependent loads overlap; the terms are still added in list order.
 constexpr uint32_t kAdjBatch = 4;
 
-// lk_i -= V sum_j (la_i - la_j) . gradW_ij            (A^T la = -D(la))
-// lx_i += sum_j vjp(r_ij, -V (k_i + k_j)(la_i - la_j))  (positions through A(k))
+// lk_i -= V sum_j (la_i - la_j) . gradW_ij + la_i . B_i          (A^T la = -D(la))
+// lx_i += sum_j vjp(r_ij, -V (k_i + k_j)(la_i - la_j)) + d/dx_i (-k_i la_i . B_i)  (positions through A(k))
+// with B_i = sum_b vb gradW(x_i - xb) the boundary samples' term
 __global__ void __launch_bounds__(kTile) k_adj_accel_T(AdjK a, const float* __restrict__ kap,
                                                        const float4* __restrict__ la, float* lk, float4* lx) {
     const uint32_t i = pid();
@@ -91,12 +149,18 @@ __global__ void __launch_bounds__(kTile) k_adj_accel_T(AdjK a, const float* __re
             gx = gx + vjp_grad(a, r, rl, g, (-a.V * (ki + kj[u])) * dl);
         }
     }
-    lk[i] -= a.V * d;
+    if (a.bodies) {
+        d = a.V * d + dot3(lai, adj_contact_sum(a, i));
+        gx = gx + adj_contact_vjp(a, i, (-ki) * lai);
+    } else {
+        d = a.V * d;
+    }
+    lk[i] -= d;
     lx[i] = st4(ld3(lx[i]) + gx);
 }
 
-// lu_i += V sum_j (lf_i + lf_j) gradW_ij               (D^T lf = -A(lf))
-// lx_i += sum_j vjp(r_ij, V (lf_i + lf_j)(u_i - u_j))   (positions through D(u))
+// lu_i += V sum_j (lf_i + lf_j) gradW_ij + lf_i B_i                       (D^T lf = -A(lf))
+// lx_i += sum_j vjp(r_ij, V (lf_i + lf_j)(u_i - u_j)) + d/dx_i (lf_i u_i . B_i)  (positions through D(u))
 __global__ void __launch_bounds__(kTile) k_adj_div_T(AdjK a, const float* __restrict__ lf,
                                                      const float4* __restrict__ u, float4* lu, float4* lx) {
     const uint32_t i = pid();
@@ -125,7 +189,12 @@ __global__ void __launch_bounds__(kTile) k_adj_div_T(AdjK a, const float* __rest
             gx = gx + vjp_grad(a, r, rl, g, (a.V * l2) * (ui - uj[q]));
         }
     }
-    lu[i] = st4(ld3(lu[i]) + a.V * s);
+    s = a.V * s;
+    if (a.bodies) {
+        s = s + lfi * adj_contact_sum(a, i);
+        gx = gx + adj_contact_vjp(a, i, lfi * ui);
+    }
+    lu[i] = st4(ld3(lu[i]) + s);
     lx[i] = st4(ld3(lx[i]) + gx);
 }
 
@@ -264,7 +333,7 @@ __global__ void __launch_bounds__(kTile) k_adj_visc_vjp(AdjK a, float h, const f
     if (threadIdx.x == 0) muPart[blockIdx.x] = v[0] / (double)a.mu;
 }
 
-// factor: S_i = V^2 sum_j |gradW_ij|^2 + |gs_i|^2, gs_i = V sum_j gradW_ij, alpha = 1/S
+// factor: S_i = V^2 sum_j |gradW_ij|^2 + |gs_i|^2, gs_i = V sum_j gradW_ij + B_i, alpha = 1/S
 __global__ void __launch_bounds__(kTile) k_adj_gsum(AdjK a, float4* gs) {
     const uint32_t i = pid();
     if (i >= a.n) return;
@@ -274,7 +343,9 @@ __global__ void __launch_bounds__(kTile) k_adj_gsum(AdjK a, float4* gs) {
         const f3 r = xi - ld3(a.x[adj_nbr(a, i, jj)]);
         s = s + adj_g(a, sqrtf(dot3(r, r))) * r;
     }
-    gs[i] = st4(a.V * s);
+    s = a.V * s;
+    if (a.bodies) s = s + adj_contact_sum(a, i);
+    gs[i] = st4(s);
 }
 __global__ void __launch_bounds__(kTile) k_adj_factor_lS(uint32_t n, const float* __restrict__ alpha,
                                                          const float* __restrict__ lalpha, float* lS) {
@@ -283,6 +354,7 @@ __global__ void __launch_bounds__(kTile) k_adj_factor_lS(uint32_t n, const float
 }
 // lx_i += sum_j vjp(r_ij, 2V [V gradW_ij (lS_i + lS_j) + lS_i gs_i - lS_j gs_j])
 //       + sum_j rho0 V (lrho_i + lrho_j) W'(|r|) r / |r|                     (densities)
+//       + d/dx_i (2 lS_i gs_i . B_i + rho0 lrho_i sum_b vb W(x_i - xb))       (boundary samples)
 __global__ void __launch_bounds__(kTile) k_adj_factor_density(AdjK a, const float* __restrict__ lS,
                                                               const float4* __restrict__ gs,
                                                               const float* __restrict__ lrho, float4* lx) {
@@ -299,7 +371,21 @@ __global__ void __launch_bounds__(kTile) k_adj_factor_density(AdjK a, const floa
         gx = gx + vjp_grad(a, r, rl, g, c);
         if (rl > 0.0f) gx = gx + (a.rho0 * a.V * (lri + lrho[j]) * adj_dW(a, rl) / rl) * r;
     }
+    if (a.bodies)
+        gx = gx + adj_contact_vjp(a, i, (2.0f * lSi) * gsi) + (a.rho0 * lri) * adj_contact_density_grad(a, i);
     lx[i] = st4(ld3(lx[i]) + gx);
+}
+
+// push-out reversed: x = x_before + delta n(x_before), v = 0  ->  lx = J^T lx, lv = 0 (rows of J in push[q n + i])
+__global__ void __launch_bounds__(kTile) k_adj_push(uint32_t n, const float4* __restrict__ push, float4* lx,
+                                                    float4* lv) {
+    const uint32_t i = pid();
+    if (i >= n) return;
+    const float4 r0 = push[i];
+    if (!(r0.w > 0.0f)) return;
+    const f3 l = ld3(lx[i]);
+    lx[i] = st4(l.x * mk3(r0.x, r0.y, r0.z) + l.y * ld3(push[n + i]) + l.z * ld3(push[2 * (size_t)n + i]));
+    lv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // BiCGSTAB helpers: per-block FP64 partial sums of a . b
@@ -430,16 +516,6 @@ __global__ void k_adj_to_orig(uint32_t n, const uint32_t* __restrict__ id, const
 }
 
 // ---- host side ---------------------------------------------------------------------------------------------------
-// flag = 1 when any particle of a recorded step has a boundary volume (contact is outside the adjoint's scope)
-__global__ void __launch_bounds__(kTile) k_adj_contact(uint32_t n, uint32_t cap, uint32_t bodies,
-                                                       const float* __restrict__ vb, uint32_t* flag) {
-    const uint32_t i = blockIdx.x * kTile + threadIdx.x;
-    if (i >= n) return;
-    bool hit = false;
-    for (uint32_t b = 0; b < bodies; ++b) hit |= vb[(size_t)b * cap + i] > 0.0f;
-    if (hit) *flag = 1u;
-}
-
 static inline dim3 grid_of(uint32_t n) { return dim3((n + kTile - 1) / kTile); }
 
 void adj_launch_accel_T(hipStream_t s, const AdjKHost& h, const float* kap, const float4* la, float* lk, float4* lx) {
@@ -504,11 +580,11 @@ void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const flo
 }
 
 AdjK AdjKHost::dev() const {
-    return AdjK{tab, R, invStep, K, L, V, rho0, m, mu, R2, n, cap, count, list, x};
+    return AdjK{tab, R, invStep, K, L, V, rho0, m, mu, R2, n, cap, count, list, x, con, bodies};
 }
 
-void adj_launch_contact(hipStream_t s, uint32_t n, uint32_t cap, uint32_t bodies, const float* vb, uint32_t* flag) {
-    if (n) hipLaunchKernelGGL(k_adj_contact, grid_of(n), dim3(kTile), 0, s, n, cap, bodies, vb, flag);
+void adj_launch_push(hipStream_t s, uint32_t n, const float4* push, float4* lx, float4* lv) {
+    if (n) hipLaunchKernelGGL(k_adj_push, grid_of(n), dim3(kTile), 0, s, n, push, lx, lv);
 }
 
 void adj_launch_bicg_iteration(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const BicgVectors& v,

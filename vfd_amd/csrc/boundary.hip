@@ -111,7 +111,8 @@ __device__ __forceinline__ void cell_nodes(const MapDev& m, uint32_t i, uint32_t
 
 // DetermineShapeFunction (SDFDeviceData.cuh:372-419): false outside the map domain.
 template <bool kGrad>
-__device__ __forceinline__ bool map_locate(const MapDev& m, f3 x, uint32_t cell[32], f3& c0, float N[32], f3 dN[32]) {
+__device__ __forceinline__ bool map_locate(const MapDev& m, f3 x, uint32_t cell[32], f3& c0, float N[32], f3 dN[32],
+                                           f3* xiOut = nullptr) {
     if (!(m.domMin[0] <= x.x && m.domMin[1] <= x.y && m.domMin[2] <= x.z && m.domMax[0] >= x.x &&
           m.domMax[1] >= x.y && m.domMax[2] >= x.z))
         return false;
@@ -131,7 +132,76 @@ __device__ __forceinline__ bool map_locate(const MapDev& m, f3 x, uint32_t cell[
     const f3 xi = cmul(c0, x) - c1;
     cell_nodes(m, mi[0], mi[1], mi[2], cell);
     shape_fn<kGrad>(xi, N, dN);
+    if (xiOut) *xiOut = xi;
     return true;
+}
+
+// sum_k c_k Hess(N_k)(xi) in the local coordinates, as (xx, yy, zz, xy, xz, yz), c_k = nodes[cell[k]]: the second
+// derivatives of the interpolant, for the adjoint's derivative of the boundary normal (no reference counterpart).
+// Corners N = q a b d / 64 (q = 9 |xi|^2 - 19, a = 1 + sx x, ...): N_xx = (18 a + 36 sx x) b d / 64,
+// N_xy = (18 x a sy + 18 y sx b + q sx sy) d / 64. Edges N = (9/64) e(s) u v with e = (1 - s^2)(1 + 3 t s):
+// e'' = -2 (1 + 3 t s) - 12 t s.
+__device__ __forceinline__ void shape_hess_sum(f3 xi, const float* __restrict__ nodes, const uint32_t cell[32],
+                                               float hs[6]) {
+    const float x = xi.x, y = xi.y, z = xi.z;
+    const float lx[2] = {1.0f - x, 1.0f + x}, ly[2] = {1.0f - y, 1.0f + y}, lz[2] = {1.0f - z, 1.0f + z};
+    const float sg[2] = {-1.0f, 1.0f};
+    const float q = 9.0f * (x * x + y * y + z * z) - 19.0f;
+    const float c = 1.0f / 64.0f;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) hs[e] = 0.0f;
+    for (int n = 0; n < 8; ++n) {
+        const int ix = n & 1, iy = (n >> 1) & 1, iz = (n >> 2) & 1;
+        const float a = lx[ix], b = ly[iy], d = lz[iz], sx = sg[ix], sy = sg[iy], sz = sg[iz];
+        const float cv = nodes[cell[n]] * c;
+        hs[0] += cv * (18.0f * a + 36.0f * sx * x) * b * d;
+        hs[1] += cv * (18.0f * b + 36.0f * sy * y) * a * d;
+        hs[2] += cv * (18.0f * d + 36.0f * sz * z) * a * b;
+        hs[3] += cv * (18.0f * x * a * sy + 18.0f * y * sx * b + q * sx * sy) * d;
+        hs[4] += cv * (18.0f * x * a * sz + 18.0f * z * sx * d + q * sx * sz) * b;
+        hs[5] += cv * (18.0f * y * b * sz + 18.0f * z * sy * d + q * sy * sz) * a;
+    }
+    const float k = 9.0f / 64.0f;
+    auto edge = [](float s, int side, float& f, float& df, float& ddf) {
+        const float t = side == 0 ? -1.0f : 1.0f;
+        const float u = 1.0f + 3.0f * t * s;
+        f = (1.0f - s * s) * u;
+        df = -2.0f * s * u + 3.0f * t * (1.0f - s * s);
+        ddf = -2.0f * u - 12.0f * t * s;
+    };
+    for (int iy = 0; iy < 2; ++iy)
+        for (int iz = 0; iz < 2; ++iz)
+            for (int side = 0; side < 2; ++side) {
+                float f, df, ddf;
+                edge(x, side, f, df, ddf);
+                const float cv = nodes[cell[8 + 2 * (iz + 2 * iy) + side]] * k;
+                hs[0] += cv * ddf * ly[iy] * lz[iz];
+                hs[3] += cv * df * sg[iy] * lz[iz];
+                hs[4] += cv * df * ly[iy] * sg[iz];
+                hs[5] += cv * f * sg[iy] * sg[iz];
+            }
+    for (int iz = 0; iz < 2; ++iz)
+        for (int ix = 0; ix < 2; ++ix)
+            for (int side = 0; side < 2; ++side) {
+                float f, df, ddf;
+                edge(y, side, f, df, ddf);
+                const float cv = nodes[cell[16 + 2 * (ix + 2 * iz) + side]] * k;
+                hs[1] += cv * ddf * lx[ix] * lz[iz];
+                hs[3] += cv * df * sg[ix] * lz[iz];
+                hs[4] += cv * f * sg[ix] * sg[iz];
+                hs[5] += cv * df * lx[ix] * sg[iz];
+            }
+    for (int ix = 0; ix < 2; ++ix)
+        for (int iy = 0; iy < 2; ++iy)
+            for (int side = 0; side < 2; ++side) {
+                float f, df, ddf;
+                edge(z, side, f, df, ddf);
+                const float cv = nodes[cell[24 + 2 * (iy + 2 * ix) + side]] * k;
+                hs[2] += cv * ddf * lx[ix] * ly[iy];
+                hs[3] += cv * f * sg[ix] * sg[iy];
+                hs[4] += cv * df * sg[ix] * ly[iy];
+                hs[5] += cv * df * lx[ix] * sg[iy];
+            }
 }
 
 __device__ __forceinline__ f3 node_position(const MapDev& m, uint32_t i) {
@@ -589,6 +659,128 @@ __global__ void __launch_bounds__(kTile) k_boundary(const SceneParams* __restric
 void launch_boundary(const Launch& L, const MapDev* maps, float4* pos, float4* vel, float4* xb, float* vb) {
     hipLaunchKernelGGL(k_boundary, dim3(L.tiles), dim3(kTile), 0, L.stream, L.P, L.tileScene, maps, pos, vel, xb, vb,
                        L.capacity);
+}
+
+// (I - n n^T) Hs / |g|: the derivative of the unit normal n = g / |g|, Hs = Hess(phi) as (xx, yy, zz, xy, xz, yz)
+__device__ __forceinline__ void normal_jacobian(f3 n, float gl, const float hs[6], float J[9]) {
+    const float Hm[9] = {hs[0], hs[3], hs[4], hs[3], hs[1], hs[5], hs[4], hs[5], hs[2]};
+    const float nv[3] = {n.x, n.y, n.z};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float s = Hm[3 * r + c];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s -= nv[r] * nv[k] * Hm[3 * k + c];
+            J[3 * r + c] = s / gl;
+        }
+}
+
+// Adjoint recording of the boundary pass of one scene (no reference counterpart): replays k_boundary from the
+// positions it started from, with the same arithmetic and branches, and stores per particle and body the contact
+// sample's derivatives (con[(5 b + q) n + i]: q = 0 {r = x - xb, vb}, 1 {grad vb, 0}, 2..4 the rows of dr/dx, with
+// r = s n, s = max(phi + r/2, 2r)) and per particle the push-out's Jacobian (push[q n + i], q = 0..2 rows; row 0's w
+// = 1 for a pushed particle). flags[0] / flags[1]: some particle has a contact sample / was pushed; flags[2]: a
+// particle was pushed twice or after a contact sample, or the replay's position differs from the forward's.
+__global__ void __launch_bounds__(kTile) k_adj_boundary_tape(const SceneParams* __restrict__ P,
+                                                             const MapDev* __restrict__ maps, uint32_t n,
+                                                             uint32_t capacity, const float4* __restrict__ xpre,
+                                                             const float4* __restrict__ posAfter,
+                                                             const float* __restrict__ vbF,
+                                                             const float4* __restrict__ xbF, float4* con,
+                                                             float4* push, uint32_t* flags) {
+    const uint32_t i = blockIdx.x * kTile + threadIdx.x;
+    if (i >= n) return;
+    const SceneParams& p = P[0];
+    f3 x = ld3(xpre[i]);
+    bool sampled = false, pushed = false, bad = false;
+    float Jp[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+    for (uint32_t b = 0; b < p.bodyCount; ++b) {
+        const MapDev& m = maps[b];
+        float4* cr = con + (size_t)5 * b * n + i;
+        cr[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        f3 normal = mk3(0.0f, 0.0f, 0.0f);
+        float dist = FLT_MAX;
+        uint32_t cell[32];
+        f3 c0, xi;
+        float N[32];
+        f3 dN[32];
+        if (map_locate<true>(m, x, cell, c0, N, dN, &xi)) {
+            float phi = 0.0f;
+            f3 g = mk3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                const float cv = m.nodes0[cell[k]];
+                phi += cv * N[k];
+                g = g + cv * dN[k];
+            }
+            normal = cmul(g, c0);
+            dist = phi;
+        } else {
+            continue;
+        }
+        const float vb = vbF[(size_t)b * capacity + i];
+        const bool contact = vb > 0.0f;
+        const bool push = !contact && dist <= 0.0f && length(normal) > 1.0e-5f;
+        if (!contact && !push) continue;
+        float hs[6];
+        shape_hess_sum(xi, m.nodes0, cell, hs);
+        const float sc[6] = {c0.x * c0.x, c0.y * c0.y, c0.z * c0.z, c0.x * c0.y, c0.x * c0.z, c0.y * c0.z};
+#pragma unroll
+        for (int e = 0; e < 6; ++e) hs[e] *= sc[e];
+        const float nl = length(normal);
+        const f3 nn = normal / nl;
+        float Jn[9];
+        normal_jacobian(nn, nl, hs, Jn);
+        const float nv[3] = {nn.x, nn.y, nn.z};
+        if (contact) {
+            const float s0 = dist + 0.5f * p.r;
+            const float s = fmaxf(s0, p.diam);
+            const f3 ds = s0 < p.diam ? mk3(0.0f, 0.0f, 0.0f) : normal;
+            f3 dv = mk3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) dv = dv + m.nodes1[cell[k]] * dN[k];
+            dv = cmul(dv, c0);
+            const f3 r = x - ld3(xbF[(size_t)b * capacity + i]);
+            const float dsv[3] = {ds.x, ds.y, ds.z};
+            cr[0] = make_float4(r.x, r.y, r.z, vb);
+            cr[n] = make_float4(dv.x, dv.y, dv.z, 0.0f);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                cr[(size_t)(2 + q) * n] = make_float4(nv[q] * dsv[0] + s * Jn[3 * q], nv[q] * dsv[1] + s * Jn[3 * q + 1],
+                                                      nv[q] * dsv[2] + s * Jn[3 * q + 2], 0.0f);
+            sampled = true;
+        } else {
+            if (sampled || pushed) bad = true;
+            const float d0 = p.diam - dist;
+            const float delta = fminf(d0, 0.1f * p.r);
+            const f3 dd = 0.1f * p.r < d0 ? mk3(0.0f, 0.0f, 0.0f) : -1.0f * normal;
+            const float ddv[3] = {dd.x, dd.y, dd.z};
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    Jp[3 * r + k] = (r == k ? 1.0f : 0.0f) + nv[r] * ddv[k] + delta * Jn[3 * r + k];
+            x = x + delta * nn;
+            pushed = true;
+        }
+    }
+    const f3 xa = ld3(posAfter[i]);
+    if (x.x != xa.x || x.y != xa.y || x.z != xa.z) bad = true;
+    push[i] = make_float4(Jp[0], Jp[1], Jp[2], pushed ? 1.0f : 0.0f);
+    push[n + i] = make_float4(Jp[3], Jp[4], Jp[5], 0.0f);
+    push[2 * (size_t)n + i] = make_float4(Jp[6], Jp[7], Jp[8], 0.0f);
+    if (sampled) flags[0] = 1u;
+    if (pushed) flags[1] = 1u;
+    if (bad) flags[2] = 1u;
+}
+
+void adj_launch_boundary_tape(hipStream_t s, const SceneParams* P, const MapDev* maps, uint32_t n, uint32_t capacity,
+                              const float4* xpre, const float4* posAfter, const float* vb, const float4* xb,
+                              float4* con, float4* push, uint32_t* flags) {
+    if (n)
+        hipLaunchKernelGGL(k_adj_boundary_tape, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, P, maps, n,
+                           capacity, xpre, posAfter, vb, xb, con, push, flags);
 }
 
 }  // namespace vfd

@@ -37,6 +37,8 @@ struct AdjKHost {
     const uint32_t* count;
     const uint32_t* list;
     const float4* x;
+    const float4* con = nullptr;  // boundary contact records of the step (adj_launch_boundary_tape); null: none
+    uint32_t bodies = 0;
     AdjK dev() const;
 };
 void adj_launch_accel_T(hipStream_t s, const AdjKHost& h, const float* kap, const float4* la, float* lk, float4* lx);
@@ -76,7 +78,14 @@ void adj_launch_copy3(hipStream_t s, uint32_t n, const float* a, float* da, cons
                       const float* c, float* dc);
 void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float4* slot);
 void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const float4* slot, float* orig);
-void adj_launch_contact(hipStream_t s, uint32_t n, uint32_t cap, uint32_t bodies, const float* vb, uint32_t* flag);
+void adj_launch_push(hipStream_t s, uint32_t n, const float4* push, float4* lx, float4* lv);
+struct MapDev;
+// boundary.hip: replay of the boundary pass of scene 0 from its starting positions, recording contact samples
+// (con: 5 * bodies * n records) and push-outs (push: 3 * n) with their derivatives; flags[3] (contact, push, out of
+// scope or replay mismatch) are set to 1 where they occur and must be cleared beforehand
+void adj_launch_boundary_tape(hipStream_t s, const SceneParams* P, const MapDev* maps, uint32_t n, uint32_t capacity,
+                              const float4* xpre, const float4* posAfter, const float* vb, const float4* xb,
+                              float4* con, float4* push, uint32_t* flags);
 
 // one-time kernel attributes (dynamic LDS above 64 KB)
 int configure_kernels();

@@ -466,13 +466,17 @@ public:
         DevBuf<float4> x, v0, v1, y, v2;
         DevBuf<float> alpha, D, rho, rhoadv, kvEnd, kpEnd;
         DevBuf<uint32_t> id, count, list;
-        DevBuf<uint32_t> contact;  // [1]: set when a particle touched a boundary during the step
+        // boundary pass: contact samples and push-outs with their derivatives (adj_launch_boundary_tape) and
+        // flags [contact, push, out of scope]
+        DevBuf<float4> con, push;
+        DevBuf<uint32_t> flags;
         std::deque<DevBuf<float>> kv, rv, kp, rp;  // per Jacobi iteration: kappa^k, residual^k
         std::deque<DevBuf<float4>> av, ap;         // per Jacobi iteration: pressure acceleration a^k
         float h = 0, H = 0;
-        uint32_t n = 0, Kd = 0, Kp = 0;
-        bool visc = false;
+        uint32_t n = 0, Kd = 0, Kp = 0, bodies = 0;
+        bool visc = false, boundaryViscosity = false;
     };
+    DevBuf<float4> tapeXpre;  // positions the boundary pass of a recorded step starts from
     // recorded steps: the first tapeCount entries of tapes; the buffers are kept and reused by later recordings
     std::deque<std::unique_ptr<DevTape>> tapes;
     size_t tapeCount = 0;
@@ -1247,6 +1251,7 @@ public:
         VFD_HIP(hipEventRecord(ev[1], stream));
         if (last == VFD_PHASE_SEARCH) return finish_partial();
         Fields F = fields();
+        if (taping && sc.size() == 1) VFD_TRY(tape_copy(tapeXpre, pos.p, sc[0]->count));
         VFD_TRY(timed("boundary", [&] { launch_boundary(L, dMaps.p, pos.p, vel[cur].p, xb.p, vb.p); }));
         // halos are built after the boundary has pushed penetrating particles out (the lists themselves come from the
         // positions before)
@@ -1269,10 +1274,17 @@ public:
             VFD_TRY(tape_copy(tp->id, id[cur].p, n));
             VFD_TRY(tape_copy(tp->count, nbrCount.p, n));
             VFD_TRY(tape_copy(tp->list, nbrList.p, (size_t)cap * kMaxNeighbors));
-            // scope: no boundary contact (every Vb = 0 after the boundary pass); checked by vfd_adjoint
-            VFD_TRY(tp->contact.alloc(1));
-            VFD_HIP(hipMemsetAsync(tp->contact.p, 0, sizeof(uint32_t), stream));
-            adj_launch_contact(stream, (uint32_t)n, cap, (uint32_t)sc[0]->bodies.size(), vb.p, tp->contact.p);
+            // boundary pass replayed with its derivatives; scope checked by vfd_adjoint from the flags
+            tp->bodies = (uint32_t)sc[0]->bodies.size();
+            tp->boundaryViscosity = sc[0]->desc.EnableViscositySolver && sc[0]->desc.BoundaryViscosity != 0.0f;
+            VFD_TRY(tp->flags.alloc(3));
+            VFD_HIP(hipMemsetAsync(tp->flags.p, 0, 3 * sizeof(uint32_t), stream));
+            if (tp->bodies) {
+                VFD_TRY(tp->con.alloc((size_t)5 * tp->bodies * n));
+                VFD_TRY(tp->push.alloc((size_t)3 * n));
+                adj_launch_boundary_tape(stream, dP.p, dMaps.p, (uint32_t)n, cap, tapeXpre.p, pos.p, vb.p, xb.p,
+                                         tp->con.p, tp->push.p, tp->flags.p);
+            }
             if (anyST) tapeError = "surface tension is outside the adjoint's scope";
         }
         VFD_TRY(timed("density_factor", [&] { launch_density_factor(L, K, F); }));
@@ -1463,11 +1475,15 @@ public:
     }
     // reverse of one recorded step: aLx, aLv hold the cotangents of its output (x', v') in its slot order on entry
     // and those of its input (x, v) on return; lmu accumulates d/dmu
-    int adj_step(const DevTape& t, double& lmu) {
+    int adj_step(const DevTape& t, const uint32_t flags[3], double& lmu) {
         const uint32_t n = t.n;
         const SceneParams& pr = sc[0]->prm;
         AdjKHost A{dTab.p, kt.R, kt.invStep, kt.K, kt.L, pr.V, pr.rho0, pr.m, pr.mu, kt.R2, n, cap,
                    t.count.p, t.list.p, t.x.p};
+        if (flags[0]) {  // some particle sampled a body: the sample terms join every operator
+            A.con = t.con.p;
+            A.bodies = t.bodies;
+        }
         const float h = t.h, H = t.H;
         for (auto* q : {&aLa, &aTmp4, &aGs, &aLy, &aLb}) VFD_TRY(q->alloc(n));
         for (auto* q : {&aLk, &aLsrc, &aLs, &aLf, &aLrho, &aLalpha, &aLd, &aLS}) VFD_TRY(q->alloc(n));
@@ -1523,18 +1539,25 @@ public:
         adj_launch_div_T(stream, A, aLd.p, t.v0.p, aLv.p, aLx.p);
         // factor and density
         adj_launch_factor_density(stream, A, t.alpha.p, aLalpha.p, aLS.p, aGs.p, aLrho.p, aLx.p);
+        // boundary push-out (the step's positions and velocities above are those after it)
+        if (flags[1]) adj_launch_push(stream, n, t.push.p, aLx.p, aLv.p);
         VFD_HIP(hipGetLastError());
         return VFD_OK;
     }
     // cotangents of the last recorded step's output -> of the first recorded step's input (original order)
     int adjoint(const float* lxOut, const float* lvOut, float* lxIn, float* lvIn, double* lmu) {
         if (tapeCount == 0) return fail(VFD_ERROR_STATE, "no recorded steps (vfd_tape)");
-        std::vector<uint32_t> contact(tapeCount);
+        std::vector<uint32_t> flags(3 * tapeCount);
         for (size_t k = 0; k < tapeCount; ++k)
-            VFD_HIP(hipMemcpyAsync(&contact[k], tapes[k]->contact.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            VFD_HIP(hipMemcpyAsync(&flags[3 * k], tapes[k]->flags.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   stream));
         VFD_HIP(hipStreamSynchronize(stream));
-        for (uint32_t c : contact)
-            if (c) tapeError = "boundary contact is outside the adjoint's scope";
+        for (size_t k = 0; k < tapeCount; ++k) {
+            if (flags[3 * k + 2])
+                tapeError = "a particle pushed out twice or after a contact sample is outside the adjoint's scope";
+            if (flags[3 * k] && tapes[k]->boundaryViscosity)
+                tapeError = "boundary viscosity at a boundary contact is outside the adjoint's scope";
+        }
         if (!tapeError.empty()) return fail(VFD_ERROR_NOT_SUPPORTED, tapeError);
         const uint32_t n = tapes[tapeCount - 1]->n;
         VFD_TRY(aOrigX.alloc((size_t)3 * n));
@@ -1551,7 +1574,7 @@ public:
             const DevTape& t = *tapes[k];
             adj_launch_to_slots(stream, n, t.id.p, aOrigX.p, aLx.p);
             adj_launch_to_slots(stream, n, t.id.p, aOrigV.p, aLv.p);
-            VFD_TRY(adj_step(t, m));
+            VFD_TRY(adj_step(t, &flags[3 * k], m));
             adj_launch_to_orig(stream, n, t.id.p, aLx.p, aOrigX.p);
             adj_launch_to_orig(stream, n, t.id.p, aLv.p, aOrigV.p);
         }
