@@ -1153,6 +1153,12 @@ public:
         // Push-out of a particle found inside a body: x <- x + delta n, v <- 0; Jpush = dx_after / dx_before
         std::vector<uint8_t> pushed;
         std::vector<M3<T>> Jpush;
+        // surface tension: the Monte-Carlo normals and curvatures of the step (their sample coverage and the
+        // classification are discrete) and the blend pass count
+        bool st = false;
+        unsigned stPasses = 0;
+        std::vector<V3<T>> nMC;
+        std::vector<T> kMC;
     };
     std::vector<StepTape> tapes;
     std::vector<V3<T>> tapePre;  // positions the boundary pass of a taped step started from
@@ -1804,6 +1810,17 @@ public:
         for (auto& q : p) q.a = info.g;  // ClearAccelerationKernel :24-37
         if (last == VFD_PHASE_CLEAR_ACCELERATION) return;
         surface_tension();
+        if (taping && desc.EnableSurfaceTensionSolver) {
+            StepTape& t = tapes.back();
+            t.st = true;
+            t.stPasses = desc.SurfaceTensionSmoothPassCount;
+            t.nMC.resize(n);
+            t.kMC.resize(n);
+            for (long i = 0; i < n; ++i) {
+                t.nMC[i] = p[i].nMC;
+                t.kMC[i] = p[i].kMC;
+            }
+        }
         if (last == VFD_PHASE_SURFACE_TENSION) return;
         viscosity_solve();
         if (taping) {
@@ -1842,9 +1859,12 @@ public:
     // transposed iteration), the viscosity solve by one transposed linear solve. Boundary contact enters through the
     // sample terms of density, factor, divergence and pressure acceleration (vb W(x - xb), vb gradW(x - xb)), whose
     // position derivatives follow the volume map's interpolant to second order (the normal's derivative is
-    // (I - n n^T) Hess(phi) / |grad phi|), and through the push-out's Jacobian. Scope: no surface tension, no
-    // boundary viscosity on a step with contact, no particle pushed out twice or pushed after a contact sample of an
-    // earlier body, symmetric lists (no particle at the 70-neighbour cap); tape_begin reports anything outside it.
+    // (I - n n^T) Hess(phi) / |grad phi|), and through the push-out's Jacobian. Surface tension enters through its
+    // smoothing pass (st_vjp): the Monte-Carlo normals and curvatures are piecewise constant in the positions (they
+    // change only where a sample's coverage or the classification flips) and are held at their recorded values.
+    // Scope: no temporal smoothing of the surface tension (it carries state between steps), no boundary viscosity on
+    // a step with contact, no particle pushed out twice or pushed after a contact sample of an earlier body,
+    // symmetric lists (no particle at the 70-neighbour cap); tape_begin reports anything outside it.
     // ==========================================================================================================
     std::string tape_error;
     void tape_begin() {
@@ -1855,7 +1875,8 @@ public:
         t.x.resize(n);
         for (long i = 0; i < n; ++i) t.x[i] = p[i].x;
         tape_velocity(t.v0);
-        if (desc.EnableSurfaceTensionSolver) tape_error = "surface tension is outside the adjoint's scope";
+        if (desc.EnableSurfaceTensionSolver && info.temporal)
+            tape_error = "temporal smoothing of the surface tension is outside the adjoint's scope";
         if (info.muB != T(0) && desc.EnableViscositySolver)
             for (auto& b : bodies)
                 for (long i = 0; i < n; ++i)
@@ -2113,6 +2134,61 @@ public:
             rho_old = rho_new;
         }
     }
+    // Surface tension of a taped step as a function of the positions, with nMC, kMC and the classification held:
+    // for a particle with nMC_i != 0 (ComputeSurfaceTensionNormalsAndCurvatureKernel :960-1006, blending :1008-1046)
+    //   w_ij = 1 - |x_i - x_j| / R over neighbours with nMC_j != 0,  nc = sum w nMC_j,  cc = sum w kMC_j,  cf = sum w,
+    //   nS = normalize((1 - sf) nMC_i + sf normalize(nc)),  kS = ((1 - sf) kMC_i + sf cc) / (1 - sf + sf cf),
+    //   a_i = -(P sigma / m) kS nS  (P blend passes, each adding the force once).
+    // Accumulates the position cotangent of a against la.
+    void st_vjp(const StepTape& t, const std::vector<V3<T>>& la, std::vector<V3<T>>& lx) const {
+        const long n = (long)t.x.size();
+        const T sf = info.smoothing, R = info.R;
+        const T c = -(T)t.stPasses * info.sigma * info.mInv;
+        std::vector<V3<T>> lnc(n, v3<T>(0, 0, 0));
+        std::vector<T> lcc(n, T(0)), lcf(n, T(0));
+        auto nrm_vjp = [](V3<T> a, V3<T> l) {  // d normalize(a) ^T l (identity where a = 0, as the forward keeps a)
+            const T a2 = length2(a);
+            if (!(a2 > T(0))) return l;
+            const V3<T> u = normalize(a);
+            return (l - dot3(u, l) * u) / std::sqrt(a2);
+        };
+        for (long i = 0; i < n; ++i) {
+            if (is_zero(t.nMC[i])) continue;
+            V3<T> nc = v3<T>(0, 0, 0);
+            T cc = 0, cf = 0;
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                if (is_zero(t.nMC[j])) continue;
+                const T w = T(1) - length(t.x[j] - t.x[i]) / R;
+                nc = nc + t.nMC[j] * w;
+                cc += t.kMC[j] * w;
+                cf += w;
+            }
+            const V3<T> m = length2(nc) > T(0) ? normalize(nc) : nc;
+            const V3<T> u = (T(1) - sf) * t.nMC[i] + sf * m;
+            const V3<T> ns = length2(u) > T(0) ? normalize(u) : u;
+            const T den = T(1) - sf + sf * cf;
+            const T ks = ((T(1) - sf) * t.kMC[i] + sf * cc) / den;
+            const T lks = c * dot3(la[i], ns);
+            lnc[i] = nrm_vjp(nc, sf * nrm_vjp(u, (c * ks) * la[i]));
+            lcc[i] = lks * sf / den;
+            lcf[i] = -lks * ks * sf / den;
+        }
+        for (long i = 0; i < n; ++i) {
+            if (is_zero(t.nMC[i])) continue;
+            for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
+                const unsigned j = t.nb.ids[t.nb.offsets[i] + jj];
+                if (is_zero(t.nMC[j])) continue;
+                const V3<T> d = t.x[i] - t.x[j];
+                const T dl = length(d);
+                if (!(dl > T(0))) continue;
+                const T lw = dot3(lnc[i], t.nMC[j]) + lcc[i] * t.kMC[j] + lcf[i];
+                const V3<T> g = (-lw / (R * dl)) * d;
+                lx[i] = lx[i] + g;
+                lx[j] = lx[j] - g;
+            }
+        }
+    }
     // Reverse of one taped step: (lx, lv) are the cotangents of the step's output (x', v') on entry and of its input
     // (x, v) on return; lmu accumulates d/dmu.
     void reverse_step(const StepTape& t, std::vector<V3<T>>& lx, std::vector<V3<T>>& lv, double& lmu) const {
@@ -2153,6 +2229,12 @@ public:
         op_A(t, ld2, tmpv);  // D^T ld2 = -A(ld2)
         for (long i = 0; i < n; ++i) lv2[i] = lv2[i] - tmpv[i];
         vjp_x_D(t, t.v2, ld2, lx);
+        // v2 includes H a_st(x) (surface tension)
+        if (t.st && t.stPasses) {
+            std::vector<V3<T>> las(n);
+            for (long i = 0; i < n; ++i) las[i] = H * lv2[i];
+            st_vjp(t, las, lx);
+        }
         // v2 = v1 + H g + (H / h)(y - v1)
         std::vector<V3<T>> ly(n), lv1(n);
         for (long i = 0; i < n; ++i) {
