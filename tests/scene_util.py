@@ -65,6 +65,23 @@ def contact_scene(kind="floor"):
     return d, x, v, bodies
 
 
+def st_scene(n_side=(8, 8, 8)):
+    """A free viscous block for the adjoint through surface tension: spaced 2.1 r (no lattice pair at the support
+    radius), surface tension 0.1, two blend passes, 16 Monte-Carlo samples per particle (CSDFix), fixed 4 ms steps,
+    3 + 3 Jacobi iterations. Returns (desc, x (float64), v (float64))."""
+    rng = np.random.default_rng(1)
+    r = 0.01
+    x = scenes.lattice_block((0.3, 0.3, 0.3), n_side, 2.1 * r, 0.01 * r, rng).astype(np.float64)
+    v = 0.3 * np.sin(7 * x[:, [1, 2, 0]]) + 0.1 * rng.normal(size=x.shape)
+    d = DFSPHSimulationDescription(
+        ParticleRadius=r, Viscosity=0.02, BoundaryViscosity=0.0, EnableSurfaceTensionSolver=1, SurfaceTension=0.1,
+        SurfaceTensionSmoothPassCount=2, CSDFix=16, EnableViscositySolver=1, MaxViscositySolverError=1e-5,
+        MaxViscositySolverIterations=200, MinDivergenceSolverIterations=3, MaxDivergenceSolverIterations=3,
+        MinPressureSolverIterations=3, MaxPressureSolverIterations=3, MinTimeStepSize=0.004, MaxTimeStepSize=0.004,
+        TimeStepSize=0.004)
+    return d, x, v
+
+
 def make_gpu(descs, fluids, bodies):
     from vfd_amd import DFSPHBatch, DFSPHSimulation
     if len(descs) == 1:

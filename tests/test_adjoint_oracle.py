@@ -14,7 +14,7 @@ import pytest
 
 from oracle.oracle import OracleSim
 from vfd_amd import scenes
-from scene_util import contact_scene
+from scene_util import contact_scene, st_scene
 from vfd_amd.simulation import DFSPHSimulationDescription
 
 STEPS = 3
@@ -135,21 +135,6 @@ def test_tape_rejects_boundary_viscosity_at_contact():
     o = _run_bodies(d, x, v, bodies, tape=True)
     with pytest.raises(RuntimeError, match="boundary viscosity"):
         o.adjoint(np.zeros((o.n, 3)))
-
-
-def st_scene():
-    """The free block spaced 2.1 r (no lattice pair at the support radius) with surface tension 0.1, two blend passes
-    and 16 Monte-Carlo samples per particle (CSDFix)."""
-    d, _, _ = _setup()
-    rng = np.random.default_rng(1)
-    r = d.ParticleRadius
-    x = scenes.lattice_block((0.3, 0.3, 0.3), (8, 8, 8), 2.1 * r, 0.01 * r, rng).astype(np.float64)
-    v = 0.3 * np.sin(7 * x[:, [1, 2, 0]]) + 0.1 * rng.normal(size=x.shape)
-    d.EnableSurfaceTensionSolver = 1
-    d.SurfaceTension = 0.1
-    d.SurfaceTensionSmoothPassCount = 2
-    d.CSDFix = 16
-    return d, x, v
 
 
 def test_reference_adjoint_with_surface_tension_matches_finite_differences():

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import OracleSim
-from scene_util import contact_scene
+from scene_util import contact_scene, st_scene
 from vfd_amd import DFSPHSimulation, scenes
 from vfd_amd.simulation import DFSPHSimulationDescription
 
@@ -129,6 +129,61 @@ def test_adjoint_through_boundary_contact_matches_oracle(kind, contacts):
     for k in list(dirs) + ["mu"]:
         assert out["fp32"][k] < 1e-3, ("fp32", k, out["fp32"][k])
         assert out["fp64"][k] < max(1e-3, 2 * mirror[k]), ("fp64", k, out["fp64"][k], mirror[k])
+
+
+@pytest.mark.parametrize("n_side", [(8, 8, 8), (16, 8, 16)])
+def test_adjoint_through_surface_tension_matches_oracle(n_side):
+    """Surface tension (two blend passes, 16 Monte-Carlo samples per particle) with its Monte-Carlo normals and
+    curvatures held, against the oracle's FP32 mirror and FP64 twin (bars as for contact)."""
+    d, x, v = st_scene(n_side)
+    x32, v32 = x.astype(np.float32), v.astype(np.float32)
+    n = len(x)
+    rng = np.random.default_rng(11)
+    wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    print("surface tension adjoint", n, "particles", flush=True)
+    gpu = DFSPHSimulation(d)
+    gpu.set_fluid(x32, v32)
+    gpu.tape(True)
+    for _ in range(STEPS):
+        gpu.OnUpdate()
+    gx, gv, gmu = gpu.adjoint(wx, wv)
+    dirs = {"x0": rng.normal(size=(n, 3)), "v0": rng.normal(size=(n, 3))}
+    out, ref = {}, {}
+    for fp64 in (False, True):
+        o = OracleSim(d, fp64=fp64)
+        o.set_fluid(x32, v32)
+        o.tape(True)
+        for _ in range(STEPS):
+            o.step()
+        if not fp64:
+            assert np.array_equal(o.field("position").astype(np.float32), gpu.field("position"))
+            assert (np.abs(o.field("mc_surface_normal")).sum(1) > 0).sum() > n // 5
+        ref[fp64] = o.adjoint(wx, wv)
+        ox, ov, omu = ref[fp64]
+        e = {k: _rel(float(((gx if k[0] == "x" else gv) * dv).sum()), float(((ox if k[0] == "x" else ov) * dv).sum()))
+             for k, dv in dirs.items()}
+        e["mu"] = _rel(gmu, omu)
+        e["x0_inf"] = float(np.abs(gx - ox).max() / np.abs(ox).max())
+        out["fp64" if fp64 else "fp32"] = e
+        print("gpu st adjoint vs oracle", "fp64" if fp64 else "fp32", {k: "%.2e" % v for k, v in e.items()}, flush=True)
+    (mx, mv, mmu), (tx, tv, tmu) = ref[False], ref[True]
+    mirror = {k: _rel(float(((mx if k[0] == "x" else mv) * dv).sum()), float(((tx if k[0] == "x" else tv) * dv).sum()))
+              for k, dv in dirs.items()}
+    mirror["mu"] = _rel(mmu, tmu)
+    for k in list(dirs) + ["mu"]:
+        assert out["fp32"][k] < 1e-3, ("fp32", k, out["fp32"][k])
+        assert out["fp64"][k] < max(1e-3, 2 * mirror[k]), ("fp64", k, out["fp64"][k], mirror[k])
+
+
+def test_adjoint_rejects_temporal_surface_tension():
+    d, x, v = st_scene()
+    d.TemporalSmoothing = 1
+    gpu = DFSPHSimulation(d)
+    gpu.set_fluid(x.astype(np.float32), v.astype(np.float32))
+    gpu.tape(True)
+    gpu.OnUpdate()
+    with pytest.raises(Exception, match="temporal smoothing"):
+        gpu.adjoint(np.zeros((gpu.GetParticleCount(), 3), np.float32))
 
 
 def test_adjoint_rejects_boundary_viscosity_at_contact():

@@ -376,6 +376,77 @@ __global__ void __launch_bounds__(kTile) k_adj_factor_density(AdjK a, const floa
     lx[i] = st4(ld3(lx[i]) + gx);
 }
 
+// Surface tension of the step with the Monte-Carlo normals nMC and curvatures kMC held (piecewise constant in the
+// positions): for nMC_i != 0, w_ij = 1 - |x_i - x_j| / R over neighbours with nMC_j != 0,
+//   nc = sum w nMC_j, cc = sum w kMC_j, cf = sum w, nS = normalize((1 - sf) nMC_i + sf normalize(nc)),
+//   kS = ((1 - sf) kMC_i + sf cc) / (1 - sf + sf cf), a_i = c kS nS with c = -(passes sigma / m).
+// Pointwise part: from la = s lv, the cotangents of nc, cc and cf (zero where nMC_i = 0).
+__device__ __forceinline__ f3 adj_normalize_T(f3 a, f3 l) {  // d normalize(a)^T l; identity where a = 0
+    const float a2 = dot3(a, a);
+    if (!(a2 > 0.0f)) return l;
+    const f3 u = normalize(a);
+    return (l - dot3(u, l) * u) / sqrtf(a2);
+}
+__global__ void __launch_bounds__(kTile) k_adj_st_point(AdjK a, float s, const float4* __restrict__ lv,
+                                                        const float4* __restrict__ nMC, const float* __restrict__ kMC,
+                                                        float sf, float c, float4* lnc, float2* lcw) {
+    const uint32_t i = pid();
+    if (i >= a.n) return;
+    const f3 ni = ld3(nMC[i]);
+    if (is_zero(ni)) {
+        lnc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        lcw[i] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    const f3 xi = ld3(a.x[i]);
+    f3 nc = mk3(0.0f, 0.0f, 0.0f);
+    float cc = 0.0f, cf = 0.0f;
+    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
+        const uint32_t j = adj_nbr(a, i, jj);
+        const f3 nj = ld3(nMC[j]);
+        if (is_zero(nj)) continue;
+        const float w = 1.0f - length(ld3(a.x[j]) - xi) / a.R;
+        nc = nc + nj * w;
+        cc += kMC[j] * w;
+        cf += w;
+    }
+    const f3 m = dot3(nc, nc) > 0.0f ? normalize(nc) : nc;
+    const f3 u = (1.0f - sf) * ni + sf * m;
+    const f3 ns = dot3(u, u) > 0.0f ? normalize(u) : u;
+    const float den = 1.0f - sf + sf * cf;
+    const float ks = ((1.0f - sf) * kMC[i] + sf * cc) / den;
+    const f3 la = s * ld3(lv[i]);
+    const float lks = c * dot3(la, ns);
+    lnc[i] = st4(adj_normalize_T(nc, sf * adj_normalize_T(u, (c * ks) * la)));
+    lcw[i] = make_float2(lks * sf / den, -lks * ks * sf / den);  // cotangents of cc and cf
+}
+// Pair part: lx_i -= sum_j (lw_ij + lw_ji) (x_i - x_j) / (R |x_i - x_j|) over pairs with both nMC != 0,
+// lw_ij = lnc_i . nMC_j + lcc_i kMC_j + lcf_i
+__global__ void __launch_bounds__(kTile) k_adj_st_pairs(AdjK a, const float4* __restrict__ nMC,
+                                                        const float* __restrict__ kMC, const float4* __restrict__ lnc,
+                                                        const float2* __restrict__ lcw, float4* lx) {
+    const uint32_t i = pid();
+    if (i >= a.n) return;
+    const f3 ni = ld3(nMC[i]);
+    if (is_zero(ni)) return;
+    const f3 xi = ld3(a.x[i]), li = ld3(lnc[i]);
+    const float ki = kMC[i];
+    const float2 ci = lcw[i];
+    f3 gx = mk3(0.0f, 0.0f, 0.0f);
+    for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
+        const uint32_t j = adj_nbr(a, i, jj);
+        const f3 nj = ld3(nMC[j]);
+        if (is_zero(nj)) continue;
+        const f3 d = xi - ld3(a.x[j]);
+        const float dl = length(d);
+        if (!(dl > 0.0f)) continue;
+        const float2 cj = lcw[j];
+        const float lw = (dot3(li, nj) + ci.x * kMC[j] + ci.y) + (dot3(ld3(lnc[j]), ni) + cj.x * ki + cj.y);
+        gx = gx - (lw / (a.R * dl)) * d;
+    }
+    lx[i] = st4(ld3(lx[i]) + gx);
+}
+
 // push-out reversed: x = x_before + delta n(x_before), v = 0  ->  lx = J^T lx, lv = 0 (rows of J in push[q n + i])
 __global__ void __launch_bounds__(kTile) k_adj_push(uint32_t n, const float4* __restrict__ push, float4* lx,
                                                     float4* lv) {
@@ -583,6 +654,11 @@ AdjK AdjKHost::dev() const {
     return AdjK{tab, R, invStep, K, L, V, rho0, m, mu, R2, n, cap, count, list, x, con, bodies};
 }
 
+void adj_launch_surface_tension(hipStream_t s, const AdjKHost& h, float scale, const float4* lv, const float4* nMC,
+                                const float* kMC, float sf, float c, float4* lnc, float2* lcw, float4* lx) {
+    hipLaunchKernelGGL(k_adj_st_point, grid_of(h.n), dim3(kTile), 0, s, h.dev(), scale, lv, nMC, kMC, sf, c, lnc, lcw);
+    hipLaunchKernelGGL(k_adj_st_pairs, grid_of(h.n), dim3(kTile), 0, s, h.dev(), nMC, kMC, lnc, lcw, lx);
+}
 void adj_launch_push(hipStream_t s, uint32_t n, const float4* push, float4* lx, float4* lv) {
     if (n) hipLaunchKernelGGL(k_adj_push, grid_of(n), dim3(kTile), 0, s, n, push, lx, lv);
 }

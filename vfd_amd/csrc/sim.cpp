@@ -470,6 +470,11 @@ public:
         // flags [contact, push, out of scope]
         DevBuf<float4> con, push;
         DevBuf<uint32_t> flags;
+        // surface tension: Monte-Carlo normals and curvatures of the step, blend passes, temporal smoothing
+        DevBuf<float4> nMC;
+        DevBuf<float> kMC;
+        uint32_t stPasses = 0;
+        bool st = false, stTemporal = false;
         std::deque<DevBuf<float>> kv, rv, kp, rp;  // per Jacobi iteration: kappa^k, residual^k
         std::deque<DevBuf<float4>> av, ap;         // per Jacobi iteration: pressure acceleration a^k
         float h = 0, H = 0;
@@ -1285,7 +1290,7 @@ public:
                 adj_launch_boundary_tape(stream, dP.p, dMaps.p, (uint32_t)n, cap, tapeXpre.p, pos.p, vb.p, xb.p,
                                          tp->con.p, tp->push.p, tp->flags.p);
             }
-            if (anyST) tapeError = "surface tension is outside the adjoint's scope";
+
         }
         VFD_TRY(timed("density_factor", [&] { launch_density_factor(L, K, F); }));
         if (tp) {
@@ -1315,6 +1320,15 @@ public:
         if (anyST) {
             VFD_TRY(upload_mc_tables(F));
             VFD_TRY(timed("surface_tension", [&] { launch_surface_tension(L, F); }));
+        }
+        if (tp) {
+            tp->st = sc[0]->desc.EnableSurfaceTensionSolver != 0;
+            tp->stPasses = sc[0]->desc.SurfaceTensionSmoothPassCount;
+            tp->stTemporal = sc[0]->desc.TemporalSmoothing != 0;
+            if (tp->st) {
+                VFD_TRY(tape_copy(tp->nMC, nMC.p, tp->n));
+                VFD_TRY(tape_copy(tp->kMC, kMC.p, tp->n));
+            }
         }
         VFD_HIP(hipEventRecord(ev[4], stream));
         if (last == VFD_PHASE_SURFACE_TENSION) return finish_partial();
@@ -1423,7 +1437,8 @@ public:
     }
 
     // ---- adjoint: reverse of the recorded steps (kernels in adjoint.hip) ----------------------------------------
-    DevBuf<float4> aLx, aLv, aLa, aTmp4, aGs, aLy, aLb, bR, bRh, bP, bV, bS, bT;
+    DevBuf<float4> aLx, aLv, aLa, aTmp4, aGs, aLy, aLb, bR, bRh, bP, bV, bS, bT, aLnc;
+    DevBuf<float2> aLcw;
     DevBuf<float> aLk, aLsrc, aLs, aLf, aLrho, aLalpha, aLd, aLS, aOrigX, aOrigV;
     DevBuf<double> aPart;
     int adj_dot(const float4* p, const float4* q, uint32_t n, double& out) {
@@ -1507,6 +1522,13 @@ public:
         adj_launch_pres_init(stream, n, H, pr.rho0, t.alpha.p, t.rhoadv.p, aLk.p, aLsrc.p, aLs.p, aLalpha.p, aLrho.p,
                              aLd.p);
         adj_launch_div_T(stream, A, aLd.p, t.v2.p, aLv.p, aLx.p);  // rhoadv's H D(v2)
+        // v2 includes H a_st(x): surface tension with its Monte-Carlo normals and curvatures held
+        if (t.st && t.stPasses) {
+            VFD_TRY(aLnc.alloc(n));
+            VFD_TRY(aLcw.alloc(n));
+            adj_launch_surface_tension(stream, A, H, aLv.p, t.nMC.p, t.kMC.p, pr.smoothing,
+                                       -(float)t.stPasses * pr.sigma * pr.mInv, aLnc.p, aLcw.p, aLx.p);
+        }
         // v2 = v1 + H g + (H / h)(y - v1)
         if (t.visc) {
             adj_launch_scale4(stream, n, aLv.p, H / h, aLy.p);
@@ -1553,6 +1575,8 @@ public:
                                    stream));
         VFD_HIP(hipStreamSynchronize(stream));
         for (size_t k = 0; k < tapeCount; ++k) {
+            if (tapes[k]->st && tapes[k]->stTemporal)
+                tapeError = "temporal smoothing of the surface tension is outside the adjoint's scope";
             if (flags[3 * k + 2])
                 tapeError = "a particle pushed out twice or after a contact sample is outside the adjoint's scope";
             if (flags[3 * k] && tapes[k]->boundaryViscosity)
