@@ -9,9 +9,9 @@ Compile-time shapes (each candidate is a separate build of the library, vfd_amd.
   VFD_GROUP_TILES    tiles per neighbour group (4: 1024-thread blocks, one per CU; 2: 512-thread blocks, two per
                      CU, halos of at most 2048 particles)
   VFD_HALO_MAX       largest halo a 4-tile group stages in LDS
-  VFD_SEARCH_CELLS   neighbour lists built one grid cell per wave (1) or one particle per thread (0)
-  VFD_SEARCH_UNROLL  candidate positions loaded per batch in the neighbour search
-  VFD_SEARCH_PREFETCH  the ranges of all 27 cells are loaded before the traversal (1) or cell by cell (0)
+  VFD_SEARCH_CELLS   neighbour lists built per 64-slot window with a lane per candidate of a cell (1) or one
+                     particle per thread (0)
+  VFD_SEARCH_UNROLL  candidate positions loaded per batch in the per-particle neighbour search
   VFD_STAGE_LISTS    neighbour lists staged in LDS and written row by row (1) or stored as found (0)
 Run-time knobs (vfd_set_tuning; their defaults are compiled in as VFD_DEFAULT_*):
   SolverCheckInterval  Jacobi / CG iterations enqueued between two convergence checks
@@ -44,7 +44,6 @@ AXES = {
     "VFD_LIST_CHUNK2": [2, 4, 8],
     "VFD_LIST_PREFETCH2": [1, 2],
     "VFD_SEARCH_UNROLL": [4, 8],
-    "VFD_SEARCH_PREFETCH": [0, 1],
     "VFD_STAGE_LISTS": [0, 1],
     "VFD_PACK_RECORDS": [0, 1, 2],
     "VFD_GROUP_TILES": [2, 4],
@@ -52,7 +51,7 @@ AXES = {
     "VFD_SEARCH_CELLS": [0, 1],
 }
 DEFAULTS = {"VFD_LIST_CHUNK": 4, "VFD_LIST_PREFETCH": 1, "VFD_LIST_CHUNK2": 2, "VFD_LIST_PREFETCH2": 2,
-            "VFD_SEARCH_UNROLL": 8, "VFD_SEARCH_PREFETCH": 1, "VFD_STAGE_LISTS": 0, "VFD_PACK_RECORDS": 2,
+            "VFD_SEARCH_UNROLL": 8, "VFD_STAGE_LISTS": 0, "VFD_PACK_RECORDS": 2,
             "VFD_GROUP_TILES": 4, "VFD_HALO_MAX": 3808, "VFD_SEARCH_CELLS": 0}
 # shapes that only make sense together: 2-tile groups take at most 2048-particle halos, two blocks per CU
 LINKED = {("VFD_GROUP_TILES", 2): {"VFD_HALO_MAX": 2048, "VFD_GROUPS_PER_CU": 2}}

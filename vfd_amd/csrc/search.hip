@@ -162,14 +162,17 @@ __global__ void __launch_bounds__(kTile) k_reorder(const SceneParams* __restrict
 #ifndef VFD_STAGE_LISTS
 #define VFD_STAGE_LISTS 0
 #endif
-#ifndef VFD_SEARCH_PREFETCH
-#define VFD_SEARCH_PREFETCH 1
-#endif
 constexpr uint32_t kSearchUnroll = VFD_SEARCH_UNROLL;
 
-// FindNeighbors (ParticleSearch.cu:95-195; kernels ParticleSearchKernels.cu:93-193) in one traversal: the 27 cells
-// in z, y, x order, each cell's particles in slot order (= ascending original id), strict 0 < d^2 < R^2, at most
-// kMaxNeighbors. out(cnt, t) receives the cnt-th neighbour.
+// FindNeighbors (ParticleSearch.cu:95-195; kernels ParticleSearchKernels.cu:93-193) in one traversal per particle:
+// the 27 cells in z, y, x order, each cell's particles in slot order (= ascending original id), strict
+// 0 < d^2 < R^2, at most kMaxNeighbors; out(cnt, t) receives the cnt-th neighbour. The ranges of all 27 cells are
+// requested before the first candidate is loaded. The three cells of a row (x - 1, x, x + 1 at fixed y, z) form one
+// stream of candidates, loaded kSearchUnroll at a time from clamped positions (every load stays inside a range) with
+// no branch around the loads; a candidate past the row's end or past the cap is simply not accepted, so the first
+// kMaxNeighbors in the reference's order are kept. Cells whose box lies farther than the support radius from the
+// particle (by a 0.1% margin: no candidate there can pass d^2 < R^2) are skipped: about a quarter of the 27 on
+// average, corner and edge cells, without changing the list.
 template <class Out>
 __device__ __forceinline__ uint32_t neighbor_list(const TileCtx& c, const SceneParams* __restrict__ P,
                                                   const SceneGrid* __restrict__ G, const float4* __restrict__ pos,
@@ -178,64 +181,51 @@ __device__ __forceinline__ uint32_t neighbor_list(const TileCtx& c, const SceneP
     const SceneGrid& g = G[c.s];
     const float R2 = P[c.s].R * P[c.s].R;
     const float4 xi = pos[c.i];
-    const int gx = (int)floorf((xi.x - g.gridMin[0]) * g.gridDelta[0]);
-    const int gy = (int)floorf((xi.y - g.gridMin[1]) * g.gridDelta[1]);
-    const int gz = (int)floorf((xi.z - g.gridMin[2]) * g.gridDelta[2]);
-    uint32_t cnt = 0;
-#if VFD_SEARCH_PREFETCH
-    // the ranges of all 27 cells are requested before the first candidate is loaded (one latency, not 27)
+    const float ux = (xi.x - g.gridMin[0]) * g.gridDelta[0], uy = (xi.y - g.gridMin[1]) * g.gridDelta[1];
+    const float uz = (xi.z - g.gridMin[2]) * g.gridDelta[2];
+    const int gx = (int)floorf(ux), gy = (int)floorf(uy), gz = (int)floorf(uz);
+    // squared distances (world units) from the particle to the slabs of the cells below / above along each axis
+    const float csx = 1.0f / g.gridDelta[0], csy = 1.0f / g.gridDelta[1], csz = 1.0f / g.gridDelta[2];
+    const float fx = ux - (float)gx, fy = uy - (float)gy, fz = uz - (float)gz;
+    const float bx[3] = {fx * csx * fx * csx, 0.0f, (1.0f - fx) * csx * (1.0f - fx) * csx};
+    const float by[3] = {fy * csy * fy * csy, 0.0f, (1.0f - fy) * csy * (1.0f - fy) * csy};
+    const float bz[3] = {fz * csz * fz * csz, 0.0f, (1.0f - fz) * csz * (1.0f - fz) * csz};
+    const float lim = R2 * 1.002f;
     uint32_t cs[27], cn[27];
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
-        const int cx = gx + k % 3 - 1, cy = gy + (k / 3) % 3 - 1, cz = gz + k / 9 - 1;
+        const int ox = k % 3, oy = (k / 3) % 3, oz = k / 9;
+        const int cx = gx + ox - 1, cy = gy + oy - 1, cz = gz + oz - 1;
         const bool in = cx >= 0 && cx < (int)g.dims[0] && cy >= 0 && cy < (int)g.dims[1] && cz >= 0 &&
-                        cz < (int)g.dims[2];
+                        cz < (int)g.dims[2] && bx[ox] + by[oy] + bz[oz] <= lim;
         const uint32_t key = in ? cell_key(g, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz) : 0u;
         cn[k] = in ? cellCount[key] : 0u;
         cs[k] = in ? (uint32_t)((int32_t)cellStart[key] + g.startCorr) : 0u;
     }
+    uint32_t cnt = 0;
 #pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        {
-            {
-                const uint32_t start = cs[k];
-                const uint32_t end = start + cn[k];
-#else
-    for (int dz = -1; dz < 2; ++dz) {
-        const int cz = gz + dz;
-        if (cz < 0 || cz >= (int)g.dims[2]) continue;
-        for (int dy = -1; dy < 2; ++dy) {
-            const int cy = gy + dy;
-            if (cy < 0 || cy >= (int)g.dims[1]) continue;
-            for (int dx = -1; dx < 2; ++dx) {
-                const int cx = gx + dx;
-                if (cx < 0 || cx >= (int)g.dims[0]) continue;
-                const uint32_t key = cell_key(g, (uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
-                const uint32_t n = cellCount[key];
-                const uint32_t start = (uint32_t)((int32_t)cellStart[key] + g.startCorr);
-                const uint32_t end = start + n;
-#endif
-                // kSearchUnroll candidate positions are loaded before any is tested (one latency per batch)
-                for (uint32_t t0 = start; t0 < end; t0 += kSearchUnroll) {
-                    float4 xj[kSearchUnroll];
+    for (int row = 0; row < 9; ++row) {
+        const uint32_t sa = cs[3 * row], sb = cs[3 * row + 1], sc = cs[3 * row + 2];
+        const uint32_t a = cn[3 * row], ab = a + cn[3 * row + 1], L = ab + cn[3 * row + 2];
+        for (uint32_t q0 = 0; q0 < L; q0 += kSearchUnroll) {
+            uint32_t t[kSearchUnroll];
+            float4 xj[kSearchUnroll];
 #pragma unroll
-                    for (uint32_t u = 0; u < kSearchUnroll; ++u)
-                        if (t0 + u < end) xj[u] = pos[t0 + u];
+            for (uint32_t u = 0; u < kSearchUnroll; ++u) {
+                const uint32_t q = min(q0 + u, L - 1u);
+                t[u] = q < a ? sa + q : (q < ab ? sb + (q - a) : sc + (q - ab));
+                xj[u] = pos[t[u]];
+            }
 #pragma unroll
-                    for (uint32_t u = 0; u < kSearchUnroll; ++u) {
-                        if (t0 + u >= end) break;
-                        const float ddx = xj[u].x - xi.x, ddy = xj[u].y - xi.y, ddz = xj[u].z - xi.z;
-                        const float d2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-                        if (d2 < R2 && d2 > 0.0f) {
-                            out(cnt, t0 + u);
-                            if (++cnt == kMaxNeighbors) goto done;
-                        }
-                    }
-                }
+            for (uint32_t u = 0; u < kSearchUnroll; ++u) {
+                const float ddx = xj[u].x - xi.x, ddy = xj[u].y - xi.y, ddz = xj[u].z - xi.z;
+                const float d2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+                const bool acc = q0 + u < L && d2 < R2 && d2 > 0.0f && cnt < kMaxNeighbors;
+                if (acc) out(cnt, t[u]);
+                cnt += acc ? 1u : 0u;
             }
         }
     }
-done:
     return cnt;
 }
 
@@ -285,24 +275,10 @@ __global__ void __launch_bounds__(kTile) k_build_lists(const float4* __restrict_
     }
 }
 
-// ---- cell-parallel list build --------------------------------------------------------------------------------------
-// The same lists as k_build_lists, built one grid cell at a time by one wave: every particle of a cell has the same 27
-// candidate ranges, so the wave loads the candidates once, 64 per chunk with a lane per candidate (consecutive slots
-// of a range are consecutive lanes), and tests them against each particle of the cell in turn; a ballot gives every
-// accepted candidate its rank in the particle's list (candidates are visited in list order: ranges in z, y, x order,
-// slots ascending), so the first kMaxNeighbors are kept exactly as the per-particle traversal keeps them. A cell's
-// lists are staged in LDS ([row][particle], kCellBatch particles at a time) and written row by row: each store
-// instruction writes 64-byte runs of four rows. A wave handles the cells whose first slot lies in its 64-slot window
-// (a cell may extend past the window); blocks take tiles in XCD-aware order, so one XCD writes one contiguous stretch
-// of every list row.
+// ---- cell-parallel list build (VFD_SEARCH_CELLS) ---------------------------------------------------------------------
 #ifndef VFD_SEARCH_CELLS
-#define VFD_SEARCH_CELLS 1
+#define VFD_SEARCH_CELLS 0
 #endif
-#ifndef VFD_CELL_BATCH
-#define VFD_CELL_BATCH 16
-#endif
-constexpr uint32_t kCellBatch = VFD_CELL_BATCH;  // particles of a cell tested together (a divisor of 64)
-static_assert(kWave % kCellBatch == 0, "the row writes cover 64 / kCellBatch rows per store");
 constexpr uint32_t kCellWaves = kTile / kWave;
 
 __device__ __forceinline__ uint32_t slot_cell(const SceneGrid& g, const float4& x) {  // k_insert's key
@@ -312,17 +288,39 @@ __device__ __forceinline__ uint32_t slot_cell(const SceneGrid& g, const float4& 
     return cell_key(g, clampu((int)fx, g.dims[0]), clampu((int)fy, g.dims[1]), clampu((int)fz, g.dims[2]));
 }
 
+// Every wave owns the 64 slots of its window and builds exactly their lists. The window's particles fall into runs of
+// one grid cell (segments: a cell may begin before the window or end after it; only the window's own particles are
+// handled). A segment's particles share the 27 candidate ranges, so the wave loads the candidates once, 64 per chunk
+// with a lane per candidate (consecutive slots of a range are consecutive lanes: few, coalesced loads instead of one
+// load per particle and candidate), and tests them against each of the segment's particles in turn; a ballot gives
+// each accepted candidate its rank in that particle's list (candidates are visited in list order: ranges in z, y, x
+// order, slots ascending), so the first kMaxNeighbors are kept exactly as the per-particle traversal keeps them.
+// Entries are staged in LDS as 16-bit {range, offset} codes in [row][lane] (rows padded so that the ranks of one
+// particle fall into different banks), decoded with the segment's range starts and written at the end as whole
+// 256-byte rows of 64 consecutive slots. A segment that does not fit the codes (a range of 2048 or more particles,
+// or a window of more than kWinSegs cells) stores its entries directly instead.
+constexpr uint32_t kWinPitch = kWave + 2;  // u16 entries per staged row
+constexpr uint32_t kWinSegs = 16;          // segments per window with staged entries
+constexpr uint32_t kWinDirect = 0xFFu;     // segment index of a lane whose entries went straight to HBM
+// lane q's value (q wave-uniform), read into a scalar register
+__device__ __forceinline__ float rdlane(float v, uint32_t q) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)q));
+}
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t q) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q);
+}
 __global__ void __launch_bounds__(kTile) k_build_lists_cells(const float4* __restrict__ pos,
-                                                             const SceneParams* __restrict__ P,
-                                                             const SceneGrid* __restrict__ G,
-                                                             const uint32_t* __restrict__ tileScene, uint32_t tiles,
-                                                             uint32_t per, const uint32_t* __restrict__ cellStart,
-                                                             const uint32_t* __restrict__ cellCount,
-                                                             uint32_t* nbrCount, uint32_t* nbrList, uint32_t capacity,
-                                                             unsigned long long* nbStats) {
-    __shared__ uint32_t stage[kCellWaves][kMaxNeighbors][kCellBatch];
-    __shared__ uint32_t pre[kCellWaves][32];  // exclusive prefix of the 27 range lengths
-    __shared__ uint32_t cst[kCellWaves][32];  // first slot of each range
+                                                           const SceneParams* __restrict__ P,
+                                                           const SceneGrid* __restrict__ G,
+                                                           const uint32_t* __restrict__ tileScene, uint32_t tiles,
+                                                           uint32_t per, const uint32_t* __restrict__ cellStart,
+                                                           const uint32_t* __restrict__ cellCount, uint32_t* nbrCount,
+                                                           uint32_t* nbrList, uint32_t capacity,
+                                                           unsigned long long* nbStats) {
+    __shared__ uint16_t stage[kCellWaves][kMaxNeighbors * kWinPitch];
+    __shared__ uint32_t tabStart[kCellWaves][kWinSegs][28];  // range starts of the staged segments
+    __shared__ uint32_t pre[kCellWaves][32];  // exclusive prefix of the 27 range lengths of the current segment
+    __shared__ uint32_t cst[kCellWaves][32];  // first slot of each range of the current segment
     const uint32_t tile = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
     if (tile >= tiles) return;  // block-uniform
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x & (kWave - 1);
@@ -330,25 +328,28 @@ __global__ void __launch_bounds__(kTile) k_build_lists_cells(const float4* __res
     const SceneParams& p = P[s];
     const SceneGrid& g = G[s];
     const uint32_t end = p.base + p.count;
-    const uint32_t i = tile * kTile + threadIdx.x;
+    const uint32_t win0 = tile * kTile + w * kWave;  // first slot of the window
+    const uint32_t i = win0 + lane;
     const bool valid = i < end;
-    const uint32_t key = valid ? slot_cell(g, pos[i]) : 0u;
-    uint32_t prevKey = __shfl_up(key, 1, kWave);
-    if (lane == 0) prevKey = valid && i > p.base ? slot_cell(g, pos[i - 1]) : 0xFFFFFFFFu;
-    unsigned long long leaders = __ballot(valid && (i == p.base || key != prevKey));
+    const float4 x = valid ? pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t key = valid ? slot_cell(g, x) : 0xFFFFFFFFu;
+    const uint32_t prevKey = __shfl_up(key, 1, kWave);
+    unsigned long long segs = __ballot(valid && (lane == 0 || key != prevKey));  // first lane of each segment
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // the valid lanes are the first nvalid
     const float R2 = p.R * p.R;
-    unsigned long long tot = 0;
-    uint32_t mx = 0;
-    while (leaders) {
-        const uint32_t b = (uint32_t)__ffsll(leaders) - 1u;
-        leaders &= leaders - 1ull;
-        const uint32_t first = tile * kTile + w * kWave + b;
-        const float4 xl = pos[first];
-        // the cell of the key (for particles inside the grid, the cell neighbor_list takes with floorf; a particle
-        // with a non-finite position sits in the clamped cell of k_insert and finds no neighbour)
-        const int gx = (int)clampu((int)((xl.x - g.gridMin[0]) * g.gridDelta[0]), g.dims[0]);
-        const int gy = (int)clampu((int)((xl.y - g.gridMin[1]) * g.gridDelta[1]), g.dims[1]);
-        const int gz = (int)clampu((int)((xl.z - g.gridMin[2]) * g.gridDelta[2]), g.dims[2]);
+    uint16_t* st = stage[w];
+    uint32_t cnt = 0;            // accepted neighbours of this lane's particle (only the first kMaxNeighbors are kept)
+    uint32_t mySeg = kWinDirect;  // this lane's staged segment
+    for (uint32_t seg = 0; segs; ++seg) {
+        const uint32_t b = (uint32_t)__ffsll(segs) - 1u;
+        segs &= segs - 1ull;
+        const uint32_t e = segs ? (uint32_t)__ffsll(segs) - 1u : nvalid;  // segment lanes [b, e)
+        const float xlx = rdlane(x.x, b), xly = rdlane(x.y, b), xlz = rdlane(x.z, b);
+        // the cell of the key (k_insert's clamped cell; for a particle inside the grid the cell the per-particle
+        // traversal takes with floorf)
+        const int gx = (int)clampu((int)((xlx - g.gridMin[0]) * g.gridDelta[0]), g.dims[0]);
+        const int gy = (int)clampu((int)((xly - g.gridMin[1]) * g.gridDelta[1]), g.dims[1]);
+        const int gz = (int)clampu((int)((xlz - g.gridMin[2]) * g.gridDelta[2]), g.dims[2]);
         uint32_t cn = 0, cs = 0;
         if (lane < 27) {
             const int cx = gx + (int)(lane % 3) - 1, cy = gy + (int)((lane / 3) % 3) - 1, cz = gz + (int)(lane / 9) - 1;
@@ -358,98 +359,94 @@ __global__ void __launch_bounds__(kTile) k_build_lists_cells(const float4* __res
                 cs = (uint32_t)((int32_t)cellStart[k] + g.startCorr);
             }
         }
+        const bool staged = seg < kWinSegs && __ballot(cn >= 2048u) == 0ull;  // wave-uniform
         uint32_t incl = cn;
 #pragma unroll
         for (int o = 1; o < 32; o <<= 1) {
             const uint32_t t = __shfl_up(incl, o, kWave);
             if ((int)lane >= o) incl += t;
         }
-        const uint32_t T = __shfl(incl, 26, kWave);  // candidates in the 27 ranges
-        const uint32_t m = __shfl(cn, 13, kWave);    // particles of this cell (the centre range)
+        const uint32_t T = rdlane(incl, 26);  // candidates in the 27 ranges
         if (lane < 27) {
             pre[w][lane] = incl - cn;
             cst[w][lane] = cs;
+            if (staged) tabStart[w][seg][lane] = cs;
+        }
+        if (lane >= b && lane < e) mySeg = staged ? seg : kWinDirect;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // candidate t of the segment's stream: its range (the last one whose prefix is <= t) and offset in it
+        auto locate = [&](uint32_t t, uint32_t& off) {
+            uint32_t r = 0;
+#pragma unroll
+            for (uint32_t step = 16; step > 0; step >>= 1)
+                if (r + step < 27 && pre[w][r + step] <= t) r += step;
+            off = t - pre[w][r];
+            return r;
+        };
+        uint32_t offN = 0, rN = lane < T ? locate(lane, offN) : 0u;
+        uint32_t slotN = lane < T ? cst[w][rN] + offN : 0u;
+        float4 xjN = lane < T ? pos[slotN] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
+            const uint32_t slot = slotN, code = (rN << 11) | offN;
+            const float4 xj = xjN;
+            const bool has = c0 + lane < T;
+            if (c0 + kWave < T) {  // the next chunk's candidates are requested before this chunk is tested
+                const uint32_t tn = c0 + kWave + lane;
+                rN = tn < T ? locate(tn, offN) : 0u;
+                slotN = tn < T ? cst[w][rN] + offN : 0u;
+                xjN = tn < T ? pos[slotN] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            for (uint32_t q = b; q < e; ++q) {  // wave-uniform: the segment's particles in slot order
+                const float xqx = rdlane(x.x, q), xqy = rdlane(x.y, q), xqz = rdlane(x.z, q);
+                const float ddx = xj.x - xqx, ddy = xj.y - xqy, ddz = xj.z - xqz;
+                const float d2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
+                const bool acc = has && d2 < R2 && d2 > 0.0f;
+                const unsigned long long bal = __ballot(acc);
+                const uint32_t cq = rdlane(cnt, q);
+                if (acc) {
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    const uint32_t rank = cq + below;  // accepted lanes before this one
+                    if (rank < kMaxNeighbors) {
+                        if (staged)
+                            st[rank * kWinPitch + q] = (uint16_t)code;
+                        else
+                            nbrList[(size_t)rank * capacity + win0 + q] = slot;
+                    }
+                }
+                if (lane == q) cnt += (uint32_t)__popcll(bal);
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (uint32_t p0 = 0; p0 < m; p0 += kCellBatch) {
-            const uint32_t mb = min(kCellBatch, m - p0);
-            float xix[kCellBatch], xiy[kCellBatch], xiz[kCellBatch];  // wave-uniform
-            uint32_t cnt[kCellBatch];
+    }
+    // the window's rows, whole: row r holds entry r of each of the 64 slots
+    const uint32_t n = valid ? min(cnt, kMaxNeighbors) : 0u;
+    uint32_t rows = n;
 #pragma unroll
-            for (uint32_t k = 0; k < kCellBatch; ++k) {
-                const float4 x = k < mb ? pos[first + p0 + k] : xl;
-                xix[k] = x.x;
-                xiy[k] = x.y;
-                xiz[k] = x.z;
-                cnt[k] = 0;
-            }
-            for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
-                const uint32_t t = c0 + lane;
-                const bool has = t < T;
-                uint32_t slot = 0;
-                float4 xj = xl;
-                if (has) {  // the range holding candidate t: the last range whose prefix is <= t
-                    uint32_t r = 0;
-#pragma unroll
-                    for (uint32_t step = 16; step > 0; step >>= 1)
-                        if (r + step < 27 && pre[w][r + step] <= t) r += step;
-                    slot = cst[w][r] + (t - pre[w][r]);
-                    xj = pos[slot];
-                }
-#pragma unroll
-                for (uint32_t k = 0; k < kCellBatch; ++k) {
-                    if (k < mb) {  // wave-uniform
-                        const float ddx = xj.x - xix[k], ddy = xj.y - xiy[k], ddz = xj.z - xiz[k];
-                        const float d2 = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx));
-                        const bool acc = has && d2 < R2 && d2 > 0.0f;
-                        const unsigned long long bal = __ballot(acc);
-                        if (acc) {
-                            const uint32_t rank = cnt[k] + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-                            if (rank < kMaxNeighbors) stage[w][rank][k] = slot;
-                        }
-                        cnt[k] += (uint32_t)__popcll(bal);
-                    }
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // this lane's particle (lane % kCellBatch) and its kept count
-            const uint32_t kk = lane % kCellBatch;
-            uint32_t mine = 0, rows = 0;
-#pragma unroll
-            for (uint32_t k = 0; k < kCellBatch; ++k) {
-                const uint32_t c = k < mb ? min(cnt[k], kMaxNeighbors) : 0u;
-                if (kk == k) mine = c;
-                rows = max(rows, c);
-            }
-            for (uint32_t r0 = 0; r0 < rows; r0 += kWave / kCellBatch) {
-                const uint32_t r = r0 + lane / kCellBatch;
-                if (kk < mb && r < mine) nbrList[(size_t)r * capacity + first + p0 + kk] = stage[w][r][kk];
-            }
-            if (lane < mb) nbrCount[first + p0 + lane] = mine;
-            if (lane < kCellBatch && lane < mb) {
-                tot += mine;
-                mx = max(mx, mine);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int o = kWave / 2; o > 0; o >>= 1) rows = max(rows, (uint32_t)__shfl_xor(rows, o, kWave));
+    const uint32_t* ts = tabStart[w][mySeg < kWinSegs ? mySeg : 0u];
+    for (uint32_t r = 0; r < rows; ++r) {
+        if (r < n && mySeg != kWinDirect) {
+            const uint32_t code = st[r * kWinPitch + lane];
+            nbrList[(size_t)r * capacity + i] = ts[code >> 11] + (code & 2047u);
         }
     }
+    if (valid) nbrCount[i] = n;
     // per-scene total and maximum neighbour count (debug statistics)
-    unsigned long long t = tot;
-    uint32_t x = mx;
+    unsigned long long t = n;
+    uint32_t mx = n;
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) {
         t += __shfl_xor(t, o, kWave);
-        x = max(x, (uint32_t)__shfl_xor(x, o, kWave));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, o, kWave));
     }
     if (lane == 0 && t > 0) {
         atomicAdd(&nbStats[2 * s], t);
-        atomicMax(&nbStats[2 * s + 1], (unsigned long long)x);
+        atomicMax(&nbStats[2 * s + 1], (unsigned long long)mx);
     }
 }
 
