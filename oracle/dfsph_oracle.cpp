@@ -106,6 +106,32 @@ template <class T> inline V3<T> mulmv(const M3<T>& A, V3<T> v) {  // glm mat3 * 
     return {A.m[0] * v.x + A.m[1] * v.y + A.m[2] * v.z, A.m[3] * v.x + A.m[4] * v.y + A.m[5] * v.z,
             A.m[6] * v.x + A.m[7] * v.y + A.m[8] * v.z};
 }
+template <class T> inline M3<T> mmul(const M3<T>& A, const M3<T>& B) {
+    M3<T> C;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            C.m[3 * r + c] = A.m[3 * r] * B.m[c] + A.m[3 * r + 1] * B.m[3 + c] + A.m[3 * r + 2] * B.m[6 + c];
+    return C;
+}
+template <class T> inline M3<T> skew(V3<T> a) {  // skew(a) b = cross(a, b)
+    M3<T> S;
+    S.m[0] = T(0), S.m[1] = -a.z, S.m[2] = a.y;
+    S.m[3] = a.z, S.m[4] = T(0), S.m[5] = -a.x;
+    S.m[6] = -a.y, S.m[7] = a.x, S.m[8] = T(0);
+    return S;
+}
+template <class T> inline M3<T> mlin(T a, const M3<T>& A, T b, const M3<T>& B) {  // a A + b B
+    M3<T> C;
+    for (int e = 0; e < 9; ++e) C.m[e] = a * A.m[e] + b * B.m[e];
+    return C;
+}
+template <class T> inline M3<T> normalize_jacobian(V3<T> u, T len) {  // d(a / |a|)/da = (I - u u^T) / |a|, u = a / |a|
+    const T uv[3] = {u.x, u.y, u.z};
+    M3<T> J;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) J.m[3 * r + c] = ((r == c ? T(1) : T(0)) - uv[r] * uv[c]) / len;
+    return J;
+}
 template <class T> inline V3<T> mulmtv(const M3<T>& A, V3<T> v) {  // A^T v
     return {A.m[0] * v.x + A.m[3] * v.y + A.m[6] * v.z, A.m[1] * v.x + A.m[4] * v.y + A.m[7] * v.z,
             A.m[2] * v.x + A.m[5] * v.y + A.m[8] * v.z};
@@ -1148,6 +1174,7 @@ public:
             T vb = T(0);
             V3<T> dvb{T(0), T(0), T(0)}, r{T(0), T(0), T(0)};
             M3<T> Jr = mzero<T>();
+            M3<T> Jn = mzero<T>();  // d n / dx (the boundary viscosity samples turn with the normal)
         };
         std::vector<std::vector<Contact>> contact;  // [body][particle]
         // Push-out of a particle found inside a body: x <- x + delta n, v <- 0; Jpush = dx_after / dx_before
@@ -1877,10 +1904,6 @@ public:
         tape_velocity(t.v0);
         if (desc.EnableSurfaceTensionSolver && info.temporal)
             tape_error = "temporal smoothing of the surface tension is outside the adjoint's scope";
-        if (info.muB != T(0) && desc.EnableViscositySolver)
-            for (auto& b : bodies)
-                for (long i = 0; i < n; ++i)
-                    if (b.vb[i] > T(0)) tape_error = "boundary viscosity at a boundary contact is outside the adjoint's scope";
         for (long i = 0; i < n; ++i)
             if (cnt(i) >= kMaxNeighbors) tape_error = "a particle reached the neighbour cap (lists not symmetric)";
         tape_boundary(t);
@@ -1928,6 +1951,7 @@ public:
                     const T nv[3] = {nn.x, nn.y, nn.z}, dv[3] = {ds.x, ds.y, ds.z};
                     for (int r = 0; r < 3; ++r)
                         for (int k = 0; k < 3; ++k) c.Jr.m[3 * r + k] = nv[r] * dv[k] + s * Jn.m[3 * r + k];
+                    c.Jn = Jn;
                     sampled = true;
                 } else if (phi <= T(0) && gl > T(1.0e-5)) {
                     if (sampled || t.pushed[i])
@@ -2082,6 +2106,62 @@ public:
             vjp_x_A(t, K[k], la, lx);
         }
     }
+    // Boundary viscosity samples of a contact (tangential(), DFSPHKernels.cu:636-681, 793-839): the offsets
+    // pd_k = r + td t1, r - td t1, r + td t2, r - td t2 from the particle to the four samples around xb, with
+    // t1 = normalize(N x v), t2 = normalize(N x (N x v)), N = -r / |r| and v the axis orthogonal_vectors picks (held),
+    // and their derivatives in the particle's position: d pd_k / dx = dr/dx +- td dt/dN dN/dx, dN/dx = -dn/dx.
+    struct TanSamples {
+        bool ok = false;
+        V3<T> pd[4];
+        M3<T> J[4];
+    };
+    TanSamples tan_samples(const typename StepTape::Contact& c) const {
+        TanSamples o;
+        V3<T> N = -c.r;
+        const T nl = length(N);
+        if (!(nl > T(0.0001))) return o;
+        N = N / nl;
+        V3<T> v = v3<T>(1, 0, 0);
+        if ((double)std::fabs(dot3(v, N)) > 0.999) v = v3<T>(0, 1, 0);
+        const V3<T> x = cross(N, v), y = cross(N, x);
+        const T xl = std::sqrt(dot3(x, x)), yl = std::sqrt(dot3(y, y));
+        const V3<T> t1 = normalize(x), t2 = normalize(y);
+        const M3<T> dxdN = mlin(T(-1), skew(v), T(0), skew(v));
+        const M3<T> dydN = mlin(T(-1), skew(x), T(-1), mmul(skew(N), skew(v)));
+        const M3<T> dNdx = mlin(T(-1), c.Jn, T(0), c.Jn);
+        const M3<T> dt1 = mmul(mmul(normalize_jacobian(t1, xl), dxdN), dNdx);
+        const M3<T> dt2 = mmul(mmul(normalize_jacobian(t2, yl), dydN), dNdx);
+        const T td = info.td;
+        o.pd[0] = c.r + td * t1;
+        o.pd[1] = c.r - td * t1;
+        o.pd[2] = c.r + td * t2;
+        o.pd[3] = c.r - td * t2;
+        o.J[0] = mlin(T(1), c.Jr, td, dt1);
+        o.J[1] = mlin(T(1), c.Jr, -td, dt1);
+        o.J[2] = mlin(T(1), c.Jr, td, dt2);
+        o.J[3] = mlin(T(1), c.Jr, -td, dt2);
+        o.ok = true;
+        return o;
+    }
+    // C w = sum_k c_k G(|pd_k|) pd_k (pd_k . w), c_k = 10 mu_b (vb / 4) / (|pd_k|^2 + 0.01 R^2): the boundary block of
+    // the viscosity matrix of particle i (symmetric), summed over the bodies it samples
+    V3<T> visc_boundary_apply(const StepTape& t, long i, V3<T> w) const {
+        V3<T> o = v3<T>(0, 0, 0);
+        if (info.muB == T(0)) return o;
+        for (const auto& cb : t.contact) {
+            const auto& c = cb[i];
+            if (!(c.vb > T(0))) continue;
+            const TanSamples s = tan_samples(c);
+            if (!s.ok) continue;
+            const T vol = T(0.25) * c.vb;
+            for (int k = 0; k < 4; ++k) {
+                const T q = length2(s.pd[k]) + T(0.01) * info.R2;
+                const T ck = T(10) * info.muB * vol / q;
+                o = o + (ck * kernel.gval(std::sqrt(dot3(s.pd[k], s.pd[k]))) * dot3(s.pd[k], w)) * s.pd[k];
+            }
+        }
+        return o;
+    }
     // viscosity system of the tape: (A y)_i = y_i - sum_j T_ij,
     // T_ij = (h / rho_i) 10 mu m / (rho_j (|r|^2 + 0.01 R^2)) G(|r|) ((y_i - y_j) . r) r
     T visc_k(const StepTape& t, const std::vector<T>& rho, long i, unsigned j, V3<T> r) const {
@@ -2100,6 +2180,8 @@ public:
                 const V3<T> u = visc_k(t, rho, i, j, r) * w[i] - visc_k(t, rho, (long)j, (unsigned)i, -r) * w[j];
                 out[i] = out[i] - dot3(r, u) * r;
             }
+        for (long i = 0; i < n; ++i)  // boundary block: (A^T w)_i -= (h / rho_i) C_i w_i (C_i symmetric)
+            out[i] = out[i] - (t.h / rho[i]) * visc_boundary_apply(t, i, w[i]);
     }
     // BiCGSTAB for A^T x = b (double precision, relative residual 1e-12)
     void solve_visc_T(const StepTape& t, const std::vector<T>& rho, const std::vector<V3<T>>& b,
@@ -2275,6 +2357,29 @@ public:
                     if (rl > T(0)) gr = gr + (ur * sr * dk / rl) * r;
                     lx[i] = lx[i] + gr;
                     lx[j] = lx[j] - gr;
+                }
+            // boundary samples: lb_i . (h / rho_i) C_i(x_i) y_i, through rho_i, the sample offsets and vb
+            if (info.muB != T(0))
+                for (long i = 0; i < n; ++i) {
+                    const T hr = h / rho[i];
+                    lrho[i] -= hr * dot3(lb[i], visc_boundary_apply(t, i, t.y[i])) / rho[i];
+                    for (const auto& cb : t.contact) {
+                        const auto& c = cb[i];
+                        if (!(c.vb > T(0))) continue;
+                        const TanSamples s = tan_samples(c);
+                        if (!s.ok) continue;
+                        const T vol = T(0.25) * c.vb;
+                        for (int k = 0; k < 4; ++k) {
+                            const V3<T> pd = s.pd[k];
+                            const T pl = std::sqrt(dot3(pd, pd)), q = pl * pl + T(0.01) * info.R2;
+                            const T ck = T(10) * info.muB * vol / q, g = kernel.gval(pl);
+                            const T lp = dot3(lb[i], pd), py = dot3(pd, t.y[i]);
+                            const T fk = ck * g * lp * py;
+                            V3<T> gp = (ck * g) * (py * lb[i] + lp * t.y[i]) + (lp * py * g * (T(-2) * ck / q)) * pd;
+                            if (pl > T(0)) gp = gp + (lp * py * ck * kernel.dG(pl) / pl) * pd;
+                            lx[i] = lx[i] + hr * (mulmtv(s.J[k], gp) + (fk / c.vb) * c.dvb);
+                        }
+                    }
                 }
         }
         // divergence final: v1 = v0 + h A(kv^K)

@@ -3,8 +3,8 @@ free and landing on a floor, and on a block landing on a sphere.
 
 The oracle runs the same forward in its FP32 mirror (identical lists, masks and iteration counts, so the two adjoints
 differ only in summation order) and in its FP64 twin (the FP64 reference gradient). The differentiated map and its
-scope are those of vfd_adjoint: fixed time step, fixed Jacobi iteration counts, boundary contact without boundary
-viscosity, no surface tension."""
+scope are those of vfd_adjoint: fixed time step, fixed Jacobi iteration counts, boundary contact (with and without
+boundary viscosity), surface tension without temporal smoothing."""
 import numpy as np
 import pytest
 
@@ -69,20 +69,23 @@ def test_adjoint_matches_oracle_reference():
             assert e[k] < 1e-3, (tag, k, e[k])
 
 
-@pytest.mark.parametrize("kind,contacts", [("floor", 512), ("sphere", 44)])
-def test_adjoint_through_boundary_contact_matches_oracle(kind, contacts):
-    """Contact samples (density, factor, divergence and pressure terms through the volume map) and the push-out of
-    one particle starting inside the solid. Directions over all particles, over the contact particles of the first
+@pytest.mark.parametrize("kind,contacts,bvisc", [("floor", 512, 0.0), ("sphere", 44, 0.0), ("floor", 512, 0.02),
+                                                 ("sphere", 44, 0.05)])
+def test_adjoint_through_boundary_contact_matches_oracle(kind, contacts, bvisc):
+    """Contact samples (density, factor, divergence and pressure terms through the volume map, and with a boundary
+    viscosity its tangential samples in the viscosity matrix) and the push-out of one particle starting inside the
+    solid. Directions over all particles, over the contact particles of the first
     step and on the pushed particle; the pushed particle's entry velocity has no effect. Against the FP32 mirror
     (same forward, bit for bit) the bar is 1e-3; against the FP64 twin it is 1e-3 or twice the mirror's own distance
     from the twin, whichever is larger: the FP32 and FP64 forwards part slightly (on the sphere the mirror's mu
     derivative is 8e-4 from the twin's), and the GPU cannot be closer to the twin than the forward it differentiates."""
     d, x, v, bodies = contact_scene(kind)
+    d.BoundaryViscosity = bvisc
     x32, v32 = x.astype(np.float32), v.astype(np.float32)
     n = len(x)
     rng = np.random.default_rng(11)
     wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
-    print("contact adjoint", kind, n, "particles", flush=True)
+    print("contact adjoint", kind, n, "particles, boundary viscosity", bvisc, flush=True)
     gpu = DFSPHSimulation(d)
     gpu.set_fluid(x32, v32)
     gpu.SetRigidBodies(bodies)
@@ -183,18 +186,6 @@ def test_adjoint_rejects_temporal_surface_tension():
     gpu.tape(True)
     gpu.OnUpdate()
     with pytest.raises(Exception, match="temporal smoothing"):
-        gpu.adjoint(np.zeros((gpu.GetParticleCount(), 3), np.float32))
-
-
-def test_adjoint_rejects_boundary_viscosity_at_contact():
-    d, x, v, bodies = contact_scene("floor")
-    d.BoundaryViscosity = 0.02
-    gpu = DFSPHSimulation(d)
-    gpu.set_fluid(x.astype(np.float32), v.astype(np.float32))
-    gpu.SetRigidBodies(bodies)
-    gpu.tape(True)
-    gpu.OnUpdate()
-    with pytest.raises(Exception, match="boundary viscosity"):
         gpu.adjoint(np.zeros((gpu.GetParticleCount(), 3), np.float32))
 
 

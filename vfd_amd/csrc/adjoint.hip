@@ -26,8 +26,9 @@ struct AdjK {
     const uint32_t* __restrict__ count;
     const uint32_t* __restrict__ list;  // [j * cap + i]
     const float4* __restrict__ x;
-    const float4* __restrict__ con;  // contact records [(5 b + q) n + i] (k_adj_boundary_tape); bodies = 0: none
-    uint32_t bodies;
+    const float4* __restrict__ con;  // contact records [(kContactRecs b + q) n + i] (k_adj_boundary_tape)
+    uint32_t bodies;                 // 0: no contact records
+    float muB, td;                   // boundary viscosity (0: its samples are not differentiated) and sample distance
 };
 
 __device__ __forceinline__ float adj_g(const AdjK& a, float rl) {  // gradW / r as the forward's table gives it
@@ -69,7 +70,7 @@ struct AdjContact {
     float vb;
 };
 __device__ __forceinline__ bool adj_contact(const AdjK& a, uint32_t b, uint32_t i, AdjContact& c) {
-    const float4* p = a.con + (size_t)5 * b * a.n + i;
+    const float4* p = a.con + (size_t)kContactRecs * b * a.n + i;
     const float4 r = p[0];
     if (!(r.w > 0.0f)) return false;
     c.r = mk3(r.x, r.y, r.z);
@@ -109,6 +110,89 @@ __device__ __forceinline__ f3 adj_contact_density_grad(const AdjK& a, uint32_t i
         const float r2 = dot3(c.r, c.r), rl = sqrtf(r2);
         const float w = adj_w(a, r2), dw = rl > 0.0f ? c.vb * adj_dW(a, rl) / rl : 0.0f;
         o = o + w * c.dv + (dw * c.r.x * c.j0 + dw * c.r.y * c.j1 + dw * c.r.z * c.j2);
+    }
+    return o;
+}
+
+// Boundary viscosity samples of a contact (tangential(), DFSPHKernels.cu:636-681, 793-839, formed from the recorded
+// r = x - xb as the forward forms them): N = -r / |r|, v the axis GetOrthogonalVectors picks (held), x = N x v,
+// y = N x x, t1 = x / |x|, t2 = y / |y|; the offsets are pd = r + td t1, r - td t1, r + td t2, r - td t2.
+struct AdjTan {
+    f3 N, v, x, t1, t2;
+    float xl, yl;
+};
+__device__ __forceinline__ bool adj_tan(f3 r, AdjTan& t) {
+    const f3 n0 = -r;
+    const float nl = sqrtf(dot3(n0, n0));
+    if (!(nl > 0.0001f)) return false;
+    t.N = n0 / nl;
+    t.v = mk3(1.0f, 0.0f, 0.0f);
+    if ((double)fabsf(dot3(t.v, t.N)) > 0.999) t.v = mk3(0.0f, 1.0f, 0.0f);
+    t.x = cross(t.N, t.v);
+    const f3 y = cross(t.N, t.x);
+    t.xl = sqrtf(dot3(t.x, t.x));
+    t.yl = sqrtf(dot3(y, y));
+    t.t1 = t.x / t.xl;
+    t.t2 = y / t.yl;
+    return true;
+}
+__device__ __forceinline__ f3 adj_tan_pd(const AdjContact& c, const AdjTan& t, float td, int k) {
+    const f3 tm = k < 2 ? t.t1 : t.t2;
+    return (k & 1) ? c.r - td * tm : c.r + td * tm;
+}
+// C_i w = sum over the particle's contacts and their four samples of c G(|pd|) pd (pd . w),
+// c = 10 mu_b (vb / 4) / (|pd|^2 + 0.01 R^2): the boundary block of the viscosity row (symmetric)
+__device__ __forceinline__ f3 adj_visc_boundary(const AdjK& a, uint32_t i, f3 w) {
+    f3 o = mk3(0.0f, 0.0f, 0.0f);
+    AdjContact c;
+    for (uint32_t b = 0; b < a.bodies; ++b) {
+        if (!adj_contact(a, b, i, c)) continue;
+        AdjTan t;
+        if (!adj_tan(c.r, t)) continue;
+        const float vol = 0.25f * c.vb;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const f3 pd = adj_tan_pd(c, t, a.td, k);
+            const float pl = sqrtf(dot3(pd, pd)), q = pl * pl + 0.01f * a.R2;
+            o = o + (10.0f * a.muB * vol / q * adj_g(a, pl) * dot3(pd, w)) * pd;
+        }
+    }
+    return o;
+}
+// d/dx_i of lb . C_i(x_i) y and its value: the samples move with the contact's offset r (dr/dx) and turn with its
+// normal (dn/dx, through t1 and t2), and scale with vb
+__device__ __forceinline__ f3 adj_visc_boundary_vjp(const AdjK& a, uint32_t i, f3 lb, f3 y) {
+    f3 o = mk3(0.0f, 0.0f, 0.0f);
+    AdjContact c;
+    for (uint32_t b = 0; b < a.bodies; ++b) {
+        if (!adj_contact(a, b, i, c)) continue;
+        AdjTan t;
+        if (!adj_tan(c.r, t)) continue;
+        const float4* p = a.con + (size_t)kContactRecs * b * a.n + i;
+        const f3 n0 = ld3(p[5 * (size_t)a.n]), n1 = ld3(p[6 * (size_t)a.n]), n2 = ld3(p[7 * (size_t)a.n]);
+        const float vol = 0.25f * c.vb;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const f3 pd = adj_tan_pd(c, t, a.td, k);
+            const float pl = sqrtf(dot3(pd, pd)), q = pl * pl + 0.01f * a.R2;
+            const float ck = 10.0f * a.muB * vol / q, g = adj_g(a, pl);
+            const float lp = dot3(lb, pd), py = dot3(pd, y);
+            const float fk = ck * g * lp * py;
+            f3 gp = (ck * g) * (py * lb + lp * y) + (lp * py * g * (-2.0f * ck / q)) * pd;
+            if (pl > 0.0f) gp = gp + (lp * py * ck * adj_dG(a, pl) / pl) * pd;
+            // (d pd / dx)^T gp = (dr/dx)^T gp -+ td (dn/dx)^T (dt/dN)^T gp, dN = -dn
+            f3 w;
+            if (k < 2) {
+                const f3 u = (gp - dot3(t.t1, gp) * t.t1) / t.xl;
+                w = cross(t.v, u);
+            } else {
+                const f3 u = (gp - dot3(t.t2, gp) * t.t2) / t.yl;
+                w = cross(t.x, u) - cross(t.v, cross(t.N, u));
+            }
+            const f3 jn = w.x * n0 + w.y * n1 + w.z * n2;
+            const f3 jr = gp.x * c.j0 + gp.y * c.j1 + gp.z * c.j2;
+            o = o + ((k & 1) ? jr + a.td * jn : jr - a.td * jn) + (fk / c.vb) * c.dv;
+        }
     }
     return o;
 }
@@ -293,6 +377,7 @@ __global__ void __launch_bounds__(kTile) k_adj_visc_T(AdjK a, float h, const flo
             o = o - dot3(r, u) * r;
         }
     }
+    if (a.bodies && a.muB != 0.0f) o = o - (h / rhoi) * adj_visc_boundary(a, i, wi);  // (A^T w)_i -= (h/rho_i) C_i w_i
     out[i] = st4(o);
 }
 // parameter cotangents of -lb^T (dA) y = lb . dT: positions, densities, and per-tile partial sums of lb . T / mu
@@ -324,6 +409,11 @@ __global__ void __launch_bounds__(kTile) k_adj_visc_vjp(AdjK a, float h, const f
             f3 t = kij * (ur * lbi + si * u) - kji * (ur * lbj + sj * u);
             if (rl > 0.0f) t = t + (ur * (si * dki - sj * dkj) / rl) * r;
             gx = gx + t;
+        }
+        if (a.bodies && a.muB != 0.0f) {  // boundary samples: lb_i . (h / rho_i) C_i(x_i) y_i
+            const float hr = h / rho[i];
+            lr -= hr * dot3(lbi, adj_visc_boundary(a, i, yi)) / rho[i];
+            gx = gx + hr * adj_visc_boundary_vjp(a, i, lbi, yi);
         }
         lx[i] = st4(ld3(lx[i]) + gx);
         lrho[i] += lr;
@@ -651,7 +741,7 @@ void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const flo
 }
 
 AdjK AdjKHost::dev() const {
-    return AdjK{tab, R, invStep, K, L, V, rho0, m, mu, R2, n, cap, count, list, x, con, bodies};
+    return AdjK{tab, R, invStep, K, L, V, rho0, m, mu, R2, n, cap, count, list, x, con, bodies, muB, td};
 }
 
 void adj_launch_surface_tension(hipStream_t s, const AdjKHost& h, float scale, const float4* lv, const float4* nMC,

@@ -678,8 +678,8 @@ __device__ __forceinline__ void normal_jacobian(f3 n, float gl, const float hs[6
 
 // Adjoint recording of the boundary pass of one scene (no reference counterpart): replays k_boundary from the
 // positions it started from, with the same arithmetic and branches, and stores per particle and body the contact
-// sample's derivatives (con[(5 b + q) n + i]: q = 0 {r = x - xb, vb}, 1 {grad vb, 0}, 2..4 the rows of dr/dx, with
-// r = s n, s = max(phi + r/2, 2r)) and per particle the push-out's Jacobian (push[q n + i], q = 0..2 rows; row 0's w
+// sample's derivatives (con[(kContactRecs b + q) n + i]: q = 0 {r = x - xb, vb}, 1 {grad vb, 0}, 2..4 the rows of
+// dr/dx, with r = s n, s = max(phi + r/2, 2r), 5..7 the rows of dn/dx) and per particle the push-out's Jacobian (push[q n + i], q = 0..2 rows; row 0's w
 // = 1 for a pushed particle). flags[0] / flags[1]: some particle has a contact sample / was pushed; flags[2]: a
 // particle was pushed twice or after a contact sample, or the replay's position differs from the forward's.
 __global__ void __launch_bounds__(kTile) k_adj_boundary_tape(const SceneParams* __restrict__ P,
@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(kTile) k_adj_boundary_tape(const SceneParams* 
     float Jp[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
     for (uint32_t b = 0; b < p.bodyCount; ++b) {
         const MapDev& m = maps[b];
-        float4* cr = con + (size_t)5 * b * n + i;
+        float4* cr = con + (size_t)kContactRecs * b * n + i;
         cr[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         f3 normal = mk3(0.0f, 0.0f, 0.0f);
         float dist = FLT_MAX;
@@ -749,6 +749,9 @@ __global__ void __launch_bounds__(kTile) k_adj_boundary_tape(const SceneParams* 
             for (int q = 0; q < 3; ++q)
                 cr[(size_t)(2 + q) * n] = make_float4(nv[q] * dsv[0] + s * Jn[3 * q], nv[q] * dsv[1] + s * Jn[3 * q + 1],
                                                       nv[q] * dsv[2] + s * Jn[3 * q + 2], 0.0f);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                cr[(size_t)(5 + q) * n] = make_float4(Jn[3 * q], Jn[3 * q + 1], Jn[3 * q + 2], 0.0f);
             sampled = true;
         } else {
             if (sampled || pushed) bad = true;

@@ -28,6 +28,7 @@ struct Launch {
 
 // ---- adjoint.hip -----------------------------------------------------------------------------------------------
 struct AdjK;
+constexpr uint32_t kContactRecs = 8;  // float4 records per particle and body of a recorded contact sample
 // Geometry and constants of one recorded step for the adjoint kernels (spline constants K = 8/(pi R^3),
 // L = 48/(pi R^3)); arrays in the step's slot order.
 struct AdjKHost {
@@ -39,6 +40,7 @@ struct AdjKHost {
     const float4* x;
     const float4* con = nullptr;  // boundary contact records of the step (adj_launch_boundary_tape); null: none
     uint32_t bodies = 0;
+    float muB = 0.0f, td = 0.0f;  // boundary viscosity (dynamic) and tangential sample distance
     AdjK dev() const;
 };
 void adj_launch_accel_T(hipStream_t s, const AdjKHost& h, const float* kap, const float4* la, float* lk, float4* lx);
@@ -85,7 +87,7 @@ void adj_launch_surface_tension(hipStream_t s, const AdjKHost& h, float scale, c
                                 const float* kMC, float sf, float c, float4* lnc, float2* lcw, float4* lx);
 struct MapDev;
 // boundary.hip: replay of the boundary pass of scene 0 from its starting positions, recording contact samples
-// (con: 5 * bodies * n records) and push-outs (push: 3 * n) with their derivatives; flags[3] (contact, push, out of
+// (con: kContactRecs * bodies * n records) and push-outs (push: 3 * n) with their derivatives; flags[3] (contact, push, out of
 // scope or replay mismatch) are set to 1 where they occur and must be cleared beforehand
 void adj_launch_boundary_tape(hipStream_t s, const SceneParams* P, const MapDev* maps, uint32_t n, uint32_t capacity,
                               const float4* xpre, const float4* posAfter, const float* vb, const float4* xb,

@@ -1285,7 +1285,7 @@ public:
             VFD_TRY(tp->flags.alloc(3));
             VFD_HIP(hipMemsetAsync(tp->flags.p, 0, 3 * sizeof(uint32_t), stream));
             if (tp->bodies) {
-                VFD_TRY(tp->con.alloc((size_t)5 * tp->bodies * n));
+                VFD_TRY(tp->con.alloc((size_t)kContactRecs * tp->bodies * n));
                 VFD_TRY(tp->push.alloc((size_t)3 * n));
                 adj_launch_boundary_tape(stream, dP.p, dMaps.p, (uint32_t)n, cap, tapeXpre.p, pos.p, vb.p, xb.p,
                                          tp->con.p, tp->push.p, tp->flags.p);
@@ -1498,6 +1498,8 @@ public:
         if (flags[0]) {  // some particle sampled a body: the sample terms join every operator
             A.con = t.con.p;
             A.bodies = t.bodies;
+            A.muB = t.boundaryViscosity ? pr.muB : 0.0f;
+            A.td = pr.td;
         }
         const float h = t.h, H = t.H;
         for (auto* q : {&aLa, &aTmp4, &aGs, &aLy, &aLb}) VFD_TRY(q->alloc(n));
@@ -1579,8 +1581,6 @@ public:
                 tapeError = "temporal smoothing of the surface tension is outside the adjoint's scope";
             if (flags[3 * k + 2])
                 tapeError = "a particle pushed out twice or after a contact sample is outside the adjoint's scope";
-            if (flags[3 * k] && tapes[k]->boundaryViscosity)
-                tapeError = "boundary viscosity at a boundary contact is outside the adjoint's scope";
         }
         if (!tapeError.empty()) return fail(VFD_ERROR_NOT_SUPPORTED, tapeError);
         const uint32_t n = tapes[tapeCount - 1]->n;
