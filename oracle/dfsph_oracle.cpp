@@ -1886,11 +1886,12 @@ public:
     // transposed iteration), the viscosity solve by one transposed linear solve. Boundary contact enters through the
     // sample terms of density, factor, divergence and pressure acceleration (vb W(x - xb), vb gradW(x - xb)), whose
     // position derivatives follow the volume map's interpolant to second order (the normal's derivative is
-    // (I - n n^T) Hess(phi) / |grad phi|), and through the push-out's Jacobian. Surface tension enters through its
-    // smoothing pass (st_vjp): the Monte-Carlo normals and curvatures are piecewise constant in the positions (they
-    // change only where a sample's coverage or the classification flips) and are held at their recorded values.
-    // Scope: no temporal smoothing of the surface tension (it carries state between steps), no boundary viscosity on
-    // a step with contact, no particle pushed out twice or pushed after a contact sample of an earlier body,
+    // (I - n n^T) Hess(phi) / |grad phi|), through the push-out's Jacobian and, with a boundary viscosity, through the
+    // tangential samples' block of the viscosity matrix (tan_samples). Surface tension enters through its smoothing
+    // pass (st_vjp): the Monte-Carlo normals and curvatures are piecewise constant in the positions (they change only
+    // where a sample's coverage or the classification flips) and are held at their recorded values. lmu is d/dmu of
+    // the fluid viscosity; the boundary viscosity is held. Scope: no temporal smoothing of the surface tension (it
+    // carries state between steps), no particle pushed out twice or pushed after a contact sample of an earlier body,
     // symmetric lists (no particle at the 70-neighbour cap); tape_begin reports anything outside it.
     // ==========================================================================================================
     std::string tape_error;
