@@ -12,6 +12,10 @@
 // pressure acceleration), whose position derivatives the recording kernel (k_adj_boundary_tape) stores per particle
 // and body from the volume map's interpolant (gradient of vb, Jacobian of r = x - xb including the normal's
 // derivative), and through the push-out's Jacobian; a pushed particle's entry velocity has no effect (it is zeroed).
+// With a boundary viscosity each contact's four tangential samples add a symmetric 3x3 block to its viscosity row;
+// the transposed solve applies it, and its derivative follows the samples' offsets, the turning of their axes with
+// the normal (the rows of dn/dx are recorded too), the sample volume and the density. Surface tension enters through
+// its smoothing pass, with the recorded Monte-Carlo normals and curvatures held.
 //
 // Per-step arrays are in that step's sorted slot order; the host maps cotangents through the recorded ids.
 #include "device.h"
