@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define VFD_ABI_VERSION 2
+#define VFD_ABI_VERSION 3
 
 /* Status codes. */
 enum {
@@ -421,6 +421,12 @@ typedef struct vfd_tuning {
     uint32_t UseGraphs;           /* 1: each chunk of iterations is one launch of a captured HIP graph */
     uint32_t PredictIterations;   /* 1: a solver's first chunk is one iteration short of what it ran in the
                                      previous step, so that few empty iterations follow its convergence (default 0) */
+    uint32_t AdjointGroupLoops;   /* vfd_adjoint's transposed neighbour loops (the pressure solves' and the
+                                     viscosity operator's): 0 one thread per particle gathering
+                                     from HBM; 1 per neighbour group with the step's halo staged in LDS (the halos
+                                     are rebuilt from the recorded lists); 2 (default) per group once there are
+                                     as many groups of 1024 particles as compute units. Results are the same bits
+                                     either way. */
 } vfd_tuning;
 int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out);
 int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* tuning);

@@ -25,7 +25,7 @@ def test_library_loads_and_exports_every_declared_symbol(vfd_lib):
     for n in names:
         assert hasattr(vfd_lib, n), n
     assert set(names) == set(_abi.EXPORTED)
-    assert vfd_lib.vfd_abi_version() == 2
+    assert vfd_lib.vfd_abi_version() == 3
 
 
 def test_default_description_matches_reference(vfd_lib):

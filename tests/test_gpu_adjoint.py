@@ -189,6 +189,45 @@ def test_adjoint_rejects_temporal_surface_tension():
         gpu.adjoint(np.zeros((gpu.GetParticleCount(), 3), np.float32))
 
 
+@pytest.mark.parametrize("scene", ["floor_contact", "falling_block_100k"])
+def test_adjoint_group_loops_give_the_same_bits(scene):
+    """The transposed neighbour loops (pressure solves, viscosity operator) per neighbour group (the step's halos
+    rebuilt from its recorded lists and staged
+    in LDS; with HaloLimit 0 every group gathers from HBM inside the same loop) against one thread per particle: the
+    same terms in the same order, so the cotangents and d/dmu agree bit for bit. The floor scene has contact samples
+    with boundary viscosity and a pushed particle; the 100k block spans 98 groups over the XCDs."""
+    if scene == "floor_contact":
+        d, x, v, bodies = contact_scene("floor")
+        d.BoundaryViscosity = 0.02
+        steps = STEPS
+    else:
+        descs, fluids, bl = scenes.falling_viscous_block(differentiable=True)
+        d, (x, v), bodies = descs[0], fluids[0], bl[0]
+        steps = 2
+    x32, v32 = x.astype(np.float32), v.astype(np.float32)
+    n = len(x)
+    rng = np.random.default_rng(5)
+    wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    gpu = DFSPHSimulation(d)
+    gpu.set_fluid(x32, v32)
+    gpu.SetRigidBodies(bodies)
+    gpu.tape(True)
+    for _ in range(steps):
+        gpu.OnUpdate()
+    out = {}
+    for name, kw in [("particles", dict(AdjointGroupLoops=0)), ("groups", dict(AdjointGroupLoops=1)),
+                     ("groups_gathered", dict(AdjointGroupLoops=1, HaloLimit=0))]:
+        gpu.set_tuning(**kw)
+        print("adjoint", scene, n, "particles,", name, flush=True)
+        out[name] = gpu.adjoint(wx, wv)
+    ref = out["particles"]
+    assert np.isfinite(ref[0]).all() and np.abs(ref[0]).max() > 0
+    for name in ("groups", "groups_gathered"):
+        gx, gv, gmu = out[name]
+        assert np.array_equal(gx, ref[0]) and np.array_equal(gv, ref[1]), name
+        assert gmu == ref[2], (name, gmu, ref[2])
+
+
 def test_checkpointed_rollout_matches_oracle_fp64():
     """torch.autograd through Rollout (60 steps, checkpoints every 10) on the 2,048-particle falling viscous block
     against the oracle's FP64 reference adjoint of the same 60 steps taken in one piece; the loss is the 64^3

@@ -185,7 +185,8 @@ class KernelStat(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """vfd_tuning: launch and solver-control knobs (no reference counterpart)."""
     _fields_ = [("SolverCheckInterval", ctypes.c_uint32), ("HaloLimit", ctypes.c_uint32),
-                ("UseGraphs", ctypes.c_uint32), ("PredictIterations", ctypes.c_uint32)]
+                ("UseGraphs", ctypes.c_uint32), ("PredictIterations", ctypes.c_uint32),
+                ("AdjointGroupLoops", ctypes.c_uint32)]
 
 
 _P = ctypes.c_void_p
@@ -252,7 +253,7 @@ _PROTOS = {
 EXPORTED = sorted(_PROTOS)
 
 _lib = None
-ABI_VERSION = 2  # VFD_ABI_VERSION of include/vfd.h, whose structs the ctypes classes above mirror
+ABI_VERSION = 3  # VFD_ABI_VERSION of include/vfd.h, whose structs the ctypes classes above mirror
 
 
 class VfdError(RuntimeError):

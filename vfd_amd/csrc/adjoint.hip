@@ -33,6 +33,7 @@ struct AdjK {
     const float4* __restrict__ con;  // contact records [(kContactRecs b + q) n + i] (k_adj_boundary_tape)
     uint32_t bodies;                 // 0: no contact records
     float muB, td;                   // boundary viscosity (0: its samples are not differentiated) and sample distance
+    float invR, dG0, LR;             // 1 / R, 3 L / R^3 (d(gradW/r)/drho for q <= 1/2), L / R
 };
 
 __device__ __forceinline__ float adj_g(const AdjK& a, float rl) {  // gradW / r as the forward's table gives it
@@ -51,17 +52,26 @@ __device__ __forceinline__ float adj_dW(const AdjK& a, float rl) {  // spline dW
     if (q <= 0.5f) return a.K * (18.0f * q * q - 12.0f * q) / a.R;
     return -6.0f * a.K * (1.0f - q) * (1.0f - q) / a.R;
 }
-__device__ __forceinline__ float adj_dG(const AdjK& a, float rl) {  // spline d(gradW/r)/drho
-    const float q = rl / a.R;
+// The pair terms of the reverse loops divide through hardware reciprocals (v_rcp_f32, about 1 ulp): the adjoint is an
+// FP32 derivative held to a relative tolerance, not a bit-exact restatement, and IEEE divisions would dominate its
+// arithmetic.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// spline d(gradW/r)/drho = (L/R) f (2 / (R rl) + f / rl^2), f = 1 - rl/R, for rl > R/2; ir = 1 / rl
+__device__ __forceinline__ float adj_dG(const AdjK& a, float rl, float ir) {
+    const float q = rl * a.invR;
     if (!(q <= 1.0f) || !(rl > 1.0e-5f)) return 0.0f;
-    if (q <= 0.5f) return 3.0f * a.L / (a.R * a.R * a.R);
+    if (q <= 0.5f) return a.dG0;
     const float f = 1.0f - q;
-    return (a.L / a.R) * (2.0f * f / (a.R * rl) + f * f / (rl * rl));
+    return a.LR * f * ir * (2.0f * a.invR + f * ir);
 }
+__device__ __forceinline__ float adj_dG(const AdjK& a, float rl) { return adj_dG(a, rl, rcp(rl)); }
 // d(c . G(|r|) r)/dr = G c + G'(|r|) (c . r) / |r| r
 __device__ __forceinline__ f3 vjp_grad(const AdjK& a, f3 r, float rl, float g, f3 c) {
     f3 o = g * c;
-    if (rl > 0.0f) o = o + (adj_dG(a, rl) * dot3(c, r) / rl) * r;
+    if (rl > 0.0f) {
+        const float ir = rcp(rl);
+        o = o + (adj_dG(a, rl, ir) * dot3(c, r) * ir) * r;
+    }
     return o;
 }
 __device__ __forceinline__ uint32_t adj_nbr(const AdjK& a, uint32_t i, uint32_t j) {
@@ -207,15 +217,37 @@ constexpr uint32_t kAdjBatch = 4;
 
 // lk_i -= V sum_j (la_i - la_j) . gradW_ij + la_i . B_i          (A^T la = -D(la))
 // lx_i += sum_j vjp(r_ij, -V (k_i + k_j)(la_i - la_j)) + d/dx_i (-k_i la_i . B_i)  (positions through A(k))
-// with B_i = sum_b vb gradW(x_i - xb) the boundary samples' term
+// with B_i = sum_b vb gradW(x_i - xb) the boundary samples' term. term() takes the neighbours in list order with
+// g = gradW / r of the pair as the forward's table gives it.
+struct AdjAccelT {
+    f3 xi, lai, gx;
+    float ki, d;
+    __device__ __forceinline__ AdjAccelT(const AdjK& a, uint32_t i, const float* __restrict__ kap,
+                                         const float4* __restrict__ la)
+        : xi(ld3(a.x[i])), lai(ld3(la[i])), gx(mk3(0.0f, 0.0f, 0.0f)), ki(kap[i]), d(0.0f) {}
+    __device__ __forceinline__ void term(const AdjK& a, f3 xj, f3 laj, float kj, float g) {
+        const f3 r = xi - xj;
+        const float rl = sqrtf(dot3(r, r));
+        const f3 dl = lai - laj;
+        d += dot3(dl, g * r);
+        gx = gx + vjp_grad(a, r, rl, g, (-a.V * (ki + kj)) * dl);
+    }
+    __device__ __forceinline__ void finish(const AdjK& a, uint32_t i, float* lk, float4* lx) {
+        if (a.bodies) {
+            d = a.V * d + dot3(lai, adj_contact_sum(a, i));
+            gx = gx + adj_contact_vjp(a, i, (-ki) * lai);
+        } else {
+            d = a.V * d;
+        }
+        lk[i] -= d;
+        lx[i] = st4(ld3(lx[i]) + gx);
+    }
+};
 __global__ void __launch_bounds__(kTile) k_adj_accel_T(AdjK a, const float* __restrict__ kap,
                                                        const float4* __restrict__ la, float* lk, float4* lx) {
     const uint32_t i = pid();
     if (i >= a.n) return;
-    const f3 xi = ld3(a.x[i]), lai = ld3(la[i]);
-    const float ki = kap[i];
-    float d = 0.0f;
-    f3 gx = mk3(0.0f, 0.0f, 0.0f);
+    AdjAccelT t(a, i, kap, la);
     const uint32_t cnt = a.count[i];
     for (uint32_t j0 = 0; j0 < cnt; j0 += kAdjBatch) {
         f3 xj[kAdjBatch], laj[kAdjBatch];
@@ -230,32 +262,43 @@ __global__ void __launch_bounds__(kTile) k_adj_accel_T(AdjK a, const float* __re
 #pragma unroll
         for (uint32_t u = 0; u < kAdjBatch; ++u) {
             if (j0 + u >= cnt) break;
-            const f3 r = xi - xj[u];
-            const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
-            const f3 dl = lai - laj[u];
-            d += dot3(dl, g * r);
-            gx = gx + vjp_grad(a, r, rl, g, (-a.V * (ki + kj[u])) * dl);
+            const f3 r = t.xi - xj[u];
+            t.term(a, xj[u], laj[u], kj[u], adj_g(a, sqrtf(dot3(r, r))));
         }
     }
-    if (a.bodies) {
-        d = a.V * d + dot3(lai, adj_contact_sum(a, i));
-        gx = gx + adj_contact_vjp(a, i, (-ki) * lai);
-    } else {
-        d = a.V * d;
-    }
-    lk[i] -= d;
-    lx[i] = st4(ld3(lx[i]) + gx);
+    t.finish(a, i, lk, lx);
 }
 
 // lu_i += V sum_j (lf_i + lf_j) gradW_ij + lf_i B_i                       (D^T lf = -A(lf))
 // lx_i += sum_j vjp(r_ij, V (lf_i + lf_j)(u_i - u_j)) + d/dx_i (lf_i u_i . B_i)  (positions through D(u))
+struct AdjDivT {
+    f3 xi, ui, s, gx;
+    float lfi;
+    __device__ __forceinline__ AdjDivT(const AdjK& a, uint32_t i, const float* __restrict__ lf,
+                                       const float4* __restrict__ u)
+        : xi(ld3(a.x[i])), ui(ld3(u[i])), s(mk3(0.0f, 0.0f, 0.0f)), gx(mk3(0.0f, 0.0f, 0.0f)), lfi(lf[i]) {}
+    __device__ __forceinline__ void term(const AdjK& a, f3 xj, f3 uj, float lfj, float g) {
+        const f3 r = xi - xj;
+        const float rl = sqrtf(dot3(r, r));
+        const float l2 = lfi + lfj;
+        s = s + (l2 * g) * r;
+        gx = gx + vjp_grad(a, r, rl, g, (a.V * l2) * (ui - uj));
+    }
+    __device__ __forceinline__ void finish(const AdjK& a, uint32_t i, float4* lu, float4* lx) {
+        s = a.V * s;
+        if (a.bodies) {
+            s = s + lfi * adj_contact_sum(a, i);
+            gx = gx + adj_contact_vjp(a, i, lfi * ui);
+        }
+        lu[i] = st4(ld3(lu[i]) + s);
+        lx[i] = st4(ld3(lx[i]) + gx);
+    }
+};
 __global__ void __launch_bounds__(kTile) k_adj_div_T(AdjK a, const float* __restrict__ lf,
                                                      const float4* __restrict__ u, float4* lu, float4* lx) {
     const uint32_t i = pid();
     if (i >= a.n) return;
-    const f3 xi = ld3(a.x[i]), ui = ld3(u[i]);
-    const float lfi = lf[i];
-    f3 s = mk3(0.0f, 0.0f, 0.0f), gx = mk3(0.0f, 0.0f, 0.0f);
+    AdjDivT t(a, i, lf, u);
     const uint32_t cnt = a.count[i];
     for (uint32_t j0 = 0; j0 < cnt; j0 += kAdjBatch) {
         f3 xj[kAdjBatch], uj[kAdjBatch];
@@ -270,20 +313,56 @@ __global__ void __launch_bounds__(kTile) k_adj_div_T(AdjK a, const float* __rest
 #pragma unroll
         for (uint32_t q = 0; q < kAdjBatch; ++q) {
             if (j0 + q >= cnt) break;
-            const f3 r = xi - xj[q];
-            const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
-            const float l2 = lfi + lfj[q];
-            s = s + (l2 * g) * r;
-            gx = gx + vjp_grad(a, r, rl, g, (a.V * l2) * (ui - uj[q]));
+            const f3 r = t.xi - xj[q];
+            t.term(a, xj[q], uj[q], lfj[q], adj_g(a, sqrtf(dot3(r, r))));
         }
     }
-    s = a.V * s;
-    if (a.bodies) {
-        s = s + lfi * adj_contact_sum(a, i);
-        gx = gx + adj_contact_vjp(a, i, lfi * ui);
-    }
-    lu[i] = st4(ld3(lu[i]) + s);
-    lx[i] = st4(ld3(lx[i]) + gx);
+    t.finish(a, i, lu, lx);
+}
+
+// The same two loops per neighbour group (the forward's group loop): the recorded step's halos, rebuilt from its lists
+// by k_build_halo, are staged in LDS with records {x_j, s_j} {v_j} and the kernel table, and each pair reads its
+// table entry through the code k_build_halo stored (the entry adj_g reads); a group whose halo does not fit gathers
+// from the global arrays. Terms and their order are those of the per-particle kernels, so both give the same bits.
+struct AdjStage {
+    TileGrid tg;
+    HaloView H;
+    const uint32_t* __restrict__ local;  // [j * cap + i] halo slot | table code << 16
+};
+constexpr size_t kLdsAdjStage = kLdsG + HaloRec<LoadXSV>::kBytes;
+template <class Make, class Fin>
+__device__ __forceinline__ void adj_group_loop(const AdjK& a, const AdjStage& g, const float* __restrict__ s,
+                                               const float4* __restrict__ v, Make&& make, Fin&& fin) {
+    extern __shared__ float4 lds[];
+    float* sG = reinterpret_cast<float*>(lds);
+    load_table_g(sG, a.tab);
+    float* rec = reinterpret_cast<float*>(lds + kLdsG / sizeof(float4));
+    const LoadXSV load{a.x, s, v};
+    const NbrSrc global{nullptr, g.local, a.list, a.count, a.cap};
+    group_loop<2>(g.tg, g.H, rec, global, load, [&](uint32_t tile, bool ok, const NbrSrc& src) {
+        const uint32_t i = tile * kTile + (threadIdx.x & (kTile - 1));
+        if (!ok || i >= a.n) return;
+        auto t = make(i);
+        each_neighbor_rec<2>(src, i, a.R, a.invStep, t.xi, sG, load,
+                             [&](float4 lo, float4 hi, float gv) { t.term(a, ld3(lo), ld3(hi), lo.w, gv); });
+        fin(t, i);
+    });
+}
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_adj_accel_T_groups(AdjK a, AdjStage g,
+                                                                                 const float* __restrict__ kap,
+                                                                                 const float4* __restrict__ la,
+                                                                                 float* lk, float4* lx) {
+    adj_group_loop(
+        a, g, kap, la, [&](uint32_t i) { return AdjAccelT(a, i, kap, la); },
+        [&](AdjAccelT& t, uint32_t i) { t.finish(a, i, lk, lx); });
+}
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_adj_div_T_groups(AdjK a, AdjStage g,
+                                                                               const float* __restrict__ lf,
+                                                                               const float4* __restrict__ u,
+                                                                               float4* lu, float4* lx) {
+    adj_group_loop(
+        a, g, lf, u, [&](uint32_t i) { return AdjDivT(a, i, lf, u); },
+        [&](AdjDivT& t, uint32_t i) { t.finish(a, i, lu, lx); });
 }
 
 // one reversed Jacobi iteration, pointwise part: kappa^{k+1} = kappa^k - s r^k, r^k = M (src - c f^k)
@@ -351,16 +430,32 @@ __global__ void k_adj_fill(float* p, float v, uint32_t n) {
 __device__ __forceinline__ float visc_k(const AdjK& a, float h, float rhoi, float rhoj, float rl, float g) {
     return (h / rhoi) * (10.0f * a.mu * a.m / rhoj) * g / (rl * rl + 0.01f * a.R2);
 }
-// out = A^T w: out_i = w_i - sum_j r r^T (k_ij w_i - k_ji w_j)
+// out = A^T w: out_i = w_i - sum_j r r^T (k_ij w_i - k_ji w_j) = w_i - sum_j k_ij r r^T (w_i - w_j), since
+// k_ij = k_ji = h 10 mu m G / (rho_i rho_j (|r|^2 + 0.01 R^2))
+struct AdjViscT {
+    f3 xi, wi, o;
+    float hci;  // h 10 mu m / rho_i
+    __device__ __forceinline__ AdjViscT(const AdjK& a, uint32_t i, float h, const float* __restrict__ rho,
+                                        const float4* __restrict__ w)
+        : xi(ld3(a.x[i])), wi(ld3(w[i])), o(wi), hci(h * (10.0f * a.mu * a.m) * rcp(rho[i])) {}
+    __device__ __forceinline__ void term(const AdjK& a, f3 xj, f3 wj, float rhoj, float g) {
+        const f3 r = xi - xj;
+        const float k = hci * rcp(rhoj) * g * rcp(dot3(r, r) + 0.01f * a.R2);
+        o = o - (k * dot3(r, wi - wj)) * r;
+    }
+    __device__ __forceinline__ void finish(const AdjK& a, uint32_t i, float h, const float* __restrict__ rho,
+                                           float4* out) {
+        if (a.bodies && a.muB != 0.0f) o = o - (h / rho[i]) * adj_visc_boundary(a, i, wi);  // -(h/rho_i) C_i w_i
+        out[i] = st4(o);
+    }
+};
 __global__ void __launch_bounds__(kTile) k_adj_visc_T(AdjK a, float h, const float* __restrict__ rho,
                                                       const float4* __restrict__ w, float4* out,
                                                       const BicgState* __restrict__ st) {
     if (st && st->done) return;  // a device-side solve that has converged (block-uniform)
     const uint32_t i = pid();
     if (i >= a.n) return;
-    const f3 xi = ld3(a.x[i]), wi = ld3(w[i]);
-    const float rhoi = rho[i];
-    f3 o = wi;
+    AdjViscT t(a, i, h, rho, w);
     const uint32_t cnt = a.count[i];
     for (uint32_t j0 = 0; j0 < cnt; j0 += kAdjBatch) {
         f3 xj[kAdjBatch], wj[kAdjBatch];
@@ -375,14 +470,21 @@ __global__ void __launch_bounds__(kTile) k_adj_visc_T(AdjK a, float h, const flo
 #pragma unroll
         for (uint32_t q = 0; q < kAdjBatch; ++q) {
             if (j0 + q >= cnt) break;
-            const f3 r = xi - xj[q];
-            const float rl = sqrtf(dot3(r, r)), g = adj_g(a, rl);
-            const f3 u = visc_k(a, h, rhoi, rj[q], rl, g) * wi - visc_k(a, h, rj[q], rhoi, rl, g) * wj[q];
-            o = o - dot3(r, u) * r;
+            const f3 r = t.xi - xj[q];
+            t.term(a, xj[q], wj[q], rj[q], adj_g(a, sqrtf(dot3(r, r))));
         }
     }
-    if (a.bodies && a.muB != 0.0f) o = o - (h / rhoi) * adj_visc_boundary(a, i, wi);  // (A^T w)_i -= (h/rho_i) C_i w_i
-    out[i] = st4(o);
+    t.finish(a, i, h, rho, out);
+}
+__global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_adj_visc_T_groups(AdjK a, AdjStage g, float h,
+                                                                                const float* __restrict__ rho,
+                                                                                const float4* __restrict__ w,
+                                                                                float4* out,
+                                                                                const BicgState* __restrict__ st) {
+    if (st && st->done) return;  // a device-side solve that has converged (block-uniform)
+    adj_group_loop(
+        a, g, rho, w, [&](uint32_t i) { return AdjViscT(a, i, h, rho, w); },
+        [&](AdjViscT& t, uint32_t i) { t.finish(a, i, h, rho, out); });
 }
 // parameter cotangents of -lb^T (dA) y = lb . dT: positions, densities, and per-tile partial sums of lb . T / mu
 __global__ void __launch_bounds__(kTile) k_adj_visc_vjp(AdjK a, float h, const float* __restrict__ rho,
@@ -683,11 +785,34 @@ __global__ void k_adj_to_orig(uint32_t n, const uint32_t* __restrict__ id, const
 // ---- host side ---------------------------------------------------------------------------------------------------
 static inline dim3 grid_of(uint32_t n) { return dim3((n + kTile - 1) / kTile); }
 
+static inline AdjStage adj_stage(const AdjKHost& h) {
+    AdjStage g;
+    g.tg.tiles = h.groups.tiles;
+    g.tg.per = h.groups.per;
+    g.H = HaloView{h.groups.haloCount, h.groups.haloIds};
+    g.local = h.groups.local;
+    return g;
+}
 void adj_launch_accel_T(hipStream_t s, const AdjKHost& h, const float* kap, const float4* la, float* lk, float4* lx) {
-    hipLaunchKernelGGL(k_adj_accel_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), kap, la, lk, lx);
+    if (h.groups.grid)
+        hipLaunchKernelGGL(k_adj_accel_T_groups, dim3(h.groups.grid), dim3(kGroupSize), kLdsAdjStage, s, h.dev(),
+                           adj_stage(h), kap, la, lk, lx);
+    else
+        hipLaunchKernelGGL(k_adj_accel_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), kap, la, lk, lx);
 }
 void adj_launch_div_T(hipStream_t s, const AdjKHost& h, const float* lf, const float4* u, float4* lu, float4* lx) {
-    hipLaunchKernelGGL(k_adj_div_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), lf, u, lu, lx);
+    if (h.groups.grid)
+        hipLaunchKernelGGL(k_adj_div_T_groups, dim3(h.groups.grid), dim3(kGroupSize), kLdsAdjStage, s, h.dev(),
+                           adj_stage(h), lf, u, lu, lx);
+    else
+        hipLaunchKernelGGL(k_adj_div_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), lf, u, lu, lx);
+}
+int configure_adjoint_kernels() {
+    for (const void* fn :
+         {(const void*)k_adj_accel_T_groups, (const void*)k_adj_div_T_groups, (const void*)k_adj_visc_T_groups})
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsAdjStage) != hipSuccess)
+            return 1;
+    return 0;
 }
 void adj_launch_jacobi_point(hipStream_t s, uint32_t n, const float* lk, const float* alpha, float div,
                              const float* res, float c, float* lsrc, float* ls, float* lf) {
@@ -714,7 +839,11 @@ void adj_launch_fill(hipStream_t s, float* p, float v, uint32_t n) {
 }
 void adj_launch_visc_T(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* w, float4* out,
                        const BicgState* st) {
-    hipLaunchKernelGGL(k_adj_visc_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), dt, rho, w, out, st);
+    if (h.groups.grid)
+        hipLaunchKernelGGL(k_adj_visc_T_groups, dim3(h.groups.grid), dim3(kGroupSize), kLdsAdjStage, s, h.dev(),
+                           adj_stage(h), dt, rho, w, out, st);
+    else
+        hipLaunchKernelGGL(k_adj_visc_T, grid_of(h.n), dim3(kTile), 0, s, h.dev(), dt, rho, w, out, st);
 }
 void adj_launch_visc_vjp(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* y,
                          const float4* lb, float4* lx, float* lrho, double* muPart) {
@@ -745,7 +874,8 @@ void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const flo
 }
 
 AdjK AdjKHost::dev() const {
-    return AdjK{tab, R, invStep, K, L, V, rho0, m, mu, R2, n, cap, count, list, x, con, bodies, muB, td};
+    return AdjK{tab, R,   invStep, K,  L,      V,   rho0, m, mu, R2, n, cap, count, list, x, con, bodies, muB, td,
+                1.0f / R, 3.0f * L / (R * R * R), L / R};
 }
 
 void adj_launch_surface_tension(hipStream_t s, const AdjKHost& h, float scale, const float4* lv, const float4* nMC,

@@ -419,8 +419,8 @@ struct LoadXSV {  // lo = {x_t, s_t}, hi = {v_t, -}
 };
 
 // f(lo, hi, t) for every neighbour of particle i in list order, t = tab[code] and code = gcode(xi - x_j) (the
-// kernel-table entry of the pair). Staged groups read their list as uint16 halo slots (2 bytes per neighbour; the
-// code is computed from the record's position) in chunks of ListShape<R>::chunk entries: the chunk's
+// kernel-table entry of the pair). Staged groups read their list as halo slot | code << 16 (4 bytes per neighbour;
+// k_build_halo computes the code once per step) in chunks of ListShape<R>::chunk entries: the chunk's
 // list loads are issued together (rows past the end repeat the last entry, so every load and LDS read stays in
 // bounds), then all of its LDS record and table reads, and only then its f calls, so that neither the list latency
 // nor the LDS latency is paid once per neighbour. With a lookahead the next chunks' list loads are issued before

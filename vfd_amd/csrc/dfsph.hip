@@ -987,6 +987,6 @@ int configure_kernels() {
     };
     for (const auto& e : k)
         if (hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e.bytes) != hipSuccess) return 1;
-    return configure_search_kernels();
+    return configure_search_kernels() || configure_adjoint_kernels();
 }
 }  // namespace vfd

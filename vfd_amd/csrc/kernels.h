@@ -41,8 +41,17 @@ struct AdjKHost {
     const float4* con = nullptr;  // boundary contact records of the step (adj_launch_boundary_tape); null: none
     uint32_t bodies = 0;
     float muB = 0.0f, td = 0.0f;  // boundary viscosity (dynamic) and tangential sample distance
+    // group loops of the transposed Jacobi and viscosity kernels: the step's halos (launch_build_halo over its lists)
+    // and the persistent grid; grid 0: one thread per particle with global gathers
+    struct {
+        uint32_t grid = 0, tiles = 0, per = 0;
+        const uint32_t* haloCount = nullptr;
+        const uint32_t* haloIds = nullptr;
+        const uint32_t* local = nullptr;
+    } groups;
     AdjK dev() const;
 };
+int configure_adjoint_kernels();  // dynamic LDS of the group kernels
 void adj_launch_accel_T(hipStream_t s, const AdjKHost& h, const float* kap, const float4* la, float* lk, float4* lx);
 void adj_launch_div_T(hipStream_t s, const AdjKHost& h, const float* lf, const float4* u, float4* lu, float4* lx);
 void adj_launch_jacobi_point(hipStream_t s, uint32_t n, const float* lk, const float* alpha, float div,

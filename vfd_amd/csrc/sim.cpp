@@ -1441,6 +1441,11 @@ public:
     DevBuf<float2> aLcw;
     DevBuf<float> aLk, aLsrc, aLs, aLf, aLrho, aLalpha, aLd, aLS, aOrigX, aOrigV;
     DevBuf<double> aPart;
+    // halos of the step being reversed, rebuilt from its recorded lists for the group kernels
+    DevBuf<uint32_t> aHaloCount, aHaloIds, aNbrLocal, aHaloStats;
+    // transposed Jacobi and viscosity loops: 0 per particle, 1 per group, 2 per group once there is a group for every
+    // CU (below that the per-particle kernels spread better and run as fast: 100k particles, 98 groups)
+    uint32_t adjointGroups = 2;
     int adj_dot(const float4* p, const float4* q, uint32_t n, double& out) {
         const uint32_t blocks = (n + kTile - 1) / kTile;
         VFD_TRY(aPart.alloc(blocks));
@@ -1500,6 +1505,21 @@ public:
             A.bodies = t.bodies;
             A.muB = t.boundaryViscosity ? pr.muB : 0.0f;
             A.td = pr.td;
+        }
+        const Launch L = launch();
+        if (adjointGroups == 1 || (adjointGroups == 2 && L.groups >= numCU)) {
+            VFD_TRY(aHaloCount.alloc(std::max<uint32_t>(L.groups, 1)));
+            VFD_TRY(aHaloIds.alloc((size_t)std::max<uint32_t>(L.groups, 1) * kHaloMax));
+            VFD_TRY(aNbrLocal.alloc((size_t)cap * kMaxNeighbors));
+            VFD_TRY(aHaloStats.alloc(3));
+            launch_build_halo(L, t.count.p, t.list.p, haloLimit, aHaloCount.p, aHaloIds.p, aNbrLocal.p, aHaloStats.p,
+                              t.x.p, ktab());
+            A.groups.grid = L.gridGroups;
+            A.groups.tiles = L.tg.tiles;
+            A.groups.per = L.tg.per;
+            A.groups.haloCount = aHaloCount.p;
+            A.groups.haloIds = aHaloIds.p;
+            A.groups.local = aNbrLocal.p;
         }
         const float h = t.h, H = t.H;
         for (auto* q : {&aLa, &aTmp4, &aGs, &aLy, &aLb}) VFD_TRY(q->alloc(n));
@@ -2234,6 +2254,7 @@ int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out) {
     out->HaloLimit = sim->impl.haloLimit;
     out->UseGraphs = sim->impl.useGraphs ? 1u : 0u;
     out->PredictIterations = sim->impl.predictIterations ? 1u : 0u;
+    out->AdjointGroupLoops = sim->impl.adjointGroups;
     return VFD_OK;
 }
 int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* t) {
@@ -2243,11 +2264,13 @@ int vfd_set_tuning(vfd_sim* sim, const vfd_tuning* t) {
         return fail(VFD_ERROR_INVALID_ARGUMENT, "SolverCheckInterval must be in [1, 1024]");
     if (t->HaloLimit > vfd::kHaloMax)
         return fail(VFD_ERROR_INVALID_ARGUMENT, "HaloLimit must be at most " + std::to_string(vfd::kHaloMax));
+    if (t->AdjointGroupLoops > 2) return fail(VFD_ERROR_INVALID_ARGUMENT, "AdjointGroupLoops must be 0, 1 or 2");
     if (t->SolverCheckInterval != sim->impl.pollChunk) sim->impl.drop_graphs();
     sim->impl.pollChunk = t->SolverCheckInterval;
     sim->impl.haloLimit = t->HaloLimit;
     sim->impl.useGraphs = t->UseGraphs != 0;
     sim->impl.predictIterations = t->PredictIterations != 0;
+    sim->impl.adjointGroups = t->AdjointGroupLoops;
     return VFD_OK;
 }
 int vfd_get_kernel_table(vfd_sim* sim, float* w, float* gradient_w, float* scalars) {

@@ -413,7 +413,8 @@ class DFSPHSimulation:
         return ox, ov, mu.value
 
     def tuning(self):
-        """{SolverCheckInterval, HaloLimit}: launch and solver-control knobs (results do not depend on them)."""
+        """{SolverCheckInterval, HaloLimit, UseGraphs, PredictIterations, AdjointGroupLoops}: launch and solver-control
+        knobs (results do not depend on them)."""
         t = _abi.Tuning()
         check(self._lib.vfd_get_tuning(self._h, _abi.ctypes.byref(t)), "vfd_get_tuning")
         return {name: getattr(t, name) for name, _ in _abi.Tuning._fields_}
