@@ -22,7 +22,7 @@ warm-up steps covers the 100-step job. `batched_gain` (second figure of merit) r
 one batch and as one simulation per scene on the same GPU: value = batched throughput / one-scene-per-launch
 throughput over the same steps. `adjoint_over_forward` (third) runs config C5, 100k particles over 60 steps with
 checkpoints every 10: value = time of the backward pass (segment re-runs with recording, plus the reverse sweeps)
-/ time of the forward rollout.
+/ time of the forward rollout. `adjoint_2m` times the same reversal on three steps of the headline workload.
 """
 import argparse
 import csv
@@ -420,6 +420,49 @@ def adjoint_ratio(reps=3):
             "dloss_dmu": float(mu.grad)}
 
 
+def adjoint_headline(sim, snap, steps=3):
+    """The adjoint on the headline workload: `steps` steps of the 2M dam break from the snapshot after the warm-up
+    (boundary contact, push-out, boundary viscosity), timed plain, then the same steps again with recording on (the
+    replay is deterministic and must end in the same state), then vfd_adjoint of a random cotangent with its
+    transposed neighbour loops per neighbour group (the default at this size) and per particle, each after one
+    untimed pass. reverse_over_forward = adjoint time / plain forward time of the same steps; backward_over_forward
+    adds the recorded re-run, as a checkpointed rollout pays it."""
+    import numpy as np
+    n = sim.GetParticleCount()
+    sim.restore(snap)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.OnUpdate()
+    fwd = time.perf_counter() - t0
+    x_plain = sim.field("position")
+    sim.restore(snap)
+    sim.tape(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.OnUpdate()
+    taped = time.perf_counter() - t0
+    sim.tape(False)
+    same = bool(np.array_equal(x_plain, sim.field("position")))
+    rng = np.random.default_rng(1)
+    wx, wv = rng.normal(size=(n, 3)).astype(np.float32), rng.normal(size=(n, 3)).astype(np.float32)
+    keep = sim.tuning()
+    rev, grads = {}, {}
+    for name, mode in (("group_loops", 1), ("particle_loops", 0)):
+        sim.set_tuning(AdjointGroupLoops=mode)
+        sim.adjoint(wx, wv)
+        t0 = time.perf_counter()
+        grads[name] = sim.adjoint(wx, wv)
+        rev[name] = time.perf_counter() - t0
+    sim.set_tuning(**keep)
+    g, p = grads["group_loops"], grads["particle_loops"]
+    return {"workload": "dam_break_2m", "particles": n, "steps": steps, "forward_ms": round(1000 * fwd, 2),
+            "recorded_forward_ms": round(1000 * taped, 2), "recorded_replay_identical": same,
+            "adjoint_ms": {k: round(1000 * v, 2) for k, v in rev.items()},
+            "loop_forms_identical": bool(np.array_equal(g[0], p[0]) and np.array_equal(g[1], p[1])),
+            "reverse_over_forward": round(rev["group_loops"] / fwd, 3),
+            "backward_over_forward": round((taped + rev["group_loops"]) / fwd, 3)}
+
+
 def frame_capture_cost(sim, snap, steps=10):
     """Frame output on the headline workload (a frame is due every step there): the same `steps` steps from `snap`
     with capture off and on. With capture on, each frame is packed in original order on the GPU and copied to pinned
@@ -681,6 +724,11 @@ def main():
         print("[bench] frame output: %d steps with capture off and on" % args.frame_steps, file=sys.stderr, flush=True)
         fc = frame_capture_cost(sim, snap, args.frame_steps)
 
+    adj2m = None
+    if not args.no_adjoint and args.config == "dam_break_2m":
+        print("[bench] adjoint of the headline workload: 3 steps", file=sys.stderr, flush=True)
+        adj2m = adjoint_headline(sim, snap)
+
     nbar = float(np.mean(nb))
     nbody = len(bodies[0])
     kernels = {}
@@ -723,6 +771,7 @@ def main():
         "gpu_kernel_ms_per_step": round(total_gpu_ms / args.steps, 3),
         "iterations_per_step": its,
         "frame_output": fc,
+        "adjoint_2m": adj2m,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline: oracle, up to %.0f s" % args.cpu_seconds, file=sys.stderr, flush=True)
