@@ -637,7 +637,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-gain measurement (config C4)")
     ap.add_argument("--batched-steps", type=int, default=20)
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement (config C5)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint/forward measurements (config C5 and the headline workload)")
+    ap.add_argument("--no-c5", action="store_true", help="skip the adjoint/forward measurement of config C5 only")
     ap.add_argument("--no-frames", action="store_true", help="skip the frame-output measurement")
     ap.add_argument("--frame-steps", type=int, default=10)
     ap.add_argument("--no-mesh", action="store_true", help="skip the mesh-input (map build, sampler) measurement")
@@ -796,7 +798,7 @@ def main():
         del sim
         print("[bench] batched gain: C4, %d steps" % args.batched_steps, file=sys.stderr, flush=True)
         out["batched_gain"] = batched_gain(args.batched_steps, 3)
-    if not args.no_adjoint:
+    if not args.no_adjoint and not args.no_c5:
         print("[bench] adjoint/forward: C5", file=sys.stderr, flush=True)
         out["adjoint_over_forward"] = adjoint_ratio()
     if not args.no_mesh:
