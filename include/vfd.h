@@ -404,10 +404,11 @@ int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint3
  * solver iteration counts and clamp masks are held at their recorded values; kernel derivatives are the spline's.
  * Boundary contact samples (with their boundary-viscosity samples) and push-outs are differentiated through the
  * volume map (their branches held as recorded); surface tension through its smoothing pass, with the Monte-Carlo
- * normals and curvatures held. d/dmu is the fluid viscosity's (BoundaryViscosity is held). Scope: one scene, no
- * particle pushed out twice in a step or after sampling another body, no temporal smoothing of the surface tension,
- * no particle at the neighbour cap (else VFD_ERROR_NOT_SUPPORTED, from vfd_adjoint). vfd_set_fluid discards the records; their device buffers are kept for
- * the next recording until vfd_destroy. */
+ * normals and curvatures held, and with temporal smoothing through the blended curvature carried from step to step
+ * (the one carried into the first recorded step is held; the loss reads positions and velocities only). d/dmu is the
+ * fluid viscosity's (BoundaryViscosity is held). Scope: one scene, no particle pushed out twice in a step or after
+ * sampling another body, no particle at the neighbour cap (else VFD_ERROR_NOT_SUPPORTED, from vfd_adjoint).
+ * vfd_set_fluid discards the records; their device buffers are kept for the next recording until vfd_destroy. */
 int vfd_tape(vfd_sim* sim, int32_t on);
 int vfd_tape_steps(vfd_sim* sim, uint32_t* steps);
 int vfd_adjoint(vfd_sim* sim, const float* lx_out, const float* lv_out, float* lx_in, float* lv_in, double* lmu);

@@ -1181,11 +1181,12 @@ public:
         std::vector<uint8_t> pushed;
         std::vector<M3<T>> Jpush;
         // surface tension: the Monte-Carlo normals and curvatures of the step (their sample coverage and the
-        // classification are discrete) and the blend pass count
-        bool st = false;
+        // classification are discrete), the blend pass count and, with temporal smoothing, the blended curvature dK
+        // each particle carried into the step
+        bool st = false, temporal = false;
         unsigned stPasses = 0;
         std::vector<V3<T>> nMC;
-        std::vector<T> kMC;
+        std::vector<T> kMC, dK0;
     };
     std::vector<StepTape> tapes;
     std::vector<V3<T>> tapePre;  // positions the boundary pass of a taped step started from
@@ -1836,10 +1837,16 @@ public:
         if (last == VFD_PHASE_DIVERGENCE) return;
         for (auto& q : p) q.a = info.g;  // ClearAccelerationKernel :24-37
         if (last == VFD_PHASE_CLEAR_ACCELERATION) return;
+        if (taping && desc.EnableSurfaceTensionSolver && info.temporal) {
+            StepTape& t = tapes.back();
+            t.dK0.resize(n);
+            for (long i = 0; i < n; ++i) t.dK0[i] = p[i].dK;
+        }
         surface_tension();
         if (taping && desc.EnableSurfaceTensionSolver) {
             StepTape& t = tapes.back();
             t.st = true;
+            t.temporal = info.temporal;
             t.stPasses = desc.SurfaceTensionSmoothPassCount;
             t.nMC.resize(n);
             t.kMC.resize(n);
@@ -1889,10 +1896,12 @@ public:
     // (I - n n^T) Hess(phi) / |grad phi|), through the push-out's Jacobian and, with a boundary viscosity, through the
     // tangential samples' block of the viscosity matrix (tan_samples). Surface tension enters through its smoothing
     // pass (st_vjp): the Monte-Carlo normals and curvatures are piecewise constant in the positions (they change only
-    // where a sample's coverage or the classification flips) and are held at their recorded values. lmu is d/dmu of
-    // the fluid viscosity; the boundary viscosity is held. Scope: no temporal smoothing of the surface tension (it
-    // carries state between steps), no particle pushed out twice or pushed after a contact sample of an earlier body,
-    // symmetric lists (no particle at the 70-neighbour cap); tape_begin reports anything outside it.
+    // where a sample's coverage or the classification flips) and are held at their recorded values; with temporal
+    // smoothing the blended curvature dK each particle carries from step to step has a cotangent of its own (ldK),
+    // passed from step to step through the reversal (zero after the last step: the loss reads x and v only; the dK
+    // carried into the first step is a constant). lmu is d/dmu of the fluid viscosity; the boundary viscosity is held.
+    // Scope: no particle pushed out twice or pushed after a contact sample of an earlier body, symmetric lists (no
+    // particle at the 70-neighbour cap); tape_begin reports anything outside it.
     // ==========================================================================================================
     std::string tape_error;
     void tape_begin() {
@@ -1903,8 +1912,6 @@ public:
         t.x.resize(n);
         for (long i = 0; i < n; ++i) t.x[i] = p[i].x;
         tape_velocity(t.v0);
-        if (desc.EnableSurfaceTensionSolver && info.temporal)
-            tape_error = "temporal smoothing of the surface tension is outside the adjoint's scope";
         for (long i = 0; i < n; ++i)
             if (cnt(i) >= kMaxNeighbors) tape_error = "a particle reached the neighbour cap (lists not symmetric)";
         tape_boundary(t);
@@ -2221,12 +2228,22 @@ public:
     // for a particle with nMC_i != 0 (ComputeSurfaceTensionNormalsAndCurvatureKernel :960-1006, blending :1008-1046)
     //   w_ij = 1 - |x_i - x_j| / R over neighbours with nMC_j != 0,  nc = sum w nMC_j,  cc = sum w kMC_j,  cf = sum w,
     //   nS = normalize((1 - sf) nMC_i + sf normalize(nc)),  kS = ((1 - sf) kMC_i + sf cc) / (1 - sf + sf cf),
-    //   a_i = -(P sigma / m) kS nS  (P blend passes, each adding the force once).
-    // Accumulates the position cotangent of a against la.
-    void st_vjp(const StepTape& t, const std::vector<V3<T>>& la, std::vector<V3<T>>& lx) const {
+    //   a_i = -(sigma / m) E_i nS with E_i = sum over the P blend passes of the pass's curvature fk_p.
+    // Without temporal smoothing fk_p = kS (E = P kS). With it fk_p = 0.05 kS + 0.95 fk_(p-1) from fk_0 = dK_i, the
+    // curvature the particle carried in, so E = A dK_i + (P - A) kS with A = sum_(p=1..P) 0.95^p, and the particle
+    // carries out dK' = 0.95^P dK_i + (1 - 0.95^P) kS (a particle with nMC_i = 0: no force, dK' = 0.95^P dK_i).
+    // Without temporal smoothing dK' = kS (0 for nMC_i = 0) whatever dK was. Accumulates the position cotangent of a
+    // against la and maps ldK, the cotangent of the carried curvature, from dK' to dK.
+    void st_vjp(const StepTape& t, const std::vector<V3<T>>& la, std::vector<V3<T>>& lx, std::vector<T>& ldK) const {
         const long n = (long)t.x.size();
         const T sf = info.smoothing, R = info.R;
-        const T c = -(T)t.stPasses * info.sigma * info.mInv;
+        const T c = -info.sigma * info.mInv;
+        const T P = (T)t.stPasses;
+        T A = 0, aP = 1;
+        for (unsigned q = 0; q < t.stPasses; ++q) {
+            aP *= T(0.95);
+            A += aP;
+        }
         std::vector<V3<T>> lnc(n, v3<T>(0, 0, 0));
         std::vector<T> lcc(n, T(0)), lcf(n, T(0));
         auto nrm_vjp = [](V3<T> a, V3<T> l) {  // d normalize(a) ^T l (identity where a = 0, as the forward keeps a)
@@ -2252,11 +2269,23 @@ public:
             const V3<T> ns = length2(u) > T(0) ? normalize(u) : u;
             const T den = T(1) - sf + sf * cf;
             const T ks = ((T(1) - sf) * t.kMC[i] + sf * cc) / den;
-            const T lks = c * dot3(la[i], ns);
-            lnc[i] = nrm_vjp(nc, sf * nrm_vjp(u, (c * ks) * la[i]));
+            const T lan = dot3(la[i], ns);
+            T E, lks;
+            if (t.temporal) {
+                E = A * t.dK0[i] + (P - A) * ks;
+                lks = c * (P - A) * lan + (T(1) - aP) * ldK[i];
+                ldK[i] = c * A * lan + aP * ldK[i];
+            } else {  // dK' = kS: the carried curvature in is overwritten
+                E = P * ks;
+                lks = c * P * lan + ldK[i];
+                ldK[i] = T(0);
+            }
+            lnc[i] = nrm_vjp(nc, sf * nrm_vjp(u, (c * E) * la[i]));
             lcc[i] = lks * sf / den;
             lcf[i] = -lks * ks * sf / den;
         }
+        for (long i = 0; i < n; ++i)
+            if (is_zero(t.nMC[i])) ldK[i] = t.temporal ? aP * ldK[i] : T(0);  // dK' = 0.95^P dK, or 0
         for (long i = 0; i < n; ++i) {
             if (is_zero(t.nMC[i])) continue;
             for (unsigned jj = 0; jj < t.nb.counts[i]; ++jj) {
@@ -2272,9 +2301,10 @@ public:
             }
         }
     }
-    // Reverse of one taped step: (lx, lv) are the cotangents of the step's output (x', v') on entry and of its input
-    // (x, v) on return; lmu accumulates d/dmu.
-    void reverse_step(const StepTape& t, std::vector<V3<T>>& lx, std::vector<V3<T>>& lv, double& lmu) const {
+    // Reverse of one taped step: (lx, lv, ldK) are the cotangents of the step's output (x', v', dK') on entry and of
+    // its input (x, v, dK) on return; lmu accumulates d/dmu.
+    void reverse_step(const StepTape& t, std::vector<V3<T>>& lx, std::vector<V3<T>>& lv, std::vector<T>& ldK,
+                      double& lmu) const {
         const long n = (long)t.x.size();
         const T h = t.h, H = t.H;
         std::vector<T> lrho(n, T(0)), lalpha(n, T(0)), tmp;
@@ -2316,7 +2346,7 @@ public:
         if (t.st && t.stPasses) {
             std::vector<V3<T>> las(n);
             for (long i = 0; i < n; ++i) las[i] = H * lv2[i];
-            st_vjp(t, las, lx);
+            st_vjp(t, las, lx, ldK);
         }
         // v2 = v1 + H g + (H / h)(y - v1)
         std::vector<V3<T>> ly(n), lv1(n);
@@ -2677,7 +2707,8 @@ int orc_adjoint(void* h, const double* lx_out, const double* lv_out, double* lx_
                 lv[i] = VT{(TT)lv_out[3 * i], (TT)lv_out[3 * i + 1], (TT)lv_out[3 * i + 2]};
             }
             double m = 0;
-            for (long k = (long)s->tapes.size() - 1; k >= 0; --k) s->reverse_step(s->tapes[k], lx, lv, m);
+            std::vector<TT> ldK(n, TT(0));
+            for (long k = (long)s->tapes.size() - 1; k >= 0; --k) s->reverse_step(s->tapes[k], lx, lv, ldK, m);
             for (size_t i = 0; i < n; ++i) {
                 lx_in[3 * i] = lx[i].x, lx_in[3 * i + 1] = lx[i].y, lx_in[3 * i + 2] = lx[i].z;
                 lv_in[3 * i] = lv[i].x, lv_in[3 * i + 1] = lv[i].y, lv_in[3 * i + 2] = lv[i].z;

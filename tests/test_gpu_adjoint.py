@@ -4,7 +4,7 @@ free and landing on a floor, and on a block landing on a sphere.
 The oracle runs the same forward in its FP32 mirror (identical lists, masks and iteration counts, so the two adjoints
 differ only in summation order) and in its FP64 twin (the FP64 reference gradient). The differentiated map and its
 scope are those of vfd_adjoint: fixed time step, fixed Jacobi iteration counts, boundary contact (with and without
-boundary viscosity), surface tension without temporal smoothing."""
+boundary viscosity), surface tension (with and without temporal smoothing)."""
 import numpy as np
 import pytest
 
@@ -134,16 +134,18 @@ def test_adjoint_through_boundary_contact_matches_oracle(kind, contacts, bvisc):
         assert out["fp64"][k] < max(1e-3, 2 * mirror[k]), ("fp64", k, out["fp64"][k], mirror[k])
 
 
-@pytest.mark.parametrize("n_side", [(8, 8, 8), (16, 8, 16)])
-def test_adjoint_through_surface_tension_matches_oracle(n_side):
+@pytest.mark.parametrize("n_side,temporal", [((8, 8, 8), 0), ((16, 8, 16), 0), ((8, 8, 8), 1), ((16, 8, 16), 1)])
+def test_adjoint_through_surface_tension_matches_oracle(n_side, temporal):
     """Surface tension (two blend passes, 16 Monte-Carlo samples per particle) with its Monte-Carlo normals and
-    curvatures held, against the oracle's FP32 mirror and FP64 twin (bars as for contact)."""
+    curvatures held, against the oracle's FP32 mirror and FP64 twin (bars as for contact). With temporal smoothing
+    the blended curvature carried from step to step has a cotangent of its own across the three recorded steps."""
     d, x, v = st_scene(n_side)
+    d.TemporalSmoothing = temporal
     x32, v32 = x.astype(np.float32), v.astype(np.float32)
     n = len(x)
     rng = np.random.default_rng(11)
     wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
-    print("surface tension adjoint", n, "particles", flush=True)
+    print("surface tension adjoint", n, "particles, temporal smoothing", temporal, flush=True)
     gpu = DFSPHSimulation(d)
     gpu.set_fluid(x32, v32)
     gpu.tape(True)
@@ -176,17 +178,6 @@ def test_adjoint_through_surface_tension_matches_oracle(n_side):
     for k in list(dirs) + ["mu"]:
         assert out["fp32"][k] < 1e-3, ("fp32", k, out["fp32"][k])
         assert out["fp64"][k] < max(1e-3, 2 * mirror[k]), ("fp64", k, out["fp64"][k], mirror[k])
-
-
-def test_adjoint_rejects_temporal_surface_tension():
-    d, x, v = st_scene()
-    d.TemporalSmoothing = 1
-    gpu = DFSPHSimulation(d)
-    gpu.set_fluid(x.astype(np.float32), v.astype(np.float32))
-    gpu.tape(True)
-    gpu.OnUpdate()
-    with pytest.raises(Exception, match="temporal smoothing"):
-        gpu.adjoint(np.zeros((gpu.GetParticleCount(), 3), np.float32))
 
 
 @pytest.mark.parametrize("scene", ["floor_contact", "falling_block_100k"])

@@ -583,15 +583,22 @@ __device__ __forceinline__ f3 adj_normalize_T(f3 a, f3 l) {  // d normalize(a)^T
     const f3 u = normalize(a);
     return (l - dot3(u, l) * u) / sqrtf(a2);
 }
+// The force is -(sigma / m) E nS with E the sum of the P passes' curvatures: E = P kS, or with temporal smoothing
+// (dK0 != null: the curvature each particle carried in) E = A dK0 + (P - A) kS, A = sum_(p=1..P) 0.95^p, and the
+// particle carries out dK' = aP dK0 + (1 - aP) kS, aP = 0.95^P (0.95^P dK0 off the surface); without it dK' = kS (0 off
+// the surface). ldK (null: not tracked) holds the cotangent of dK' on entry and of dK on return.
 __global__ void __launch_bounds__(kTile) k_adj_st_point(AdjK a, float s, const float4* __restrict__ lv,
                                                         const float4* __restrict__ nMC, const float* __restrict__ kMC,
-                                                        float sf, float c, float4* lnc, float2* lcw) {
+                                                        float sf, float c, float P, float A, float aP,
+                                                        const float* __restrict__ dK0, float* ldK, float4* lnc,
+                                                        float2* lcw) {
     const uint32_t i = pid();
     if (i >= a.n) return;
     const f3 ni = ld3(nMC[i]);
     if (is_zero(ni)) {
         lnc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         lcw[i] = make_float2(0.0f, 0.0f);
+        if (ldK) ldK[i] = dK0 ? aP * ldK[i] : 0.0f;
         return;
     }
     const f3 xi = ld3(a.x[i]);
@@ -612,8 +619,18 @@ __global__ void __launch_bounds__(kTile) k_adj_st_point(AdjK a, float s, const f
     const float den = 1.0f - sf + sf * cf;
     const float ks = ((1.0f - sf) * kMC[i] + sf * cc) / den;
     const f3 la = s * ld3(lv[i]);
-    const float lks = c * dot3(la, ns);
-    lnc[i] = st4(adj_normalize_T(nc, sf * adj_normalize_T(u, (c * ks) * la)));
+    const float lan = dot3(la, ns), l = ldK ? ldK[i] : 0.0f;
+    float E, lks;
+    if (dK0) {
+        E = A * dK0[i] + (P - A) * ks;
+        lks = c * (P - A) * lan + (1.0f - aP) * l;
+        if (ldK) ldK[i] = c * A * lan + aP * l;
+    } else {
+        E = P * ks;
+        lks = c * P * lan + l;
+        if (ldK) ldK[i] = 0.0f;
+    }
+    lnc[i] = st4(adj_normalize_T(nc, sf * adj_normalize_T(u, (c * E) * la)));
     lcw[i] = make_float2(lks * sf / den, -lks * ks * sf / den);  // cotangents of cc and cf
 }
 // Pair part: lx_i -= sum_j (lw_ij + lw_ji) (x_i - x_j) / (R |x_i - x_j|) over pairs with both nMC != 0,
@@ -761,7 +778,17 @@ __global__ void k_adj_copy3(uint32_t n, const float* __restrict__ a, float* da, 
     }
 }
 
-// cotangents between the step's slot order and the original particle order
+// cotangents between the step's slot order and the original particle order (three floats per particle, or one)
+__global__ void k_adj_to_slots1(uint32_t n, const uint32_t* __restrict__ id, const float* __restrict__ orig,
+                                float* slot) {
+    const uint32_t i = pid();
+    if (i < n) slot[i] = orig[id[i]];
+}
+__global__ void k_adj_to_orig1(uint32_t n, const uint32_t* __restrict__ id, const float* __restrict__ slot,
+                               float* orig) {
+    const uint32_t i = pid();
+    if (i < n) orig[id[i]] = slot[i];
+}
 __global__ void k_adj_to_slots(uint32_t n, const uint32_t* __restrict__ id, const float* __restrict__ orig,
                                float4* slot) {
     const uint32_t i = pid();
@@ -872,6 +899,12 @@ void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const fl
 void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const float4* slot, float* orig) {
     hipLaunchKernelGGL(k_adj_to_orig, grid_of(n), dim3(kTile), 0, s, n, id, slot, orig);
 }
+void adj_launch_to_slots1(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float* slot) {
+    hipLaunchKernelGGL(k_adj_to_slots1, grid_of(n), dim3(kTile), 0, s, n, id, orig, slot);
+}
+void adj_launch_to_orig1(hipStream_t s, uint32_t n, const uint32_t* id, const float* slot, float* orig) {
+    hipLaunchKernelGGL(k_adj_to_orig1, grid_of(n), dim3(kTile), 0, s, n, id, slot, orig);
+}
 
 AdjK AdjKHost::dev() const {
     return AdjK{tab, R,   invStep, K,  L,      V,   rho0, m, mu, R2, n, cap, count, list, x, con, bodies, muB, td,
@@ -879,8 +912,15 @@ AdjK AdjKHost::dev() const {
 }
 
 void adj_launch_surface_tension(hipStream_t s, const AdjKHost& h, float scale, const float4* lv, const float4* nMC,
-                                const float* kMC, float sf, float c, float4* lnc, float2* lcw, float4* lx) {
-    hipLaunchKernelGGL(k_adj_st_point, grid_of(h.n), dim3(kTile), 0, s, h.dev(), scale, lv, nMC, kMC, sf, c, lnc, lcw);
+                                const float* kMC, float sf, float c, uint32_t passes, const float* dK0, float* ldK,
+                                float4* lnc, float2* lcw, float4* lx) {
+    float A = 0.0f, aP = 1.0f;
+    for (uint32_t q = 0; q < passes; ++q) {
+        aP *= 0.95f;
+        A += aP;
+    }
+    hipLaunchKernelGGL(k_adj_st_point, grid_of(h.n), dim3(kTile), 0, s, h.dev(), scale, lv, nMC, kMC, sf, c,
+                       (float)passes, A, aP, dK0, ldK, lnc, lcw);
     hipLaunchKernelGGL(k_adj_st_pairs, grid_of(h.n), dim3(kTile), 0, s, h.dev(), nMC, kMC, lnc, lcw, lx);
 }
 void adj_launch_push(hipStream_t s, uint32_t n, const float4* push, float4* lx, float4* lv) {

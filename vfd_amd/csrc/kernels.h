@@ -89,11 +89,16 @@ void adj_launch_copy3(hipStream_t s, uint32_t n, const float* a, float* da, cons
                       const float* c, float* dc);
 void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float4* slot);
 void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const float4* slot, float* orig);
+void adj_launch_to_slots1(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float* slot);
+void adj_launch_to_orig1(hipStream_t s, uint32_t n, const uint32_t* id, const float* slot, float* orig);
 void adj_launch_push(hipStream_t s, uint32_t n, const float4* push, float4* lx, float4* lv);
-// position cotangent of the step's surface tension acceleration against scale * lv (nMC, kMC held); lnc, lcw are
-// n-element scratch
+// position cotangent of the step's surface tension acceleration -(sigma / m) E nS (c = -sigma / m) against
+// scale * lv over `passes` blend passes (nMC, kMC held); with temporal smoothing dK0 is the blended curvature the
+// particles carried into the step (null: none); ldK (null: not tracked) maps the carried curvature's cotangent from
+// the step's output to its input; lnc, lcw are n-element scratch
 void adj_launch_surface_tension(hipStream_t s, const AdjKHost& h, float scale, const float4* lv, const float4* nMC,
-                                const float* kMC, float sf, float c, float4* lnc, float2* lcw, float4* lx);
+                                const float* kMC, float sf, float c, uint32_t passes, const float* dK0, float* ldK,
+                                float4* lnc, float2* lcw, float4* lx);
 struct MapDev;
 // boundary.hip: replay of the boundary pass of scene 0 from its starting positions, recording contact samples
 // (con: kContactRecs * bodies * n records) and push-outs (push: 3 * n) with their derivatives; flags[3] (contact, push, out of

@@ -472,7 +472,7 @@ public:
         DevBuf<uint32_t> flags;
         // surface tension: Monte-Carlo normals and curvatures of the step, blend passes, temporal smoothing
         DevBuf<float4> nMC;
-        DevBuf<float> kMC;
+        DevBuf<float> kMC, dK0;  // dK0: the blended curvature carried into the step (temporal smoothing)
         uint32_t stPasses = 0;
         bool st = false, stTemporal = false;
         std::deque<DevBuf<float>> kv, rv, kp, rp;  // per Jacobi iteration: kappa^k, residual^k
@@ -1318,6 +1318,8 @@ public:
         VFD_HIP(hipEventRecord(ev[3], stream));
         if (last == VFD_PHASE_DIVERGENCE || last == VFD_PHASE_CLEAR_ACCELERATION) return finish_partial();
         if (anyST) {
+            if (tp && sc[0]->desc.EnableSurfaceTensionSolver && sc[0]->desc.TemporalSmoothing)
+                VFD_TRY(tape_copy(tp->dK0, dk[cur].p, tp->n));
             VFD_TRY(upload_mc_tables(F));
             VFD_TRY(timed("surface_tension", [&] { launch_surface_tension(L, F); }));
         }
@@ -1439,7 +1441,7 @@ public:
     // ---- adjoint: reverse of the recorded steps (kernels in adjoint.hip) ----------------------------------------
     DevBuf<float4> aLx, aLv, aLa, aTmp4, aGs, aLy, aLb, bR, bRh, bP, bV, bS, bT, aLnc;
     DevBuf<float2> aLcw;
-    DevBuf<float> aLk, aLsrc, aLs, aLf, aLrho, aLalpha, aLd, aLS, aOrigX, aOrigV;
+    DevBuf<float> aLk, aLsrc, aLs, aLf, aLrho, aLalpha, aLd, aLS, aOrigX, aOrigV, aOrigK, aLdK;
     DevBuf<double> aPart;
     // halos of the step being reversed, rebuilt from its recorded lists for the group kernels
     DevBuf<uint32_t> aHaloCount, aHaloIds, aNbrLocal, aHaloStats;
@@ -1494,8 +1496,9 @@ public:
         return VFD_OK;
     }
     // reverse of one recorded step: aLx, aLv hold the cotangents of its output (x', v') in its slot order on entry
-    // and those of its input (x, v) on return; lmu accumulates d/dmu
-    int adj_step(const DevTape& t, const uint32_t flags[3], double& lmu) {
+    // and those of its input (x, v) on return, and so does ldK (null: not tracked) for the blended curvature the
+    // surface tension's temporal smoothing carries; lmu accumulates d/dmu
+    int adj_step(const DevTape& t, const uint32_t flags[3], double& lmu, float* ldK) {
         const uint32_t n = t.n;
         const SceneParams& pr = sc[0]->prm;
         AdjKHost A{dTab.p, kt.R, kt.invStep, kt.K, kt.L, pr.V, pr.rho0, pr.m, pr.mu, kt.R2, n, cap,
@@ -1548,8 +1551,8 @@ public:
         if (t.st && t.stPasses) {
             VFD_TRY(aLnc.alloc(n));
             VFD_TRY(aLcw.alloc(n));
-            adj_launch_surface_tension(stream, A, H, aLv.p, t.nMC.p, t.kMC.p, pr.smoothing,
-                                       -(float)t.stPasses * pr.sigma * pr.mInv, aLnc.p, aLcw.p, aLx.p);
+            adj_launch_surface_tension(stream, A, H, aLv.p, t.nMC.p, t.kMC.p, pr.smoothing, -pr.sigma * pr.mInv,
+                                       t.stPasses, t.stTemporal ? t.dK0.p : nullptr, ldK, aLnc.p, aLcw.p, aLx.p);
         }
         // v2 = v1 + H g + (H / h)(y - v1)
         if (t.visc) {
@@ -1596,9 +1599,9 @@ public:
             VFD_HIP(hipMemcpyAsync(&flags[3 * k], tapes[k]->flags.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                    stream));
         VFD_HIP(hipStreamSynchronize(stream));
+        bool carry = false;  // some step's temporal smoothing reads the curvature carried from the step before
         for (size_t k = 0; k < tapeCount; ++k) {
-            if (tapes[k]->st && tapes[k]->stTemporal)
-                tapeError = "temporal smoothing of the surface tension is outside the adjoint's scope";
+            carry = carry || (tapes[k]->st && tapes[k]->stTemporal);
             if (flags[3 * k + 2])
                 tapeError = "a particle pushed out twice or after a contact sample is outside the adjoint's scope";
         }
@@ -1613,14 +1616,21 @@ public:
             VFD_HIP(hipMemcpyAsync(aOrigV.p, lvOut, (size_t)3 * n * sizeof(float), hipMemcpyHostToDevice, stream));
         else
             VFD_HIP(hipMemsetAsync(aOrigV.p, 0, (size_t)3 * n * sizeof(float), stream));
+        if (carry) {  // the loss reads x and v: the carried curvature after the last step has cotangent 0
+            VFD_TRY(aOrigK.alloc(n));
+            VFD_TRY(aLdK.alloc(n));
+            adj_launch_fill(stream, aOrigK.p, 0.0f, n);
+        }
         double m = 0;
         for (size_t k = tapeCount; k-- > 0;) {
             const DevTape& t = *tapes[k];
             adj_launch_to_slots(stream, n, t.id.p, aOrigX.p, aLx.p);
             adj_launch_to_slots(stream, n, t.id.p, aOrigV.p, aLv.p);
-            VFD_TRY(adj_step(t, &flags[3 * k], m));
+            if (carry) adj_launch_to_slots1(stream, n, t.id.p, aOrigK.p, aLdK.p);
+            VFD_TRY(adj_step(t, &flags[3 * k], m, carry ? aLdK.p : nullptr));
             adj_launch_to_orig(stream, n, t.id.p, aLx.p, aOrigX.p);
             adj_launch_to_orig(stream, n, t.id.p, aLv.p, aOrigV.p);
+            if (carry) adj_launch_to_orig1(stream, n, t.id.p, aLdK.p, aOrigK.p);
         }
         VFD_HIP(hipMemcpyAsync(lxIn, aOrigX.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, stream));
         VFD_HIP(hipMemcpyAsync(lvIn, aOrigV.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, stream));
