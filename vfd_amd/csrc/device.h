@@ -83,10 +83,30 @@ __device__ __forceinline__ float getW_scalar(const KTab& k, float r) {
 }
 
 // ---- wave / block reductions (deterministic order) -------------------------------------------------------------
+// A 64-bit lane value moved by the DPP pattern kCtrl (two 32-bit moves); lanes of rows outside kRowMask read 0.
+template <int kCtrl, int kRowMask = 0xF>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, kCtrl, kRowMask, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), kCtrl, kRowMask, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+// Sum over the wave's 64 lanes through DPP lane moves (no LDS), the same value in every lane; called with every lane
+// active. Within each row of 16 lanes: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror (each
+// lane then holds its row's sum: the pairs added are the same in both lanes, so every lane has the same bits); then
+// row_bcast:15 adds row 0 into row 1 and row 2 into row 3, row_bcast:31 adds rows 0-1 into row 3, and lane 63 holds
+// the wave's sum. A fixed tree, so the per-tile partial sums are deterministic.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    v += dpp_f64<0xB1>(v);        // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);        // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);       // row_half_mirror
+    v += dpp_f64<0x140>(v);       // row_mirror
+    v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, kWave - 1);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), kWave - 1);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
