@@ -3,7 +3,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config dam_break_2m] [--no-cpu-baseline]
     python bench.py --copy-probe                      # plain 512 MiB device copies, for counter calibration
-    python bench.py --pmc-summary FETCH WRITE COPY_FETCH COPY_WRITE   # rocprofv3 --pmc output dirs -> traffic table
+    python bench.py --pmc-summary FETCH WRITE COPY_FETCH COPY_WRITE [TRACE]   # rocprofv3 output dirs -> traffic table
     python bench.py --trace-summary DIR               # rocprofv3 --kernel-trace output -> per-kernel means
     python bench.py --counter-summary LDS_DIR VALU_DIR  # rocprofv3 --pmc LDS / instruction counters -> table
 
@@ -301,9 +301,18 @@ def trace_summary(path):
     return "\n".join(lines)
 
 
-def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, trace_dir=None):
     """Markdown table of HBM traffic per launch. FETCH_SIZE/WRITE_SIZE (KiB per dispatch) are scaled by what the
-    same counters read for a device-to-device copy of COPY_PROBE_BYTES (`--copy-probe`)."""
+    same counters read for a device-to-device copy of COPY_PROBE_BYTES (`--copy-probe`). With a --kernel-trace output
+    of the same command, each kernel's measured traffic over its mean duration there (live dispatches of the
+    solver-loop kernels) gives the HBM bandwidth it achieves and its share of the MI355X's 8 TB/s peak."""
+    dur = {}
+    if trace_dir:
+        acc = {}
+        for k, _, us in _trace_rows(trace_dir):
+            if us >= LIVE_US or k not in LOOP_KERNELS:
+                acc.setdefault(k, []).append(us)
+        dur = {k: sum(v) / len(v) for k, v in acc.items()}
     cf, cw = _pmc_table(copy_fetch_dir), _pmc_table(copy_write_dir)
     copy_k = [k for k in cf if "copyBuffer" in k][0]
     f_scale = COPY_PROBE_BYTES / (cf[copy_k]["FETCH_SIZE"] * 1024.0)
@@ -312,11 +321,14 @@ def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
     lines = ["calibration: %s of %d bytes reads FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB -> scale fetch x%.4f, "
              "write x%.4f" % (copy_k, COPY_PROBE_BYTES, cf[copy_k]["FETCH_SIZE"], cw[copy_k]["WRITE_SIZE"], f_scale,
                               w_scale), "",
-             "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes per launch (scaled) |", "|---|---:|---:|---:|"]
+             "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes per launch (scaled) | mean us | HBM GB/s | "
+             "of peak |", "|---|---:|---:|---:|---:|---:|---:|"]
     for k in sorted(fe):
         if k in wr and k.startswith("k_"):
             t = fe[k]["FETCH_SIZE"] * 1024.0 * f_scale + wr[k]["WRITE_SIZE"] * 1024.0 * w_scale
-            lines.append("| %s | %.1f | %.1f | %d |" % (k[2:], fe[k]["FETCH_SIZE"], wr[k]["WRITE_SIZE"], t))
+            us = dur.get(k)
+            rate = "| %.1f | %.0f | %.3f |" % (us, t / (us * 1e3), t / (us * 1e3) / HBM_PEAK_GBS) if us else "| - | - | - |"
+            lines.append("| %s | %.1f | %.1f | %d %s" % (k[2:], fe[k]["FETCH_SIZE"], wr[k]["WRITE_SIZE"], t, rate))
     return "\n".join(lines)
 
 
@@ -342,6 +354,26 @@ def counter_summary(lds_dir, valu_dir, kernels=("div_iter", "div_accel", "pres_i
             100.0 * a["SQ_LDS_IDX_ACTIVE"] / (SIMDS / 4) / max(dur, 1.0), a["SQ_INSTS_LDS"], a["SQ_WAIT_INST_LDS"],
             b["SQ_INSTS_VALU"], 100.0 * 4.0 * b["SQ_INSTS_VALU"] / SIMDS / max(dur, 1.0), b["SQ_INSTS_VMEM_RD"],
             b["SQ_WAIT_INST_ANY"]))
+    return "\n".join(lines)
+
+
+CUS = 256  # MI355X compute units; the TA and TD counters' _sum adds their per-CU busy cycles
+
+
+def memory_unit_summary(path, kernels=("build_lists", "div_iter", "div_accel", "cg_matvec", "build_halo")):
+    """Markdown table of the vector memory units from one rocprofv3 --pmc run of TA_TA_BUSY_sum TD_TD_BUSY_sum
+    SQ_BUSY_CYCLES SQ_INSTS_VMEM_RD: per CU busy share = counter / CUS over the duration SQ_BUSY_CYCLES / shader
+    engines (the texture addresser TA takes the load addresses, the texture data unit TD returns the data)."""
+    t = _pmc_table(path)
+    lines = ["| kernel | duration cycles | TA busy | TD busy | VMEM reads |", "|---|---:|---:|---:|---:|"]
+    for k in kernels:
+        a = t.get("k_" + k)
+        if not a:
+            continue
+        dur = a["SQ_BUSY_CYCLES"] / SHADER_ENGINES
+        lines.append("| %s | %.0f | %.0f%% | %.0f%% | %.3g |" % (
+            k, dur, 100.0 * a["TA_TA_BUSY_sum"] / CUS / max(dur, 1.0), 100.0 * a["TD_TD_BUSY_sum"] / CUS / max(dur, 1.0),
+            a["SQ_INSTS_VMEM_RD"]))
     return "\n".join(lines)
 
 
@@ -647,8 +679,10 @@ def main():
     ap.add_argument("--no-lattices", action="store_true",
                     help="skip the 2M dam break on the cubic and MaxDensity lattices (lattice_variants)")
     ap.add_argument("--mesh-only", action="store_true", help="only measure the mesh inputs")
-    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"))
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
+                    help="FETCH WRITE COPY_FETCH COPY_WRITE [KERNEL_TRACE]: rocprofv3 output dirs -> traffic table")
     ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_DIR")
+    ap.add_argument("--memory-unit-summary", metavar="TA_TD_COUNTERS_DIR")
     ap.add_argument("--counter-summary", nargs=2, metavar=("LDS_COUNTERS_DIR", "VALU_COUNTERS_DIR"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -660,6 +694,9 @@ def main():
         return
     if args.trace_summary:
         print(trace_summary(args.trace_summary))
+        return
+    if args.memory_unit_summary:
+        print(memory_unit_summary(args.memory_unit_summary))
         return
     if args.counter_summary:
         print(counter_summary(*args.counter_summary))
