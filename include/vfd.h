@@ -406,12 +406,16 @@ int vfd_get_profile(vfd_sim* sim, vfd_kernel_stat* out, uint32_t capacity, uint3
  * volume map (their branches held as recorded); surface tension through its smoothing pass, with the Monte-Carlo
  * normals and curvatures held, and with temporal smoothing through the blended curvature carried from step to step
  * (the one carried into the first recorded step is held; the loss reads positions and velocities only). d/dmu is the
- * fluid viscosity's (BoundaryViscosity is held). Scope: one scene, no particle pushed out twice in a step or after
- * sampling another body, no particle at the neighbour cap (else VFD_ERROR_NOT_SUPPORTED, from vfd_adjoint).
- * vfd_set_fluid discards the records; their device buffers are kept for the next recording until vfd_destroy. */
+ * fluid viscosity's (BoundaryViscosity is held). Scope: no particle pushed out twice in a step or after sampling
+ * another body, no particle at the neighbour cap (else VFD_ERROR_NOT_SUPPORTED, from vfd_adjoint). vfd_set_fluid
+ * discards the records; their device buffers are kept for the next recording until vfd_destroy. In a batch every
+ * scene is recorded; vfd_adjoint reverses scene 0 and vfd_batch_adjoint scene `scene` (its own time steps, solver
+ * iteration counts, viscosity, bodies and surface tension; cotangents of that scene's particles only, its d/dmu). */
 int vfd_tape(vfd_sim* sim, int32_t on);
 int vfd_tape_steps(vfd_sim* sim, uint32_t* steps);
 int vfd_adjoint(vfd_sim* sim, const float* lx_out, const float* lv_out, float* lx_in, float* lv_in, double* lmu);
+int vfd_batch_adjoint(vfd_sim* sim, uint32_t scene, const float* lx_out, const float* lv_out, float* lx_in,
+                      float* lv_in, double* lmu);
 
 /* Launch and solver-control knobs; no counterpart in the reference (which synchronises after every iteration).
  * Results do not depend on them: a converged scene's extra iterations are skipped on the device, and a staged halo

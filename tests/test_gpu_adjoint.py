@@ -219,6 +219,47 @@ def test_adjoint_group_loops_give_the_same_bits(scene):
         assert gmu == ref[2], (name, gmu, ref[2])
 
 
+def test_batch_adjoint_matches_each_scene_alone():
+    """Three scenes recorded in one batch and reversed one at a time (vfd_adjoint for scene 0, vfd_batch_adjoint for
+    the others): a free viscous block; a block landing on a floor with boundary viscosity and a pushed particle; a
+    block with surface tension and temporal smoothing and its own viscosity. Each scene has its own bodies, switches
+    and iteration counts; its cotangents and d/dmu equal those of the scene recorded and reversed alone, bit for bit
+    (the single-scene adjoint is the one compared with the oracle above)."""
+    from vfd_amd import DFSPHBatch
+    d0, x0, v0 = _setup((8, 8, 8))
+    d1, x1, v1, b1 = contact_scene("floor")
+    d1.BoundaryViscosity = 0.02
+    d2, x2, v2 = st_scene()
+    d2.TemporalSmoothing = 1
+    d2.Viscosity = 0.05
+    parts = [(d0, x0, v0, []), (d1, x1, v1, b1), (d2, x2, v2, [])]
+    batch = DFSPHBatch([p[0] for p in parts])
+    for s, (d, x, v, b) in enumerate(parts):
+        batch.scenes[s].set_fluid(x.astype(np.float32), v.astype(np.float32))
+        batch.scenes[s].SetRigidBodies(b)
+    batch.tape(True)
+    for _ in range(STEPS):
+        batch.step()
+    assert batch.taped_steps() == STEPS
+    rng = np.random.default_rng(13)
+    for s, (d, x, v, b) in enumerate(parts):
+        n = len(x)
+        wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+        print("batch adjoint, scene", s, n, "particles", flush=True)
+        gx, gv, gmu = batch.adjoint(s, wx, wv)
+        one = DFSPHSimulation(d)
+        one.set_fluid(x.astype(np.float32), v.astype(np.float32))
+        one.SetRigidBodies(b)
+        one.tape(True)
+        for _ in range(STEPS):
+            one.OnUpdate()
+        assert np.array_equal(one.field("position"), batch.scenes[s].field("position")), s
+        ox, ov, omu = one.adjoint(wx, wv)
+        assert np.isfinite(gx).all() and np.abs(gx).max() > 0, s
+        assert np.array_equal(gx, ox) and np.array_equal(gv, ov), s
+        assert gmu == omu, (s, gmu, omu)
+
+
 def test_checkpointed_rollout_matches_oracle_fp64():
     """torch.autograd through Rollout (60 steps, checkpoints every 10) on the 2,048-particle falling viscous block
     against the oracle's FP64 reference adjoint of the same 60 steps taken in one piece; the loss is the 64^3

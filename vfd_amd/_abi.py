@@ -246,6 +246,7 @@ _PROTOS = {
     "vfd_tape": (ctypes.c_int, [_P, _I32]),
     "vfd_tape_steps": (ctypes.c_int, [_P, ctypes.POINTER(_U32)]),
     "vfd_adjoint": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
+    "vfd_batch_adjoint": (ctypes.c_int, [_P, _U32, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double)]),
     "vfd_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "vfd_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
 }

@@ -26,11 +26,12 @@ namespace vfd {
 struct AdjK {
     const float2* __restrict__ tab;
     float R, invStep, K, L, V, rho0, m, mu, R2;
-    uint32_t n, cap;
+    uint32_t n, cap;                    // particles of the scene being reversed, list capacity
+    uint32_t base, stride;              // the scene's first slot; slots per record of the step (contact records)
     const uint32_t* __restrict__ count;
     const uint32_t* __restrict__ list;  // [j * cap + i]
     const float4* __restrict__ x;
-    const float4* __restrict__ con;  // contact records [(kContactRecs b + q) n + i] (k_adj_boundary_tape)
+    const float4* __restrict__ con;  // contact records [(kContactRecs b + q) stride + i] (k_adj_boundary_tape)
     uint32_t bodies;                 // 0: no contact records
     float muB, td;                   // boundary viscosity (0: its samples are not differentiated) and sample distance
     float invR, dG0, LR;             // 1 / R, 3 L / R^3 (d(gradW/r)/drho for q <= 1/2), L / R
@@ -78,21 +79,27 @@ __device__ __forceinline__ uint32_t adj_nbr(const AdjK& a, uint32_t i, uint32_t 
     return a.list[(size_t)j * a.cap + i];
 }
 __device__ __forceinline__ uint32_t pid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+// The neighbour-loop kernels run over the scene's slots [base, base + n) (one thread each) and index every array by
+// slot, as the lists do; the pointwise kernels take arrays offset to the scene's first slot.
+__device__ __forceinline__ bool adj_slot(const AdjK& a, uint32_t& i) {
+    i = a.base + pid();
+    return pid() < a.n;
+}
 // Boundary sample records of particle i, body b: {r = x - xb, vb}, {grad vb}, rows of dr/dx.
 struct AdjContact {
     f3 r, dv, j0, j1, j2;
     float vb;
 };
 __device__ __forceinline__ bool adj_contact(const AdjK& a, uint32_t b, uint32_t i, AdjContact& c) {
-    const float4* p = a.con + (size_t)kContactRecs * b * a.n + i;
+    const float4* p = a.con + (size_t)kContactRecs * b * a.stride + i;
     const float4 r = p[0];
     if (!(r.w > 0.0f)) return false;
     c.r = mk3(r.x, r.y, r.z);
     c.vb = r.w;
-    c.dv = ld3(p[a.n]);
-    c.j0 = ld3(p[2 * (size_t)a.n]);
-    c.j1 = ld3(p[3 * (size_t)a.n]);
-    c.j2 = ld3(p[4 * (size_t)a.n]);
+    c.dv = ld3(p[a.stride]);
+    c.j0 = ld3(p[2 * (size_t)a.stride]);
+    c.j1 = ld3(p[3 * (size_t)a.stride]);
+    c.j2 = ld3(p[4 * (size_t)a.stride]);
     return true;
 }
 // sum over bodies of vb gradW(r): the boundary part of particle i's kernel-gradient sum
@@ -182,8 +189,8 @@ __device__ __forceinline__ f3 adj_visc_boundary_vjp(const AdjK& a, uint32_t i, f
         if (!adj_contact(a, b, i, c)) continue;
         AdjTan t;
         if (!adj_tan(c.r, t)) continue;
-        const float4* p = a.con + (size_t)kContactRecs * b * a.n + i;
-        const f3 n0 = ld3(p[5 * (size_t)a.n]), n1 = ld3(p[6 * (size_t)a.n]), n2 = ld3(p[7 * (size_t)a.n]);
+        const float4* p = a.con + (size_t)kContactRecs * b * a.stride + i;
+        const f3 n0 = ld3(p[5 * (size_t)a.stride]), n1 = ld3(p[6 * (size_t)a.stride]), n2 = ld3(p[7 * (size_t)a.stride]);
         const float vol = 0.25f * c.vb;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -245,8 +252,8 @@ struct AdjAccelT {
 };
 __global__ void __launch_bounds__(kTile) k_adj_accel_T(AdjK a, const float* __restrict__ kap,
                                                        const float4* __restrict__ la, float* lk, float4* lx) {
-    const uint32_t i = pid();
-    if (i >= a.n) return;
+    uint32_t i;
+    if (!adj_slot(a, i)) return;
     AdjAccelT t(a, i, kap, la);
     const uint32_t cnt = a.count[i];
     for (uint32_t j0 = 0; j0 < cnt; j0 += kAdjBatch) {
@@ -296,8 +303,8 @@ struct AdjDivT {
 };
 __global__ void __launch_bounds__(kTile) k_adj_div_T(AdjK a, const float* __restrict__ lf,
                                                      const float4* __restrict__ u, float4* lu, float4* lx) {
-    const uint32_t i = pid();
-    if (i >= a.n) return;
+    uint32_t i;
+    if (!adj_slot(a, i)) return;
     AdjDivT t(a, i, lf, u);
     const uint32_t cnt = a.count[i];
     for (uint32_t j0 = 0; j0 < cnt; j0 += kAdjBatch) {
@@ -341,7 +348,7 @@ __device__ __forceinline__ void adj_group_loop(const AdjK& a, const AdjStage& g,
     const NbrSrc global{nullptr, g.local, a.list, a.count, a.cap};
     group_loop<2>(g.tg, g.H, rec, global, load, [&](uint32_t tile, bool ok, const NbrSrc& src) {
         const uint32_t i = tile * kTile + (threadIdx.x & (kTile - 1));
-        if (!ok || i >= a.n) return;
+        if (!ok || i - a.base >= a.n) return;  // (unsigned: slots below base wrap past n)
         auto t = make(i);
         each_neighbor_rec<2>(src, i, a.R, a.invStep, t.xi, sG, load,
                              [&](float4 lo, float4 hi, float gv) { t.term(a, ld3(lo), ld3(hi), lo.w, gv); });
@@ -453,8 +460,8 @@ __global__ void __launch_bounds__(kTile) k_adj_visc_T(AdjK a, float h, const flo
                                                       const float4* __restrict__ w, float4* out,
                                                       const BicgState* __restrict__ st) {
     if (st && st->done) return;  // a device-side solve that has converged (block-uniform)
-    const uint32_t i = pid();
-    if (i >= a.n) return;
+    uint32_t i;
+    if (!adj_slot(a, i)) return;
     AdjViscT t(a, i, h, rho, w);
     const uint32_t cnt = a.count[i];
     for (uint32_t j0 = 0; j0 < cnt; j0 += kAdjBatch) {
@@ -490,9 +497,9 @@ __global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_adj_visc_T_groups(
 __global__ void __launch_bounds__(kTile) k_adj_visc_vjp(AdjK a, float h, const float* __restrict__ rho,
                                                         const float4* __restrict__ y, const float4* __restrict__ lb,
                                                         float4* lx, float* lrho, double* muPart) {
-    const uint32_t i = pid();
+    uint32_t i;
     double part = 0.0;
-    if (i < a.n) {
+    if (adj_slot(a, i)) {
         const f3 xi = ld3(a.x[i]), yi = ld3(y[i]), lbi = ld3(lb[i]);
         const float e = 0.01f * a.R2;
         f3 gx = mk3(0.0f, 0.0f, 0.0f);
@@ -531,8 +538,8 @@ __global__ void __launch_bounds__(kTile) k_adj_visc_vjp(AdjK a, float h, const f
 
 // factor: S_i = V^2 sum_j |gradW_ij|^2 + |gs_i|^2, gs_i = V sum_j gradW_ij + B_i, alpha = 1/S
 __global__ void __launch_bounds__(kTile) k_adj_gsum(AdjK a, float4* gs) {
-    const uint32_t i = pid();
-    if (i >= a.n) return;
+    uint32_t i;
+    if (!adj_slot(a, i)) return;
     const f3 xi = ld3(a.x[i]);
     f3 s = mk3(0.0f, 0.0f, 0.0f);
     for (uint32_t jj = 0; jj < a.count[i]; ++jj) {
@@ -554,8 +561,8 @@ __global__ void __launch_bounds__(kTile) k_adj_factor_lS(uint32_t n, const float
 __global__ void __launch_bounds__(kTile) k_adj_factor_density(AdjK a, const float* __restrict__ lS,
                                                               const float4* __restrict__ gs,
                                                               const float* __restrict__ lrho, float4* lx) {
-    const uint32_t i = pid();
-    if (i >= a.n) return;
+    uint32_t i;
+    if (!adj_slot(a, i)) return;
     const f3 xi = ld3(a.x[i]), gsi = ld3(gs[i]);
     const float lSi = lS[i], lri = lrho[i];
     f3 gx = mk3(0.0f, 0.0f, 0.0f);
@@ -592,8 +599,8 @@ __global__ void __launch_bounds__(kTile) k_adj_st_point(AdjK a, float s, const f
                                                         float sf, float c, float P, float A, float aP,
                                                         const float* __restrict__ dK0, float* ldK, float4* lnc,
                                                         float2* lcw) {
-    const uint32_t i = pid();
-    if (i >= a.n) return;
+    uint32_t i;
+    if (!adj_slot(a, i)) return;
     const f3 ni = ld3(nMC[i]);
     if (is_zero(ni)) {
         lnc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -638,8 +645,8 @@ __global__ void __launch_bounds__(kTile) k_adj_st_point(AdjK a, float s, const f
 __global__ void __launch_bounds__(kTile) k_adj_st_pairs(AdjK a, const float4* __restrict__ nMC,
                                                         const float* __restrict__ kMC, const float4* __restrict__ lnc,
                                                         const float2* __restrict__ lcw, float4* lx) {
-    const uint32_t i = pid();
-    if (i >= a.n) return;
+    uint32_t i;
+    if (!adj_slot(a, i)) return;
     const f3 ni = ld3(nMC[i]);
     if (is_zero(ni)) return;
     const f3 xi = ld3(a.x[i]), li = ld3(lnc[i]);
@@ -660,15 +667,16 @@ __global__ void __launch_bounds__(kTile) k_adj_st_pairs(AdjK a, const float4* __
     lx[i] = st4(ld3(lx[i]) + gx);
 }
 
-// push-out reversed: x = x_before + delta n(x_before), v = 0  ->  lx = J^T lx, lv = 0 (rows of J in push[q n + i])
-__global__ void __launch_bounds__(kTile) k_adj_push(uint32_t n, const float4* __restrict__ push, float4* lx,
-                                                    float4* lv) {
+// push-out reversed: x = x_before + delta n(x_before), v = 0  ->  lx = J^T lx, lv = 0 (rows of J in
+// push[q stride + i])
+__global__ void __launch_bounds__(kTile) k_adj_push(uint32_t n, uint32_t stride, const float4* __restrict__ push,
+                                                    float4* lx, float4* lv) {
     const uint32_t i = pid();
     if (i >= n) return;
     const float4 r0 = push[i];
     if (!(r0.w > 0.0f)) return;
     const f3 l = ld3(lx[i]);
-    lx[i] = st4(l.x * mk3(r0.x, r0.y, r0.z) + l.y * ld3(push[n + i]) + l.z * ld3(push[2 * (size_t)n + i]));
+    lx[i] = st4(l.x * mk3(r0.x, r0.y, r0.z) + l.y * ld3(push[stride + i]) + l.z * ld3(push[2 * (size_t)stride + i]));
     lv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -879,7 +887,8 @@ void adj_launch_visc_vjp(hipStream_t s, const AdjKHost& h, float dt, const float
 void adj_launch_factor_density(hipStream_t s, const AdjKHost& h, const float* alpha, const float* lalpha, float* lS,
                                float4* gs, const float* lrho, float4* lx) {
     hipLaunchKernelGGL(k_adj_gsum, grid_of(h.n), dim3(kTile), 0, s, h.dev(), gs);
-    hipLaunchKernelGGL(k_adj_factor_lS, grid_of(h.n), dim3(kTile), 0, s, h.n, alpha, lalpha, lS);
+    hipLaunchKernelGGL(k_adj_factor_lS, grid_of(h.n), dim3(kTile), 0, s, h.n, alpha + h.base, lalpha + h.base,
+                       lS + h.base);
     hipLaunchKernelGGL(k_adj_factor_density, grid_of(h.n), dim3(kTile), 0, s, h.dev(), lS, gs, lrho, lx);
 }
 void adj_launch_dot(hipStream_t s, uint32_t n, const float4* p, const float4* q, double* part) {
@@ -907,8 +916,9 @@ void adj_launch_to_orig1(hipStream_t s, uint32_t n, const uint32_t* id, const fl
 }
 
 AdjK AdjKHost::dev() const {
-    return AdjK{tab, R,   invStep, K,  L,      V,   rho0, m, mu, R2, n, cap, count, list, x, con, bodies, muB, td,
-                1.0f / R, 3.0f * L / (R * R * R), L / R};
+    return AdjK{tab,   R,    invStep, K,     L,    V,     rho0, m,   mu,   R2,     n,   cap,
+                base,  stride ? stride : n,   count, list, x,     con,  bodies, muB, td, 1.0f / R,
+                3.0f * L / (R * R * R), L / R};
 }
 
 void adj_launch_surface_tension(hipStream_t s, const AdjKHost& h, float scale, const float4* lv, const float4* nMC,
@@ -923,26 +933,28 @@ void adj_launch_surface_tension(hipStream_t s, const AdjKHost& h, float scale, c
                        (float)passes, A, aP, dK0, ldK, lnc, lcw);
     hipLaunchKernelGGL(k_adj_st_pairs, grid_of(h.n), dim3(kTile), 0, s, h.dev(), nMC, kMC, lnc, lcw, lx);
 }
-void adj_launch_push(hipStream_t s, uint32_t n, const float4* push, float4* lx, float4* lv) {
-    if (n) hipLaunchKernelGGL(k_adj_push, grid_of(n), dim3(kTile), 0, s, n, push, lx, lv);
+void adj_launch_push(hipStream_t s, uint32_t n, uint32_t stride, const float4* push, float4* lx, float4* lv) {
+    if (n) hipLaunchKernelGGL(k_adj_push, grid_of(n), dim3(kTile), 0, s, n, stride, push, lx, lv);
 }
 
 void adj_launch_bicg_iteration(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const BicgVectors& v,
                                double* part, BicgState* st) {
-    const uint32_t n = h.n;
+    // the vectors are slot-indexed arrays: the operator takes them whole, the pointwise kernels from the scene's
+    // first slot
+    const uint32_t n = h.n, b = h.base;
     const uint32_t blocks = grid_of(n).x;
-    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.rh, v.r, nullptr, nullptr, part, st);
+    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.rh + b, v.r + b, nullptr, nullptr, part, st);
     hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(kTile), 0, s, part, blocks, st, (int)kBicgRho);
-    hipLaunchKernelGGL(k_bicg_p, grid_of(n), dim3(kTile), 0, s, n, v.r, v.v, v.p, st);
+    hipLaunchKernelGGL(k_bicg_p, grid_of(n), dim3(kTile), 0, s, n, v.r + b, v.v + b, v.p + b, st);
     adj_launch_visc_T(s, h, dt, rho, v.p, v.v, st);
-    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.rh, v.v, nullptr, nullptr, part, st);
+    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.rh + b, v.v + b, nullptr, nullptr, part, st);
     hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(kTile), 0, s, part, blocks, st, (int)kBicgRv);
-    hipLaunchKernelGGL(k_bicg_s, grid_of(n), dim3(kTile), 0, s, n, v.r, v.v, v.s, st);
+    hipLaunchKernelGGL(k_bicg_s, grid_of(n), dim3(kTile), 0, s, n, v.r + b, v.v + b, v.s + b, st);
     adj_launch_visc_T(s, h, dt, rho, v.s, v.t, st);
-    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.t, v.s, v.t, v.t, part, st);
+    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.t + b, v.s + b, v.t + b, v.t + b, part, st);
     hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(kTile), 0, s, part, blocks, st, (int)kBicgTsTt);
-    hipLaunchKernelGGL(k_bicg_xr, grid_of(n), dim3(kTile), 0, s, n, v.p, v.s, v.t, v.x, v.r, st);
-    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.r, v.r, nullptr, nullptr, part, st);
+    hipLaunchKernelGGL(k_bicg_xr, grid_of(n), dim3(kTile), 0, s, n, v.p + b, v.s + b, v.t + b, v.x + b, v.r + b, st);
+    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, v.r + b, v.r + b, nullptr, nullptr, part, st);
     hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(kTile), 0, s, part, blocks, st, (int)kBicgRr);
 }
 

@@ -683,15 +683,20 @@ __device__ __forceinline__ void normal_jacobian(f3 n, float gl, const float hs[6
 // = 1 for a pushed particle). flags[0] / flags[1]: some particle has a contact sample / was pushed; flags[2]: a
 // particle was pushed twice or after a contact sample, or the replay's position differs from the forward's.
 __global__ void __launch_bounds__(kTile) k_adj_boundary_tape(const SceneParams* __restrict__ P,
-                                                             const MapDev* __restrict__ maps, uint32_t n,
+                                                             const uint32_t* __restrict__ tileScene,
+                                                             const MapDev* __restrict__ mapsAll, uint32_t n,
                                                              uint32_t capacity, const float4* __restrict__ xpre,
                                                              const float4* __restrict__ posAfter,
                                                              const float* __restrict__ vbF,
                                                              const float4* __restrict__ xbF, float4* con,
-                                                             float4* push, uint32_t* flags) {
+                                                             float4* push, uint32_t* flagsAll) {
     const uint32_t i = blockIdx.x * kTile + threadIdx.x;
     if (i >= n) return;
-    const SceneParams& p = P[0];
+    const uint32_t scene = tileScene[blockIdx.x];
+    const SceneParams& p = P[scene];
+    if (i >= p.base + p.count) return;  // a slot past the scene's last particle
+    const MapDev* maps = mapsAll + (size_t)scene * kMaxBodies;
+    uint32_t* flags = flagsAll + 3 * (size_t)scene;
     f3 x = ld3(xpre[i]);
     bool sampled = false, pushed = false, bad = false;
     float Jp[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
@@ -778,12 +783,12 @@ __global__ void __launch_bounds__(kTile) k_adj_boundary_tape(const SceneParams* 
     if (bad) flags[2] = 1u;
 }
 
-void adj_launch_boundary_tape(hipStream_t s, const SceneParams* P, const MapDev* maps, uint32_t n, uint32_t capacity,
-                              const float4* xpre, const float4* posAfter, const float* vb, const float4* xb,
-                              float4* con, float4* push, uint32_t* flags) {
+void adj_launch_boundary_tape(hipStream_t s, const SceneParams* P, const uint32_t* tileScene, const MapDev* maps,
+                              uint32_t n, uint32_t capacity, const float4* xpre, const float4* posAfter, const float* vb,
+                              const float4* xb, float4* con, float4* push, uint32_t* flags) {
     if (n)
-        hipLaunchKernelGGL(k_adj_boundary_tape, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, P, maps, n,
-                           capacity, xpre, posAfter, vb, xb, con, push, flags);
+        hipLaunchKernelGGL(k_adj_boundary_tape, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, s, P, tileScene, maps,
+                           n, capacity, xpre, posAfter, vb, xb, con, push, flags);
 }
 
 }  // namespace vfd

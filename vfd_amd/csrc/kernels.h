@@ -29,8 +29,9 @@ struct Launch {
 // ---- adjoint.hip -----------------------------------------------------------------------------------------------
 struct AdjK;
 constexpr uint32_t kContactRecs = 8;  // float4 records per particle and body of a recorded contact sample
-// Geometry and constants of one recorded step for the adjoint kernels (spline constants K = 8/(pi R^3),
-// L = 48/(pi R^3)); arrays in the step's slot order.
+// Geometry and constants of one recorded step and one scene for the adjoint kernels (spline constants
+// K = 8/(pi R^3), L = 48/(pi R^3)); arrays in the step's slot order, the scene's n particles in slots
+// [base, base + n).
 struct AdjKHost {
     const float2* tab;
     float R, invStep, K, L, V, rho0, m, mu, R2;
@@ -41,6 +42,7 @@ struct AdjKHost {
     const float4* con = nullptr;  // boundary contact records of the step (adj_launch_boundary_tape); null: none
     uint32_t bodies = 0;
     float muB = 0.0f, td = 0.0f;  // boundary viscosity (dynamic) and tangential sample distance
+    uint32_t base = 0, stride = 0;  // the scene's first slot; slots per record of the step (contact records; 0: n)
     // group loops of the transposed Jacobi and viscosity kernels: the step's halos (launch_build_halo over its lists)
     // and the persistent grid; grid 0: one thread per particle with global gathers
     struct {
@@ -91,7 +93,7 @@ void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const fl
 void adj_launch_to_orig(hipStream_t s, uint32_t n, const uint32_t* id, const float4* slot, float* orig);
 void adj_launch_to_slots1(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float* slot);
 void adj_launch_to_orig1(hipStream_t s, uint32_t n, const uint32_t* id, const float* slot, float* orig);
-void adj_launch_push(hipStream_t s, uint32_t n, const float4* push, float4* lx, float4* lv);
+void adj_launch_push(hipStream_t s, uint32_t n, uint32_t stride, const float4* push, float4* lx, float4* lv);
 // position cotangent of the step's surface tension acceleration -(sigma / m) E nS (c = -sigma / m) against
 // scale * lv over `passes` blend passes (nMC, kMC held); with temporal smoothing dK0 is the blended curvature the
 // particles carried into the step (null: none); ldK (null: not tracked) maps the carried curvature's cotangent from
@@ -100,12 +102,13 @@ void adj_launch_surface_tension(hipStream_t s, const AdjKHost& h, float scale, c
                                 const float* kMC, float sf, float c, uint32_t passes, const float* dK0, float* ldK,
                                 float4* lnc, float2* lcw, float4* lx);
 struct MapDev;
-// boundary.hip: replay of the boundary pass of scene 0 from its starting positions, recording contact samples
-// (con: kContactRecs * bodies * n records) and push-outs (push: 3 * n) with their derivatives; flags[3] (contact, push, out of
-// scope or replay mismatch) are set to 1 where they occur and must be cleared beforehand
-void adj_launch_boundary_tape(hipStream_t s, const SceneParams* P, const MapDev* maps, uint32_t n, uint32_t capacity,
-                              const float4* xpre, const float4* posAfter, const float* vb, const float4* xb,
-                              float4* con, float4* push, uint32_t* flags);
+// boundary.hip: replay of the boundary pass of every scene from its starting positions over slots [0, n), recording
+// contact samples (con: kContactRecs * kMaxBodies * n records, [(kContactRecs b + q) n + i]) and push-outs (push:
+// 3 * n) with their derivatives; flags[3 * scene + k] (contact, push, out of scope or replay mismatch) are set to 1
+// where they occur and must be cleared beforehand
+void adj_launch_boundary_tape(hipStream_t s, const SceneParams* P, const uint32_t* tileScene, const MapDev* maps,
+                              uint32_t n, uint32_t capacity, const float4* xpre, const float4* posAfter, const float* vb,
+                              const float4* xb, float4* con, float4* push, uint32_t* flags);
 
 // one-time kernel attributes (dynamic LDS above 64 KB)
 int configure_kernels();

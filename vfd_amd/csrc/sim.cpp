@@ -470,16 +470,20 @@ public:
         // flags [contact, push, out of scope]
         DevBuf<float4> con, push;
         DevBuf<uint32_t> flags;
-        // surface tension: Monte-Carlo normals and curvatures of the step, blend passes, temporal smoothing
+        // surface tension: Monte-Carlo normals and curvatures of the step (scenes that run it)
         DevBuf<float4> nMC;
         DevBuf<float> kMC, dK0;  // dK0: the blended curvature carried into the step (temporal smoothing)
-        uint32_t stPasses = 0;
-        bool st = false, stTemporal = false;
         std::deque<DevBuf<float>> kv, rv, kp, rp;  // per Jacobi iteration: kappa^k, residual^k
         std::deque<DevBuf<float4>> av, ap;         // per Jacobi iteration: pressure acceleration a^k
-        float h = 0, H = 0;
-        uint32_t n = 0, Kd = 0, Kp = 0, bodies = 0;
-        bool visc = false, boundaryViscosity = false;
+        uint32_t n = 0;                            // slots recorded (every scene's)
+        bool st = false, stTemporal = false;       // some scene ran surface tension, some with temporal smoothing
+        // per scene: time steps before and after the step's CFL update, Jacobi iteration counts, bodies, switches
+        struct SceneRec {
+            float h = 0, H = 0;
+            uint32_t Kd = 0, Kp = 0, bodies = 0, stPasses = 0;
+            bool visc = false, boundaryViscosity = false, st = false, stTemporal = false, atCap = false;
+        };
+        std::vector<SceneRec> scn;
     };
     DevBuf<float4> tapeXpre;  // positions the boundary pass of a recorded step starts from
     // recorded steps: the first tapeCount entries of tapes; the buffers are kept and reused by later recordings
@@ -1256,7 +1260,7 @@ public:
         VFD_HIP(hipEventRecord(ev[1], stream));
         if (last == VFD_PHASE_SEARCH) return finish_partial();
         Fields F = fields();
-        if (taping && sc.size() == 1) VFD_TRY(tape_copy(tapeXpre, pos.p, sc[0]->count));
+        if (taping) VFD_TRY(tape_copy(tapeXpre, pos.p, cap));
         VFD_TRY(timed("boundary", [&] { launch_boundary(L, dMaps.p, pos.p, vel[cur].p, xb.p, vb.p); }));
         // halos are built after the boundary has pushed penetrating particles out (the lists themselves come from the
         // positions before)
@@ -1267,30 +1271,41 @@ public:
         if (last == VFD_PHASE_BOUNDARY) return finish_partial();
         DevTape* tp = nullptr;
         if (taping) {
-            if (sc.size() != 1) return fail(VFD_ERROR_NOT_SUPPORTED, "adjoint recording needs a single scene");
             if (tapeCount == tapes.size()) tapes.emplace_back(new DevTape());
             tp = tapes[tapeCount++].get();
-            const size_t n = sc[0]->count;
+            const size_t n = cap;  // every scene's slots, each scene reversed over its own range
             tp->n = (uint32_t)n;
-            tp->visc = false;
-            tp->h = sc[0]->st.dt;
+            tp->scn.assign(sc.size(), DevTape::SceneRec{});
+            tp->st = tp->stTemporal = false;
+            uint32_t maxBodies = 0;
+            for (size_t s = 0; s < sc.size(); ++s) {
+                DevTape::SceneRec& r = tp->scn[s];
+                const vfd_desc& d = sc[s]->desc;
+                r.h = sc[s]->st.dt;
+                r.bodies = (uint32_t)sc[s]->bodies.size();
+                r.boundaryViscosity = d.EnableViscositySolver && d.BoundaryViscosity != 0.0f;
+                r.visc = d.EnableViscositySolver != 0;
+                r.st = d.EnableSurfaceTensionSolver != 0;
+                r.stPasses = d.SurfaceTensionSmoothPassCount;
+                r.stTemporal = r.st && d.TemporalSmoothing != 0;
+                tp->st = tp->st || r.st;
+                tp->stTemporal = tp->stTemporal || r.stTemporal;
+                maxBodies = std::max(maxBodies, r.bodies);
+            }
             VFD_TRY(tape_copy(tp->x, pos.p, n));
             VFD_TRY(tape_copy(tp->v0, vel[cur].p, n));
             VFD_TRY(tape_copy(tp->id, id[cur].p, n));
             VFD_TRY(tape_copy(tp->count, nbrCount.p, n));
             VFD_TRY(tape_copy(tp->list, nbrList.p, (size_t)cap * kMaxNeighbors));
             // boundary pass replayed with its derivatives; scope checked by vfd_adjoint from the flags
-            tp->bodies = (uint32_t)sc[0]->bodies.size();
-            tp->boundaryViscosity = sc[0]->desc.EnableViscositySolver && sc[0]->desc.BoundaryViscosity != 0.0f;
-            VFD_TRY(tp->flags.alloc(3));
-            VFD_HIP(hipMemsetAsync(tp->flags.p, 0, 3 * sizeof(uint32_t), stream));
-            if (tp->bodies) {
-                VFD_TRY(tp->con.alloc((size_t)kContactRecs * tp->bodies * n));
+            VFD_TRY(tp->flags.alloc(3 * sc.size()));
+            VFD_HIP(hipMemsetAsync(tp->flags.p, 0, 3 * sc.size() * sizeof(uint32_t), stream));
+            if (maxBodies) {
+                VFD_TRY(tp->con.alloc((size_t)kContactRecs * kMaxBodies * n));
                 VFD_TRY(tp->push.alloc((size_t)3 * n));
-                adj_launch_boundary_tape(stream, dP.p, dMaps.p, (uint32_t)n, cap, tapeXpre.p, pos.p, vb.p, xb.p,
-                                         tp->con.p, tp->push.p, tp->flags.p);
+                adj_launch_boundary_tape(stream, dP.p, dTileScene.p, dMaps.p, (uint32_t)n, cap, tapeXpre.p, pos.p,
+                                         vb.p, xb.p, tp->con.p, tp->push.p, tp->flags.p);
             }
-
         }
         VFD_TRY(timed("density_factor", [&] { launch_density_factor(L, K, F); }));
         if (tp) {
@@ -1318,19 +1333,13 @@ public:
         VFD_HIP(hipEventRecord(ev[3], stream));
         if (last == VFD_PHASE_DIVERGENCE || last == VFD_PHASE_CLEAR_ACCELERATION) return finish_partial();
         if (anyST) {
-            if (tp && sc[0]->desc.EnableSurfaceTensionSolver && sc[0]->desc.TemporalSmoothing)
-                VFD_TRY(tape_copy(tp->dK0, dk[cur].p, tp->n));
+            if (tp && tp->stTemporal) VFD_TRY(tape_copy(tp->dK0, dk[cur].p, tp->n));
             VFD_TRY(upload_mc_tables(F));
             VFD_TRY(timed("surface_tension", [&] { launch_surface_tension(L, F); }));
         }
-        if (tp) {
-            tp->st = sc[0]->desc.EnableSurfaceTensionSolver != 0;
-            tp->stPasses = sc[0]->desc.SurfaceTensionSmoothPassCount;
-            tp->stTemporal = sc[0]->desc.TemporalSmoothing != 0;
-            if (tp->st) {
-                VFD_TRY(tape_copy(tp->nMC, nMC.p, tp->n));
-                VFD_TRY(tape_copy(tp->kMC, kMC.p, tp->n));
-            }
+        if (tp && tp->st) {
+            VFD_TRY(tape_copy(tp->nMC, nMC.p, tp->n));
+            VFD_TRY(tape_copy(tp->kMC, kMC.p, tp->n));
         }
         VFD_HIP(hipEventRecord(ev[4], stream));
         if (last == VFD_PHASE_SURFACE_TENSION) return finish_partial();
@@ -1348,10 +1357,7 @@ public:
                 },
                 minVisc, 2));
             VFD_TRY(timed("visc_apply", [&] { launch_visc_apply(L, F); }));
-            if (tp) {
-                VFD_TRY(tape_copy(tp->y, cgX.p, tp->n));
-                tp->visc = true;
-            }
+            if (tp) VFD_TRY(tape_copy(tp->y, cgX.p, tp->n));
         }
         VFD_HIP(hipEventRecord(ev[5], stream));
         if (last == VFD_PHASE_VISCOSITY) return finish_partial();
@@ -1387,13 +1393,14 @@ public:
         int nonfiniteScene = -1;  // per-step non-finite check (no reference counterpart)
         for (size_t k = 0; k < sc.size() && nonfiniteScene < 0; ++k)
             if (sc[k]->count && sc[k]->st.nonfinite) nonfiniteScene = (int)k;
-        if (tp) {
-            tp->H = sc[0]->st.dt;
-            tp->Kd = sc[0]->desc.EnableDivergenceSolverError ? sc[0]->st.div.it : 0;
-            tp->Kp = sc[0]->st.pres.it;
-            if (sc[0]->dbg.MaxNeighborCount >= kMaxNeighbors)
-                tapeError = "a particle reached the neighbour cap (lists not symmetric)";
-        }
+        if (tp)
+            for (size_t s = 0; s < sc.size(); ++s) {
+                DevTape::SceneRec& r = tp->scn[s];
+                r.H = sc[s]->st.dt;
+                r.Kd = sc[s]->desc.EnableDivergenceSolverError ? sc[s]->st.div.it : 0;
+                r.Kp = sc[s]->st.pres.it;
+                r.atCap = sc[s]->count && sc[s]->dbg.MaxNeighborCount >= kMaxNeighbors;
+            }
         float t[6] = {0, 0, 0, 0, 0, 0};
         VFD_HIP(hipEventElapsedTime(&t[0], ev[0], ev[1]));
         VFD_HIP(hipEventElapsedTime(&t[1], ev[1], ev[2]));
@@ -1466,20 +1473,21 @@ public:
     BicgState hBicg{};
     static constexpr int kBicgCheck = 4;
     int adj_visc_solve(const AdjKHost& A, float h, const float* rho, const float4* b, float4* x) {
-        const uint32_t n = A.n;
-        for (auto* q : {&bR, &bRh, &bP, &bV, &bS, &bT}) VFD_TRY(q->alloc(n));
+        // slot-indexed vectors (A.stride slots); the scene's rows are [A.base, A.base + A.n)
+        const uint32_t n = A.n, o = A.base;
+        for (auto* q : {&bR, &bRh, &bP, &bV, &bS, &bT}) VFD_TRY(q->alloc(A.stride));
         VFD_TRY(dBicg.alloc(1));
-        VFD_HIP(hipMemcpyAsync(x, b, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        VFD_HIP(hipMemcpyAsync(x + o, b + o, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
         adj_launch_visc_T(stream, A, h, rho, x, bT.p);
-        VFD_HIP(hipMemcpyAsync(bR.p, b, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-        adj_launch_axpy4(stream, n, -1.0f, bT.p, bR.p);
-        VFD_HIP(hipMemcpyAsync(bRh.p, bR.p, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-        VFD_HIP(hipMemsetAsync(bP.p, 0, n * sizeof(float4), stream));
-        VFD_HIP(hipMemsetAsync(bV.p, 0, n * sizeof(float4), stream));
+        VFD_HIP(hipMemcpyAsync(bR.p + o, b + o, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        adj_launch_axpy4(stream, n, -1.0f, bT.p + o, bR.p + o);
+        VFD_HIP(hipMemcpyAsync(bRh.p + o, bR.p + o, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        VFD_HIP(hipMemsetAsync(bP.p + o, 0, n * sizeof(float4), stream));
+        VFD_HIP(hipMemsetAsync(bV.p + o, 0, n * sizeof(float4), stream));
         double bn2 = 0, rr = 0;
-        VFD_TRY(adj_dot(b, b, n, bn2));
+        VFD_TRY(adj_dot(b + o, b + o, n, bn2));
         if (bn2 == 0) return VFD_OK;
-        VFD_TRY(adj_dot(bR.p, bR.p, n, rr));
+        VFD_TRY(adj_dot(bR.p + o, bR.p + o, n, rr));
         const double tol2 = 1e-12 * bn2;
         if (!(rr > tol2)) return VFD_OK;
         hBicg = BicgState{1.0, 0.0, 1.0, 1.0, 0.0, rr, tol2, 0u, 300u, 0u, 0u};
@@ -1495,22 +1503,28 @@ public:
         }
         return VFD_OK;
     }
-    // reverse of one recorded step: aLx, aLv hold the cotangents of its output (x', v') in its slot order on entry
-    // and those of its input (x, v) on return, and so does ldK (null: not tracked) for the blended curvature the
-    // surface tension's temporal smoothing carries; lmu accumulates d/dmu
-    int adj_step(const DevTape& t, const uint32_t flags[3], double& lmu, float* ldK) {
-        const uint32_t n = t.n;
-        const SceneParams& pr = sc[0]->prm;
+    // reverse of scene sc_ of one recorded step: aLx, aLv hold the cotangents of its output (x', v') on entry and
+    // those of its input (x, v) on return, in the step's slot order (the scene's particles in slots
+    // [base, base + count)), and so does ldK (null: not tracked) for the blended curvature the surface tension's
+    // temporal smoothing carries; lmu accumulates d/dmu. Neighbour kernels take slot-indexed arrays whole (the lists
+    // hold slots), pointwise kernels take them from the scene's first slot.
+    int adj_step(const DevTape& t, uint32_t sc_, const uint32_t flags[3], double& lmu, float* ldK) {
+        const Scene& S = *sc[sc_];
+        const DevTape::SceneRec& r = t.scn[sc_];
+        const uint32_t N = t.n, n = S.count, o = S.base;
+        const SceneParams& pr = S.prm;
         AdjKHost A{dTab.p, kt.R, kt.invStep, kt.K, kt.L, pr.V, pr.rho0, pr.m, pr.mu, kt.R2, n, cap,
                    t.count.p, t.list.p, t.x.p};
+        A.base = o;
+        A.stride = N;
         if (flags[0]) {  // some particle sampled a body: the sample terms join every operator
             A.con = t.con.p;
-            A.bodies = t.bodies;
-            A.muB = t.boundaryViscosity ? pr.muB : 0.0f;
+            A.bodies = r.bodies;
+            A.muB = r.boundaryViscosity ? pr.muB : 0.0f;
             A.td = pr.td;
         }
         const Launch L = launch();
-        if (adjointGroups == 1 || (adjointGroups == 2 && L.groups >= numCU)) {
+        if (sc.size() == 1 && (adjointGroups == 1 || (adjointGroups == 2 && L.groups >= numCU))) {
             VFD_TRY(aHaloCount.alloc(std::max<uint32_t>(L.groups, 1)));
             VFD_TRY(aHaloIds.alloc((size_t)std::max<uint32_t>(L.groups, 1) * kHaloMax));
             VFD_TRY(aNbrLocal.alloc((size_t)cap * kMaxNeighbors));
@@ -1524,43 +1538,45 @@ public:
             A.groups.haloIds = aHaloIds.p;
             A.groups.local = aNbrLocal.p;
         }
-        const float h = t.h, H = t.H;
-        for (auto* q : {&aLa, &aTmp4, &aGs, &aLy, &aLb}) VFD_TRY(q->alloc(n));
-        for (auto* q : {&aLk, &aLsrc, &aLs, &aLf, &aLrho, &aLalpha, &aLd, &aLS}) VFD_TRY(q->alloc(n));
-        adj_launch_fill(stream, aLrho.p, 0.0f, n);
-        adj_launch_fill(stream, aLalpha.p, 0.0f, n);
+        const float h = r.h, H = r.H;
+        for (auto* q : {&aLa, &aTmp4, &aGs, &aLy, &aLb}) VFD_TRY(q->alloc(N));
+        for (auto* q : {&aLk, &aLsrc, &aLs, &aLf, &aLrho, &aLalpha, &aLd, &aLS}) VFD_TRY(q->alloc(N));
+        float4 *lx = aLx.p + o, *lv = aLv.p + o, *la = aLa.p + o, *ly = aLy.p + o, *lb = aLb.p + o;
+        float *lk = aLk.p + o, *lsrc = aLsrc.p + o, *ls = aLs.p + o, *lf = aLf.p + o, *lrho = aLrho.p + o;
+        float *lalpha = aLalpha.p + o, *ld = aLd.p + o;
+        adj_launch_fill(stream, lrho, 0.0f, n);
+        adj_launch_fill(stream, lalpha, 0.0f, n);
         // x' = x + H v3  ->  lv3 = lv + H lx (aLv becomes lv3, then lv2)
-        adj_launch_axpy4(stream, n, H, aLx.p, aLv.p);
+        adj_launch_axpy4(stream, n, H, lx, lv);
         // pressure final: v3 = v2 + H A(kpEnd)
-        adj_launch_scale4(stream, n, aLv.p, H, aLa.p);
-        adj_launch_fill(stream, aLk.p, 0.0f, n);
+        adj_launch_scale4(stream, n, lv, H, la);
+        adj_launch_fill(stream, lk, 0.0f, n);
         adj_launch_accel_T(stream, A, t.kpEnd.p, aLa.p, aLk.p, aLx.p);
         // pressure iterations, gamma = alpha / H^2, source 1 - rhoadv, c = H^2
-        adj_launch_fill(stream, aLsrc.p, 0.0f, n);
-        adj_launch_fill(stream, aLs.p, 0.0f, n);
-        for (int k = (int)t.Kp - 1; k >= 0; --k) {
-            adj_launch_jacobi_point(stream, n, aLk.p, t.alpha.p, H * H, t.rp[k].p, H * H, aLsrc.p, aLs.p, aLf.p);
-            VFD_HIP(hipMemsetAsync(aLa.p, 0, n * sizeof(float4), stream));
+        adj_launch_fill(stream, lsrc, 0.0f, n);
+        adj_launch_fill(stream, ls, 0.0f, n);
+        for (int k = (int)r.Kp - 1; k >= 0; --k) {
+            adj_launch_jacobi_point(stream, n, lk, t.alpha.p + o, H * H, t.rp[k].p + o, H * H, lsrc, ls, lf);
+            VFD_HIP(hipMemsetAsync(la, 0, n * sizeof(float4), stream));
             adj_launch_div_T(stream, A, aLf.p, t.ap[k].p, aLa.p, aLx.p);
             adj_launch_accel_T(stream, A, t.kp[k].p, aLa.p, aLk.p, aLx.p);
         }
-        adj_launch_pres_init(stream, n, H, pr.rho0, t.alpha.p, t.rhoadv.p, aLk.p, aLsrc.p, aLs.p, aLalpha.p, aLrho.p,
-                             aLd.p);
+        adj_launch_pres_init(stream, n, H, pr.rho0, t.alpha.p + o, t.rhoadv.p + o, lk, lsrc, ls, lalpha, lrho, ld);
         adj_launch_div_T(stream, A, aLd.p, t.v2.p, aLv.p, aLx.p);  // rhoadv's H D(v2)
         // v2 includes H a_st(x): surface tension with its Monte-Carlo normals and curvatures held
-        if (t.st && t.stPasses) {
-            VFD_TRY(aLnc.alloc(n));
-            VFD_TRY(aLcw.alloc(n));
+        if (r.st && r.stPasses) {
+            VFD_TRY(aLnc.alloc(N));
+            VFD_TRY(aLcw.alloc(N));
             adj_launch_surface_tension(stream, A, H, aLv.p, t.nMC.p, t.kMC.p, pr.smoothing, -pr.sigma * pr.mInv,
-                                       t.stPasses, t.stTemporal ? t.dK0.p : nullptr, ldK, aLnc.p, aLcw.p, aLx.p);
+                                       r.stPasses, r.stTemporal ? t.dK0.p : nullptr, ldK, aLnc.p, aLcw.p, aLx.p);
         }
         // v2 = v1 + H g + (H / h)(y - v1)
-        if (t.visc) {
-            adj_launch_scale4(stream, n, aLv.p, H / h, aLy.p);
-            adj_launch_scale4(stream, n, aLv.p, 1.0f - H / h, aLv.p);
+        if (r.visc) {
+            adj_launch_scale4(stream, n, lv, H / h, ly);
+            adj_launch_scale4(stream, n, lv, 1.0f - H / h, lv);
             // A y = v1: lb = A^-T ly; lv1 += lb; positions, densities and mu through lb . dT
             VFD_TRY(adj_visc_solve(A, h, t.rho.p, aLy.p, aLb.p));
-            adj_launch_axpy4(stream, n, 1.0f, aLb.p, aLv.p);
+            adj_launch_axpy4(stream, n, 1.0f, lb, lv);
             const uint32_t blocks = (n + kTile - 1) / kTile;
             VFD_TRY(aPart.alloc(blocks));
             adj_launch_visc_vjp(stream, A, h, t.rho.p, t.y.p, aLb.p, aLx.p, aLrho.p, aPart.p);
@@ -1570,47 +1586,56 @@ public:
             for (double v : hp) lmu += v;
         }
         // divergence final: v1 = v0 + h A(kvEnd)
-        adj_launch_scale4(stream, n, aLv.p, h, aLa.p);
-        adj_launch_fill(stream, aLk.p, 0.0f, n);
+        adj_launch_scale4(stream, n, lv, h, la);
+        adj_launch_fill(stream, lk, 0.0f, n);
         adj_launch_accel_T(stream, A, t.kvEnd.p, aLa.p, aLk.p, aLx.p);
         // divergence iterations, beta = alpha / h, source -D, c = h
-        adj_launch_fill(stream, aLsrc.p, 0.0f, n);
-        adj_launch_fill(stream, aLs.p, 0.0f, n);
-        for (int k = (int)t.Kd - 1; k >= 0; --k) {
-            adj_launch_jacobi_point(stream, n, aLk.p, t.alpha.p, h, t.rv[k].p, h, aLsrc.p, aLs.p, aLf.p);
-            VFD_HIP(hipMemsetAsync(aLa.p, 0, n * sizeof(float4), stream));
+        adj_launch_fill(stream, lsrc, 0.0f, n);
+        adj_launch_fill(stream, ls, 0.0f, n);
+        for (int k = (int)r.Kd - 1; k >= 0; --k) {
+            adj_launch_jacobi_point(stream, n, lk, t.alpha.p + o, h, t.rv[k].p + o, h, lsrc, ls, lf);
+            VFD_HIP(hipMemsetAsync(la, 0, n * sizeof(float4), stream));
             adj_launch_div_T(stream, A, aLf.p, t.av[k].p, aLa.p, aLx.p);
             adj_launch_accel_T(stream, A, t.kv[k].p, aLa.p, aLk.p, aLx.p);
         }
-        adj_launch_div_init(stream, n, h, t.alpha.p, t.D.p, aLk.p, aLsrc.p, aLs.p, aLalpha.p, aLd.p);
+        adj_launch_div_init(stream, n, h, t.alpha.p + o, t.D.p + o, lk, lsrc, ls, lalpha, ld);
         adj_launch_div_T(stream, A, aLd.p, t.v0.p, aLv.p, aLx.p);
         // factor and density
         adj_launch_factor_density(stream, A, t.alpha.p, aLalpha.p, aLS.p, aGs.p, aLrho.p, aLx.p);
         // boundary push-out (the step's positions and velocities above are those after it)
-        if (flags[1]) adj_launch_push(stream, n, t.push.p, aLx.p, aLv.p);
+        if (flags[1]) adj_launch_push(stream, n, N, t.push.p + o, lx, lv);
         VFD_HIP(hipGetLastError());
         return VFD_OK;
     }
-    // cotangents of the last recorded step's output -> of the first recorded step's input (original order)
-    int adjoint(const float* lxOut, const float* lvOut, float* lxIn, float* lvIn, double* lmu) {
+    // cotangents of scene sc_'s particles (original order) after the last recorded step -> before the first
+    int adjoint(uint32_t sc_, const float* lxOut, const float* lvOut, float* lxIn, float* lvIn, double* lmu) {
         if (tapeCount == 0) return fail(VFD_ERROR_STATE, "no recorded steps (vfd_tape)");
+        if (sc_ >= sc.size()) return fail(VFD_ERROR_INVALID_ARGUMENT, "scene index out of range");
+        for (size_t k = 0; k < tapeCount; ++k)
+            if (tapes[k]->scn.size() != sc.size())
+                return fail(VFD_ERROR_STATE, "the scenes changed since the steps were recorded");
         std::vector<uint32_t> flags(3 * tapeCount);
         for (size_t k = 0; k < tapeCount; ++k)
-            VFD_HIP(hipMemcpyAsync(&flags[3 * k], tapes[k]->flags.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                   stream));
+            VFD_HIP(hipMemcpyAsync(&flags[3 * k], tapes[k]->flags.p + 3 * (size_t)sc_, 3 * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, stream));
         VFD_HIP(hipStreamSynchronize(stream));
         bool carry = false;  // some step's temporal smoothing reads the curvature carried from the step before
+        std::string err = tapeError;
         for (size_t k = 0; k < tapeCount; ++k) {
-            carry = carry || (tapes[k]->st && tapes[k]->stTemporal);
+            const DevTape::SceneRec& r = tapes[k]->scn[sc_];
+            carry = carry || r.stTemporal;
             if (flags[3 * k + 2])
-                tapeError = "a particle pushed out twice or after a contact sample is outside the adjoint's scope";
+                err = "a particle pushed out twice or after a contact sample is outside the adjoint's scope";
+            if (r.atCap) err = "a particle reached the neighbour cap (lists not symmetric)";
         }
-        if (!tapeError.empty()) return fail(VFD_ERROR_NOT_SUPPORTED, tapeError);
-        const uint32_t n = tapes[tapeCount - 1]->n;
+        if (!err.empty()) return fail(VFD_ERROR_NOT_SUPPORTED, err);
+        const Scene& S = *sc[sc_];
+        const uint32_t N = tapes[tapeCount - 1]->n, n = S.count, o = S.base;
+        if (n == 0) return fail(VFD_ERROR_STATE, "the scene has no particles");
         VFD_TRY(aOrigX.alloc((size_t)3 * n));
         VFD_TRY(aOrigV.alloc((size_t)3 * n));
-        VFD_TRY(aLx.alloc(n));
-        VFD_TRY(aLv.alloc(n));
+        VFD_TRY(aLx.alloc(N));
+        VFD_TRY(aLv.alloc(N));
         VFD_HIP(hipMemcpyAsync(aOrigX.p, lxOut, (size_t)3 * n * sizeof(float), hipMemcpyHostToDevice, stream));
         if (lvOut)
             VFD_HIP(hipMemcpyAsync(aOrigV.p, lvOut, (size_t)3 * n * sizeof(float), hipMemcpyHostToDevice, stream));
@@ -1618,19 +1643,19 @@ public:
             VFD_HIP(hipMemsetAsync(aOrigV.p, 0, (size_t)3 * n * sizeof(float), stream));
         if (carry) {  // the loss reads x and v: the carried curvature after the last step has cotangent 0
             VFD_TRY(aOrigK.alloc(n));
-            VFD_TRY(aLdK.alloc(n));
+            VFD_TRY(aLdK.alloc(N));
             adj_launch_fill(stream, aOrigK.p, 0.0f, n);
         }
         double m = 0;
         for (size_t k = tapeCount; k-- > 0;) {
             const DevTape& t = *tapes[k];
-            adj_launch_to_slots(stream, n, t.id.p, aOrigX.p, aLx.p);
-            adj_launch_to_slots(stream, n, t.id.p, aOrigV.p, aLv.p);
-            if (carry) adj_launch_to_slots1(stream, n, t.id.p, aOrigK.p, aLdK.p);
-            VFD_TRY(adj_step(t, &flags[3 * k], m, carry ? aLdK.p : nullptr));
-            adj_launch_to_orig(stream, n, t.id.p, aLx.p, aOrigX.p);
-            adj_launch_to_orig(stream, n, t.id.p, aLv.p, aOrigV.p);
-            if (carry) adj_launch_to_orig1(stream, n, t.id.p, aLdK.p, aOrigK.p);
+            adj_launch_to_slots(stream, n, t.id.p + o, aOrigX.p, aLx.p + o);
+            adj_launch_to_slots(stream, n, t.id.p + o, aOrigV.p, aLv.p + o);
+            if (carry) adj_launch_to_slots1(stream, n, t.id.p + o, aOrigK.p, aLdK.p + o);
+            VFD_TRY(adj_step(t, sc_, &flags[3 * k], m, carry ? aLdK.p : nullptr));
+            adj_launch_to_orig(stream, n, t.id.p + o, aLx.p + o, aOrigX.p);
+            adj_launch_to_orig(stream, n, t.id.p + o, aLv.p + o, aOrigV.p);
+            if (carry) adj_launch_to_orig1(stream, n, t.id.p + o, aLdK.p + o, aOrigK.p);
         }
         VFD_HIP(hipMemcpyAsync(lxIn, aOrigX.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, stream));
         VFD_HIP(hipMemcpyAsync(lvIn, aOrigV.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, stream));
@@ -2255,7 +2280,13 @@ int vfd_tape_steps(vfd_sim* sim, uint32_t* steps) {
 int vfd_adjoint(vfd_sim* sim, const float* lx_out, const float* lv_out, float* lx_in, float* lv_in, double* lmu) {
     DRIVE(sim);
     if (!lx_out || !lx_in || !lv_in) return fail(VFD_ERROR_INVALID_ARGUMENT, "cotangent buffers must not be NULL");
-    return sim->impl.adjoint(lx_out, lv_out, lx_in, lv_in, lmu);
+    return sim->impl.adjoint(0, lx_out, lv_out, lx_in, lv_in, lmu);
+}
+int vfd_batch_adjoint(vfd_sim* sim, uint32_t scene, const float* lx_out, const float* lv_out, float* lx_in,
+                      float* lv_in, double* lmu) {
+    DRIVE_SCENE(sim, scene);
+    if (!lx_out || !lx_in || !lv_in) return fail(VFD_ERROR_INVALID_ARGUMENT, "cotangent buffers must not be NULL");
+    return sim->impl.adjoint(scene, lx_out, lv_out, lx_in, lv_in, lmu);
 }
 int vfd_get_tuning(vfd_sim* sim, vfd_tuning* out) {
     DRIVE(sim);

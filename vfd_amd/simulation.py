@@ -402,14 +402,18 @@ class DFSPHSimulation:
 
     def adjoint(self, lx_out, lv_out=None):
         """Cotangents of the last recorded step's positions/velocities (original order) -> those of the first
-        recorded step's input, and d/dmu (mu = dynamic viscosity)."""
+        recorded step's input, and d/dmu (mu = dynamic viscosity). For a scene of a batch, that scene's particles
+        and viscosity (vfd_batch_adjoint)."""
         n = self.GetParticleCount()
         lx = np.ascontiguousarray(lx_out, np.float32).reshape(n, 3)
         lv = None if lv_out is None else np.ascontiguousarray(lv_out, np.float32).reshape(n, 3)
         ox, ov = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
         mu = _abi.ctypes.c_double()
-        check(self._lib.vfd_adjoint(self._h, ptr(lx), None if lv is None else ptr(lv), ptr(ox), ptr(ov),
-                                    _abi.ctypes.byref(mu)), "vfd_adjoint")
+        args = (ptr(lx), None if lv is None else ptr(lv), ptr(ox), ptr(ov), _abi.ctypes.byref(mu))
+        if self._scene == 0:
+            check(self._lib.vfd_adjoint(self._h, *args), "vfd_adjoint")
+        else:
+            check(self._lib.vfd_batch_adjoint(self._h, self._scene, *args), "vfd_batch_adjoint")
         return ox, ov, mu.value
 
     def tuning(self):
@@ -460,6 +464,17 @@ class DFSPHBatch:
 
     def set_frame_capture(self, enabled):
         check(self._lib.vfd_set_frame_capture(self._h, 1 if enabled else 0), "vfd_set_frame_capture")
+
+    def tape(self, on):
+        """Start (clearing earlier records) or stop recording the batch's steps for adjoint()."""
+        self.scenes[0].tape(on)
+
+    def taped_steps(self):
+        return self.scenes[0].taped_steps()
+
+    def adjoint(self, scene, lx_out, lv_out=None):
+        """Scene `scene`'s cotangents through the recorded steps and its d/dmu (DFSPHSimulation.adjoint)."""
+        return self.scenes[scene].adjoint(lx_out, lv_out)
 
     def set_profiling(self, enabled):
         self.scenes[0].set_profiling(enabled)
