@@ -680,21 +680,6 @@ __global__ void __launch_bounds__(kTile) k_adj_push(uint32_t n, uint32_t stride,
     lv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// BiCGSTAB helpers: per-block FP64 partial sums of a . b
-__global__ void __launch_bounds__(kTile) k_adj_dot(uint32_t n, const float4* __restrict__ p,
-                                                   const float4* __restrict__ q, double* part) {
-    const uint32_t i = pid();
-    double v[1] = {i < n ? (double)dot3(ld3(p[i]), ld3(q[i])) : 0.0};
-    block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
-}
-// p = r + beta (p - omega v)
-__global__ void k_adj_bicg_p(uint32_t n, const float4* __restrict__ r, float beta, float omega,
-                             const float4* __restrict__ v, float4* p) {
-    const uint32_t i = pid();
-    if (i < n) p[i] = st4(ld3(r[i]) + beta * (ld3(p[i]) - omega * ld3(v[i])));
-}
-
 // ---- BiCGSTAB with its scalars on the device --------------------------------------------------------------------
 // The solve of the transposed viscosity system runs a fixed sequence of launches per iteration; the scalars live in
 // one BicgState, every kernel returns at once when `done` is set, and the host reads `done` only every few
@@ -722,12 +707,17 @@ __global__ void __launch_bounds__(kTile) k_bicg_scalars(const double* __restrict
     double v[2] = {0.0, 0.0};
     for (uint32_t k = threadIdx.x; k < blocks; k += blockDim.x) {
         v[0] += part[k];
-        if (which == kBicgTsTt) v[1] += part[blocks + k];
+        if (which == kBicgTsTt || which == kBicgBegin) v[1] += part[blocks + k];
     }
     block_sum<2>(v);
     if (threadIdx.x != 0) return;
     const double a = v[0], b = v[1];
     switch (which) {
+        case kBicgBegin:  // a = |b|^2, b = |r0|^2
+            st->rr = b;
+            st->tol2 = 1e-12 * a;
+            if (a == 0.0 || !(b > st->tol2)) st->done = 1u;
+            break;
         case kBicgRho:
             st->rhoNew = a;
             st->beta = (a / st->rhoOld) * (st->alpha / st->omega);
@@ -745,6 +735,16 @@ __global__ void __launch_bounds__(kTile) k_bicg_scalars(const double* __restrict
             if (st->omega == 0.0 || !(st->it < st->maxIt && a > st->tol2)) st->done = 1u;
         }
     }
+}
+__global__ void k_bicg_reset(BicgState* st, uint32_t maxIt) {
+    if (threadIdx.x == 0) *st = BicgState{1.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0u, maxIt, 0u, 0u};
+}
+// acc[0] += sum of part[0, blocks): strided per thread, then block_sum's tree
+__global__ void __launch_bounds__(kTile) k_adj_mu_acc(const double* __restrict__ part, uint32_t blocks, double* acc) {
+    double v[1] = {0.0};
+    for (uint32_t k = threadIdx.x; k < blocks; k += blockDim.x) v[0] += part[k];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) acc[0] += v[0];
 }
 // p = r + beta (p - omega v)
 __global__ void k_bicg_p(uint32_t n, const float4* __restrict__ r, const float4* __restrict__ v, float4* p,
@@ -891,13 +891,6 @@ void adj_launch_factor_density(hipStream_t s, const AdjKHost& h, const float* al
                        lS + h.base);
     hipLaunchKernelGGL(k_adj_factor_density, grid_of(h.n), dim3(kTile), 0, s, h.dev(), lS, gs, lrho, lx);
 }
-void adj_launch_dot(hipStream_t s, uint32_t n, const float4* p, const float4* q, double* part) {
-    hipLaunchKernelGGL(k_adj_dot, grid_of(n), dim3(kTile), 0, s, n, p, q, part);
-}
-void adj_launch_bicg_p(hipStream_t s, uint32_t n, const float4* r, float beta, float omega, const float4* v,
-                       float4* p) {
-    hipLaunchKernelGGL(k_adj_bicg_p, grid_of(n), dim3(kTile), 0, s, n, r, beta, omega, v, p);
-}
 void adj_launch_copy3(hipStream_t s, uint32_t n, const float* a, float* da, const float4* b, float4* db,
                       const float* c, float* dc) {
     if (n) hipLaunchKernelGGL(k_adj_copy3, grid_of(n), dim3(kTile), 0, s, n, a, da, b, db, c, dc);
@@ -937,6 +930,16 @@ void adj_launch_push(hipStream_t s, uint32_t n, uint32_t stride, const float4* p
     if (n) hipLaunchKernelGGL(k_adj_push, grid_of(n), dim3(kTile), 0, s, n, stride, push, lx, lv);
 }
 
+void adj_launch_bicg_begin(hipStream_t s, const AdjKHost& h, const float4* b, const float4* r, double* part,
+                           BicgState* st, uint32_t maxIt) {
+    const uint32_t n = h.n, o = h.base;
+    hipLaunchKernelGGL(k_bicg_reset, dim3(1), dim3(kWave), 0, s, st, maxIt);
+    hipLaunchKernelGGL(k_bicg_dot, grid_of(n), dim3(kTile), 0, s, n, b + o, b + o, r + o, r + o, part, st);
+    hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(kTile), 0, s, part, grid_of(n).x, st, (int)kBicgBegin);
+}
+void adj_launch_mu_acc(hipStream_t s, const double* part, uint32_t blocks, double* acc) {
+    hipLaunchKernelGGL(k_adj_mu_acc, dim3(1), dim3(kTile), 0, s, part, blocks, acc);
+}
 void adj_launch_bicg_iteration(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const BicgVectors& v,
                                double* part, BicgState* st) {
     // the vectors are slot-indexed arrays: the operator takes them whole, the pointwise kernels from the scene's

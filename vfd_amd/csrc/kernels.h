@@ -71,12 +71,18 @@ struct BicgState {
     double rhoOld, rhoNew, alpha, omega, beta, rr, tol2;  // tol2: continue while rr > tol2
     uint32_t it, maxIt, done, pad;
 };
-enum { kBicgRho = 0, kBicgRv = 1, kBicgTsTt = 2, kBicgRr = 3 };
+enum { kBicgRho = 0, kBicgRv = 1, kBicgTsTt = 2, kBicgRr = 3, kBicgBegin = 4 };
 struct BicgVectors {
     float4 *x, *r, *rh, *p, *v, *s, *t;
 };
 void adj_launch_visc_T(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const float4* w, float4* out,
                        const BicgState* st = nullptr);
+// the solve's start on the device: state reset, |b|^2 and |r0|^2, the stopping threshold 1e-12 |b|^2, and done
+// when b = 0 or r0 already meets it (b, r0 slot-indexed; part holds 2 * ceil(n / kTile) doubles)
+void adj_launch_bicg_begin(hipStream_t s, const AdjKHost& h, const float4* b, const float4* r, double* part,
+                           BicgState* st, uint32_t maxIt);
+// acc[0] += the sum of part[0, blocks) (per-block d/dmu partials of one step), in a fixed order
+void adj_launch_mu_acc(hipStream_t s, const double* part, uint32_t blocks, double* acc);
 // one BiCGSTAB iteration for A^T x = b on the device; part holds 2 * ceil(n / kTile) doubles
 void adj_launch_bicg_iteration(hipStream_t s, const AdjKHost& h, float dt, const float* rho, const BicgVectors& v,
                                double* part, BicgState* st);
@@ -84,9 +90,6 @@ void adj_launch_visc_vjp(hipStream_t s, const AdjKHost& h, float dt, const float
                          const float4* lb, float4* lx, float* lrho, double* muPart);
 void adj_launch_factor_density(hipStream_t s, const AdjKHost& h, const float* alpha, const float* lalpha, float* lS,
                                float4* gs, const float* lrho, float4* lx);
-void adj_launch_dot(hipStream_t s, uint32_t n, const float4* p, const float4* q, double* part);
-void adj_launch_bicg_p(hipStream_t s, uint32_t n, const float4* r, float beta, float omega, const float4* v,
-                       float4* p);
 void adj_launch_copy3(hipStream_t s, uint32_t n, const float* a, float* da, const float4* b, float4* db,
                       const float* c, float* dc);
 void adj_launch_to_slots(hipStream_t s, uint32_t n, const uint32_t* id, const float* orig, float4* slot);

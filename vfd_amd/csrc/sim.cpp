@@ -1449,23 +1449,12 @@ public:
     DevBuf<float4> aLx, aLv, aLa, aTmp4, aGs, aLy, aLb, bR, bRh, bP, bV, bS, bT, aLnc;
     DevBuf<float2> aLcw;
     DevBuf<float> aLk, aLsrc, aLs, aLf, aLrho, aLalpha, aLd, aLS, aOrigX, aOrigV, aOrigK, aLdK;
-    DevBuf<double> aPart;
+    DevBuf<double> aPart, aMu;  // aMu: d/dmu accumulated over the reversed steps
     // halos of the step being reversed, rebuilt from its recorded lists for the group kernels
     DevBuf<uint32_t> aHaloCount, aHaloIds, aNbrLocal, aHaloStats;
     // transposed Jacobi and viscosity loops: 0 per particle, 1 per group, 2 per group once there is a group for every
     // CU (below that the per-particle kernels spread better and run as fast: 100k particles, 98 groups)
     uint32_t adjointGroups = 2;
-    int adj_dot(const float4* p, const float4* q, uint32_t n, double& out) {
-        const uint32_t blocks = (n + kTile - 1) / kTile;
-        VFD_TRY(aPart.alloc(blocks));
-        adj_launch_dot(stream, n, p, q, aPart.p);
-        std::vector<double> h(blocks);
-        VFD_HIP(hipMemcpyAsync(h.data(), aPart.p, blocks * sizeof(double), hipMemcpyDeviceToHost, stream));
-        VFD_HIP(hipStreamSynchronize(stream));
-        out = 0;
-        for (double v : h) out += v;
-        return VFD_OK;
-    }
     // BiCGSTAB for A^T x = b (the transposed viscosity system of one recorded step), relative residual 1e-6. The
     // iterations run on the device with their scalars in a BicgState; the host reads the state every
     // kBicgCheck iterations.
@@ -1484,15 +1473,9 @@ public:
         VFD_HIP(hipMemcpyAsync(bRh.p + o, bR.p + o, n * sizeof(float4), hipMemcpyDeviceToDevice, stream));
         VFD_HIP(hipMemsetAsync(bP.p + o, 0, n * sizeof(float4), stream));
         VFD_HIP(hipMemsetAsync(bV.p + o, 0, n * sizeof(float4), stream));
-        double bn2 = 0, rr = 0;
-        VFD_TRY(adj_dot(b + o, b + o, n, bn2));
-        if (bn2 == 0) return VFD_OK;
-        VFD_TRY(adj_dot(bR.p + o, bR.p + o, n, rr));
-        const double tol2 = 1e-12 * bn2;
-        if (!(rr > tol2)) return VFD_OK;
-        hBicg = BicgState{1.0, 0.0, 1.0, 1.0, 0.0, rr, tol2, 0u, 300u, 0u, 0u};
-        VFD_HIP(hipMemcpyAsync(dBicg.p, &hBicg, sizeof(BicgState), hipMemcpyHostToDevice, stream));
+        // |b|^2, |r0|^2 and the threshold on the device (done at once when b = 0 or r0 = b - A^T b meets it)
         VFD_TRY(aPart.alloc(2 * (size_t)((n + kTile - 1) / kTile)));
+        adj_launch_bicg_begin(stream, A, b, bR.p, aPart.p, dBicg.p, 300u);
         const BicgVectors vec{x, bR.p, bRh.p, bP.p, bV.p, bS.p, bT.p};
         for (uint32_t done = 0; !done;) {
             for (int k = 0; k < kBicgCheck; ++k) adj_launch_bicg_iteration(stream, A, h, rho, vec, aPart.p, dBicg.p);
@@ -1506,9 +1489,9 @@ public:
     // reverse of scene sc_ of one recorded step: aLx, aLv hold the cotangents of its output (x', v') on entry and
     // those of its input (x, v) on return, in the step's slot order (the scene's particles in slots
     // [base, base + count)), and so does ldK (null: not tracked) for the blended curvature the surface tension's
-    // temporal smoothing carries; lmu accumulates d/dmu. Neighbour kernels take slot-indexed arrays whole (the lists
-    // hold slots), pointwise kernels take them from the scene's first slot.
-    int adj_step(const DevTape& t, uint32_t sc_, const uint32_t flags[3], double& lmu, float* ldK) {
+    // temporal smoothing carries; lmu (one device double) accumulates d/dmu. Neighbour kernels take slot-indexed
+    // arrays whole (the lists hold slots), pointwise kernels take them from the scene's first slot.
+    int adj_step(const DevTape& t, uint32_t sc_, const uint32_t flags[3], double* lmu, float* ldK) {
         const Scene& S = *sc[sc_];
         const DevTape::SceneRec& r = t.scn[sc_];
         const uint32_t N = t.n, n = S.count, o = S.base;
@@ -1580,10 +1563,7 @@ public:
             const uint32_t blocks = (n + kTile - 1) / kTile;
             VFD_TRY(aPart.alloc(blocks));
             adj_launch_visc_vjp(stream, A, h, t.rho.p, t.y.p, aLb.p, aLx.p, aLrho.p, aPart.p);
-            std::vector<double> hp(blocks);
-            VFD_HIP(hipMemcpyAsync(hp.data(), aPart.p, blocks * sizeof(double), hipMemcpyDeviceToHost, stream));
-            VFD_HIP(hipStreamSynchronize(stream));
-            for (double v : hp) lmu += v;
+            adj_launch_mu_acc(stream, aPart.p, blocks, lmu);
         }
         // divergence final: v1 = v0 + h A(kvEnd)
         adj_launch_scale4(stream, n, lv, h, la);
@@ -1646,19 +1626,22 @@ public:
             VFD_TRY(aLdK.alloc(N));
             adj_launch_fill(stream, aOrigK.p, 0.0f, n);
         }
-        double m = 0;
+        VFD_TRY(aMu.alloc(1));
+        VFD_HIP(hipMemsetAsync(aMu.p, 0, sizeof(double), stream));
         for (size_t k = tapeCount; k-- > 0;) {
             const DevTape& t = *tapes[k];
             adj_launch_to_slots(stream, n, t.id.p + o, aOrigX.p, aLx.p + o);
             adj_launch_to_slots(stream, n, t.id.p + o, aOrigV.p, aLv.p + o);
             if (carry) adj_launch_to_slots1(stream, n, t.id.p + o, aOrigK.p, aLdK.p + o);
-            VFD_TRY(adj_step(t, sc_, &flags[3 * k], m, carry ? aLdK.p : nullptr));
+            VFD_TRY(adj_step(t, sc_, &flags[3 * k], aMu.p, carry ? aLdK.p : nullptr));
             adj_launch_to_orig(stream, n, t.id.p + o, aLx.p + o, aOrigX.p);
             adj_launch_to_orig(stream, n, t.id.p + o, aLv.p + o, aOrigV.p);
             if (carry) adj_launch_to_orig1(stream, n, t.id.p + o, aLdK.p + o, aOrigK.p);
         }
         VFD_HIP(hipMemcpyAsync(lxIn, aOrigX.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, stream));
         VFD_HIP(hipMemcpyAsync(lvIn, aOrigV.p, (size_t)3 * n * sizeof(float), hipMemcpyDeviceToHost, stream));
+        double m = 0;
+        VFD_HIP(hipMemcpyAsync(&m, aMu.p, sizeof(double), hipMemcpyDeviceToHost, stream));
         VFD_HIP(hipStreamSynchronize(stream));
         if (lmu) *lmu = m;
         return VFD_OK;
