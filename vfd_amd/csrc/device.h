@@ -534,9 +534,11 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
         using HR = HaloRec<Load>;
         const uint32_t* __restrict__ li = src.local + i;
         const uint32_t last = n - 1u;
+        // the list is read once per launch: non-temporal loads keep it from displacing the halo records, which
+        // neighbouring groups gather again, from the caches
         auto fetch = [&](uint32_t (&e)[C], uint32_t b) {
 #pragma unroll
-            for (uint32_t u = 0; u < C; ++u) e[u] = li[(size_t)min(b + u, last) * cap];
+            for (uint32_t u = 0; u < C; ++u) e[u] = __builtin_nontemporal_load(li + (size_t)min(b + u, last) * cap);
         };
         auto consume = [&](const uint32_t (&e)[C], uint32_t b) {
             float4 lo[C], hi[C];

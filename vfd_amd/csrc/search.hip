@@ -705,8 +705,10 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
     const f3 xi = n ? ld3(pos[i]) : mk3(0.0f, 0.0f, 0.0f);
     for (uint32_t j0 = 0; j0 < nr; j0 += kHaloBatch) {
         uint32_t tb[kHaloBatch];
+        // the list's last read in the step: non-temporal loads (it would only displace cached positions)
 #pragma unroll
-        for (uint32_t u = 0; u < kHaloBatch; ++u) tb[u] = j0 + u < n ? nbrList[(size_t)(j0 + u) * cap + i] : 0u;
+        for (uint32_t u = 0; u < kHaloBatch; ++u)
+            tb[u] = j0 + u < n ? __builtin_nontemporal_load(nbrList + (size_t)(j0 + u) * cap + i) : 0u;
 #pragma unroll
         for (uint32_t u = 0; u < kHaloBatch; ++u) {
             if (j0 + u >= n) continue;
