@@ -289,14 +289,41 @@ __device__ __forceinline__ void each_neighbor(const uint32_t* __restrict__ list,
 // both LDS tables hold one entry past the kernel table, for code kGCodeZero (see gcode)
 constexpr unsigned kLdsG = ((kTableEntries + 1u) * sizeof(float) + 15u) / 16u * 16u;
 constexpr unsigned kLdsWG = ((kTableEntries + 1u) * sizeof(float2) + 15u) / 16u * 16u;
+// Each thread's loads are all issued before its first LDS store (kTableBatch per round: one round for a 1024-thread
+// block), so the copy pays one memory latency per launch instead of one per entry it handles.
+constexpr uint32_t kTableBatch = 16;
 __device__ __forceinline__ void load_table_g(float* sG, const float2* __restrict__ tab) {
-    for (uint32_t p = threadIdx.x; p < kTableEntries; p += blockDim.x) sG[p] = tab[p].y;
+    for (uint32_t p0 = threadIdx.x; p0 < kTableEntries; p0 += kTableBatch * blockDim.x) {
+        float v[kTableBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kTableBatch; ++u) {
+            const uint32_t p = p0 + u * blockDim.x;
+            v[u] = tab[min(p, kTableEntries - 1u)].y;  // clamped: every load in bounds, none behind a branch
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kTableBatch; ++u) {
+            const uint32_t p = p0 + u * blockDim.x;
+            if (p < kTableEntries) sG[p] = v[u];
+        }
+    }
     if (threadIdx.x == 0) sG[kTableEntries] = 0.0f;  // code kGCodeZero: beyond the gradient cutoff
     __syncthreads();
 }
 // {W, G} pairs; entry kGCodeZero = {W of the last entry (what GetW reads beyond R), 0}
 __device__ __forceinline__ void load_table_wg(float2* sWG, const float2* __restrict__ tab) {
-    for (uint32_t p = threadIdx.x; p < kTableEntries; p += blockDim.x) sWG[p] = tab[p];
+    for (uint32_t p0 = threadIdx.x; p0 < kTableEntries; p0 += kTableBatch * blockDim.x) {
+        float2 v[kTableBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kTableBatch; ++u) {
+            const uint32_t p = p0 + u * blockDim.x;
+            v[u] = tab[min(p, kTableEntries - 1u)];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kTableBatch; ++u) {
+            const uint32_t p = p0 + u * blockDim.x;
+            if (p < kTableEntries) sWG[p] = v[u];
+        }
+    }
     if (threadIdx.x == 0) sWG[kTableEntries] = make_float2(tab[kTableEntries - 1u].x, 0.0f);
     __syncthreads();
 }
