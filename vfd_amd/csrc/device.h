@@ -124,10 +124,11 @@ __device__ __forceinline__ int wave_max_i(int v) {
     return v;
 }
 
-// Block (kTile threads) sum of NV doubles; result valid in thread 0. Fixed order: wave tree, then waves 0..3.
-template <int NV>
+// Block (kThreads threads) sum of NV doubles; result valid in thread 0. Fixed order: wave tree, then the waves in
+// order.
+template <int NV, int kThreads = kTile>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double sh[kTile / kWave][NV];
+    __shared__ double sh[kThreads / kWave][NV];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
@@ -139,7 +140,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             double s = sh[0][i];
-            for (int j = 1; j < (int)(kTile / kWave); ++j) s += sh[j][i];
+            for (int j = 1; j < kThreads / kWave; ++j) s += sh[j][i];
             v[i] = s;
         }
     }
@@ -158,30 +159,31 @@ __device__ __forceinline__ void tile_partials(double (&v)[NV], double* partials,
     __syncthreads();  // the block-sum scratch is reused by the next tile of a persistent block
 }
 
-// Inside a finalize block (kTile threads) for scene s: sum of the scene's tile partials, valid in thread 0. Each
-// thread adds the tiles threadIdx.x, threadIdx.x + kTile, ... in that order; the loads of eight of them are issued
-// together before they are added, so the latency is paid once per eight.
-template <int NV>
+// Inside a finalize block (kThreads threads) for scene s: sum of the scene's tile partials, valid in thread 0. Each
+// thread adds the tiles threadIdx.x, threadIdx.x + kThreads, ... in that order; the loads of eight of them are issued
+// together before they are added, so the latency is paid once per eight (once in all for a 1024-thread block and up
+// to 8192 tiles, 2M particles).
+template <int NV, int kThreads = kTile>
 __device__ __forceinline__ void scene_sum(const double* __restrict__ partials, uint32_t tileBegin, uint32_t tileEnd,
                                           double (&out)[NV]) {
     constexpr uint32_t kBatch = 8;
 #pragma unroll
     for (int i = 0; i < NV; ++i) out[i] = 0.0;
-    for (uint32_t t0 = tileBegin + threadIdx.x; t0 < tileEnd; t0 += kBatch * kTile) {
+    for (uint32_t t0 = tileBegin + threadIdx.x; t0 < tileEnd; t0 += kBatch * kThreads) {
         double v[kBatch][NV];
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u) {
-            const uint32_t t = t0 + u * kTile;
+            const uint32_t t = t0 + u * kThreads;
 #pragma unroll
             for (int i = 0; i < NV; ++i) v[u][i] = t < tileEnd ? partials[(size_t)t * kPartialSlots + i] : 0.0;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kBatch; ++u)
-            if (t0 + u * kTile < tileEnd)
+            if (t0 + u * kThreads < tileEnd)
 #pragma unroll
                 for (int i = 0; i < NV; ++i) out[i] += v[u][i];
     }
-    block_sum<NV>(out);
+    block_sum<NV, kThreads>(out);
 }
 
 __device__ __forceinline__ float block_max(float v) {

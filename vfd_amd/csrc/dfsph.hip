@@ -153,6 +153,9 @@ __device__ __forceinline__ float density_pressure_force(const SceneParams& p, co
     float* rec = reinterpret_cast<float*>(lds + kLdsG / sizeof(float4));
 
 // ---- Jacobi solver control (DFSPHImplementation.cu:455-461, 526-532) -------------------------------------------
+// The per-scene finalisers (Jacobi, CG) run one 1024-thread block per scene: the tile partials of a 2M scene are then
+// loaded in one round.
+constexpr int kFinishThreads = 1024;
 // which = 0: divergence (eta = (1/dt) tol 1e-4 rho0), 1: pressure (eta = tol 1e-4 rho0)
 // One block; also counts the scenes that iterate into live[which] (and clears live[2], the viscosity count, at the
 // start of a step).
@@ -185,13 +188,13 @@ __global__ void __launch_bounds__(kTile) k_jacobi_begin(const SceneParams* __res
 
 // Residual error -rho0 * sum(r) / N, iteration count and the loop condition (DFSPHImplementation.cu:461, 483-492,
 // 532, 554-563), one block per scene after every iteration kernel.
-__global__ void __launch_bounds__(kTile) k_jacobi_finish(const SceneParams* __restrict__ P, Fields F, int which) {
+__global__ void __launch_bounds__(kFinishThreads) k_jacobi_finish(const SceneParams* __restrict__ P, Fields F, int which) {
     const uint32_t s = blockIdx.x;
     const SceneParams& p = P[s];
     JacobiState& J = which == 0 ? F.state[s].div : F.state[s].pres;
     if (!J.active || p.count == 0) return;  // uniform per block
     double out[1];
-    scene_sum<1>(F.partials, p.tileBegin, p.tileEnd, out);
+    scene_sum<1, kFinishThreads>(F.partials, p.tileBegin, p.tileEnd, out);
     if (threadIdx.x == 0) {
         const uint32_t minIt = which == 0 ? p.minDiv : p.minPres, maxIt = which == 0 ? p.maxDiv : p.maxPres;
         const float err = (float)(-out[0]) / (float)p.count;
@@ -296,7 +299,7 @@ void launch_div_iter(const Launch& L, const KTabHost& kt, const Fields& F) {
 }
 // the residual reduction and loop condition after a Jacobi iteration; which = 0 divergence, 1 pressure
 void launch_jacobi_finish(const Launch& L, const Fields& F, int which) {
-    hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F, which);
+    hipLaunchKernelGGL(k_jacobi_finish, dim3(L.sceneCount), dim3(kFinishThreads), 0, L.stream, L.P, F, which);
 }
 void launch_div_final(const Launch& L, const KTabHost& kt, const Fields& F) {
     hipLaunchKernelGGL(k_div_final, dim3(L.gridGroups), dim3(kGroupSize), kLdsXS, L.stream, L.P, L.tileScene, kt, F, L.capacity,
@@ -663,12 +666,12 @@ __global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_cg_begin(const Sce
     });
 }
 
-__global__ void __launch_bounds__(kTile) k_cg_begin_finish(const SceneParams* __restrict__ P, Fields F) {
+__global__ void __launch_bounds__(kFinishThreads) k_cg_begin_finish(const SceneParams* __restrict__ P, Fields F) {
     const uint32_t s = blockIdx.x;
     const SceneParams& p = P[s];
     if (!p.enableVisc || p.count == 0) return;
     double out[3];
-    scene_sum<3>(F.partials, p.tileBegin, p.tileEnd, out);
+    scene_sum<3, kFinishThreads>(F.partials, p.tileBegin, p.tileEnd, out);
     if (threadIdx.x == 0) {
         CgState& G = F.state[s].cg;
         G.it = 0;
@@ -713,14 +716,14 @@ __global__ void __launch_bounds__(kGroupSize, kGroupsPerCU) k_cg_matvec(const Sc
     });
 }
 
-__global__ void __launch_bounds__(kTile) k_cg_alpha(const SceneParams* __restrict__ P, Fields F) {
+__global__ void __launch_bounds__(kFinishThreads) k_cg_alpha(const SceneParams* __restrict__ P, Fields F) {
     if (F.live[2] == 0u) return;  // every scene has left this solve (block-uniform)
     const uint32_t s = blockIdx.x;
     const SceneParams& p = P[s];
     CgState& G = F.state[s].cg;
     if (!G.active || p.count == 0) return;
     double out[1];
-    scene_sum<1>(F.partials, p.tileBegin, p.tileEnd, out);
+    scene_sum<1, kFinishThreads>(F.partials, p.tileBegin, p.tileEnd, out);
     if (threadIdx.x == 0) G.alpha = G.absNew / (float)out[0];
 }
 
@@ -748,13 +751,13 @@ __global__ void __launch_bounds__(kTile) k_cg_update(const SceneParams* __restri
     });
 }
 
-__global__ void __launch_bounds__(kTile) k_cg_update_finish(const SceneParams* __restrict__ P, Fields F) {
+__global__ void __launch_bounds__(kFinishThreads) k_cg_update_finish(const SceneParams* __restrict__ P, Fields F) {
     const uint32_t s = blockIdx.x;
     const SceneParams& p = P[s];
     CgState& G = F.state[s].cg;
     if (!G.active || p.count == 0) return;
     double out[2];
-    scene_sum<2>(F.partials, p.tileBegin, p.tileEnd, out);
+    scene_sum<2, kFinishThreads>(F.partials, p.tileBegin, p.tileEnd, out);
     if (threadIdx.x == 0) {
         G.loops += 1u;
         G.res2 = (float)out[0];
@@ -814,20 +817,20 @@ void launch_cg_begin(const Launch& L, const KTabHost& kt, const Fields& F) {
                        TileGrid(L.tg));
 }
 void launch_cg_begin_finish(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_begin_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
+    hipLaunchKernelGGL(k_cg_begin_finish, dim3(L.sceneCount), dim3(kFinishThreads), 0, L.stream, L.P, F);
 }
 void launch_cg_matvec(const Launch& L, const KTabHost& kt, const Fields& F) {
     hipLaunchKernelGGL(k_cg_matvec, dim3(L.gridGroups), dim3(kGroupSize), kLdsXSV, L.stream, L.P, L.tileScene, kt, F,
                        L.capacity, TileGrid(L.tg));
 }
 void launch_cg_alpha(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_alpha, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(L.sceneCount), dim3(kFinishThreads), 0, L.stream, L.P, F);
 }
 void launch_cg_update(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_cg_update, dim3(L.gridAll), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, L.capacity, TileGrid(L.tg));
 }
 void launch_cg_update_finish(const Launch& L, const Fields& F) {
-    hipLaunchKernelGGL(k_cg_update_finish, dim3(L.sceneCount), dim3(kTile), 0, L.stream, L.P, F);
+    hipLaunchKernelGGL(k_cg_update_finish, dim3(L.sceneCount), dim3(kFinishThreads), 0, L.stream, L.P, F);
 }
 void launch_cg_direction(const Launch& L, const Fields& F) {
     hipLaunchKernelGGL(k_cg_direction, dim3(L.gridAll), dim3(kTile), 0, L.stream, L.P, L.tileScene, F, TileGrid(L.tg));
