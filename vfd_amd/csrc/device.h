@@ -550,6 +550,15 @@ template <class Load> struct HaloRec {
         if (H) put(rec + L * kHaloMax + H * s, H, h);
     }
 };
+// Word 3 of a CDNA buffer resource for raw (stride 0) 32-bit data: data format 32, no swizzle, no index stride.
+constexpr int kRawBufferWord3 = 0x00020000;
+// An offset at or past a raw buffer's num_records reads 0 without touching memory; adding this to a lane's offset
+// keeps the load in the instruction stream (so the compiler still counts it in flight) but takes it off the memory
+// path. Valid offsets stay below 2^31 bytes.
+constexpr uint32_t kBufferOutOfRange = 0x80000000u;
+// Cache policy bits of a buffer load on gfx950: bit 1 = non-temporal.
+constexpr int kBufferNonTemporal = 2;
+
 template <int R, class TV, class Load, class Fn>
 __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i, float cutR, float invStep, f3 xi,
                                                   const TV* tab, const Load& load, Fn&& f) {
@@ -561,13 +570,21 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
         if (n == 0) return;
         const float* rec = src.rec;
         using HR = HaloRec<Load>;
-        const uint32_t* __restrict__ li = src.local + i;
         const uint32_t last = n - 1u;
         // the list is read once per launch: non-temporal loads keep it from displacing the halo records, which
-        // neighbouring groups gather again, from the caches
+        // neighbouring groups gather again, from the caches. Row r of the wave (r is the same in every lane) is one
+        // buffer of cap entries; a lane whose list ended before row r reads past the buffer's end, which returns 0
+        // without a memory access, so the rows past a short list cost the memory path nothing.
         auto fetch = [&](uint32_t (&e)[C], uint32_t b) {
 #pragma unroll
-            for (uint32_t u = 0; u < C; ++u) e[u] = __builtin_nontemporal_load(li + (size_t)min(b + u, last) * cap);
+            for (uint32_t u = 0; u < C; ++u) {
+                const uint32_t r = b + u;
+                const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
+                    const_cast<uint32_t*>(src.local + (size_t)r * cap), (short)0, (int)(cap * sizeof(uint32_t)),
+                    kRawBufferWord3);
+                const uint32_t off = i * (uint32_t)sizeof(uint32_t) + (r <= last ? 0u : kBufferOutOfRange);
+                e[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(row, (int)off, 0, kBufferNonTemporal);
+            }
         };
         auto consume = [&](const uint32_t (&e)[C], uint32_t b) {
             float4 lo[C], hi[C];
@@ -590,9 +607,9 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
             }
         };
         // ring of D chunk buffers: chunk c lives in e[c % D]; chunk c + D - 1 is requested before chunk c is consumed.
-        // The requests are unconditional (rows past a list's end repeat its last row): under a per-lane condition
-        // the compiler cannot count the loads in flight and waits for all of them, the prefetch included, before
-        // the current chunk's first use.
+        // The requests are unconditional (rows past a list's end are out-of-range reads that return 0): under a
+        // per-lane condition the compiler cannot count the loads in flight and waits for all of them, the prefetch
+        // included, before the current chunk's first use.
         constexpr uint32_t D = ListShape<R>::depth;
         uint32_t e[D][C];
 #pragma unroll
