@@ -294,7 +294,7 @@ int vfd_batch_get_info(const vfd_sim* sim, uint32_t scene, vfd_info* info);
 
 /* SetFluidObjects (DFSPHImplementation.cu:172-253): positions of every fluid object concatenated, one velocity per
  * particle (the reference holds one per object, FluidObject.h:21-31; expand it per particle). A simulation holds at
- * most 2^29 particle slots (every scene's count rounded up to a multiple of 256); beyond that the call that allocates
+ * most 2^28 particle slots (every scene's count rounded up to a multiple of 256); beyond that the call that allocates
  * the particle arrays returns VFD_ERROR_NOT_SUPPORTED. */
 int vfd_set_fluid(vfd_sim* sim, const float* positions, const float* velocities, uint32_t count);
 int vfd_batch_set_fluid(vfd_sim* sim, uint32_t scene, const float* positions, const float* velocities,

@@ -554,7 +554,7 @@ template <class Load> struct HaloRec {
 constexpr int kRawBufferWord3 = 0x00020000;
 // An offset at or past a raw buffer's num_records reads 0 without touching memory; adding this to a lane's offset
 // keeps the load in the instruction stream (so the compiler still counts it in flight) but takes it off the memory
-// path. Valid offsets stay below 2^31 bytes.
+// path. Valid offsets stay below 2^31 bytes (a simulation holds at most 2^28 slots: 8-byte row-pair entries).
 constexpr uint32_t kBufferOutOfRange = 0x80000000u;
 // Cache policy bits of a buffer load on gfx950: bit 1 = non-temporal.
 constexpr int kBufferNonTemporal = 2;
@@ -572,18 +572,22 @@ __device__ __forceinline__ void each_neighbor_rec(const NbrSrc& src, uint32_t i,
         using HR = HaloRec<Load>;
         const uint32_t last = n - 1u;
         // the list is read once per launch: non-temporal loads keep it from displacing the halo records, which
-        // neighbouring groups gather again, from the caches. Row r of the wave (r is the same in every lane) is one
-        // buffer of cap entries; a lane whose list ended before row r reads past the buffer's end, which returns 0
+        // neighbouring groups gather again, from the caches. The staged lists hold rows in pairs ([j/2][cap][2]):
+        // rows r, r + 1 of the wave (r even, the same in every lane) are one buffer of cap 8-byte entries, read with
+        // one load per lane; a lane whose list ended before row r reads past the buffer's end, which returns 0
         // without a memory access, so the rows past a short list cost the memory path nothing.
+        static_assert(C % 2 == 0, "list chunks cover whole row pairs");
         auto fetch = [&](uint32_t (&e)[C], uint32_t b) {
 #pragma unroll
-            for (uint32_t u = 0; u < C; ++u) {
-                const uint32_t r = b + u;
-                const __amdgpu_buffer_rsrc_t row = __builtin_amdgcn_make_buffer_rsrc(
-                    const_cast<uint32_t*>(src.local + (size_t)r * cap), (short)0, (int)(cap * sizeof(uint32_t)),
+            for (uint32_t u = 0; u < C / 2; ++u) {
+                const uint32_t r = b + 2 * u;
+                const __amdgpu_buffer_rsrc_t rows = __builtin_amdgcn_make_buffer_rsrc(
+                    const_cast<uint32_t*>(src.local + (size_t)(r >> 1) * cap * 2), (short)0, (int)(cap * 8),
                     kRawBufferWord3);
-                const uint32_t off = i * (uint32_t)sizeof(uint32_t) + (r <= last ? 0u : kBufferOutOfRange);
-                e[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(row, (int)off, 0, kBufferNonTemporal);
+                const uint32_t off = i * 8u + (r <= last ? 0u : kBufferOutOfRange);
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rows, (int)off, 0, kBufferNonTemporal);
+                e[2 * u] = v[0];
+                e[2 * u + 1] = v[1];
             }
         };
         auto consume = [&](const uint32_t (&e)[C], uint32_t b) {

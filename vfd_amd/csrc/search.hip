@@ -701,7 +701,9 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
     }
     __syncthreads();
     // staged entries: the halo number of each neighbour | its table code << 16, in list order (the neighbour's
-    // position from the halo's copy in LDS)
+    // position from the halo's copy in LDS), rows in pairs ([j/2][cap][2]: one 8-byte store per row pair; the entry
+    // after a list's last one is 0 and never read)
+    static_assert(kHaloBatch % 2 == 0 && kMaxNeighbors % 2 == 0, "staged rows are stored in pairs");
     const f3 xi = n ? ld3(pos[i]) : mk3(0.0f, 0.0f, 0.0f);
     for (uint32_t j0 = 0; j0 < nr; j0 += kHaloBatch) {
         uint32_t tb[kHaloBatch];
@@ -709,8 +711,10 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
 #pragma unroll
         for (uint32_t u = 0; u < kHaloBatch; ++u)
             tb[u] = j0 + u < n ? __builtin_nontemporal_load(nbrList + (size_t)(j0 + u) * cap + i) : 0u;
+        uint32_t en[kHaloBatch];
 #pragma unroll
         for (uint32_t u = 0; u < kHaloBatch; ++u) {
+            en[u] = 0u;
             if (j0 + u >= n) continue;
             const uint32_t t = tb[u];
             const uint32_t b = t >> 5, below = (1u << (t & 31u)) - 1u;
@@ -718,8 +722,12 @@ __global__ void __launch_bounds__(kGroupSize) k_build_halo(const SceneParams* __
             while (bkey[h] != b) h = (h + 1) & (kBucketSlots - 1);
             const uint32_t s = bbase[h] + __popc(bmask[h] & below);
             const uint32_t code = gcode<false>(R, invStep, xi - mk3(hpos[3 * s], hpos[3 * s + 1], hpos[3 * s + 2]));
-            nbrLocal[(size_t)(j0 + u) * cap + i] = s | (code << 16);
+            en[u] = s | (code << 16);
         }
+        uint2* out = reinterpret_cast<uint2*>(nbrLocal) + (size_t)(j0 >> 1) * cap + i;
+#pragma unroll
+        for (uint32_t u = 0; u < kHaloBatch; u += 2)
+            if (j0 + u < n) out[(size_t)(u >> 1) * cap] = make_uint2(en[u], en[u + 1]);
     }
 }
 

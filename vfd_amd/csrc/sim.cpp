@@ -975,9 +975,9 @@ public:
             s->prm.bodyCount = (uint32_t)s->bodies.size();
         }
         cap = tiles * kTile;
-        // a neighbour-list row of cap entries is read as one buffer whose byte offsets must stay below 2^31
-        if ((uint64_t)cap * sizeof(uint32_t) >= (1ull << 31))
-            return fail(VFD_ERROR_NOT_SUPPORTED, "at most 2^29 particle slots in one simulation");
+        // a pair of staged-list rows (cap 8-byte entries) is read as one buffer whose byte offsets stay below 2^31
+        if ((uint64_t)cap * 2 * sizeof(uint32_t) >= (1ull << 31))
+            return fail(VFD_ERROR_NOT_SUPPORTED, "at most 2^28 particle slots in one simulation");
         const uint32_t S = (uint32_t)sc.size();
         std::vector<uint32_t> ts(tiles);
         for (uint32_t k = 0; k < S; ++k)
