@@ -42,7 +42,7 @@ AXES = {
     "VFD_LIST_CHUNK": [4, 8],
     "VFD_LIST_PREFETCH": [0, 1, 2],
     "VFD_LIST_CHUNK2": [2, 4, 8],
-    "VFD_LIST_PREFETCH2": [1, 2],
+    "VFD_LIST_PREFETCH2": [1, 2, 3],
     "VFD_SEARCH_UNROLL": [4, 8],
     "VFD_STAGE_LISTS": [0, 1],
     "VFD_PACK_RECORDS": [0, 1, 2],
